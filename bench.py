@@ -1,0 +1,138 @@
+"""Benchmark: ray-bounces/s of the stochastic split ray tracer on one MI355X.
+
+Workload: the seeded polyhedral hall (fan-shaped plan extruded to 12 m,
+~2,000 jittered triangles, per-surface random absorption and scattering per
+band, 16 receivers r = 0.5 m), 1e6 geodesic source rays, 3 s cut-off, 1 ms
+bins, 32^3 voxel grid, FP32 traversal with FP64 histograms.  One step is one
+complete trace (SplitRayTracer.Begin -> Combine) of all rays; the inputs are
+resident in HBM before the timed region starts.
+
+Prints one JSON line (see DESIGN.md for the field definitions).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+BYTES_OCT_BOUNCE = 128      # read ray 40 B + write ray 40 B + hit triangle 48 B
+BYTES_BROAD_BOUNCE = 184    # broadband ray 68 B x 2 + hit triangle 48 B
+BYTES_SPAWN = 40            # each spawned octave ray written once
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--rays", type=int, default=1_000_000)
+    ap.add_argument("--cutoff", type=float, default=3.0)
+    ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
+    ap.add_argument("--grid", type=int, default=32)
+    ap.add_argument("--cpu-rays", type=int, default=400, help="source rays of the CPU sample")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build traces on one GPU only; the multi-GPU split (disjoint "
+              "source-ray ranges plus one histogram all-reduce) is not implemented", file=sys.stderr)
+        sys.exit(2)
+
+    from pachydermacoustic_universal_amd import _abi as A
+    from pachydermacoustic_universal_amd import rooms
+
+    room = rooms.fan_hall()
+    at, c, rho = A.uniform_medium()
+    prec = A.PREC_FP64 if args.precision == "fp64" else A.PREC_FP32
+    tr = A.Tracer(0)
+    sr = 1000
+    sct = int(np.floor(args.cutoff * 1000 * sr / 1000))
+    tr.upload_scene(room.verts, room.surf, room.alpha, room.scat, room.trans, (args.grid,) * 3)
+    tr.set_source(room.source, np.ones(8), A.DIR_GEODESIC)
+    tr.set_receivers(room.receivers, room.radius, at, c, c * rho, sr, sct)
+
+    def step(seed):
+        tr.trace(rays=args.rays, cutoff_m=args.cutoff * c, seed=seed, precision=prec)
+        return tr.stats()
+
+    for w in range(args.warmup):
+        step(args.seed + 1000 + w)
+        print(f"[bench] warmup {w} done", file=sys.stderr, flush=True)
+    tot = dict(bounces=0, oct=0, broad=0, spawn=0, ms_oct=0.0, n_oct=0, ms_p1=0.0)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        st = step(args.seed + k)
+        tot["bounces"] += st["bounces"]
+        tot["oct"] += st["bounces_late"] + st["bounces_verylate"]
+        tot["broad"] += st["bounces_broad"]
+        tot["spawn"] += st["spawned"]
+        tot["ms_oct"] += st["ms_octave"]
+        tot["n_oct"] += st["launches_octave"]
+        tot["ms_p1"] += st["ms_phase1"]
+        print(f"[bench] step {k}: {st['bounces']:.3e} bounces in {st['ms_trace']:.1f} ms",
+              file=sys.stderr, flush=True)
+    dt = time.perf_counter() - t0
+    value = tot["bounces"] / dt
+    # dominant kernel: the persistent octave-ray kernel
+    oct_bytes = tot["oct"] * BYTES_OCT_BOUNCE + tot["spawn"] * BYTES_SPAWN
+    achieved = oct_bytes / (tot["ms_oct"] / 1e3) / 1e9 if tot["ms_oct"] > 0 else 0.0
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as O
+
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        med = O.Medium(c=c, atten=at, rho=rho)
+        t1 = time.perf_counter()
+        _, cnt = O.trace(room, rays=args.cpu_rays, cutoff_s=args.cutoff, seed=args.seed,
+                         medium=med, grid=(args.grid,) * 3, threads=threads)
+        cdt = time.perf_counter() - t1
+        cb = cnt["bounces_broad"] + cnt["bounces_late"] + cnt["bounces_verylate"]
+        cpu = {"value": cb / cdt, "unit": "ray-bounces/s", "cores": threads, "kind": "port",
+               "sample": f"{args.cpu_rays} source rays of the same hall workload "
+                         f"({cb:.3e} bounces, {cdt:.1f} s), C FP64 restatement, "
+                         f"{threads} threads on {cpu_model()} (nproc={os.cpu_count()})"}
+    out = {
+        "metric": "ray-bounces/s", "value": value, "unit": "ray-bounces/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": args.precision, "data": "synthetic",
+        "config": {"workload": "seeded polyhedral hall, ~2000 tris, 16 receivers, "
+                               f"{args.rays} rays, {args.cutoff} s cut-off, {args.grid}^3 grid",
+                   "rays": args.rays, "triangles": room.n_tris, "receivers": 16,
+                   "bounces_per_step": tot["bounces"] / args.steps,
+                   "rays_per_s": args.rays * args.steps / dt},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_octave", "avg_launch_ms": tot["ms_oct"] / max(tot["n_oct"], 1),
+                     "launches": tot["n_oct"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    tr.close()
+
+
+if __name__ == "__main__":
+    main()

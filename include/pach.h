@@ -1,0 +1,178 @@
+/*
+ * pach.h -- C ABI of the MI355X stochastic split ray tracer (libpach.so).
+ *
+ * The library re-implements Pachyderm's Complete_Trace ray loop
+ * (SplitRayTracer, "Classes_Complete_Trace - TaskAll.cs":33-536) on one
+ * gfx950 GPU.  A host in any language (the C# GpuSplitRayTracer shim, Python
+ * ctypes, C++) flattens Scene / Source / Receiver_Bank into the plain arrays
+ * below and calls these entry points.
+ *
+ * Conventions
+ *   - Every entry point returns 0 on success and a negative PACH_E* code on
+ *     failure; pach_last_error(ctx) then describes the failure.  No C++
+ *     exception or longjmp crosses this boundary.
+ *   - The caller owns every buffer.  Inputs are copied during the call;
+ *     outputs are written into caller-allocated arrays.
+ *   - A context is used from one host thread at a time.  Several contexts
+ *     may coexist; each owns its own HIP stream.
+ *   - Histogram layout is [R][8][sample_ct] doubles, band-major per
+ *     receiver, the layout of Spherical_Receiver.Recs (Classes_Receivers.cs:1694-1760).
+ */
+#ifndef PACH_H
+#define PACH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PACH_ABI_VERSION 1
+
+/* error codes */
+#define PACH_OK 0
+#define PACH_EINVAL (-1)   /* bad argument */
+#define PACH_EHIP (-2)     /* HIP runtime failure */
+#define PACH_ESTATE (-3)   /* call out of order (no scene, trace running, ...) */
+#define PACH_ENOMEM (-4)   /* device allocation failed */
+#define PACH_ECANCEL (-5)  /* trace was cancelled (results hold completed rays) */
+
+/* trace modes */
+#define PACH_MODE_STOCHASTIC 0  /* Lambert split scheme of Lambert_Scattering */
+#define PACH_MODE_SPECULAR 1    /* deterministic: scattering/transmission off */
+
+/* geometry precision of the traversal and reflection arithmetic */
+#define PACH_PREC_FP32 0
+#define PACH_PREC_FP64 1
+
+/* source direction modes */
+#define PACH_DIR_GEODESIC 0   /* GeodesicSource (Classes_Sources.cs:272-404) */
+#define PACH_DIR_RANDOM 1     /* RandomSource (Classes_Sources.cs:238-266) */
+#define PACH_DIR_SUPPLIED 2   /* caller's direction list, source ray i uses dirs[i % n] */
+
+/* histogram planes (Directional_Histogram.Add, Classes_Receivers.cs:1813-1825,
+ * and Reflection_Analysis_Data, :857-875) */
+#define PACH_PLANE_ENERGY 0
+#define PACH_PLANE_PRESSURE 1
+#define PACH_PLANE_POS_X 2
+#define PACH_PLANE_POS_Y 3
+#define PACH_PLANE_POS_Z 4
+#define PACH_PLANE_NEG_X 5
+#define PACH_PLANE_NEG_Y 6
+#define PACH_PLANE_NEG_Z 7
+#define PACH_PLANE_SPEC 8        /* specular, never scattered */
+#define PACH_PLANE_DIFFUSE 9     /* currently scattered */
+#define PACH_PLANE_SPEC_SCAT 10  /* specular after an earlier scatter */
+#define PACH_NPLANES 11
+
+typedef struct pach_ctx pach_ctx;
+
+typedef struct pach_params {
+    double cutoff_m;      /* SplitRayTracer ctor `cutoffTime`: a path length in m; COTime = cutoff_m / c (TaskAll:83) */
+    int oct_lo, oct_hi;   /* octaveRange[0], octaveRange[1] (TaskAll:81-82) */
+    int is_order;         /* IS_Order: broadband deposits only for order > is_order (TaskAll:366, :378) */
+    int mode;             /* PACH_MODE_* */
+    int precision;        /* PACH_PREC_* */
+    int async;            /* 0: return when done; 1: return at once, use pach_poll/pach_wait */
+    int64_t ray_count;    /* Raycount (> 0) */
+    uint64_t seed;        /* Philox key of every ray stream */
+    int p1_max_bounces;   /* broadband-phase bounce cap (0 = 128) */
+    int max_bounces;      /* per octave ray bounce guard, phases 2+3 (0 = 100000) */
+    int64_t batch_rays;   /* source rays per phase-1 launch (0 = auto) */
+} pach_params;
+
+typedef struct pach_progress {
+    int64_t rays_done;    /* _currentRay (TaskAll:40): completed source rays */
+    int64_t rays_total;
+    int64_t octave_rays;  /* _rayTotal analogue (sub-rays traced) */
+    int running;          /* 1 = ThreadState Running, 0 = Stopped */
+    int status;           /* last trace status (0 or PACH_E*) */
+} pach_progress;
+
+typedef struct pach_stats {
+    int64_t bounces_broad;     /* closest-hit queries by broadband rays (incl. misses) */
+    int64_t bounces_late;      /* ... by octave rays in the split phase (TaskAll:434-473) */
+    int64_t bounces_verylate;  /* ... in End_Trace (TaskAll:498-534) */
+    int64_t octave_rays;       /* octave rays created (split + scatter + transmission) */
+    int64_t spawned;           /* of those, created by Scatter_Early/Late or transmission */
+    int64_t misses;
+    int64_t deposits;          /* receiver histogram additions */
+    int64_t p1_cap_hits;       /* broadband rays stopped by p1_max_bounces */
+    int64_t guard_hits;        /* octave rays stopped by max_bounces */
+    int64_t spills;            /* rays moved to the overflow queue */
+    int64_t resumes;           /* broadband rays resumed after an output-buffer overflow */
+    double lost;               /* energy of rays leaving the model (TaskAll:372, :440, :530) */
+    double e_init;             /* sum of initial ray energies (TaskAll:350) */
+    int h_oct;                 /* band steering the broadband phase (TaskAll:88-109) */
+    int64_t rays_done;
+    double ms_phase1;          /* summed HIP-event durations of the broadband kernel */
+    double ms_octave;          /* ... of the persistent octave-ray kernel */
+    int64_t launches_phase1;
+    int64_t launches_octave;
+    double ms_trace;           /* wall time of the last trace (host clock) */
+} pach_stats;
+
+/* Context lifetime.  device: HIP device ordinal. */
+int pach_create(pach_ctx **out, int device);
+void pach_destroy(pach_ctx *ctx);
+const char *pach_last_error(const pach_ctx *ctx);
+int pach_abi_version(void);
+
+/* Uniform_Medium (Classes_Medium_Props.cs:283-296, :42-190; Classes_Math.cs:272):
+ * octave attenuation in dB/m, speed of sound and density of air. */
+int pach_uniform_medium(int air_choice, double temp_c, double hr, double pa_hpa,
+                        int edge_correction, double atten_db_per_m[8], double *c,
+                        double *rho);
+
+/* Polygon_Scene.Construct + partition (Classes_Scenes.cs:447-630, :709-755).
+ * verts [T][3][3]: triangles of the triangulated polygons; surf_of_tri [T]:
+ * polygon index (Surf_ID) of each triangle; alpha/scat/trans [S][8] per
+ * polygon: Coefficient_A_Broad, Scattering Coefficient, TransmissionValue.
+ * grid_{x,y,z}: voxel counts of the uniform grid (<= 0: chosen automatically). */
+int pach_scene_upload(pach_ctx *ctx, const double *verts, int T, const int *surf_of_tri,
+                      int S, const double *alpha, const double *scat, const double *trans,
+                      int grid_x, int grid_y, int grid_z);
+
+/* Source origin, per-band power in W (Source.SoundPower, Classes_Sources.cs:96-104)
+ * and direction mode.  dirs [n_dirs][3] only for PACH_DIR_SUPPLIED. */
+int pach_source_set(pach_ctx *ctx, const double origin[3], const double power_w[8],
+                    int dir_mode, const double *dirs, int64_t n_dirs);
+
+/* Receiver_Bank of spherical receivers (Classes_Receivers.cs:139-164, :748-767).
+ * radius: the reference hard-codes 1 m (:751); SizeMod = 1/(pi r^3) reduces to
+ * the reference's 1/pi at r = 1.  atten: medium attenuation in dB/m; c, rho_c
+ * as Uniform_Medium; sample_ct = floor(COTime_ms * fs / 1000) (:144). */
+int pach_receivers_set(pach_ctx *ctx, const double *centers, int R, double radius,
+                       const double atten_db_per_m[8], double c, double rho_c,
+                       int sample_rate, int sample_ct);
+
+/* SplitRayTracer(...).Begin() (TaskAll:68-240).  Clears the histograms first. */
+int pach_trace(pach_ctx *ctx, const pach_params *params);
+/* ThreadState / ProgressMsg analogue (TaskAll:246-252, :742-763). */
+int pach_poll(pach_ctx *ctx, pach_progress *out);
+/* Block until an async trace ends; returns its status. */
+int pach_wait(pach_ctx *ctx);
+/* Cooperative cancellation (I_Conv_Progress.CancellationToken, TaskAll:276-280):
+ * the trace stops after the batch in flight; results cover completed rays. */
+int pach_cancel(pach_ctx *ctx);
+
+/* Copy one histogram plane [R][8][sample_ct].  scaled != 0 applies
+ * Combine_ThreadLocal_Results' Receiver_Bank.Scale(rays_done)
+ * (Classes_Receivers.cs:1088-1102, :1774-1786): pressure / sqrt(N), other
+ * planes / N.  scaled == 0 returns raw sums. */
+int pach_histogram(pach_ctx *ctx, int plane, double *out, int scaled);
+int pach_stats_get(pach_ctx *ctx, pach_stats *out);
+/* PercentLost (TaskAll:773-782) */
+int pach_percent_lost(pach_ctx *ctx, double *out);
+
+/* Decay-curve integration, FP64 (Classes_Math.cs:282-585). */
+int pach_schroeder_integral(const double *etc, int n, double *out);
+int pach_t_x(const double *schroeder, int n, int decay_depth, int sample_rate, double *out);
+int pach_early_decay_time(const double *schroeder, int n, int sample_rate, double *out);
+int pach_clarity(const double *etc, int n, double sample_rate, double seconds,
+                 double start_time, int pressure, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

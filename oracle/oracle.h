@@ -1,0 +1,137 @@
+/*
+ * oracle.h -- CPU FP64 restatement of Pachyderm's stochastic split ray tracer.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP tracer in
+ * pachydermacoustic_universal_amd/.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it, and only to compute expected values
+ * or to time the CPU baseline.  The product never links or calls it.
+ *
+ * Reference files restated (paths relative to Pachyderm_Acoustic_Universal/):
+ *   Classes_Complete_Trace - TaskAll.cs   (SplitRayTracer: ctor :68-177,
+ *                                          cast_ray_distributed :335-486,
+ *                                          End_Trace :489-536, Combine :768-774)
+ *   Classes_Rays.cs                        (OctaveRay/BroadRay SplitRay, masks)
+ *   Classes_Materials.cs                   (Basic_Material.Absorb :155-190,
+ *                                          Lambert_Scattering :4463-4702)
+ *   Classes_Receivers.cs                   (Spherical_Receiver.CheckRay :803-877,
+ *                                          Directional_Histogram.Add :1813-1825,
+ *                                          Scale :1088-1102, :1774-1786)
+ *   Classes_Sources.cs                     (Source power :50-104, RandomSource
+ *                                          :238-266, GeodesicSource :272-404)
+ *   Classes_Medium_Props.cs                (Calculate_Attenuation :52-190,
+ *                                          Calculate_Density :42-49, Uniform_Medium)
+ *   Classes_Math.cs                        (SoundSpeed :272, Schroeder/T_X/EDT/
+ *                                          Clarity :282-585, Sabine/Eyring :216-270)
+ *
+ * Parity status of each part (how it is pinned) is listed in oracle.c's header.
+ */
+#ifndef PACH_ORACLE_H
+#define PACH_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OR_NBANDS 8
+#define OR_NPLANES 11   /* E, P, +x,+y,+z, -x,-y,-z, spec, diffuse, spec-after-scatter */
+
+enum { OR_MODE_STOCHASTIC = 0, OR_MODE_SPECULAR = 1 };
+enum { OR_DIR_GEODESIC = 0, OR_DIR_RANDOM = 1, OR_DIR_SUPPLIED = 2 };
+
+typedef struct or_scene or_scene;
+
+/* --- RNG ---------------------------------------------------------------- */
+void or_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+uint64_t or_mix64(uint64_t z);
+uint64_t or_child_uid(uint64_t parent_uid, uint64_t tag);
+
+/* --- medium --------------------------------------------------------------- */
+/* Octave-band attenuation in dB/m exactly as Calculate_Attenuation computes it
+ * (C# integer-division exponents included).  pa_hpa is the Scene pressure in
+ * hPa; Uniform_Medium passes pa_hpa/10 (kPa) to the formula. */
+void or_attenuation(int air_choice, double temp_c, double hr, double pa_hpa,
+                    int edge_correction, double out_db_per_m[8]);
+double or_density(double temp_c, double hr, double pa_hpa);
+double or_sound_speed(double temp_c);
+
+/* --- scene ---------------------------------------------------------------- */
+/* verts: [T][3][3]; surf: [T] polygon id; alpha/scat/trans: [S][8]. */
+or_scene *or_scene_create(const double *verts, int T, const int *surf, int S,
+                          const double *alpha, const double *scat, const double *trans,
+                          int gx, int gy, int gz);
+void or_scene_destroy(or_scene *s);
+/* closest hit with t > 1e-10; returns triangle index or -1 */
+int or_shoot_brute(const or_scene *s, const double o[3], const double d[3], double *t_out);
+int or_shoot_grid(const or_scene *s, const double o[3], const double d[3], double *t_out);
+double or_surface_area(const or_scene *s, int surf);
+void or_tri_normal(const or_scene *s, int tri, double n[3]);
+
+/* --- sources -------------------------------------------------------------- */
+/* 80-face order-1 geodesic sphere of GeodesicSource.GeoSphere(1); out [80][3][3] */
+void or_geodesic_faces(double out[80][3][3]);
+void or_source_direction(int dir_mode, uint64_t seed, int64_t src_id,
+                         const double *dirs, int64_t n_dirs, double d[3]);
+
+/* --- tracer --------------------------------------------------------------- */
+typedef struct {
+    double cutoff_m;        /* SplitRayTracer cutoffTime: path length in m (COTime = cutoff/c) */
+    int oct_lo, oct_hi;     /* octaveRange */
+    int is_order;           /* IS_Order */
+    int64_t ray_begin;      /* first source-ray index traced */
+    int64_t ray_end;        /* one past the last */
+    uint64_t seed;
+    int mode;               /* OR_MODE_* */
+    int dir_mode;           /* OR_DIR_* */
+    const double *dirs;     /* supplied directions [n_dirs][3] */
+    int64_t n_dirs;
+    int p1_max_bounces;     /* broadband-phase bounce cap */
+    int max_bounces;        /* octave-ray bounce guard (phases 2+3) */
+    int use_grid;           /* 1: voxel DDA, 0: brute force */
+} or_params;
+
+typedef struct {
+    double src[3];
+    double power[8];        /* W per band (Source.SoundPower) */
+    int R;
+    const double *centers;  /* [R][3] */
+    double radius;
+    double c;               /* speed of sound m/s */
+    double atten[8];        /* dB/m */
+    double rho_c;
+    int sample_rate;
+    int sample_ct;
+} or_setup;
+
+typedef struct {
+    int64_t bounces_broad, bounces_late, bounces_verylate;
+    int64_t spawned, misses, deposits, octave_rays;
+    int64_t p1_cap_hits, guard_hits;
+    double lost, e_init;
+    int h_oct;
+} or_counters;
+
+/* planes: OR_NPLANES pointers (NULL to skip), each [R][8][sample_ct], accumulated (+=). */
+int or_trace(const or_scene *s, const or_setup *su, const or_params *p,
+             double **planes, or_counters *cnt);
+/* multi-threaded CPU baseline: splits [ray_begin, ray_end) over n_threads */
+int or_trace_mt(const or_scene *s, const or_setup *su, const or_params *p,
+                double **planes, or_counters *cnt, int n_threads);
+int or_h_oct(const or_scene *s, const double power[8], int oct_lo, int oct_hi);
+
+/* --- decay-curve integration (Classes_Math.cs) ---------------------------- */
+void or_schroeder(const double *etc, int n, double *out);
+void or_log10data(const double *x, int n, double limit_db, double *out);
+void or_schroeder_limits(const double *log_sch, int n, double first, double second, int lim[2]);
+void or_lsf(const double *mag, int n, double fs, double out[2]);
+double or_t_x(const double *sch, int n, int depth, int fs);
+double or_edt(const double *sch, int n, int fs);
+double or_clarity(const double *etc, int n, double fs, double seconds, double start, int pressure);
+void or_sabine(const or_scene *s, double volume, const double atten[8], double out[8]);
+void or_eyring(const or_scene *s, double volume, const double atten[8], double out[8]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

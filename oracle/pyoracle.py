@@ -1,0 +1,299 @@
+"""ctypes front end of the CPU FP64 oracle (oracle/oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+
+NPLANES = 11
+PLANE_NAMES = ("E", "P", "px", "py", "pz", "nx", "ny", "nz", "spec", "diff", "spec_scat")
+MODE_STOCHASTIC, MODE_SPECULAR = 0, 1
+DIR_GEODESIC, DIR_RANDOM, DIR_SUPPLIED = 0, 1, 2
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class Params(C.Structure):
+    _fields_ = [("cutoff_m", C.c_double), ("oct_lo", C.c_int), ("oct_hi", C.c_int),
+                ("is_order", C.c_int), ("ray_begin", C.c_int64), ("ray_end", C.c_int64),
+                ("seed", C.c_uint64), ("mode", C.c_int), ("dir_mode", C.c_int),
+                ("dirs", _dp), ("n_dirs", C.c_int64), ("p1_max_bounces", C.c_int),
+                ("max_bounces", C.c_int), ("use_grid", C.c_int)]
+
+
+class Setup(C.Structure):
+    _fields_ = [("src", C.c_double * 3), ("power", C.c_double * 8), ("R", C.c_int),
+                ("centers", _dp), ("radius", C.c_double), ("c", C.c_double),
+                ("atten", C.c_double * 8), ("rho_c", C.c_double), ("sample_rate", C.c_int),
+                ("sample_ct", C.c_int)]
+
+
+class Counters(C.Structure):
+    _fields_ = [("bounces_broad", C.c_int64), ("bounces_late", C.c_int64),
+                ("bounces_verylate", C.c_int64), ("spawned", C.c_int64),
+                ("misses", C.c_int64), ("deposits", C.c_int64), ("octave_rays", C.c_int64),
+                ("p1_cap_hits", C.c_int64), ("guard_hits", C.c_int64), ("lost", C.c_double),
+                ("e_init", C.c_double), ("h_oct", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        L.or_philox4x32_10.argtypes = [C.POINTER(C.c_uint32)] * 3
+        L.or_mix64.restype = C.c_uint64
+        L.or_mix64.argtypes = [C.c_uint64]
+        L.or_child_uid.restype = C.c_uint64
+        L.or_child_uid.argtypes = [C.c_uint64, C.c_uint64]
+        L.or_attenuation.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, _dp]
+        L.or_density.restype = C.c_double
+        L.or_density.argtypes = [C.c_double] * 3
+        L.or_sound_speed.restype = C.c_double
+        L.or_sound_speed.argtypes = [C.c_double]
+        L.or_scene_create.restype = C.c_void_p
+        L.or_scene_create.argtypes = [_dp, C.c_int, _ip, C.c_int, _dp, _dp, _dp,
+                                      C.c_int, C.c_int, C.c_int]
+        L.or_scene_destroy.argtypes = [C.c_void_p]
+        L.or_shoot_brute.restype = C.c_int
+        L.or_shoot_brute.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.or_shoot_grid.restype = C.c_int
+        L.or_shoot_grid.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.or_surface_area.restype = C.c_double
+        L.or_surface_area.argtypes = [C.c_void_p, C.c_int]
+        L.or_geodesic_faces.argtypes = [_dp]
+        L.or_source_direction.argtypes = [C.c_int, C.c_uint64, C.c_int64, _dp, C.c_int64, _dp]
+        L.or_trace.restype = C.c_int
+        L.or_trace.argtypes = [C.c_void_p, C.POINTER(Setup), C.POINTER(Params),
+                               C.POINTER(_dp), C.POINTER(Counters)]
+        L.or_trace_mt.restype = C.c_int
+        L.or_trace_mt.argtypes = [C.c_void_p, C.POINTER(Setup), C.POINTER(Params),
+                                  C.POINTER(_dp), C.POINTER(Counters), C.c_int]
+        L.or_h_oct.restype = C.c_int
+        L.or_h_oct.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int]
+        for name in ("or_schroeder",):
+            getattr(L, name).argtypes = [_dp, C.c_int, _dp]
+        L.or_log10data.argtypes = [_dp, C.c_int, C.c_double, _dp]
+        L.or_t_x.restype = C.c_double
+        L.or_t_x.argtypes = [_dp, C.c_int, C.c_int, C.c_int]
+        L.or_edt.restype = C.c_double
+        L.or_edt.argtypes = [_dp, C.c_int, C.c_int]
+        L.or_clarity.restype = C.c_double
+        L.or_clarity.argtypes = [_dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int]
+        L.or_sabine.argtypes = [C.c_void_p, C.c_double, _dp, _dp]
+        L.or_eyring.argtypes = [C.c_void_p, C.c_double, _dp, _dp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().or_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def attenuation(air_choice=0, temp_c=20.0, hr=50.0, pa_hpa=1013.25, edge=False):
+    out = np.zeros(8)
+    lib().or_attenuation(air_choice, temp_c, hr, pa_hpa, int(edge), _p(out))
+    return out
+
+
+def density(temp_c=20.0, hr=50.0, pa_hpa=1013.25):
+    return lib().or_density(temp_c, hr, pa_hpa)
+
+
+def sound_speed(temp_c=20.0):
+    return lib().or_sound_speed(temp_c)
+
+
+@dataclass
+class Medium:
+    c: float
+    atten: np.ndarray
+    rho: float
+
+    @property
+    def rho_c(self):
+        return self.c * self.rho
+
+
+def standard_medium(temp_c=20.0, hr=50.0, pa_hpa=1013.25, air_choice=0):
+    return Medium(c=sound_speed(temp_c), atten=attenuation(air_choice, temp_c, hr, pa_hpa),
+                  rho=density(temp_c, hr, pa_hpa))
+
+
+class Scene:
+    """Oracle scene (triangles + per-polygon materials + its own voxel grid)."""
+
+    def __init__(self, room, grid=(16, 16, 16)):
+        self._v = np.ascontiguousarray(room.verts, dtype=np.float64)
+        self._s = np.ascontiguousarray(room.surf, dtype=np.int32)
+        self._a = np.ascontiguousarray(room.alpha, dtype=np.float64)
+        self._sc = np.ascontiguousarray(room.scat, dtype=np.float64)
+        self._t = np.ascontiguousarray(room.trans, dtype=np.float64)
+        g = grid if hasattr(grid, "__len__") else (grid, grid, grid)
+        self.h = lib().or_scene_create(_p(self._v), self._v.shape[0],
+                                       self._s.ctypes.data_as(_ip), self._a.shape[0],
+                                       _p(self._a), _p(self._sc), _p(self._t), *map(int, g))
+        if not self.h:
+            raise ValueError("oracle scene creation failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_scene_destroy(self.h)
+            self.h = None
+
+    def shoot(self, o, d, grid=True):
+        o = np.ascontiguousarray(o, dtype=np.float64)
+        d = np.ascontiguousarray(d, dtype=np.float64)
+        t = np.zeros(1)
+        f = lib().or_shoot_grid if grid else lib().or_shoot_brute
+        tri = f(self.h, _p(o), _p(d), _p(t))
+        return tri, (float(t[0]) if tri >= 0 else None)
+
+    def surface_area(self, s):
+        return lib().or_surface_area(self.h, s)
+
+    def h_oct(self, power, oct_lo=0, oct_hi=7):
+        pw = np.ascontiguousarray(power, dtype=np.float64)
+        return lib().or_h_oct(self.h, _p(pw), oct_lo, oct_hi)
+
+    def sabine(self, volume, atten):
+        out = np.zeros(8)
+        a = np.ascontiguousarray(atten, dtype=np.float64)
+        lib().or_sabine(self.h, volume, _p(a), _p(out))
+        return out
+
+    def eyring(self, volume, atten):
+        out = np.zeros(8)
+        a = np.ascontiguousarray(atten, dtype=np.float64)
+        lib().or_eyring(self.h, volume, _p(a), _p(out))
+        return out
+
+
+def source_direction(dir_mode, seed, src_id, dirs=None):
+    d = np.zeros(3)
+    if dirs is not None:
+        dd = np.ascontiguousarray(dirs, dtype=np.float64)
+        lib().or_source_direction(dir_mode, seed, src_id, _p(dd), dd.shape[0], _p(d))
+    else:
+        lib().or_source_direction(dir_mode, seed, src_id, None, 0, _p(d))
+    return d
+
+
+def geodesic_faces():
+    out = np.zeros((80, 3, 3))
+    lib().or_geodesic_faces(_p(out))
+    return out
+
+
+def trace(room, *, rays, cutoff_s, seed=1, mode=MODE_STOCHASTIC, dir_mode=DIR_GEODESIC,
+          dirs=None, oct_lo=0, oct_hi=7, is_order=-1, sample_rate=1000, sample_ct=None,
+          power=None, medium=None, grid=(16, 16, 16), use_grid=True, ray_begin=0,
+          p1_max_bounces=128, max_bounces=100000, threads=1, planes=NPLANES, scene=None):
+    """Trace ``rays`` source rays; returns (planes[11, R, 8, sample_ct], counters dict).
+
+    Histograms are raw sums (not divided by the ray count), i.e. what the
+    receivers hold before Combine_ThreadLocal_Results' Scale."""
+    med = medium or standard_medium()
+    if sample_ct is None:
+        sample_ct = int(np.floor(cutoff_s * 1000 * sample_rate / 1000))
+    sc = scene or Scene(room, grid)
+    R = room.receivers.shape[0]
+    su = Setup()
+    su.src[:] = list(map(float, room.source))
+    pw = np.ones(8) if power is None else np.asarray(power, dtype=np.float64)
+    su.power[:] = list(pw)
+    centers = np.ascontiguousarray(room.receivers, dtype=np.float64)
+    su.R = R
+    su.centers = _p(centers)
+    su.radius = room.radius
+    su.c = med.c
+    su.atten[:] = list(med.atten)
+    su.rho_c = med.rho_c
+    su.sample_rate = sample_rate
+    su.sample_ct = sample_ct
+    p = Params()
+    p.cutoff_m = cutoff_s * med.c
+    p.oct_lo, p.oct_hi, p.is_order = oct_lo, oct_hi, is_order
+    p.ray_begin, p.ray_end = ray_begin, ray_begin + rays
+    p.seed = seed
+    p.mode, p.dir_mode = mode, dir_mode
+    dd = None
+    if dirs is not None:
+        dd = np.ascontiguousarray(dirs, dtype=np.float64)
+        p.dirs = _p(dd)
+        p.n_dirs = dd.shape[0]
+    p.p1_max_bounces, p.max_bounces, p.use_grid = p1_max_bounces, max_bounces, int(use_grid)
+    out = np.zeros((NPLANES, R, 8, sample_ct))
+    ptrs = (_dp * NPLANES)()
+    for k in range(NPLANES):
+        ptrs[k] = _p(out[k]) if k < planes else None
+    cnt = Counters()
+    if threads > 1:
+        rc = lib().or_trace_mt(sc.h, C.byref(su), C.byref(p), ptrs, C.byref(cnt), threads)
+    else:
+        rc = lib().or_trace(sc.h, C.byref(su), C.byref(p), ptrs, C.byref(cnt))
+    if rc != 0:
+        raise RuntimeError(f"or_trace failed: {rc}")
+    return out, cnt.as_dict()
+
+
+def scale(planes, n_rays):
+    """Combine_ThreadLocal_Results -> Receiver_Bank.Scale (Classes_Receivers.cs:1774-1786,
+    :1088-1102): energy, direction and analysis planes / N, pressure / sqrt(N)."""
+    out = planes / float(n_rays)
+    out[1] = planes[1] / np.sqrt(float(n_rays))
+    return out
+
+
+def schroeder(etc):
+    e = np.ascontiguousarray(etc, dtype=np.float64)
+    out = np.zeros_like(e)
+    lib().or_schroeder(_p(e), e.size, _p(out))
+    return out
+
+
+def t_x(sch, depth=30, fs=1000):
+    s = np.ascontiguousarray(sch, dtype=np.float64)
+    return lib().or_t_x(_p(s), s.size, depth, fs)
+
+
+def edt(sch, fs=1000):
+    s = np.ascontiguousarray(sch, dtype=np.float64)
+    return lib().or_edt(_p(s), s.size, fs)
+
+
+def clarity(etc, fs=1000.0, seconds=0.08, start=0.0, pressure=False):
+    e = np.ascontiguousarray(etc, dtype=np.float64)
+    return lib().or_clarity(_p(e), e.size, fs, seconds, start, int(pressure))
+
+
+def log10data(x, limit_db=-60.0):
+    a = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros_like(a)
+    lib().or_log10data(_p(a), a.size, limit_db, _p(out))
+    return out

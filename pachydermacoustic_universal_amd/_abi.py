@@ -1,0 +1,266 @@
+"""ctypes binding of libpach.so (include/pach.h).
+
+The library is the only compute path: if it is missing or cannot be loaded
+this module raises, and nothing falls back to a CPU implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libpach.so")
+
+PACH_OK = 0
+ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ESTATE", -4: "ENOMEM", -5: "ECANCEL"}
+MODE_STOCHASTIC, MODE_SPECULAR = 0, 1
+PREC_FP32, PREC_FP64 = 0, 1
+DIR_GEODESIC, DIR_RANDOM, DIR_SUPPLIED = 0, 1, 2
+NPLANES = 11
+PLANES = {"energy": 0, "pressure": 1, "pos_x": 2, "pos_y": 3, "pos_z": 4, "neg_x": 5,
+          "neg_y": 6, "neg_z": 7, "spec": 8, "diffuse": 9, "spec_scat": 10}
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class Params(C.Structure):
+    _fields_ = [("cutoff_m", C.c_double), ("oct_lo", C.c_int), ("oct_hi", C.c_int),
+                ("is_order", C.c_int), ("mode", C.c_int), ("precision", C.c_int),
+                ("async_", C.c_int), ("ray_count", C.c_int64), ("seed", C.c_uint64),
+                ("p1_max_bounces", C.c_int), ("max_bounces", C.c_int), ("batch_rays", C.c_int64)]
+
+
+class Progress(C.Structure):
+    _fields_ = [("rays_done", C.c_int64), ("rays_total", C.c_int64), ("octave_rays", C.c_int64),
+                ("running", C.c_int), ("status", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("bounces_broad", C.c_int64), ("bounces_late", C.c_int64),
+                ("bounces_verylate", C.c_int64), ("octave_rays", C.c_int64),
+                ("spawned", C.c_int64), ("misses", C.c_int64), ("deposits", C.c_int64),
+                ("p1_cap_hits", C.c_int64), ("guard_hits", C.c_int64), ("spills", C.c_int64),
+                ("resumes", C.c_int64), ("lost", C.c_double), ("e_init", C.c_double),
+                ("h_oct", C.c_int), ("rays_done", C.c_int64), ("ms_phase1", C.c_double),
+                ("ms_octave", C.c_double), ("launches_phase1", C.c_int64),
+                ("launches_octave", C.c_int64), ("ms_trace", C.c_double)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["bounces"] = d["bounces_broad"] + d["bounces_late"] + d["bounces_verylate"]
+        return d
+
+
+# (name, restype, argtypes) of every entry point declared in include/pach.h
+SIGNATURES = [
+    ("pach_abi_version", C.c_int, []),
+    ("pach_create", C.c_int, [C.POINTER(C.c_void_p), C.c_int]),
+    ("pach_destroy", None, [C.c_void_p]),
+    ("pach_last_error", C.c_char_p, [C.c_void_p]),
+    ("pach_uniform_medium", C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, _dp,
+                                      _dp, _dp]),
+    ("pach_scene_upload", C.c_int, [C.c_void_p, _dp, C.c_int, _ip, C.c_int, _dp, _dp, _dp,
+                                    C.c_int, C.c_int, C.c_int]),
+    ("pach_source_set", C.c_int, [C.c_void_p, _dp, _dp, C.c_int, _dp, C.c_int64]),
+    ("pach_receivers_set", C.c_int, [C.c_void_p, _dp, C.c_int, C.c_double, _dp, C.c_double,
+                                     C.c_double, C.c_int, C.c_int]),
+    ("pach_trace", C.c_int, [C.c_void_p, C.POINTER(Params)]),
+    ("pach_poll", C.c_int, [C.c_void_p, C.POINTER(Progress)]),
+    ("pach_wait", C.c_int, [C.c_void_p]),
+    ("pach_cancel", C.c_int, [C.c_void_p]),
+    ("pach_histogram", C.c_int, [C.c_void_p, C.c_int, _dp, C.c_int]),
+    ("pach_stats_get", C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    ("pach_percent_lost", C.c_int, [C.c_void_p, _dp]),
+    ("pach_schroeder_integral", C.c_int, [_dp, C.c_int, _dp]),
+    ("pach_t_x", C.c_int, [_dp, C.c_int, C.c_int, C.c_int, _dp]),
+    ("pach_early_decay_time", C.c_int, [_dp, C.c_int, C.c_int, _dp]),
+    ("pach_clarity", C.c_int, [_dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, _dp]),
+]
+
+_lib = None
+
+
+class PachError(RuntimeError):
+    def __init__(self, code, msg=""):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libpach.so (raises if it is absent: there is no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"libpach.so not built at {LIB_PATH}; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _chk(rc, ctx=None):
+    if rc != PACH_OK:
+        msg = lib().pach_last_error(ctx).decode() if ctx else ""
+        raise PachError(rc, msg)
+
+
+def uniform_medium(temp_c=20.0, hr=50.0, pa_hpa=1013.25, air_choice=0, edge_correction=False):
+    """(attenuation dB/m [8], speed of sound m/s, density kg/m^3) of Uniform_Medium."""
+    at = np.zeros(8)
+    c = C.c_double()
+    rho = C.c_double()
+    _chk(lib().pach_uniform_medium(air_choice, temp_c, hr, pa_hpa, int(edge_correction), _p(at),
+                                   C.byref(c), C.byref(rho)))
+    return at, c.value, rho.value
+
+
+def schroeder_integral(etc):
+    e = np.ascontiguousarray(etc, dtype=np.float64)
+    out = np.zeros_like(e)
+    _chk(lib().pach_schroeder_integral(_p(e), e.size, _p(out)))
+    return out
+
+
+def t_x(sch, decay_depth=30, sample_rate=1000):
+    s = np.ascontiguousarray(sch, dtype=np.float64)
+    out = C.c_double()
+    _chk(lib().pach_t_x(_p(s), s.size, decay_depth, sample_rate, C.byref(out)))
+    return out.value
+
+
+def early_decay_time(sch, sample_rate=1000):
+    s = np.ascontiguousarray(sch, dtype=np.float64)
+    out = C.c_double()
+    _chk(lib().pach_early_decay_time(_p(s), s.size, sample_rate, C.byref(out)))
+    return out.value
+
+
+def clarity(etc, sample_rate=1000.0, seconds=0.08, start_time=0.0, pressure=False):
+    e = np.ascontiguousarray(etc, dtype=np.float64)
+    out = C.c_double()
+    _chk(lib().pach_clarity(_p(e), e.size, sample_rate, seconds, start_time, int(pressure),
+                            C.byref(out)))
+    return out.value
+
+
+class Tracer:
+    """One libpach context on one GPU (a HIP stream of its own)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        _chk(lib().pach_create(C.byref(h), device))
+        self.h = h
+        self.R = 0
+        self.sample_ct = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pach_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, rc):
+        _chk(rc, self.h)
+
+    def upload_scene(self, verts, surf, alpha, scat, trans=None, grid=(0, 0, 0)):
+        v = np.ascontiguousarray(verts, dtype=np.float64).reshape(-1, 3, 3)
+        s = np.ascontiguousarray(surf, dtype=np.int32)
+        a = np.ascontiguousarray(alpha, dtype=np.float64).reshape(-1, 8)
+        sc = np.ascontiguousarray(scat, dtype=np.float64).reshape(-1, 8)
+        tr = None if trans is None else np.ascontiguousarray(trans, dtype=np.float64).reshape(-1, 8)
+        g = grid if hasattr(grid, "__len__") else (grid, grid, grid)
+        self._chk(lib().pach_scene_upload(self.h, _p(v), v.shape[0], s.ctypes.data_as(_ip),
+                                          a.shape[0], _p(a), _p(sc),
+                                          _p(tr) if tr is not None else None, *map(int, g)))
+
+    def set_source(self, origin, power_w, dir_mode=DIR_GEODESIC, dirs=None):
+        o = np.ascontiguousarray(origin, dtype=np.float64)
+        pw = np.ascontiguousarray(power_w, dtype=np.float64)
+        if dirs is not None:
+            d = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+            self._chk(lib().pach_source_set(self.h, _p(o), _p(pw), dir_mode, _p(d), d.shape[0]))
+        else:
+            self._chk(lib().pach_source_set(self.h, _p(o), _p(pw), dir_mode, None, 0))
+
+    def set_receivers(self, centers, radius, atten, c, rho_c, sample_rate, sample_ct):
+        ctr = np.ascontiguousarray(centers, dtype=np.float64).reshape(-1, 3)
+        at = np.ascontiguousarray(atten, dtype=np.float64)
+        self._chk(lib().pach_receivers_set(self.h, _p(ctr), ctr.shape[0], radius, _p(at), c, rho_c,
+                                           sample_rate, sample_ct))
+        self.R = ctr.shape[0]
+        self.sample_ct = sample_ct
+
+    def trace(self, *, rays, cutoff_m, seed=1, mode=MODE_STOCHASTIC, precision=PREC_FP32,
+              oct_lo=0, oct_hi=7, is_order=-1, p1_max_bounces=128, max_bounces=100000,
+              batch_rays=0, async_=False):
+        p = Params()
+        p.cutoff_m = cutoff_m
+        p.oct_lo, p.oct_hi, p.is_order = oct_lo, oct_hi, is_order
+        p.mode, p.precision, p.async_ = mode, precision, int(async_)
+        p.ray_count, p.seed = rays, seed
+        p.p1_max_bounces, p.max_bounces, p.batch_rays = p1_max_bounces, max_bounces, batch_rays
+        rc = lib().pach_trace(self.h, C.byref(p))
+        if rc == -5:
+            return rc
+        self._chk(rc)
+        return rc
+
+    def poll(self):
+        pr = Progress()
+        self._chk(lib().pach_poll(self.h, C.byref(pr)))
+        return {k: getattr(pr, k) for k, _ in pr._fields_}
+
+    def wait(self):
+        return lib().pach_wait(self.h)
+
+    def cancel(self):
+        self._chk(lib().pach_cancel(self.h))
+
+    def histogram(self, plane=0, scaled=True):
+        out = np.zeros((self.R, 8, self.sample_ct))
+        self._chk(lib().pach_histogram(self.h, plane, _p(out), int(scaled)))
+        return out
+
+    def planes(self, scaled=False):
+        return np.stack([self.histogram(k, scaled) for k in range(NPLANES)])
+
+    def stats(self):
+        st = Stats()
+        self._chk(lib().pach_stats_get(self.h, C.byref(st)))
+        return st.as_dict()
+
+    def percent_lost(self):
+        out = C.c_double()
+        self._chk(lib().pach_percent_lost(self.h, C.byref(out)))
+        return out.value
+
+
+def trace_room(tr: Tracer, room, *, rays, cutoff_s, seed=1, mode=MODE_STOCHASTIC,
+               precision=PREC_FP32, sample_rate=1000, grid=(0, 0, 0), dir_mode=DIR_GEODESIC,
+               dirs=None, power_w=None, medium=None, **kw):
+    """Upload a rooms.Room and trace it; returns the context's stats."""
+    at, c, rho = medium if medium is not None else uniform_medium()
+    tr.upload_scene(room.verts, room.surf, room.alpha, room.scat, room.trans, grid)
+    tr.set_source(room.source, np.ones(8) if power_w is None else power_w, dir_mode, dirs)
+    sct = int(np.floor(cutoff_s * 1000 * sample_rate / 1000))
+    tr.set_receivers(room.receivers, room.radius, at, c, c * rho, sample_rate, sct)
+    tr.trace(rays=rays, cutoff_m=cutoff_s * c, seed=seed, mode=mode, precision=precision, **kw)
+    return tr.stats()

@@ -1,0 +1,919 @@
+// runtime.cpp -- host side of libpach.so: the C ABI of include/pach.h.
+//
+// Owns the HIP stream, the scene / receiver buffers in HBM, and the trace
+// orchestration: the source rays are traced in batches; each batch's
+// broadband phase (k_phase1) feeds the persistent octave-ray kernel
+// (k_octave) together with the rays the previous launch handed over, until
+// every ray of every batch has terminated.
+#include "../../include/pach.h"
+#include "trace_types.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+using namespace pach;
+
+namespace {
+
+template <class R> struct SceneBuf {
+    typename V4<R>::T *v0 = nullptr, *e1 = nullptr, *e2 = nullptr, *nrm = nullptr, *mat = nullptr;
+    R *trans = nullptr;
+};
+
+template <class R> struct WorkBuf {
+    ORay<R> *p1_out = nullptr;
+    unsigned p1_cap = 0;
+    P1State<R> *res[2] = {nullptr, nullptr};
+    unsigned res_cap = 0;
+    ORay<R> *stacks = nullptr;
+    int stack_cap = 0;
+    size_t n_waves = 0;
+    ORay<R> *dump[2] = {nullptr, nullptr};
+    unsigned dump_cap = 0;
+};
+
+}  // namespace
+
+struct pach_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    int n_cu = 0;
+    std::string err;
+
+    // scene
+    bool has_scene = false;
+    int T = 0, S = 0;
+    std::vector<double> area;                 // [S]
+    std::vector<double> alpha, scat, trans;   // [S*8]
+    bool any_trans = false;
+    int g[3] = {0, 0, 0};
+    double gmin[3] = {0, 0, 0}, cs[3] = {1, 1, 1};
+    int *d_surf = nullptr;
+    uint32_t *d_cell_start = nullptr, *d_cell_tris = nullptr;
+    SceneBuf<float> s32;
+    SceneBuf<double> s64;
+
+    // source
+    bool has_src = false;
+    double origin[3] = {0, 0, 0}, power[8] = {0};
+    int dir_mode = PACH_DIR_GEODESIC;
+    double *d_dirs = nullptr;
+    int64_t n_dirs = 0;
+    double *d_geo = nullptr;
+
+    // receivers
+    bool has_recv = false;
+    int R = 0;
+    double radius = 1, atten[8] = {0}, c = 343, rho_c = 411, fs = 1000;
+    int sct = 0;
+    float4 *d_ctr32 = nullptr;
+    double4 *d_ctr64 = nullptr;
+    double *d_hist = nullptr;
+    size_t hist_n = 0;  // per plane
+
+    // counters and small device scalars
+    DevCounters *d_cnt = nullptr;
+    unsigned *d_scal = nullptr;  // [0] p1_count [1] res_count [2] in_head [3..4] dump_count[2]
+
+    WorkBuf<float> w32;
+    WorkBuf<double> w64;
+
+    // trace state
+    std::thread worker;
+    std::atomic<bool> running{false};
+    std::atomic<bool> cancel{false};
+    std::atomic<int64_t> rays_done{0};
+    std::atomic<int64_t> rays_total{0};
+    std::atomic<int64_t> oct_rays{0};
+    int status = 0;
+    pach_stats stats{};
+    std::mutex mu;
+};
+
+// ------------------------------------------------------------------ helpers
+#define HIPCHK(ctx, expr)                                                              \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);            \
+            return PACH_EHIP;                                                          \
+        }                                                                              \
+    } while (0)
+
+static int fail(pach_ctx *c, int code, const std::string &msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+template <class T> static void dfree(T *&p)
+{
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+template <class T> static hipError_t dalloc(T *&p, size_t n)
+{
+    dfree(p);
+    if (n == 0) n = 1;
+    return hipMalloc((void **)&p, n * sizeof(T));
+}
+
+template <class R> static void free_scene(SceneBuf<R> &s)
+{
+    dfree(s.v0); dfree(s.e1); dfree(s.e2); dfree(s.nrm); dfree(s.mat); dfree(s.trans);
+}
+
+template <class R> static void free_work(WorkBuf<R> &w)
+{
+    dfree(w.p1_out); dfree(w.res[0]); dfree(w.res[1]); dfree(w.stacks); dfree(w.dump[0]); dfree(w.dump[1]);
+    w.p1_cap = w.res_cap = w.dump_cap = 0;
+    w.stack_cap = 0;
+    w.n_waves = 0;
+}
+
+// ------------------------------------------------------------------ medium
+extern "C" int pach_uniform_medium(int air_choice, double temp_c, double hr, double pa_hpa,
+                                   int edge_correction, double atten[8], double *c, double *rho)
+{
+    // Calculate_Attenuation (Classes_Medium_Props.cs:52-190) with the pressure
+    // Uniform_Medium passes (Pa/10, :289) and C# integer-division exponents
+    // (-1/2 == 0, -1/3 == 0, -5/2 == -2, 1/2 == 0).
+    if (!atten) return PACH_EINVAL;
+    static const double F[8] = {62.5, 125, 250, 500, 1000, 2000, 4000, 8000};
+    const double Tk = temp_c + 273.15, Pa = pa_hpa / 10.0;
+    for (int i = 0; i < 8; ++i) atten[i] = 0;
+    if (air_choice == 0) {
+        const double h = hr * std::pow(10.0, -6.8346 * std::pow(273.16 / Tk, 1.261) + 4.6151) / (Pa / 101.325);
+        for (int o = 0; o < 8; ++o) {
+            const double frO = (Pa / 101.325) * (24 + 4.04e4 * h * ((0.02 + h) / (0.391 + h)));
+            const double frN = (Pa / 101.325) * (9 + 280 * h);
+            const double z = 0.1068 * std::exp(-3352 / Tk) / (frN + F[o] * F[o] / frN);
+            const double y = std::pow(Tk / 293.15, -2.0) *
+                             (0.01275 * std::exp(-2239.1 / Tk) / (frO + F[o] * F[o] / frO) + z);
+            atten[o] = 8.686 * F[o] * F[o] * ((1.84e-11 / (Pa / 101.325)) + y);
+        }
+    } else if (air_choice == 1) {
+        for (int q = 0; q < 8; ++q) {
+            double a = ((85 / hr) * ((F[q] / 1000) * (F[q] / 1000) * 0.0001)) / 8.7;
+            a -= 0.04 * (Tk - 273.15 - 20) * a;
+            atten[q] = a * 0.50248756218905477;
+        }
+    } else if (air_choice == 2) {
+        static const double A[6][8] = {
+            {0.0001, 0.0001, 0.0003, 0.0006, 0.001, 0.0019, 0.0058, 0.0203},
+            {0.0001, 0.0001, 0.0003, 0.0006, 0.001, 0.0017, 0.0041, 0.0135},
+            {0.0001, 0.0001, 0.0003, 0.0006, 0.0011, 0.0017, 0.0035, 0.0106},
+            {0.0001, 0.0001, 0.0002, 0.0005, 0.0011, 0.0027, 0.0094, 0.029},
+            {0.0001, 0.0001, 0.0002, 0.0005, 0.0008, 0.0018, 0.0059, 0.0211},
+            {0.0001, 0.0001, 0.0002, 0.0005, 0.0007, 0.0014, 0.0044, 0.0158}};
+        const int base = (Tk - 273.15 > 15) ? 0 : 3;
+        int row = -1;
+        if (hr < 50) row = base;
+        else if (hr < 70) row = base + 1;
+        else if (hr > 70) row = base + 2;
+        if (row >= 0)
+            for (int q = 0; q < 8; ++q) atten[q] = A[row][q];
+        if (edge_correction)
+            for (int q = 0; q < 8; ++q) atten[q] /= std::sqrt(2.0);
+    } else {
+        return PACH_EINVAL;
+    }
+    if (c) *c = 331 + 0.6 * temp_c;  // AcousticalMath.SoundSpeed (Classes_Math.cs:272-275)
+    if (rho) {                         // Calculate_Density(Tk, hPa*100, hr) (:42-49)
+        const double P = pa_hpa * 100.0, T = temp_c;
+        const double Pv = hr * 6.1078 * std::pow(10.0, (7.5 * T) / (237.3 + T));
+        *rho = (P / (287.05 * Tk)) * (1 - 0.378 * Pv / P);
+    }
+    return PACH_OK;
+}
+
+// ------------------------------------------------------------------ lifetime
+extern "C" int pach_abi_version(void) { return PACH_ABI_VERSION; }
+
+extern "C" int pach_create(pach_ctx **out, int device)
+{
+    if (!out) return PACH_EINVAL;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return PACH_EHIP;
+    if (device < 0 || device >= n) return PACH_EINVAL;
+    pach_ctx *c = new pach_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return PACH_EHIP;
+    }
+    for (auto &e : c->ev) (void)hipEventCreate(&e);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->n_cu = prop.multiProcessorCount;
+    if (c->n_cu <= 0) c->n_cu = 256;
+    if (dalloc(c->d_cnt, 1) != hipSuccess || dalloc(c->d_scal, 8) != hipSuccess || dalloc(c->d_geo, 80 * 9) != hipSuccess) {
+        pach_destroy(c);
+        return PACH_ENOMEM;
+    }
+    // GeodesicSource.GeoSphere(1): icosahedron subdivided once (Classes_Sources.cs:290-370)
+    std::vector<double> faces;
+    {
+        const double sqr5 = std::sqrt(5.0), phi = (1.0 + sqr5) * 0.5;
+        const double ratio = std::sqrt(10.0 + 2.0 * sqr5) / (4.0 * phi);
+        const double a = (.25 / ratio) * 0.5, b = (.25 / ratio) / (2.0 * phi);
+        double P[12][3] = {{0, b, -a}, {b, a, 0}, {-b, a, 0}, {0, b, a}, {0, -b, a}, {-a, 0, b},
+                           {0, -b, -a}, {a, 0, -b}, {a, 0, b}, {-a, 0, -b}, {b, -a, 0}, {-b, -a, 0}};
+        auto nrm = [](double *v) {
+            const double l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            if (l > 0) { v[0] /= l; v[1] /= l; v[2] /= l; }
+        };
+        for (auto &p : P) nrm(p);
+        static const int F[20][3] = {{0, 1, 2}, {3, 2, 1}, {3, 4, 5}, {3, 8, 4}, {0, 6, 7},
+                                     {0, 9, 6}, {4, 10, 11}, {6, 11, 10}, {2, 5, 9}, {11, 9, 5},
+                                     {1, 7, 8}, {10, 8, 7}, {3, 5, 2}, {3, 1, 8}, {0, 2, 9},
+                                     {0, 7, 1}, {6, 9, 11}, {6, 10, 7}, {4, 11, 5}, {4, 8, 10}};
+        for (auto &f : F) {
+            const double *p0 = P[f[0]], *p1 = P[f[1]], *p2 = P[f[2]];
+            double m01[3], m12[3], m20[3];
+            for (int k = 0; k < 3; ++k) {
+                m01[k] = (p0[k] + p1[k]) / 2;
+                m12[k] = (p1[k] + p2[k]) / 2;
+                m20[k] = (p2[k] + p0[k]) / 2;
+            }
+            nrm(m01); nrm(m12); nrm(m20);
+            const double *tri[4][3] = {{p0, m01, m20}, {p1, m12, m01}, {p2, m20, m12}, {m01, m12, m20}};
+            for (auto &t : tri)
+                for (auto *v : t) faces.insert(faces.end(), v, v + 3);
+        }
+    }
+    if (hipMemcpy(c->d_geo, faces.data(), faces.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+        pach_destroy(c);
+        return PACH_EHIP;
+    }
+    *out = c;
+    return PACH_OK;
+}
+
+extern "C" void pach_destroy(pach_ctx *c)
+{
+    if (!c) return;
+    if (c->worker.joinable()) {
+        c->cancel = true;
+        c->worker.join();
+    }
+    (void)hipSetDevice(c->device);
+    free_scene(c->s32);
+    free_scene(c->s64);
+    free_work(c->w32);
+    free_work(c->w64);
+    dfree(c->d_surf); dfree(c->d_cell_start); dfree(c->d_cell_tris);
+    dfree(c->d_dirs); dfree(c->d_geo);
+    dfree(c->d_ctr32); dfree(c->d_ctr64); dfree(c->d_hist);
+    dfree(c->d_cnt); dfree(c->d_scal);
+    for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" const char *pach_last_error(const pach_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+// ------------------------------------------------------------------ scene + grid
+namespace {
+
+// Triangle / axis-aligned box overlap by the separating-axis theorem
+// (Akenine-Moller 2001); the box is inflated by `pad` so the FP32 walk never
+// misses a triangle that grazes a voxel face.
+bool tri_box_overlap(const double *bc, const double *bh, const double *tv)
+{
+    double v[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) v[i][k] = tv[3 * i + k] - bc[k];
+    const double e[3][3] = {{v[1][0] - v[0][0], v[1][1] - v[0][1], v[1][2] - v[0][2]},
+                            {v[2][0] - v[1][0], v[2][1] - v[1][1], v[2][2] - v[1][2]},
+                            {v[0][0] - v[2][0], v[0][1] - v[2][1], v[0][2] - v[2][2]}};
+    // 9 edge cross axes
+    for (int i = 0; i < 3; ++i) {
+        for (int a = 0; a < 3; ++a) {
+            double ax[3] = {0, 0, 0};
+            double unit[3] = {0, 0, 0};
+            unit[a] = 1;
+            ax[0] = unit[1] * e[i][2] - unit[2] * e[i][1];
+            ax[1] = unit[2] * e[i][0] - unit[0] * e[i][2];
+            ax[2] = unit[0] * e[i][1] - unit[1] * e[i][0];
+            double mn = 1e300, mx = -1e300;
+            for (int j = 0; j < 3; ++j) {
+                const double p = v[j][0] * ax[0] + v[j][1] * ax[1] + v[j][2] * ax[2];
+                mn = std::min(mn, p);
+                mx = std::max(mx, p);
+            }
+            const double rad = bh[0] * std::fabs(ax[0]) + bh[1] * std::fabs(ax[1]) + bh[2] * std::fabs(ax[2]);
+            if (mn > rad || mx < -rad) return false;
+        }
+    }
+    // box face normals
+    for (int k = 0; k < 3; ++k) {
+        const double mn = std::min({v[0][k], v[1][k], v[2][k]}), mx = std::max({v[0][k], v[1][k], v[2][k]});
+        if (mn > bh[k] || mx < -bh[k]) return false;
+    }
+    // triangle plane
+    double n[3] = {e[0][1] * e[1][2] - e[0][2] * e[1][1], e[0][2] * e[1][0] - e[0][0] * e[1][2],
+                   e[0][0] * e[1][1] - e[0][1] * e[1][0]};
+    const double d = -(n[0] * v[0][0] + n[1] * v[0][1] + n[2] * v[0][2]);
+    const double r = bh[0] * std::fabs(n[0]) + bh[1] * std::fabs(n[1]) + bh[2] * std::fabs(n[2]);
+    if (std::fabs(d) > r) return false;
+    return true;
+}
+
+template <class R, class V>
+hipError_t upload_vec4(V *&dst, const std::vector<double> &src4)
+{
+    const size_t n = src4.size() / 4;
+    std::vector<V> tmp(n);
+    for (size_t i = 0; i < n; ++i) {
+        tmp[i].x = (R)src4[4 * i];
+        tmp[i].y = (R)src4[4 * i + 1];
+        tmp[i].z = (R)src4[4 * i + 2];
+        tmp[i].w = (R)src4[4 * i + 3];
+    }
+    hipError_t e = dalloc(dst, n);
+    if (e != hipSuccess) return e;
+    return hipMemcpy(dst, tmp.data(), n * sizeof(V), hipMemcpyHostToDevice);
+}
+
+template <class R> hipError_t upload_scalars(R *&dst, const std::vector<double> &src)
+{
+    std::vector<R> tmp(src.begin(), src.end());
+    hipError_t e = dalloc(dst, tmp.size());
+    if (e != hipSuccess) return e;
+    return hipMemcpy(dst, tmp.data(), tmp.size() * sizeof(R), hipMemcpyHostToDevice);
+}
+
+}  // namespace
+
+extern "C" int pach_scene_upload(pach_ctx *c, const double *verts, int T, const int *surf, int S,
+                                 const double *alpha, const double *scat, const double *trans,
+                                 int gx, int gy, int gz)
+{
+    if (!c) return PACH_EINVAL;
+    if (c->running) return fail(c, PACH_ESTATE, "trace running");
+    if (!verts || T <= 0 || !surf || S <= 0 || !alpha || !scat)
+        return fail(c, PACH_EINVAL, "scene: null array or empty scene");
+    for (int t = 0; t < T; ++t)
+        if (surf[t] < 0 || surf[t] >= S) return fail(c, PACH_EINVAL, "scene: surf_of_tri out of range");
+    for (int i = 0; i < 8 * S; ++i) {
+        if (!(alpha[i] >= 0 && alpha[i] <= 1) || !(scat[i] >= 0 && scat[i] <= 1) ||
+            (trans && !(trans[i] >= 0 && trans[i] <= 1)))
+            return fail(c, PACH_EINVAL, "scene: coefficients must lie in [0, 1]");
+    }
+    (void)hipSetDevice(c->device);
+    c->has_scene = false;
+    c->T = T;
+    c->S = S;
+    c->alpha.assign(alpha, alpha + 8 * S);
+    c->scat.assign(scat, scat + 8 * S);
+    c->trans.assign(8 * S, 0.0);
+    c->any_trans = false;
+    if (trans) {
+        c->trans.assign(trans, trans + 8 * S);
+        for (double t : c->trans) c->any_trans |= t > 0;
+    }
+    c->area.assign(S, 0.0);
+    std::vector<double> v0(4 * T), e1(4 * T), e2(4 * T), nr(4 * T);
+    double bmin[3] = {1e300, 1e300, 1e300}, bmax[3] = {-1e300, -1e300, -1e300};
+    for (int t = 0; t < T; ++t) {
+        const double *p = verts + 9 * t;
+        double a[3], b[3];
+        for (int k = 0; k < 3; ++k) {
+            if (!std::isfinite(p[k]) || !std::isfinite(p[3 + k]) || !std::isfinite(p[6 + k]))
+                return fail(c, PACH_EINVAL, "scene: non-finite vertex");
+            v0[4 * t + k] = p[k];
+            a[k] = e1[4 * t + k] = p[3 + k] - p[k];
+            b[k] = e2[4 * t + k] = p[6 + k] - p[k];
+            for (int j = 0; j < 3; ++j) {
+                bmin[k] = std::min(bmin[k], p[3 * j + k]);
+                bmax[k] = std::max(bmax[k], p[3 * j + k]);
+            }
+        }
+        double n[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+        const double l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        c->area[surf[t]] += 0.5 * l;
+        for (int k = 0; k < 3; ++k) nr[4 * t + k] = l > 0 ? n[k] / l : 0.0;
+        v0[4 * t + 3] = e1[4 * t + 3] = e2[4 * t + 3] = nr[4 * t + 3] = 0;
+    }
+    // grid resolution: explicit, or ~8 cells per triangle shaped like the bbox
+    int G[3] = {gx, gy, gz};
+    double ext[3];
+    for (int k = 0; k < 3; ++k) ext[k] = std::max(bmax[k] - bmin[k], 1e-6);
+    if (G[0] <= 0 || G[1] <= 0 || G[2] <= 0) {
+        const double target = std::min(std::max(8.0 * T, 4096.0), 2.0e6);
+        const double unit = std::cbrt(ext[0] * ext[1] * ext[2] / target);
+        for (int k = 0; k < 3; ++k) G[k] = std::max(1, std::min(256, (int)std::lround(ext[k] / unit)));
+    }
+    for (int k = 0; k < 3; ++k) {
+        if (G[k] > 1024) return fail(c, PACH_EINVAL, "scene: grid dimension above 1024");
+        const double pad = 1e-4 * std::max(ext[k], 1.0);
+        c->g[k] = G[k];
+        c->gmin[k] = bmin[k] - pad;
+        c->cs[k] = (ext[k] + 2 * pad) / G[k];
+    }
+    const size_t ncell = (size_t)G[0] * G[1] * G[2];
+    std::vector<std::vector<uint32_t>> lists(ncell);
+    for (int t = 0; t < T; ++t) {
+        const double *p = verts + 9 * t;
+        int lo[3], hi[3];
+        for (int k = 0; k < 3; ++k) {
+            const double mn = std::min({p[k], p[3 + k], p[6 + k]}), mx = std::max({p[k], p[3 + k], p[6 + k]});
+            const double e = 1e-3 * c->cs[k];
+            lo[k] = std::max(0, (int)std::floor((mn - e - c->gmin[k]) / c->cs[k]));
+            hi[k] = std::min(G[k] - 1, (int)std::floor((mx + e - c->gmin[k]) / c->cs[k]));
+        }
+        for (int z = lo[2]; z <= hi[2]; ++z)
+            for (int y = lo[1]; y <= hi[1]; ++y)
+                for (int x = lo[0]; x <= hi[0]; ++x) {
+                    const double bc[3] = {c->gmin[0] + (x + 0.5) * c->cs[0], c->gmin[1] + (y + 0.5) * c->cs[1],
+                                          c->gmin[2] + (z + 0.5) * c->cs[2]};
+                    const double bh[3] = {0.5 * c->cs[0] * (1 + 2e-3), 0.5 * c->cs[1] * (1 + 2e-3),
+                                          0.5 * c->cs[2] * (1 + 2e-3)};
+                    if (tri_box_overlap(bc, bh, p)) lists[((size_t)z * G[1] + y) * G[0] + x].push_back((uint32_t)t);
+                }
+    }
+    std::vector<uint32_t> start(ncell + 1, 0), tris;
+    for (size_t i = 0; i < ncell; ++i) start[i + 1] = start[i] + (uint32_t)lists[i].size();
+    tris.reserve(start[ncell]);
+    for (auto &l : lists) tris.insert(tris.end(), l.begin(), l.end());
+    // materials: (1 - alpha, s, lo, hi) per (polygon, band), Lambert_Scattering
+    // split bounds (Classes_Materials.cs:4466-4484)
+    std::vector<double> mat(4 * 8 * (size_t)S);
+    for (int i = 0; i < 8 * S; ++i) {
+        const double s = scat[i];
+        double lo, hi;
+        if (s < 0.25) { lo = 0.1; hi = 0.4; }
+        else { const double m = std::fabs(1 - s) / 5; lo = s - m; hi = s + m; }
+        mat[4 * i] = 1 - alpha[i];
+        mat[4 * i + 1] = s;
+        mat[4 * i + 2] = lo;
+        mat[4 * i + 3] = hi;
+    }
+    std::vector<double> surfd(surf, surf + T);
+    hipError_t e;
+    if ((e = upload_vec4<float>(c->s32.v0, v0)) != hipSuccess || (e = upload_vec4<float>(c->s32.e1, e1)) != hipSuccess ||
+        (e = upload_vec4<float>(c->s32.e2, e2)) != hipSuccess || (e = upload_vec4<float>(c->s32.nrm, nr)) != hipSuccess ||
+        (e = upload_vec4<float>(c->s32.mat, mat)) != hipSuccess || (e = upload_scalars<float>(c->s32.trans, c->trans)) != hipSuccess ||
+        (e = upload_vec4<double>(c->s64.v0, v0)) != hipSuccess || (e = upload_vec4<double>(c->s64.e1, e1)) != hipSuccess ||
+        (e = upload_vec4<double>(c->s64.e2, e2)) != hipSuccess || (e = upload_vec4<double>(c->s64.nrm, nr)) != hipSuccess ||
+        (e = upload_vec4<double>(c->s64.mat, mat)) != hipSuccess || (e = upload_scalars<double>(c->s64.trans, c->trans)) != hipSuccess)
+        return fail(c, PACH_EHIP, std::string("scene upload: ") + hipGetErrorString(e));
+    if ((e = dalloc(c->d_surf, T)) != hipSuccess || (e = hipMemcpy(c->d_surf, surf, T * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = dalloc(c->d_cell_start, start.size())) != hipSuccess ||
+        (e = hipMemcpy(c->d_cell_start, start.data(), start.size() * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = dalloc(c->d_cell_tris, tris.size())) != hipSuccess ||
+        (e = hipMemcpy(c->d_cell_tris, tris.data(), std::max<size_t>(tris.size(), 1) * 4, hipMemcpyHostToDevice)) != hipSuccess)
+        return fail(c, PACH_EHIP, std::string("grid upload: ") + hipGetErrorString(e));
+    c->has_scene = true;
+    return PACH_OK;
+}
+
+extern "C" int pach_source_set(pach_ctx *c, const double origin[3], const double power[8], int dir_mode,
+                               const double *dirs, int64_t n_dirs)
+{
+    if (!c) return PACH_EINVAL;
+    if (c->running) return fail(c, PACH_ESTATE, "trace running");
+    if (!origin || !power) return fail(c, PACH_EINVAL, "source: null origin or power");
+    if (dir_mode < 0 || dir_mode > 2) return fail(c, PACH_EINVAL, "source: bad direction mode");
+    if (dir_mode == PACH_DIR_SUPPLIED && (!dirs || n_dirs <= 0))
+        return fail(c, PACH_EINVAL, "source: supplied mode needs directions");
+    for (int o = 0; o < 8; ++o)
+        if (!(power[o] >= 0) || !std::isfinite(power[o])) return fail(c, PACH_EINVAL, "source: bad power");
+    (void)hipSetDevice(c->device);
+    std::memcpy(c->origin, origin, sizeof(c->origin));
+    std::memcpy(c->power, power, sizeof(c->power));
+    c->dir_mode = dir_mode;
+    c->n_dirs = 0;
+    dfree(c->d_dirs);
+    if (dir_mode == PACH_DIR_SUPPLIED) {
+        HIPCHK(c, dalloc(c->d_dirs, 3 * (size_t)n_dirs));
+        HIPCHK(c, hipMemcpy(c->d_dirs, dirs, 3 * (size_t)n_dirs * sizeof(double), hipMemcpyHostToDevice));
+        c->n_dirs = n_dirs;
+    }
+    c->has_src = true;
+    return PACH_OK;
+}
+
+extern "C" int pach_receivers_set(pach_ctx *c, const double *centers, int R, double radius,
+                                  const double atten[8], double cs, double rho_c, int fs, int sct)
+{
+    if (!c) return PACH_EINVAL;
+    if (c->running) return fail(c, PACH_ESTATE, "trace running");
+    if (!centers || R <= 0 || !atten) return fail(c, PACH_EINVAL, "receivers: null array or none");
+    if (!(radius > 0) || !(cs > 0) || !(rho_c > 0) || fs <= 0 || sct <= 0)
+        return fail(c, PACH_EINVAL, "receivers: radius, c, rho_c, sample rate and count must be positive");
+    (void)hipSetDevice(c->device);
+    c->R = R;
+    c->radius = radius;
+    std::memcpy(c->atten, atten, sizeof(c->atten));
+    c->c = cs;
+    c->rho_c = rho_c;
+    c->fs = fs;
+    c->sct = sct;
+    std::vector<float4> c32(R);
+    std::vector<double4> c64(R);
+    for (int r = 0; r < R; ++r) {
+        c32[r] = make_float4((float)centers[3 * r], (float)centers[3 * r + 1], (float)centers[3 * r + 2], 0.f);
+        c64[r] = make_double4(centers[3 * r], centers[3 * r + 1], centers[3 * r + 2], 0.0);
+    }
+    HIPCHK(c, dalloc(c->d_ctr32, R));
+    HIPCHK(c, dalloc(c->d_ctr64, R));
+    HIPCHK(c, hipMemcpy(c->d_ctr32, c32.data(), R * sizeof(float4), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_ctr64, c64.data(), R * sizeof(double4), hipMemcpyHostToDevice));
+    c->hist_n = (size_t)R * 8 * sct;
+    if (dalloc(c->d_hist, c->hist_n * kPlanes) != hipSuccess) return fail(c, PACH_ENOMEM, "histogram allocation");
+    HIPCHK(c, hipMemset(c->d_hist, 0, c->hist_n * kPlanes * sizeof(double)));
+    c->has_recv = true;
+    return PACH_OK;
+}
+
+// ------------------------------------------------------------------ trace
+namespace {
+
+int h_oct(const pach_ctx *c, int lo, int hi)
+{
+    // SplitRayTracer ctor (TaskAll:88-109): band with the least total
+    // absorption among bands carrying >= 1e-6 of the strongest band's power.
+    double total[8] = {0};
+    for (int s = 0; s < c->S; ++s)
+        for (int o = lo; o <= hi; ++o) total[o] += c->area[s] * c->alpha[8 * s + o];
+    double mn = HUGE_VAL, op = 0;
+    int h = 0;
+    for (int o = lo; o <= hi; ++o) op = std::max(op, c->power[o]);
+    for (int o = lo; o <= hi; ++o) {
+        if (c->power[o] < op * 1e-6) continue;
+        if (mn > total[o]) { mn = total[o]; h = o; }
+    }
+    return h;
+}
+
+template <class R> SceneBuf<R> &scene_of(pach_ctx *c);
+template <> SceneBuf<float> &scene_of<float>(pach_ctx *c) { return c->s32; }
+template <> SceneBuf<double> &scene_of<double>(pach_ctx *c) { return c->s64; }
+template <class R> WorkBuf<R> &work_of(pach_ctx *c);
+template <> WorkBuf<float> &work_of<float>(pach_ctx *c) { return c->w32; }
+template <> WorkBuf<double> &work_of<double>(pach_ctx *c) { return c->w64; }
+template <class R> const typename V4<R>::T *ctr_of(pach_ctx *c);
+template <> const float4 *ctr_of<float>(pach_ctx *c) { return c->d_ctr32; }
+template <> const double4 *ctr_of<double>(pach_ctx *c) { return c->d_ctr64; }
+
+template <class R> int ensure_work(pach_ctx *c, int64_t batch, int grid)
+{
+    WorkBuf<R> &w = work_of<R>(c);
+    const unsigned p1_cap = (unsigned)std::min<int64_t>(batch * 200 + 4096, (int64_t)1 << 31);
+    const unsigned res_cap = (unsigned)batch;
+    const size_t waves = (size_t)grid * (kBlock / 64);
+    const int stack_cap = 4096;
+    const unsigned dump_cap = (unsigned)std::min<size_t>(waves * (stack_cap + 64) + (1u << 20), (size_t)1 << 31);
+    hipError_t e = hipSuccess;
+    if (w.p1_cap < p1_cap) {
+        if ((e = dalloc(w.p1_out, p1_cap)) != hipSuccess) return fail(c, PACH_ENOMEM, "phase-1 buffer");
+        w.p1_cap = p1_cap;
+    }
+    if (w.res_cap < res_cap) {
+        if ((e = dalloc(w.res[0], res_cap)) != hipSuccess || (e = dalloc(w.res[1], res_cap)) != hipSuccess)
+            return fail(c, PACH_ENOMEM, "resume buffers");
+        w.res_cap = res_cap;
+    }
+    if (w.n_waves < waves || w.stack_cap != stack_cap) {
+        if ((e = dalloc(w.stacks, waves * (size_t)stack_cap)) != hipSuccess) return fail(c, PACH_ENOMEM, "wave stacks");
+        w.n_waves = waves;
+        w.stack_cap = stack_cap;
+    }
+    if (w.dump_cap < dump_cap) {
+        if ((e = dalloc(w.dump[0], dump_cap)) != hipSuccess || (e = dalloc(w.dump[1], dump_cap)) != hipSuccess)
+            return fail(c, PACH_ENOMEM, "dump buffers");
+        w.dump_cap = dump_cap;
+    }
+    return PACH_OK;
+}
+
+template <class R> int run_trace(pach_ctx *c, const pach_params p)
+{
+    (void)hipSetDevice(c->device);
+    auto t0 = std::chrono::steady_clock::now();
+    TraceConst tc{};
+    tc.mode = p.mode;
+    tc.oct_lo = p.oct_lo;
+    tc.oct_hi = p.oct_hi;
+    tc.is_order = p.is_order;
+    tc.h = h_oct(c, p.oct_lo, p.oct_hi);
+    tc.dir_mode = c->dir_mode;
+    tc.p1_max = p.p1_max_bounces > 0 ? p.p1_max_bounces : 128;
+    tc.max_bounces = p.max_bounces > 0 ? p.max_bounces : 100000;
+    tc.cotime = p.cutoff_m / c->c;  // TaskAll:83
+    tc.T1 = c->power[tc.h] * 1e-2;  // TaskAll:351
+    for (int o = 0; o < 8; ++o) {
+        tc.power[o] = c->power[o];
+        const bool in = o >= p.oct_lo && o <= p.oct_hi;
+        tc.T2[o] = in ? c->power[o] * std::pow(10, -4.5) : 0.0;  // TaskAll:354
+        tc.T3[o] = in ? c->power[o] * 1e-9 : 0.0;                // TaskAll:355
+    }
+    for (int k = 0; k < 3; ++k) tc.src[k] = c->origin[k];
+    tc.key0 = (uint32_t)p.seed;
+    tc.key1 = (uint32_t)(p.seed >> 32);
+    tc.dirs = c->d_dirs;
+    tc.n_dirs = c->n_dirs;
+    tc.geo = c->d_geo;
+    tc.has_trans = c->any_trans ? 1 : 0;
+
+    SceneBuf<R> &sb = scene_of<R>(c);
+    DevScene<R> s{};
+    s.v0 = sb.v0; s.e1 = sb.e1; s.e2 = sb.e2; s.nrm = sb.nrm; s.surf = c->d_surf;
+    s.cell_start = c->d_cell_start; s.cell_tris = c->d_cell_tris;
+    s.gx = c->g[0]; s.gy = c->g[1]; s.gz = c->g[2];
+    for (int k = 0; k < 3; ++k) {
+        s.gmin[k] = (R)c->gmin[k];
+        s.cs[k] = (R)c->cs[k];
+        s.inv_cs[k] = (R)(1.0 / c->cs[k]);
+    }
+    s.mat = sb.mat;
+    s.trans = sb.trans;
+
+    DevRecv<R> rv{};
+    rv.ctr = ctr_of<R>(c);
+    rv.n = c->R;
+    rv.r2 = (R)(c->radius * c->radius);
+    rv.sizemod = 1.0 / (M_PI * c->radius * c->radius * c->radius);
+    rv.c = c->c;
+    rv.inv_c = 1 / c->c;
+    for (int o = 0; o < 8; ++o) {
+        rv.atten[o] = c->atten[o];
+        rv.kexp2[o] = (float)(-0.1 * c->atten[o] * std::log2(10.0));
+    }
+    rv.rho_c = c->rho_c;
+    rv.fs = (int)c->fs;
+    rv.sct = c->sct;
+    rv.hist = c->d_hist;
+    rv.plane_stride = c->hist_n;
+
+    const int64_t N = p.ray_count;
+    int64_t batch = p.batch_rays > 0 ? p.batch_rays : std::min<int64_t>(N, 32768);
+    batch = std::max<int64_t>(1, std::min<int64_t>(batch, 1 << 22));
+    const int grid = c->n_cu * octave_blocks_per_cu<R>();
+    int rc = ensure_work<R>(c, batch, grid);
+    if (rc) return rc;
+    WorkBuf<R> &w = work_of<R>(c);
+    unsigned *p1_count = c->d_scal + 0, *res_count = c->d_scal + 1, *in_head = c->d_scal + 2;
+    unsigned *dump_count = c->d_scal + 3;  // [2]
+    hipStream_t st = c->stream;
+    HIPCHK(c, hipMemsetAsync(c->d_hist, 0, c->hist_n * kPlanes * sizeof(double), st));
+    HIPCHK(c, hipMemsetAsync(c->d_cnt, 0, sizeof(DevCounters), st));
+    HIPCHK(c, hipMemsetAsync(c->d_scal, 0, 8 * sizeof(unsigned), st));
+
+    double ms_p1 = 0, ms_oct = 0;
+    int64_t n_p1 = 0, n_oct = 0;
+    int cur = 0;            // dump buffer holding the previous launch's leftovers
+    unsigned pending = 0;   // rays in dump[cur]
+    unsigned n_resume = 0;  // broadband rays waiting in res[res_cur]
+    int res_cur = 0;
+    int64_t next = 0;       // next fresh source ray
+    int64_t started = 0;
+    const int min_iters = 256;
+    for (;;) {
+        const bool stop_new = c->cancel.load();
+        bool p1_launched = false;
+        HIPCHK(c, hipMemsetAsync(p1_count, 0, 2 * sizeof(unsigned), st));  // p1_count, res_count
+        if (n_resume > 0) {
+            HIPCHK(c, hipEventRecord(c->ev[0], st));
+            HIPCHK(c, launch_phase1<R>(st, tc, s, rv, 0, 0, w.res[res_cur], (int)n_resume, w.p1_out, p1_count,
+                                       w.p1_cap, w.res[1 - res_cur], res_count, c->d_cnt));
+            HIPCHK(c, hipEventRecord(c->ev[1], st));
+            res_cur = 1 - res_cur;
+            p1_launched = true;
+        } else if (next < N && !stop_new) {
+            const int nb = (int)std::min<int64_t>(batch, N - next);
+            HIPCHK(c, hipEventRecord(c->ev[0], st));
+            HIPCHK(c, launch_phase1<R>(st, tc, s, rv, next, nb, nullptr, 0, w.p1_out, p1_count, w.p1_cap,
+                                       w.res[1 - res_cur], res_count, c->d_cnt));
+            HIPCHK(c, hipEventRecord(c->ev[1], st));
+            res_cur = 1 - res_cur;
+            next += nb;
+            started += nb;
+            p1_launched = true;
+        }
+        if (!p1_launched && pending == 0) break;
+        HIPCHK(c, hipMemsetAsync(in_head, 0, sizeof(unsigned), st));
+        HIPCHK(c, hipMemsetAsync(dump_count + (1 - cur), 0, sizeof(unsigned), st));
+        HIPCHK(c, hipEventRecord(c->ev[2], st));
+        HIPCHK(c, launch_octave<R>(st, grid, tc, s, rv, pending ? w.dump[cur] : nullptr, dump_count + cur,
+                                   w.dump_cap, p1_launched ? w.p1_out : nullptr, p1_count, w.p1_cap, in_head,
+                                   w.stacks, w.stack_cap, w.dump[1 - cur], dump_count + (1 - cur), w.dump_cap,
+                                   min_iters, c->d_cnt));
+        HIPCHK(c, hipEventRecord(c->ev[3], st));
+        unsigned h[8];
+        HIPCHK(c, hipMemcpyAsync(h, c->d_scal, sizeof(h), hipMemcpyDeviceToHost, st));
+        HIPCHK(c, hipStreamSynchronize(st));
+        float ms = 0;
+        if (p1_launched && hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) { ms_p1 += ms; ++n_p1; }
+        if (hipEventElapsedTime(&ms, c->ev[2], c->ev[3]) == hipSuccess) { ms_oct += ms; ++n_oct; }
+        n_resume = h[1];
+        cur = 1 - cur;
+        pending = std::min(h[3 + cur], w.dump_cap);
+        if (n_resume > w.res_cap) return fail(c, PACH_EHIP, "resume buffer overflow");
+        // a source ray is complete once its broadband phase and every ray it spawned have ended
+        if (n_resume == 0 && pending == 0) c->rays_done = started;
+    }
+    c->rays_done = started;
+    DevCounters hc{};
+    HIPCHK(c, hipMemcpy(&hc, c->d_cnt, sizeof(hc), hipMemcpyDeviceToHost));
+    pach_stats &S = c->stats;
+    S = pach_stats{};
+    S.bounces_broad = (int64_t)hc.v[C_BROAD];
+    S.bounces_late = (int64_t)hc.v[C_LATE];
+    S.bounces_verylate = (int64_t)hc.v[C_VLATE];
+    S.octave_rays = (int64_t)hc.v[C_OCT];
+    S.spawned = (int64_t)hc.v[C_SPAWN];
+    S.misses = (int64_t)hc.v[C_MISS];
+    S.deposits = (int64_t)hc.v[C_DEP];
+    S.p1_cap_hits = (int64_t)hc.v[C_P1CAP];
+    S.guard_hits = (int64_t)hc.v[C_GUARD];
+    S.spills = (int64_t)hc.v[C_SPILL];
+    S.resumes = (int64_t)hc.v[C_RESUME];
+    S.lost = hc.lost;
+    double psum = 0;
+    for (int o = 0; o < 8; ++o) psum += c->power[o];
+    S.e_init = psum * (double)started;
+    S.h_oct = tc.h;
+    S.rays_done = started;
+    S.ms_phase1 = ms_p1;
+    S.ms_octave = ms_oct;
+    S.launches_phase1 = n_p1;
+    S.launches_octave = n_oct;
+    S.ms_trace = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    c->oct_rays = S.octave_rays;
+    if (hc.v[C_DROP]) return fail(c, PACH_ENOMEM, "octave-ray queue overflow: rays dropped");
+    if (c->cancel.load() && started < N) return PACH_ECANCEL;
+    return PACH_OK;
+}
+
+}  // namespace
+
+extern "C" int pach_trace(pach_ctx *c, const pach_params *p)
+{
+    if (!c || !p) return PACH_EINVAL;
+    if (c->running) return fail(c, PACH_ESTATE, "trace already running");
+    if (c->worker.joinable()) c->worker.join();
+    if (!c->has_scene || !c->has_src || !c->has_recv)
+        return fail(c, PACH_ESTATE, "scene, source and receivers must be set before tracing");
+    if (p->ray_count <= 0 || p->ray_count > ((int64_t)1 << 32))
+        return fail(c, PACH_EINVAL, "ray_count must lie in [1, 2^32]");
+    if (p->oct_lo < 0 || p->oct_hi > 7 || p->oct_lo > p->oct_hi) return fail(c, PACH_EINVAL, "bad octave range");
+    if (!(p->cutoff_m > 0)) return fail(c, PACH_EINVAL, "cutoff must be positive");
+    if (p->mode != PACH_MODE_STOCHASTIC && p->mode != PACH_MODE_SPECULAR) return fail(c, PACH_EINVAL, "bad mode");
+    if (p->precision != PACH_PREC_FP32 && p->precision != PACH_PREC_FP64) return fail(c, PACH_EINVAL, "bad precision");
+    c->cancel = false;
+    c->rays_done = 0;
+    c->rays_total = p->ray_count;
+    c->status = 0;
+    c->running = true;
+    const pach_params pp = *p;
+    auto body = [c, pp]() {
+        const int rc = pp.precision == PACH_PREC_FP64 ? run_trace<double>(c, pp) : run_trace<float>(c, pp);
+        c->status = rc;
+        c->running = false;
+        return rc;
+    };
+    if (!p->async) return body();
+    c->worker = std::thread(body);
+    return PACH_OK;
+}
+
+extern "C" int pach_poll(pach_ctx *c, pach_progress *out)
+{
+    if (!c || !out) return PACH_EINVAL;
+    out->rays_done = c->rays_done.load();
+    out->rays_total = c->rays_total.load();
+    out->octave_rays = c->oct_rays.load();
+    out->running = c->running ? 1 : 0;
+    out->status = c->status;
+    return PACH_OK;
+}
+
+extern "C" int pach_wait(pach_ctx *c)
+{
+    if (!c) return PACH_EINVAL;
+    if (c->worker.joinable()) c->worker.join();
+    return c->status;
+}
+
+extern "C" int pach_cancel(pach_ctx *c)
+{
+    if (!c) return PACH_EINVAL;
+    c->cancel = true;
+    return PACH_OK;
+}
+
+extern "C" int pach_histogram(pach_ctx *c, int plane, double *out, int scaled)
+{
+    if (!c || !out) return PACH_EINVAL;
+    if (plane < 0 || plane >= kPlanes) return fail(c, PACH_EINVAL, "bad plane");
+    if (!c->has_recv) return fail(c, PACH_ESTATE, "no receivers");
+    if (c->running) return fail(c, PACH_ESTATE, "trace running");
+    (void)hipSetDevice(c->device);
+    HIPCHK(c, hipMemcpy(out, c->d_hist + (size_t)plane * c->hist_n, c->hist_n * sizeof(double), hipMemcpyDeviceToHost));
+    if (scaled) {
+        const double n = (double)c->rays_done.load();
+        if (n > 0) {
+            const double f = plane == PACH_PLANE_PRESSURE ? 1.0 / std::sqrt(n) : 1.0 / n;
+            for (size_t i = 0; i < c->hist_n; ++i) out[i] *= f;
+        }
+    }
+    return PACH_OK;
+}
+
+extern "C" int pach_stats_get(pach_ctx *c, pach_stats *out)
+{
+    if (!c || !out) return PACH_EINVAL;
+    *out = c->stats;
+    return PACH_OK;
+}
+
+extern "C" int pach_percent_lost(pach_ctx *c, double *out)
+{
+    if (!c || !out) return PACH_EINVAL;
+    *out = c->stats.e_init > 0 ? 100 * c->stats.lost / c->stats.e_init : 0.0;
+    return PACH_OK;
+}
+
+// ------------------------------------------------------------------ decay curves (Classes_Math.cs)
+extern "C" int pach_schroeder_integral(const double *etc, int n, double *out)
+{
+    // Schroeder_Integral (:282-299): 1 - cumulative / total
+    if (!etc || !out || n <= 0) return PACH_EINVAL;
+    std::vector<double> b(n);
+    double sum = 0;
+    for (int i = 0; i < n; ++i) b[i] = sum += etc[i];
+    for (int i = 0; i < n; ++i) out[i] = sum == 0 ? 0.0 : 1 - b[i] / sum;
+    return PACH_OK;
+}
+
+static double decay_fit(const double *sch, int n, double first, double second, int fs)
+{
+    // Log10Data (:392-407; index 0 stays 0 dB), Schroeder_Limits (:417-440),
+    // Least_Square_Fit (:446-478): value = -60 / slope
+    std::vector<double> ls(n, 0.0);
+    const double lim = std::pow(10, -60.0 / 10);
+    for (int i = 1; i < n; ++i) ls[i] = sch[i] >= lim ? 10 * std::log10(sch[i]) : -60.0;
+    int l0 = 0, l1 = 0;
+    for (int i = 1; i < n; ++i)
+        if (ls[i] < first) { l0 = i; break; }
+    for (int i = l0; i < n; ++i)
+        if (ls[i] < second) { l1 = i; break; }
+    const int m = l1 - l0 + 1;
+    if (m <= 0) return NAN;
+    double ts = 0, ms = 0, tt = 0, tm = 0;
+    for (int i = 0; i < m; ++i) {
+        const double q = i / (double)fs;
+        ts += q; ms += ls[l0 + i]; tt += q * q; tm += q * ls[l0 + i];
+    }
+    const double slope = ((m * tm) - (ts * ms)) / ((m * tt) - (ts * ts));
+    return -60 / slope;
+}
+
+extern "C" int pach_t_x(const double *sch, int n, int depth, int fs, double *out)
+{
+    // T_X (:511-527): fit from -5 dB to -(5 + depth) dB
+    if (!sch || !out || n <= 0 || fs <= 0) return PACH_EINVAL;
+    *out = decay_fit(sch, n, -5, -5 - std::abs(depth), fs);
+    return PACH_OK;
+}
+
+extern "C" int pach_early_decay_time(const double *sch, int n, int fs, double *out)
+{
+    // EarlyDecayTime (:534-551): fit from -1 dB to -10.0001 dB
+    if (!sch || !out || n <= 0 || fs <= 0) return PACH_EINVAL;
+    *out = decay_fit(sch, n, -1, -10.0001, fs);
+    return PACH_OK;
+}
+
+extern "C" int pach_clarity(const double *etc, int n, double fs, double seconds, double start, int pressure,
+                            double *out)
+{
+    // Clarity (:559-585): early/late energy with the boundary bin split
+    if (!etc || !out || n <= 0 || !(fs > 0)) return PACH_EINVAL;
+    if (pressure) seconds += 8192.0 / fs;
+    const double bw = 1 / fs;
+    const int si = (int)std::floor(start * fs), ei = (int)std::floor((seconds + start) * fs);
+    if (si < 0 || ei >= n) return PACH_EINVAL;
+    double early = 0, late = 0;
+    for (int q = si; q < ei; ++q) early += etc[q];
+    const double split = (seconds + start - (ei * bw)) / bw;
+    early += split * etc[ei];
+    for (int q = ei + 1; q < n; ++q) late += etc[q];
+    late += (1 - split) * etc[ei];
+    *out = 10 * std::log10(early / late);
+    return PACH_OK;
+}

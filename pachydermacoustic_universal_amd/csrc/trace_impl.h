@@ -1,0 +1,760 @@
+// trace_impl.h -- gfx950 kernels of the stochastic split ray tracer.
+//
+// Included by trace_f32.hip (FP32 geometry, the throughput path) and
+// trace_f64.hip (FP64 geometry, built with -ffp-contract=off, the path the
+// deterministic per-bin parity runs on).  Each file instantiates one precision.
+//
+// Reference behaviour restated here (TaskAll = "Classes_Complete_Trace - TaskAll.cs"):
+//   broadband phase        TaskAll:360-424
+//   split phase            TaskAll:427-480
+//   End_Trace              TaskAll:489-536
+//   closest hit            Polygon_Scene.shoot (Classes_Scenes.cs:777-808) over a
+//                          voxel DDA (Classes_VoxelGrid_PolyRefractive.cs:84-313)
+//   receivers              Spherical_Receiver.CheckRay/CheckBroadbandRay
+//                          (Classes_Receivers.cs:803-877), Directional_Histogram.Add (:1813-1825)
+//   absorption/scattering  Basic_Material.Absorb (Classes_Materials.cs:155-190),
+//                          Lambert_Scattering (Classes_Materials.cs:4463-4702)
+//   sources                GeodesicSource/RandomSource.Directions (Classes_Sources.cs:256-403)
+//
+// Randomness: every ray owns a Philox4x32-10 stream keyed by the trace seed with
+// counter (block, uid_lo, uid_hi, source ray).  A spawned ray's uid is derived
+// from its parent's uid and the spawning event, so results do not depend on
+// which lane or wave traces which ray.
+#pragma once
+
+#include "trace_types.h"
+
+namespace pach {
+
+// ---------------------------------------------------------------- RNG
+__device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                              uint32_t k0, uint32_t k1, uint32_t out[4])
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+        uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ uint64_t child_uid(uint64_t parent, uint64_t tag)
+{
+    return mix64(parent + 0x9E3779B97F4A7C15ull * (tag + 1ull));
+}
+
+#define PACH_TAG_EARLY(k, o) ((1ull << 40) | ((uint64_t)(k) << 8) | (uint64_t)(o))
+#define PACH_TAG_EARLY_TR(k, o) ((2ull << 40) | ((uint64_t)(k) << 8) | (uint64_t)(o))
+#define PACH_TAG_EARLY_TD(k, o) ((3ull << 40) | ((uint64_t)(k) << 8) | (uint64_t)(o))
+#define PACH_TAG_SPLIT(o) ((4ull << 40) | (uint64_t)(o))
+#define PACH_TAG_LATE(k) ((5ull << 40) | (uint64_t)(k))
+
+template <class R> __device__ __forceinline__ R unif(uint32_t w);
+template <> __device__ __forceinline__ double unif<double>(uint32_t w) { return (double)w * (1.0 / 4294967296.0); }
+template <> __device__ __forceinline__ float unif<float>(uint32_t w) { return (float)(w >> 8) * (1.0f / 16777216.0f); }
+
+// ---------------------------------------------------------------- math helpers
+template <class R> __device__ __forceinline__ R rcp(R x);
+template <> __device__ __forceinline__ float rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template <> __device__ __forceinline__ double rcp<double>(double x) { return 1.0 / x; }
+
+template <class R> __device__ __forceinline__ void normalize3(R &x, R &y, R &z)
+{
+    R l = sqrt(x * x + y * y + z * z);
+    if (l > R(0)) { x /= l; y /= l; z /= l; }
+}
+
+// Smallest accepted hit distance.  FP64 keeps Hare's t > 1e-10
+// (Classes_VoxelGrid_PolyRefractive.cs:166).  FP32 instead excludes the
+// triangle the ray leaves (planar polygons cannot be re-hit) and accepts t > 0.
+template <class R> struct Tmin;
+template <> struct Tmin<double> { static constexpr double v = 1e-10; static constexpr bool excl = false; };
+template <> struct Tmin<float> { static constexpr float v = 0.0f; static constexpr bool excl = true; };
+
+// ---------------------------------------------------------------- closest hit
+// Moller-Trumbore with inclusive edges.
+template <class R>
+__device__ __forceinline__ bool tri_hit(const DevScene<R> &s, int t, R ox, R oy, R oz, R dx, R dy,
+                                        R dz, R &tout)
+{
+    const auto a = s.v0[t];
+    const auto e1 = s.e1[t];
+    const auto e2 = s.e2[t];
+    R px = dy * e2.z - dz * e2.y;
+    R py = dz * e2.x - dx * e2.z;
+    R pz = dx * e2.y - dy * e2.x;
+    R det = e1.x * px + e1.y * py + e1.z * pz;
+    if (det == R(0)) return false;
+    R inv = rcp<R>(det);
+    R sx = ox - a.x, sy = oy - a.y, sz = oz - a.z;
+    R u = (sx * px + sy * py + sz * pz) * inv;
+    if (u < R(0) || u > R(1)) return false;
+    R qx = sy * e1.z - sz * e1.y;
+    R qy = sz * e1.x - sx * e1.z;
+    R qz = sx * e1.y - sy * e1.x;
+    R v = (dx * qx + dy * qy + dz * qz) * inv;
+    if (v < R(0) || u + v > R(1)) return false;
+    R tt = (e2.x * qx + e2.y * qy + e2.z * qz) * inv;
+    if (!(tt > R(Tmin<R>::v))) return false;
+    tout = tt;
+    return true;
+}
+
+// Amanatides-Woo walk over the uniform grid; the closest hit ends the walk once
+// it lies before the current voxel's exit.  Returns the triangle or -1.
+template <class R>
+__device__ int shoot(const DevScene<R> &s, R ox, R oy, R oz, R dx, R dy, R dz, int excl, R &tbest)
+{
+    const R INF = R(__builtin_huge_val());
+    int cx = (int)floor((ox - s.gmin[0]) * s.inv_cs[0]);
+    int cy = (int)floor((oy - s.gmin[1]) * s.inv_cs[1]);
+    int cz = (int)floor((oz - s.gmin[2]) * s.inv_cs[2]);
+    cx = min(max(cx, 0), s.gx - 1);
+    cy = min(max(cy, 0), s.gy - 1);
+    cz = min(max(cz, 0), s.gz - 1);
+    int sx = 0, sy = 0, sz = 0;
+    R tmx = INF, tmy = INF, tmz = INF, tdx = INF, tdy = INF, tdz = INF;
+    if (dx > R(0)) { sx = 1; tmx = (s.gmin[0] + (cx + 1) * s.cs[0] - ox) / dx; tdx = s.cs[0] / dx; }
+    else if (dx < R(0)) { sx = -1; tmx = (s.gmin[0] + cx * s.cs[0] - ox) / dx; tdx = -s.cs[0] / dx; }
+    if (dy > R(0)) { sy = 1; tmy = (s.gmin[1] + (cy + 1) * s.cs[1] - oy) / dy; tdy = s.cs[1] / dy; }
+    else if (dy < R(0)) { sy = -1; tmy = (s.gmin[1] + cy * s.cs[1] - oy) / dy; tdy = -s.cs[1] / dy; }
+    if (dz > R(0)) { sz = 1; tmz = (s.gmin[2] + (cz + 1) * s.cs[2] - oz) / dz; tdz = s.cs[2] / dz; }
+    else if (dz < R(0)) { sz = -1; tmz = (s.gmin[2] + cz * s.cs[2] - oz) / dz; tdz = -s.cs[2] / dz; }
+    int best = -1;
+    tbest = INF;
+    for (;;) {
+        const int cell = (cz * s.gy + cy) * s.gx + cx;
+        const uint32_t b = s.cell_start[cell], e = s.cell_start[cell + 1];
+        for (uint32_t j = b; j < e; ++j) {
+            const int t = (int)s.cell_tris[j];
+            if (Tmin<R>::excl && t == excl) continue;
+            R tt;
+            if (tri_hit<R>(s, t, ox, oy, oz, dx, dy, dz, tt) && (tt < tbest || (tt == tbest && t < best))) {
+                tbest = tt;
+                best = t;
+            }
+        }
+        const int ax = (tmx < tmy) ? (tmx < tmz ? 0 : 2) : (tmy < tmz ? 1 : 2);
+        const R texit = ax == 0 ? tmx : (ax == 1 ? tmy : tmz);
+        if (best >= 0 && tbest <= texit) break;
+        if (ax == 0) {
+            cx += sx;
+            if (sx == 0 || cx < 0 || cx >= s.gx) break;
+            tmx += tdx;
+        } else if (ax == 1) {
+            cy += sy;
+            if (sy == 0 || cy < 0 || cy >= s.gy) break;
+            tmy += tdy;
+        } else {
+            cz += sz;
+            if (sz == 0 || cz < 0 || cz >= s.gz) break;
+            tmz += tdz;
+        }
+    }
+    return best;
+}
+
+// ---------------------------------------------------------------- receivers
+struct LaneCounters {
+    uint32_t v[C_N];
+    double lost;
+};
+
+__device__ __forceinline__ void hist_add(const double *base_unused, double *hist, uint64_t stride,
+                                         uint64_t idx, double e, double rho_c, double ndx,
+                                         double ndy, double ndz, int ana)
+{
+    unsafeAtomicAdd(hist + idx, e);
+    unsafeAtomicAdd(hist + stride + idx, sqrt(e * rho_c));
+    // signed direction sums through a float cast, as Directional_Histogram.Add does
+    const double vx = (double)(float)(ndx * e), vy = (double)(float)(ndy * e), vz = (double)(float)(ndz * e);
+    unsafeAtomicAdd(hist + (ndx > 0 ? 2 : 5) * stride + idx, vx);
+    unsafeAtomicAdd(hist + (ndy > 0 ? 3 : 6) * stride + idx, vy);
+    unsafeAtomicAdd(hist + (ndz > 0 ? 4 : 7) * stride + idx, vz);
+    unsafeAtomicAdd(hist + (8 + ana) * stride + idx, e);
+}
+
+template <class R> __device__ __forceinline__ double band_gain(const DevRecv<R> &rv, int band, double dist);
+template <> __device__ __forceinline__ double band_gain<double>(const DevRecv<double> &rv, int band, double dist)
+{
+    return pow(10.0, -.1 * rv.atten[band] * dist);
+}
+template <> __device__ __forceinline__ double band_gain<float>(const DevRecv<float> &rv, int band, double dist)
+{
+    return (double)exp2f(rv.kexp2[band] * (float)dist);
+}
+
+// Receiver_Bank.CheckRay / CheckBroadbandRay over every receiver.
+// sqd: squared distance to EndPt; E: intensity (octave) or energies (broadband).
+template <class R>
+__device__ void check_receivers(const DevRecv<R> &rv, R ox, R oy, R oz, R dx, R dy, R dz, R sqd,
+                                const R *E, int band, int lo, int hi, double tsum, int ana,
+                                LaneCounters &lc)
+{
+    for (int r = 0; r < rv.n; ++r) {
+        const auto C = rv.ctr[r];
+        const R mx = ox - C.x, my = oy - C.y, mz = oz - C.z;
+        const R b = mx * dx + my * dy + mz * dz;
+        const R c = (mx * mx + my * my + mz * mz) - rv.r2;
+        if (c > R(0) && b > R(0)) continue;
+        const R disc = b * b - c;
+        if (disc < R(0)) continue;
+        const R sq = sqrt(disc);
+        const R t1 = -b - sq, t2 = -b + sq;
+        const R t = (t1 + t2) * R(.5);
+        if (!(t > R(0) && t * t < sqd)) continue;
+        const double ts = (double)((t2 - t1) * R(0.5));
+        const double rt = (double)t * rv.inv_c + tsum;
+        const int sample = (int)(rt * rv.fs);
+        if (sample >= rv.sct || sample < 0) continue;
+        const double dist = rt * rv.c;
+        const double ndx = -(double)dx, ndy = -(double)dy, ndz = -(double)dz;
+        if (band >= 0) {
+            const uint64_t idx = ((uint64_t)r * kBands + band) * rv.sct + sample;
+            const double e = (double)E[0] * band_gain<R>(rv, band, dist) * rv.sizemod * ts;
+            hist_add(nullptr, rv.hist, rv.plane_stride, idx, e, rv.rho_c, ndx, ndy, ndz, ana);
+            lc.v[C_DEP]++;
+        } else {
+            for (int o = lo; o <= hi; ++o) {
+                const uint64_t idx = ((uint64_t)r * kBands + o) * rv.sct + sample;
+                const double e = (double)E[o] * band_gain<R>(rv, o, dist) * rv.sizemod * ts;
+                hist_add(nullptr, rv.hist, rv.plane_stride, idx, e, rv.rho_c, ndx, ndy, ndz, 0);
+                lc.v[C_DEP]++;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- reflection
+// sin(2 pi u), cos(2 pi u) for u in [0, 1): quarter-turn reduction and the
+// Taylor series of sin / cos on [-pi/4, pi/4] to degree 17/18 in a fixed Horner
+// order.  FP64 only: +, -, * are IEEE-exact operations, so the FP64 kernel and
+// the CPU oracle produce identical bits.
+__device__ __forceinline__ void sincos_turn(double u, double &s, double &c)
+{
+    const int k = (int)floor(4.0 * u + 0.5);
+    const double r = u - 0.25 * k;
+    const double a = r * 6.283185307179586476925286766559;
+    const double z = a * a;
+    const double sp = 1.0 + z * (-1.0 / 6.0 + z * (1.0 / 120.0 + z * (-1.0 / 5040.0 + z * (1.0 / 362880.0 +
+                      z * (-1.0 / 39916800.0 + z * (1.0 / 6227020800.0 + z * (-1.0 / 1307674368000.0 +
+                      z * (1.0 / 355687428096000.0))))))));
+    const double cp = 1.0 + z * (-1.0 / 2.0 + z * (1.0 / 24.0 + z * (-1.0 / 720.0 + z * (1.0 / 40320.0 +
+                      z * (-1.0 / 3628800.0 + z * (1.0 / 479001600.0 + z * (-1.0 / 87178291200.0 +
+                      z * (1.0 / 20922789888000.0 + z * (-1.0 / 6402373705728000.0)))))))));
+    const double sa = a * sp, ca = cp;
+    switch (k & 3) {
+    case 0: s = sa; c = ca; break;
+    case 1: s = ca; c = -sa; break;
+    case 2: s = -sa; c = -ca; break;
+    default: s = -ca; c = sa; break;
+    }
+}
+
+// Lambert diffuse direction about the incoming-side normal: azimuth 2 pi U,
+// cos(zenith) = U (uniform over the hemisphere, Classes_Materials.cs:4519-4549).
+// cos(acos U) is U and sin(acos U) is sqrt((1-U)(1+U)).
+template <class R>
+__device__ __forceinline__ void diffuse_dir(R nx, R ny, R nz, R uaz, R uzen, R &ox, R &oy, R &oz)
+{
+    R zx = nx, zy = ny, zz = nz;
+    R xx = 0, xy = 0, xz = 1;
+    const R proj = fabs(zz);
+    if (R(0.99) < proj && R(1.01) > proj) { xx = 1; xz = 0; }
+    // y = z cross x ; x = y cross z
+    R yx = zy * xz - zz * xy, yy = zz * xx - zx * xz, yz = zx * xy - zy * xx;
+    xx = yy * zz - yz * zy;
+    xy = yz * zx - yx * zz;
+    xz = yx * zy - yy * zx;
+    normalize3(xx, xy, xz);
+    normalize3(yx, yy, yz);
+    normalize3(zx, zy, zz);
+    R sa, ca, st;
+    if constexpr (sizeof(R) == 8) {
+        sincos_turn(uaz, sa, ca);
+        st = sqrt((1.0 - uzen) * (1.0 + uzen));
+    } else {
+        sincospif(2.0f * uaz, &sa, &ca);
+        st = sqrtf((1.0f - uzen) * (1.0f + uzen));
+    }
+    const R x = ca * st, y = sa * st, z = uzen;
+    ox = xx * x + yx * y + zx * z;
+    oy = xy * x + yy * y + zy * z;
+    oz = xz * x + yz * y + zz * z;
+    normalize3(ox, oy, oz);
+}
+
+// ---------------------------------------------------------------- phase 1
+template <class R>
+__device__ __forceinline__ void source_dir(const TraceConst &tc, uint32_t src, R &dx, R &dy, R &dz)
+{
+    uint32_t w[4];
+    philox4x32_10(0u, 0xFFFFFFFFu, 0xFFFFFFFFu, src, tc.key0, tc.key1, w);
+    double x, y, z;
+    if (tc.dir_mode == 2 && tc.dirs && tc.n_dirs > 0) {
+        const double *p = tc.dirs + 3 * (src % (uint64_t)tc.n_dirs);
+        x = p[0]; y = p[1]; z = p[2];
+    } else if (tc.dir_mode == 1) {
+        double st, ct, sp, cp;  // theta, phi ~ U(0, 2 pi)
+        sincos_turn(unif<double>(w[0]), st, ct);
+        sincos_turn(unif<double>(w[1]), sp, cp);
+        x = st * cp; y = st * sp; z = ct;
+    } else {
+        const double *f = tc.geo + 9 * (src % 80u);
+        const double su = sqrt(unif<double>(w[0])), v = unif<double>(w[1]);
+        x = (1 - su) * f[0] + su * (1 - v) * f[3] + su * v * f[6];
+        y = (1 - su) * f[1] + su * (1 - v) * f[4] + su * v * f[7];
+        z = (1 - su) * f[2] + su * (1 - v) * f[5] + su * v * f[8];
+    }
+    double l = sqrt(x * x + y * y + z * z);
+    if (l > 0) { x /= l; y /= l; z /= l; }
+    dx = (R)x; dy = (R)y; dz = (R)z;
+}
+
+template <class R>
+__device__ __forceinline__ void make_oray(ORay<R> &r, R ox, R oy, R oz, R dx, R dy, R dz, R I, double t,
+                                          int band, int tri, uint32_t src, uint64_t uid, uint32_t mask)
+{
+    r.o[0] = ox; r.o[1] = oy; r.o[2] = oz; r.I = I;
+    r.d[0] = dx; r.d[1] = dy; r.d[2] = dz; r.tri = tri;
+    r.src = src; r.t = t;
+    r.uid_lo = (uint32_t)uid; r.uid_hi = (uint32_t)(uid >> 32);
+    r.k = 0;
+    r.meta = (uint32_t)band | mask;
+}
+
+template <class R>
+__device__ __forceinline__ void store_dead(ORay<R> *out, unsigned from, unsigned to)
+{
+    for (unsigned q = from; q < to; ++q) out[q].meta = M_DEAD;
+}
+
+// cast_ray_distributed's broadband phase and split (TaskAll:345-424) for one
+// source ray; may suspend into resume_out when the output buffer is full.
+template <class R>
+__global__ void __launch_bounds__(kBlock)
+k_phase1(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv, int64_t src_begin,
+         int n_fresh, const P1State<R> *__restrict__ resume_in, int n_resume, ORay<R> *out,
+         unsigned *out_count, unsigned out_cap, P1State<R> *resume_out, unsigned *resume_count,
+         DevCounters *cnt)
+{
+    LaneCounters lc = {};
+    lc.lost = 0;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = resume_in ? n_resume : n_fresh;
+    if (i < n) {
+        P1State<R> st;
+        if (resume_in) {
+            st = resume_in[i];
+        } else {
+            st.src = (uint32_t)(src_begin + i);
+            st.o[0] = (R)tc.src[0]; st.o[1] = (R)tc.src[1]; st.o[2] = (R)tc.src[2];
+            source_dir<R>(tc, st.src, st.d[0], st.d[1], st.d[2]);
+            for (int o = 0; o < kBands; ++o) st.E[o] = (R)tc.power[o];
+            st.tb = 0; st.order = 0; st.tri = -1; st.k = 0; st.flags = 0;
+        }
+        const bool stoch = tc.mode == 0;
+        const int excl_on = Tmin<R>::excl ? 1 : 0;
+        bool split = true, suspended = false;
+        if (!(st.flags & F_SPLIT_PENDING)) {
+            for (;;) {
+                R leg;
+                const int ht = shoot<R>(s, st.o[0], st.o[1], st.o[2], st.d[0], st.d[1], st.d[2],
+                                        excl_on ? st.tri : -1, leg);
+                if (ht < 0) {
+                    lc.v[C_BROAD]++;
+                    if (st.order > tc.is_order) {
+                        const R ex = st.o[0] + st.d[0] * R(1000000), ey = st.o[1] + st.d[1] * R(1000000),
+                                ez = st.o[2] + st.d[2] * R(1000000);
+                        const R qx = ex - st.o[0], qy = ey - st.o[1], qz = ez - st.o[2];
+                        check_receivers<R>(rv, st.o[0], st.o[1], st.o[2], st.d[0], st.d[1], st.d[2],
+                                           qx * qx + qy * qy + qz * qz, st.E, -1, tc.oct_lo,
+                                           tc.oct_hi, st.tb, 0, lc);
+                    }
+                    double l = 0;
+                    for (int j = 0; j < kBands; ++j) l += (double)st.E[j];
+                    lc.lost += l;
+                    lc.v[C_MISS]++;
+                    split = false;
+                    break;
+                }
+                const int sf = s.surf[ht];
+                unsigned nch = 0;
+                if (stoch) {
+                    for (int oc = 0; oc < kBands; ++oc) {
+                        const R e2 = st.E[oc] * s.mat[sf * kBands + oc].x;
+                        if (e2 != R(0)) nch += (s.trans[sf * kBands + oc] > R(0)) ? 3u : 1u;
+                    }
+                }
+                unsigned base = 0;
+                if (nch) {
+                    base = atomicAdd(out_count, nch);
+                    if (base + nch > out_cap) {
+                        if (base < out_cap) store_dead<R>(out, base, out_cap);
+                        resume_out[atomicAdd(resume_count, 1u)] = st;
+                        lc.v[C_RESUME]++;
+                        suspended = true;
+                        break;
+                    }
+                }
+                lc.v[C_BROAD]++;
+                const R hx = st.o[0] + st.d[0] * leg, hy = st.o[1] + st.d[1] * leg, hz = st.o[2] + st.d[2] * leg;
+                if (st.order > tc.is_order) {
+                    const R qx = hx - st.o[0], qy = hy - st.o[1], qz = hz - st.o[2];
+                    check_receivers<R>(rv, st.o[0], st.o[1], st.o[2], st.d[0], st.d[1], st.d[2],
+                                       qx * qx + qy * qy + qz * qz, st.E, -1, tc.oct_lo, tc.oct_hi,
+                                       st.tb, 0, lc);
+                }
+                st.tb += (double)leg / rv.c;
+                st.o[0] = hx; st.o[1] = hy; st.o[2] = hz;
+                st.order++;
+                st.tri = ht;
+                const auto N = s.nrm[ht];
+                const R cos_t = st.d[0] * N.x + st.d[1] * N.y + st.d[2] * N.z;
+                for (int o = 0; o < kBands; ++o) st.E[o] *= s.mat[sf * kBands + o].x;
+                if (stoch) {
+                    // Scatter_Early (Classes_Materials.cs:4498-4570)
+                    R fx = N.x, fy = N.y, fz = N.z, cf = cos_t;
+                    if (cf > R(0)) { fx = -fx; fy = -fy; fz = -fz; cf = -cf; }
+                    unsigned q = base;
+                    for (int oc = 0; oc < kBands; ++oc) {
+                        if (st.E[oc] == R(0)) continue;
+                        uint32_t w[4];
+                        philox4x32_10(4u * st.k + (uint32_t)(oc >> 1), 0u, 0u, st.src, tc.key0, tc.key1, w);
+                        const R sc = s.mat[sf * kBands + oc].y;
+                        R childI = st.E[oc] * sc;
+                        st.E[oc] *= (R(1) - sc);
+                        R ddx, ddy, ddz;
+                        diffuse_dir<R>(fx, fy, fz, unif<R>(w[2 * (oc & 1)]), unif<R>(w[2 * (oc & 1) + 1]),
+                                       ddx, ddy, ddz);
+                        const R tau = s.trans[sf * kBands + oc];
+                        if (tau > R(0)) {
+                            ORay<R> tr;
+                            make_oray<R>(tr, hx, hy, hz, st.d[0], st.d[1], st.d[2], st.E[oc] * tau, st.tb,
+                                         oc, ht, st.src, child_uid(0, PACH_TAG_EARLY_TR(st.k, oc)), 0u);
+                            st.E[oc] *= (R(1) - tau);
+                            out[q++] = tr;
+                            ORay<R> td;
+                            make_oray<R>(td, hx, hy, hz, -ddx, -ddy, -ddz, childI * tau, st.tb, oc, ht,
+                                         st.src, child_uid(0, PACH_TAG_EARLY_TD(st.k, oc)), M_SCAT | M_EVER);
+                            childI *= (R(1) - tau);
+                            out[q++] = td;
+                            lc.v[C_SPAWN] += 2;
+                            lc.v[C_OCT] += 2;
+                        }
+                        ORay<R> ch;
+                        make_oray<R>(ch, hx, hy, hz, ddx, ddy, ddz, childI, st.tb, oc, ht, st.src,
+                                     child_uid(0, PACH_TAG_EARLY(st.k, oc)), M_SCAT | M_EVER);
+                        out[q++] = ch;
+                        lc.v[C_SPAWN]++;
+                        lc.v[C_OCT]++;
+                    }
+                    st.d[0] -= fx * cf * R(2);
+                    st.d[1] -= fy * cf * R(2);
+                    st.d[2] -= fz * cf * R(2);
+                } else {
+                    st.d[0] -= N.x * cos_t * R(2);
+                    st.d[1] -= N.y * cos_t * R(2);
+                    st.d[2] -= N.z * cos_t * R(2);
+                }
+                st.k++;
+                if (!((double)st.E[tc.h] > tc.T1)) break;
+                if (st.order >= tc.p1_max) { lc.v[C_P1CAP]++; break; }
+            }
+        }
+        if (split && !suspended) {
+            const unsigned nb = (unsigned)(tc.oct_hi - tc.oct_lo + 1);
+            const unsigned base = atomicAdd(out_count, nb);
+            if (base + nb > out_cap) {
+                if (base < out_cap) store_dead<R>(out, base, out_cap);
+                st.flags |= F_SPLIT_PENDING;
+                resume_out[atomicAdd(resume_count, 1u)] = st;
+                lc.v[C_RESUME]++;
+            } else {
+                // BroadRay.SplitRay(o) per octave (Classes_Rays.cs:331-337)
+                for (int oc = tc.oct_lo; oc <= tc.oct_hi; ++oc) {
+                    ORay<R> r;
+                    make_oray<R>(r, st.o[0], st.o[1], st.o[2], st.d[0], st.d[1], st.d[2], st.E[oc], st.tb,
+                                 oc, st.tri, st.src, child_uid(0, PACH_TAG_SPLIT(oc)), 0u);
+                    out[base + (oc - tc.oct_lo)] = r;
+                    lc.v[C_OCT]++;
+                }
+            }
+        }
+    }
+    for (int c = 0; c < C_N; ++c)
+        if (lc.v[c]) atomicAdd(&cnt->v[c], (unsigned long long)lc.v[c]);
+    if (lc.lost != 0) unsafeAtomicAdd(&cnt->lost, lc.lost);
+}
+
+// ---------------------------------------------------------------- phases 2, 3
+// One bounce of an octave ray.  Returns false when the ray terminates; sets
+// has_child when Scatter_Late spawned a specular child.
+template <class R>
+__device__ __forceinline__ bool octave_bounce(const TraceConst &tc, const DevScene<R> &s,
+                                              const DevRecv<R> &rv, ORay<R> &r, bool &has_child,
+                                              ORay<R> &child, LaneCounters &lc)
+{
+    has_child = false;
+    if ((int)r.k >= tc.max_bounces) { lc.v[C_GUARD]++; return false; }
+    const int band = (int)(r.meta & M_BAND);
+    const bool late3 = (r.meta & M_LATE3) != 0;
+    const int ana = (r.meta & M_SCAT) ? 1 : ((r.meta & M_EVER) ? 2 : 0);
+    R leg;
+    const int ht = shoot<R>(s, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2],
+                            Tmin<R>::excl ? r.tri : -1, leg);
+    lc.v[late3 ? C_VLATE : C_LATE]++;
+    if (ht < 0) {
+        const R ex = r.o[0] + r.d[0] * R(1000000), ey = r.o[1] + r.d[1] * R(1000000),
+                ez = r.o[2] + r.d[2] * R(1000000);
+        const R qx = ex - r.o[0], qy = ey - r.o[1], qz = ez - r.o[2];
+        check_receivers<R>(rv, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2],
+                           qx * qx + qy * qy + qz * qz, &r.I, band, 0, 0, r.t, ana, lc);
+        lc.lost += (double)r.I;
+        lc.v[C_MISS]++;
+        return false;
+    }
+    const R hx = r.o[0] + r.d[0] * leg, hy = r.o[1] + r.d[1] * leg, hz = r.o[2] + r.d[2] * leg;
+    {
+        const R qx = hx - r.o[0], qy = hy - r.o[1], qz = hz - r.o[2];
+        check_receivers<R>(rv, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2],
+                           qx * qx + qy * qy + qz * qz, &r.I, band, 0, 0, r.t, ana, lc);
+    }
+    r.t += (double)leg / rv.c;
+    r.o[0] = hx; r.o[1] = hy; r.o[2] = hz;
+    r.tri = ht;
+    const int sf = s.surf[ht];
+    const auto N = s.nrm[ht];
+    const auto M = s.mat[sf * kBands + band];  // (1 - alpha, s, lo, hi)
+    const R cos_t = N.x * r.d[0] + N.y * r.d[1] + N.z * r.d[2];
+    r.I *= M.x;
+    const uint32_t kev = r.k++;
+    if (tc.mode == 0) {
+        uint32_t w[4];
+        philox4x32_10(kev, r.uid_lo, r.uid_hi, r.src, tc.key0, tc.key1, w);
+        const bool transmit = tc.has_trans && unif<R>(w[0]) < s.trans[sf * kBands + band];
+        bool diffuse;
+        if (!late3) {
+            // Scatter_Late (Classes_Materials.cs:4629-4702)
+            const R sel = unif<R>(w[1]);
+            if (sel > M.w) {
+                diffuse = false;
+            } else {
+                diffuse = true;
+                if (sel > M.z) {
+                    const R emc = R(1) - M.y;
+                    child = r;
+                    child.I = r.I * emc;
+                    r.I *= (R(1) - emc);
+                    child.d[0] -= N.x * cos_t * R(2);
+                    child.d[1] -= N.y * cos_t * R(2);
+                    child.d[2] -= N.z * cos_t * R(2);
+                    if (transmit) { child.d[0] = -child.d[0]; child.d[1] = -child.d[1]; child.d[2] = -child.d[2]; }
+                    child.meta &= ~M_SCAT;
+                    const uint64_t cu = child_uid(((uint64_t)r.uid_hi << 32) | r.uid_lo, PACH_TAG_LATE(kev));
+                    child.uid_lo = (uint32_t)cu;
+                    child.uid_hi = (uint32_t)(cu >> 32);
+                    child.k = 0;
+                    has_child = true;
+                    lc.v[C_SPAWN]++;
+                    lc.v[C_OCT]++;
+                }
+            }
+        } else {
+            // Scatter_VeryLate (Classes_Materials.cs:4572-4627)
+            diffuse = unif<R>(w[1]) < M.y;
+        }
+        if (diffuse) {
+            R fx = N.x, fy = N.y, fz = N.z;
+            if (cos_t > R(0)) { fx = -fx; fy = -fy; fz = -fz; }
+            diffuse_dir<R>(fx, fy, fz, unif<R>(w[2]), unif<R>(w[3]), r.d[0], r.d[1], r.d[2]);
+            r.meta |= (M_SCAT | M_EVER);
+        } else {
+            r.d[0] -= N.x * cos_t * R(2);
+            r.d[1] -= N.y * cos_t * R(2);
+            r.d[2] -= N.z * cos_t * R(2);
+            r.meta &= ~M_SCAT;
+        }
+        if (transmit) { r.d[0] = -r.d[0]; r.d[1] = -r.d[1]; r.d[2] = -r.d[2]; }
+    } else {
+        r.d[0] -= N.x * cos_t * R(2);
+        r.d[1] -= N.y * cos_t * R(2);
+        r.d[2] -= N.z * cos_t * R(2);
+    }
+    if (!late3) {
+        if (!(r.t < tc.cotime && (double)r.I > tc.T2[band])) r.meta |= M_LATE3;  // hand-off to End_Trace
+        return true;
+    }
+    return r.t < tc.cotime && (double)r.I > tc.T3[band];
+}
+
+// Persistent, wavefront-compacting octave-ray kernel.  Each wave keeps a LIFO
+// stack of spawned rays in its own HBM slab; lanes whose ray terminated are
+// refilled from that stack first, then from the shared input (the previous
+// launch's leftovers in0, then the new broadband-phase output in1).  Once the
+// input is drained and the wave has done at least min_iters bounce steps, it
+// moves its stack and in-flight rays to the shared `dump` queue and exits, so
+// the next launch spreads the leftover work over every wave.  No wave ever
+// waits on another, so every launch ends after a bounded number of steps.
+template <class R>
+__global__ void __launch_bounds__(kBlock)
+k_octave(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv,
+         const ORay<R> *__restrict__ in0, const unsigned *n0_ptr, unsigned cap0,
+         const ORay<R> *__restrict__ in1, const unsigned *n1_ptr, unsigned cap1, unsigned *in_head,
+         ORay<R> *stacks, int stack_cap, ORay<R> *dump, unsigned *dump_count, unsigned dump_cap,
+         int min_iters, DevCounters *cnt)
+{
+    LaneCounters lc = {};
+    lc.lost = 0;
+    const int lane = (int)__lane_id();
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint64_t wave = (uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    ORay<R> *ws = stacks + wave * (uint64_t)stack_cap;
+    const unsigned n0 = in0 ? min(*n0_ptr, cap0) : 0u;
+    const unsigned n1 = in1 ? min(*n1_ptr, cap1) : 0u;
+    const unsigned n_in = n0 + n1;
+    int stack_n = 0;
+    int iters = 0;
+    bool input_done = false;
+    bool act = false;
+    ORay<R> r;
+    for (;;) {
+        // ---- refill idle lanes: own stack first, then the input queue
+        const uint64_t empty = __ballot(!act);
+        if (empty) {
+            const int n_empty = __popcll(empty);
+            const int rank = __popcll(empty & lt_mask);
+            const int from_stack = min(n_empty, stack_n);
+            if (!act && rank < from_stack) {
+                r = ws[stack_n - 1 - rank];
+                act = true;
+            }
+            stack_n -= from_stack;
+            const int need = n_empty - from_stack;
+            if (need > 0 && !input_done) {
+                unsigned base = 0;
+                if (lane == 0) base = atomicAdd(in_head, (unsigned)need);
+                base = __shfl(base, 0);
+                const int avail = base >= n_in ? 0 : (int)min((unsigned)need, n_in - base);
+                if (avail < need) input_done = true;
+                const int j = rank - from_stack;
+                if (!act && j >= 0 && j < avail) {
+                    const unsigned q = base + (unsigned)j;
+                    r = q < n0 ? in0[q] : in1[q - n0];
+                    act = (r.meta & M_DEAD) == 0;
+                }
+            }
+        }
+        const uint64_t am = __ballot(act);
+        if (am == 0) {
+            if (input_done) break;
+            continue;  // the claim returned only dead slots: claim again
+        }
+        if (input_done && iters >= min_iters) {
+            // ---- hand the wave's remaining work to the next launch
+            const int n_act = __popcll(am);
+            const unsigned n_dump = (unsigned)(stack_n + n_act);
+            unsigned base = 0;
+            if (lane == 0) base = atomicAdd(dump_count, n_dump);
+            base = __shfl(base, 0);
+            for (int i = lane; i < stack_n; i += 64) {
+                const unsigned q = base + (unsigned)i;
+                if (q < dump_cap) dump[q] = ws[i];
+                else { lc.v[C_DROP]++; lc.lost += (double)ws[i].I; }
+            }
+            if (act) {
+                const unsigned q = base + (unsigned)stack_n + (unsigned)__popcll(am & lt_mask);
+                if (q < dump_cap) dump[q] = r;
+                else { lc.v[C_DROP]++; lc.lost += (double)r.I; }
+            }
+            lc.v[C_SPILL] += (lane == 0) ? n_dump : 0u;
+            break;
+        }
+        // ---- one bounce per active lane
+        bool has_child = false;
+        ORay<R> child;
+        if (act) act = octave_bounce<R>(tc, s, rv, r, has_child, child, lc);
+        ++iters;
+        // ---- push spawned rays onto the wave's stack (overflow goes to the dump queue)
+        const uint64_t cm = __ballot(has_child);
+        if (cm) {
+            const int n = __popcll(cm);
+            if (has_child) {
+                const int pos = stack_n + __popcll(cm & lt_mask);
+                if (pos < stack_cap) {
+                    ws[pos] = child;
+                } else {
+                    const unsigned q = atomicAdd(dump_count, 1u);
+                    if (q < dump_cap) { dump[q] = child; lc.v[C_SPILL]++; }
+                    else { lc.v[C_DROP]++; lc.lost += (double)child.I; }
+                }
+            }
+            stack_n = min(stack_n + n, stack_cap);
+            __threadfence_block();
+        }
+    }
+    for (int c = 0; c < C_N; ++c)
+        if (lc.v[c]) atomicAdd(&cnt->v[c], (unsigned long long)lc.v[c]);
+    if (lc.lost != 0) unsafeAtomicAdd(&cnt->lost, lc.lost);
+}
+
+// ---------------------------------------------------------------- launchers
+template <class R>
+hipError_t launch_phase1(hipStream_t st, const TraceConst &tc, const DevScene<R> &s,
+                         const DevRecv<R> &rv, int64_t src_begin, int n_fresh,
+                         const P1State<R> *resume_in, int n_resume, ORay<R> *out,
+                         unsigned *out_count, unsigned out_cap, P1State<R> *resume_out,
+                         unsigned *resume_count, DevCounters *cnt)
+{
+    const int n = resume_in ? n_resume : n_fresh;
+    if (n <= 0) return hipSuccess;
+    const int grid = (n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_phase1<R>, dim3(grid), dim3(kBlock), 0, st, tc, s, rv, src_begin, n_fresh,
+                       resume_in, n_resume, out, out_count, out_cap, resume_out, resume_count, cnt);
+    return hipGetLastError();
+}
+
+template <class R>
+hipError_t launch_octave(hipStream_t st, int grid, const TraceConst &tc, const DevScene<R> &s,
+                         const DevRecv<R> &rv, const ORay<R> *in0, const unsigned *n0, unsigned cap0,
+                         const ORay<R> *in1, const unsigned *n1, unsigned cap1, unsigned *in_head,
+                         ORay<R> *stacks, int stack_cap, ORay<R> *dump, unsigned *dump_count,
+                         unsigned dump_cap, int min_iters, DevCounters *cnt)
+{
+    hipLaunchKernelGGL(k_octave<R>, dim3(grid), dim3(kBlock), 0, st, tc, s, rv, in0, n0, cap0, in1, n1,
+                       cap1, in_head, stacks, stack_cap, dump, dump_count, dump_cap, min_iters, cnt);
+    return hipGetLastError();
+}
+
+template <class R> int octave_blocks_per_cu()
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_octave<R>, kBlock, 0) != hipSuccess || nb < 1)
+        nb = 1;
+    return nb;
+}
+
+#define PACH_INSTANTIATE(R)                                                                        \
+    template hipError_t launch_phase1<R>(hipStream_t, const TraceConst &, const DevScene<R> &,     \
+                                         const DevRecv<R> &, int64_t, int, const P1State<R> *, int, \
+                                         ORay<R> *, unsigned *, unsigned, P1State<R> *, unsigned *, \
+                                         DevCounters *);                                           \
+    template hipError_t launch_octave<R>(hipStream_t, int, const TraceConst &, const DevScene<R> &, \
+                                         const DevRecv<R> &, const ORay<R> *, const unsigned *,     \
+                                         unsigned, const ORay<R> *, const unsigned *, unsigned,     \
+                                         unsigned *, ORay<R> *, int, ORay<R> *, unsigned *,         \
+                                         unsigned, int, DevCounters *);                             \
+    template int octave_blocks_per_cu<R>();
+
+}  // namespace pach

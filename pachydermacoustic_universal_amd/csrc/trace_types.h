@@ -1,0 +1,134 @@
+// trace_types.h -- plain data shared by the host runtime and the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pach {
+
+constexpr int kBands = 8;
+constexpr int kPlanes = 11;
+constexpr int kBlock = 256;  // 4 wave64 per workgroup
+
+template <class R> struct V4;
+template <> struct V4<float> { using T = float4; };
+template <> struct V4<double> { using T = double4; };
+
+// Scene in HBM: structure-of-arrays triangle records, per-(polygon, band)
+// material records and a uniform voxel grid in CSR form.
+template <class R> struct DevScene {
+    const typename V4<R>::T *v0;   // [T] vertex 0 (w unused)
+    const typename V4<R>::T *e1;   // [T] v1 - v0
+    const typename V4<R>::T *e2;   // [T] v2 - v0
+    const typename V4<R>::T *nrm;  // [T] unit normal (Topology.Normal)
+    const int *surf;               // [T] polygon index (Surf_ID)
+    const uint32_t *cell_start;    // [ncell + 1]
+    const uint32_t *cell_tris;     // triangle ids per cell
+    int gx, gy, gz;
+    R gmin[3], cs[3], inv_cs[3];
+    const typename V4<R>::T *mat;  // [S*8] (1 - alpha, s, lo, hi) of Lambert_Scattering
+    const R *trans;                // [S*8] transmission
+};
+
+template <class R> struct DevRecv {
+    const typename V4<R>::T *ctr;  // [R] centre (w unused)
+    int n;
+    R r2;
+    double sizemod;   // 1 / (pi r^3)
+    double c, inv_c;  // speed of sound and 1/c as Spherical_Receiver holds them
+    double atten[kBands];
+    float kexp2[kBands];  // -0.1 * atten * log2(10): 10^(-0.1 a x) = exp2(k x)
+    double rho_c;
+    int fs, sct;
+    double *hist;         // [kPlanes][n][8][sct]
+    uint64_t plane_stride;
+};
+
+struct TraceConst {
+    int mode, oct_lo, oct_hi, is_order, h, dir_mode;
+    int p1_max, max_bounces;
+    double cotime, T1;
+    double T2[kBands], T3[kBands];
+    double power[kBands];
+    double src[3];
+    uint32_t key0, key1;
+    const double *dirs;   // supplied directions [n_dirs][3]
+    int64_t n_dirs;
+    const double *geo;    // 80 geodesic faces [80][9]
+    int has_trans;        // any transmission coefficient > 0
+};
+
+// Octave ray (OctaveRay, Classes_Rays.cs:30-181).  meta bits: 0-2 band,
+// 3 currently scattered, 4 ever scattered, 5 End_Trace phase, 6 dead slot.
+template <class R> struct alignas(16) ORay {
+    R o[3];
+    R I;
+    R d[3];
+    int tri;   // last surface hit (triangle), -1 at the source
+    uint32_t src;
+    double t;  // t_sum in seconds
+    uint32_t uid_lo, uid_hi, k, meta;
+};
+
+// Broadband ray between launches (only when the phase-1 output buffer fills).
+template <class R> struct alignas(16) P1State {
+    R o[3];
+    R d[3];
+    R E[kBands];
+    double tb;
+    int order, tri;
+    uint32_t k, src;
+    int flags;
+};
+
+enum : uint32_t {
+    M_BAND = 7u,
+    M_SCAT = 1u << 3,
+    M_EVER = 1u << 4,
+    M_LATE3 = 1u << 5,
+    M_DEAD = 1u << 6,
+};
+
+enum : int { F_SPLIT_PENDING = 1 };
+
+// device counters (one set per trace)
+enum : int {
+    C_BROAD = 0,
+    C_LATE,
+    C_VLATE,
+    C_OCT,
+    C_SPAWN,
+    C_MISS,
+    C_DEP,
+    C_P1CAP,
+    C_GUARD,
+    C_SPILL,
+    C_RESUME,
+    C_DROP,
+    C_N
+};
+
+struct DevCounters {
+    unsigned long long v[16];
+    double lost;
+    double pad;
+};
+
+// launchers (trace_f32.hip / trace_f64.hip)
+template <class R>
+hipError_t launch_phase1(hipStream_t st, const TraceConst &tc, const DevScene<R> &s,
+                         const DevRecv<R> &rv, int64_t src_begin, int n_fresh,
+                         const P1State<R> *resume_in, int n_resume, ORay<R> *out,
+                         unsigned *out_count, unsigned out_cap, P1State<R> *resume_out,
+                         unsigned *resume_count, DevCounters *cnt);
+
+template <class R>
+hipError_t launch_octave(hipStream_t st, int grid, const TraceConst &tc, const DevScene<R> &s,
+                         const DevRecv<R> &rv, const ORay<R> *in0, const unsigned *n0, unsigned cap0,
+                         const ORay<R> *in1, const unsigned *n1, unsigned cap1, unsigned *in_head,
+                         ORay<R> *stacks, int stack_cap, ORay<R> *dump, unsigned *dump_count,
+                         unsigned dump_cap, int min_iters, DevCounters *cnt);
+
+template <class R> int octave_blocks_per_cu();
+
+}  // namespace pach

@@ -1,0 +1,180 @@
+"""Seeded synthetic rooms for the stochastic tracer.
+
+Each generator returns a :class:`Room`: a triangulated, watertight polygon
+scene with one material entry per polygon (the layout ``Polygon_Scene.Construct``
+builds, Classes_Scenes.cs:447-630: absorption, scattering and transmission are
+stored per polygon at :484-487), plus the source origin and receiver centres
+of the workload.  Polygons are quads or triangles; every polygon is split into
+triangles that keep the polygon's index as their surface id, so ``Surf_ID``
+semantics (material lookup per polygon) are unchanged.
+
+The medium of every workload is 20 C, 50 % RH, 1013.25 hPa (ISO 9613-1,
+Air_Choice 0); the source is omni at 120 dB per octave band (1 W).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+NBANDS = 8
+
+
+@dataclass
+class Room:
+    name: str
+    verts: np.ndarray          # [T, 3, 3] float64 triangle vertices
+    surf: np.ndarray           # [T] int32 polygon (surface) index of each triangle
+    alpha: np.ndarray          # [S, 8] absorption per polygon and octave band
+    scat: np.ndarray           # [S, 8] scattering coefficient
+    trans: np.ndarray          # [S, 8] transmission coefficient
+    source: np.ndarray         # [3]
+    receivers: np.ndarray      # [R, 3]
+    radius: float = 0.5
+    volume: float = 0.0
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n_tris(self) -> int:
+        return int(self.verts.shape[0])
+
+    @property
+    def n_surfaces(self) -> int:
+        return int(self.alpha.shape[0])
+
+    def bbox(self):
+        v = self.verts.reshape(-1, 3)
+        return v.min(0), v.max(0)
+
+
+def _tri_volume(verts: np.ndarray) -> float:
+    """Signed-volume sum (divergence theorem) of a closed triangle mesh."""
+    a, b, c = verts[:, 0], verts[:, 1], verts[:, 2]
+    return float(abs(np.einsum("ij,ij->i", a, np.cross(b, c)).sum()) / 6.0)
+
+
+def _quads_to_tris(quads: np.ndarray):
+    """[Q, 4, 3] quads -> [2Q, 3, 3] triangles and the quad index of each."""
+    t0 = quads[:, [0, 1, 2]]
+    t1 = quads[:, [0, 2, 3]]
+    tris = np.stack([t0, t1], axis=1).reshape(-1, 3, 3)
+    surf = np.repeat(np.arange(quads.shape[0], dtype=np.int32), 2)
+    return tris, surf
+
+
+def shoebox(lx=10.0, ly=8.0, lz=4.0, alpha=0.10, scat=0.20, trans=0.0,
+            source=(2.0, 3.0, 1.5), n_receivers=3, radius=0.5, seed=1,
+            receivers=None, name="shoebox") -> Room:
+    """Axis-aligned box as 6 quad polygons (12 triangles), normals inward.
+
+    The plain-method small workload: 10 x 8 x 4 m, uniform absorption 0.10,
+    scattering 0.20, 3 receivers of radius 0.5 m drawn uniformly from the box
+    shrunk by 1 m (seed 1).
+    """
+    x0, y0, z0, x1, y1, z1 = 0.0, 0.0, 0.0, lx, ly, lz
+    P = np.array([[x0, y0, z0], [x1, y0, z0], [x1, y1, z0], [x0, y1, z0],
+                  [x0, y0, z1], [x1, y0, z1], [x1, y1, z1], [x0, y1, z1]])
+    # vertex order gives inward normals (e1 x e2 points into the room)
+    faces = np.array([[0, 1, 2, 3],   # floor  z=0, normal +z
+                      [4, 7, 6, 5],   # ceiling z=lz, normal -z
+                      [0, 4, 5, 1],   # y=0, normal +y
+                      [3, 2, 6, 7],   # y=ly, normal -y
+                      [0, 3, 7, 4],   # x=0, normal +x
+                      [1, 5, 6, 2]])  # x=lx, normal -x
+    tris, surf = _quads_to_tris(P[faces])
+    S = faces.shape[0]
+    A = np.broadcast_to(np.asarray(alpha, dtype=np.float64), (S, NBANDS)).copy()
+    Sc = np.broadcast_to(np.asarray(scat, dtype=np.float64), (S, NBANDS)).copy()
+    Tr = np.broadcast_to(np.asarray(trans, dtype=np.float64), (S, NBANDS)).copy()
+    if receivers is None:
+        rng = np.random.default_rng(seed)
+        lo = np.array([x0 + 1, y0 + 1, z0 + 1])
+        hi = np.array([x1 - 1, y1 - 1, z1 - 1])
+        receivers = lo + (hi - lo) * rng.random((n_receivers, 3))
+    return Room(name=name, verts=tris, surf=surf, alpha=A, scat=Sc, trans=Tr,
+                source=np.asarray(source, dtype=np.float64),
+                receivers=np.asarray(receivers, dtype=np.float64).reshape(-1, 3),
+                radius=float(radius), volume=lx * ly * lz,
+                meta={"dims": (lx, ly, lz)})
+
+
+def _deformed_box(mapping, nu, nv, nw, jitter, rng):
+    """Closed, watertight quad mesh of the unit cube's surface pushed through
+    ``mapping(u, v, w) -> xyz``; every shared vertex is jittered once, so
+    neighbouring quads keep common vertices.  Returns [Q, 4, 3] quads whose
+    normals point into the room."""
+    N = (nu, nv, nw)
+    verts = {}
+
+    def vid(i, j, k):
+        key = (i, j, k)
+        if key not in verts:
+            p = np.asarray(mapping(i / nu, j / nv, k / nw), dtype=np.float64)
+            verts[key] = p + (rng.uniform(-jitter, jitter, 3) if jitter > 0 else 0.0)
+        return verts[key]
+
+    quads = []
+    # each face: fixed axis a at value 0 or N[a]; (b, c) spans the face.
+    for a in range(3):
+        b, c = [x for x in range(3) if x != a]
+        for side in (0, 1):
+            fixed = 0 if side == 0 else N[a]
+            for ib in range(N[b]):
+                for ic in range(N[c]):
+                    corners = []
+                    for db, dc in ((0, 0), (1, 0), (1, 1), (0, 1)):
+                        idx = [0, 0, 0]
+                        idx[a] = fixed
+                        idx[b] = ib + db
+                        idx[c] = ic + dc
+                        corners.append(vid(*idx))
+                    # orientation: (b, c) order gives normal along +a for a cyclic
+                    # (a, b, c); flip so normals point inward.
+                    cyclic = (b - a) % 3 == 1
+                    inward_positive = side == 0
+                    if cyclic != inward_positive:
+                        corners = corners[::-1]
+                    quads.append(corners)
+    return np.asarray(quads, dtype=np.float64)
+
+
+def fan_hall(seed=2, nu=14, nv=18, nw=8, height=12.0, front_half=8.0, back_half=18.0,
+             depth=30.0, jitter=0.15, n_receivers=16, radius=0.5,
+             source=(0.0, 4.0, 1.5), name="fan_hall") -> Room:
+    """Seeded polyhedral hall: a fan-shaped plan (16 m stage wall splaying to a
+    36 m back wall over 30 m) extruded to 12 m, every wall, floor and ceiling
+    subdivided into quads and vertex-jittered (+-0.15 m) to ~2,000 triangles.
+    Each quad is one polygon with absorption U(0.02, 0.40) and scattering
+    U(0.05, 0.70) per band; 16 receivers 1.2 m over the audience area."""
+    rng = np.random.default_rng(seed)
+
+    def mapping(u, v, w):
+        half = front_half + (back_half - front_half) * v
+        return (-half + 2 * half * u, depth * v, height * w)
+
+    quads = _deformed_box(mapping, nu, nv, nw, jitter, rng)
+    tris, surf = _quads_to_tris(quads)
+    S = quads.shape[0]
+    alpha = rng.uniform(0.02, 0.40, (S, NBANDS))
+    scat = rng.uniform(0.05, 0.70, (S, NBANDS))
+    trans = np.zeros((S, NBANDS))
+    recs = []
+    for _ in range(n_receivers):
+        u = rng.uniform(0.15, 0.85)
+        v = rng.uniform(0.35, 0.95)
+        x, y, _z = mapping(u, v, 0.0)
+        recs.append((x, y, 1.2))
+    return Room(name=name, verts=tris, surf=surf, alpha=alpha, scat=scat, trans=trans,
+                source=np.asarray(source, dtype=np.float64),
+                receivers=np.asarray(recs, dtype=np.float64), radius=float(radius),
+                volume=_tri_volume(tris), meta={"grid": (nu, nv, nw), "jitter": jitter})
+
+
+def small_workload() -> Room:
+    """Plain method, small: the 12-triangle shoebox."""
+    return shoebox()
+
+
+def hall_workload() -> Room:
+    """Seeded polyhedral hall (~2,000 triangles, 16 receivers)."""
+    return fan_hall()

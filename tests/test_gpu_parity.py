@@ -1,0 +1,115 @@
+"""GPU parity of the HIP tracer against the FP64 oracle (through the C ABI).
+
+Tolerances:
+  * FP64 geometry, deterministic specular mode: every histogram bin within
+    1e-5 relative of the oracle (the bar for the deterministic mode), and
+    bounce / deposit counters equal.
+  * FP64 geometry, stochastic mode: the same per-ray Philox streams drive both,
+    so bins again agree within 1e-5 relative.
+  * FP32 geometry, stochastic mode: T30 and EDT within 2 % and C80 within
+    0.2 dB per band of the FP64 result on the same seeds.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as O
+from pachydermacoustic_universal_amd import _abi as A
+from pachydermacoustic_universal_amd import rooms
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def tracer():
+    t = A.Tracer(0)
+    yield t
+    t.close()
+
+
+@pytest.fixture(scope="module")
+def medium():
+    at, c, rho = A.uniform_medium()
+    return at, c, rho
+
+
+def seeded_dirs(n, seed):
+    rng = np.random.default_rng(seed)
+    d = rng.normal(size=(n, 3))
+    return d / np.linalg.norm(d, axis=1, keepdims=True)
+
+
+def run_both(tracer, medium, room, *, rays, cut, seed, mode, precision, dir_mode=A.DIR_GEODESIC,
+             dirs=None, grid=(0, 0, 0), **kw):
+    at, c, rho = medium
+    print(f"[case] {room.name} rays={rays} cut={cut} seed={seed} mode={mode} prec={precision}",
+          flush=True)
+    st = A.trace_room(tracer, room, rays=rays, cutoff_s=cut, seed=seed, mode=mode,
+                      precision=precision, dir_mode=dir_mode, dirs=dirs, grid=grid,
+                      medium=medium, **kw)
+    g = tracer.planes(scaled=False)
+    ora, cnt = O.trace(room, rays=rays, cutoff_s=cut, seed=seed, mode=mode, dir_mode=dir_mode,
+                       dirs=dirs, medium=O.Medium(c=c, atten=at, rho=rho), threads=8, **kw)
+    return st, g, cnt, ora
+
+
+def assert_bins(g, o, rtol, what):
+    scale = max(np.abs(o).max(), 1e-300)
+    diff = np.abs(g - o)
+    bad = diff > rtol * np.abs(o) + 1e-12 * scale
+    assert not bad.any(), (f"{what}: {bad.sum()} of {bad.size} bins off; worst rel "
+                           f"{(diff / np.maximum(np.abs(o), 1e-300))[bad].max():.3e}")
+
+
+def bounces(cnt):
+    return cnt["bounces_broad"] + cnt["bounces_late"] + cnt["bounces_verylate"]
+
+
+@pytest.mark.parametrize("room_fn,rays,cut", [(rooms.shoebox, 64, 2.0), (rooms.fan_hall, 48, 1.5)])
+def test_specular_fp64_per_bin(tracer, medium, room_fn, rays, cut):
+    room = room_fn()
+    dirs = seeded_dirs(rays, 11)
+    st, g, cnt, ora = run_both(tracer, medium, room, rays=rays, cut=cut, seed=3,
+                               mode=A.MODE_SPECULAR, precision=A.PREC_FP64,
+                               dir_mode=A.DIR_SUPPLIED, dirs=dirs)
+    assert st["bounces"] == bounces(cnt)
+    assert st["deposits"] == cnt["deposits"]
+    assert st["deposits"] > 0
+    for k in range(A.NPLANES):
+        assert_bins(g[k], ora[k], 1e-5, f"plane {k}")
+
+
+@pytest.mark.parametrize("room_fn,rays,cut", [(rooms.shoebox, 6, 0.6), (rooms.fan_hall, 12, 1.0)])
+def test_stochastic_fp64_per_bin(tracer, medium, room_fn, rays, cut):
+    room = room_fn()
+    st, g, cnt, ora = run_both(tracer, medium, room, rays=rays, cut=cut, seed=5,
+                               mode=A.MODE_STOCHASTIC, precision=A.PREC_FP64)
+    assert st["bounces"] == bounces(cnt)
+    assert st["octave_rays"] == cnt["octave_rays"]
+    assert st["deposits"] == cnt["deposits"]
+    for k in range(A.NPLANES):
+        assert_bins(g[k], ora[k], 1e-5, f"plane {k}")
+    assert abs(st["lost"] - cnt["lost"]) <= 1e-9 * max(cnt["e_init"], 1)
+
+
+def decay_params(E, fs=1000):
+    """T30, EDT, C80 per (receiver, band) from an energy histogram [R, 8, n]."""
+    out = np.zeros(E.shape[:2] + (3,))
+    for r in range(E.shape[0]):
+        for b in range(E.shape[1]):
+            sch = A.schroeder_integral(E[r, b])
+            out[r, b] = (A.t_x(sch, 30, fs), A.early_decay_time(sch, fs), A.clarity(E[r, b], fs))
+    return out
+
+
+def test_fp32_matches_fp64_statistics(tracer, medium):
+    room = rooms.shoebox()
+    kw = dict(rays=1000, cutoff_s=1.0, seed=9, medium=medium)
+    A.trace_room(tracer, room, precision=A.PREC_FP64, **kw)
+    E64 = tracer.histogram(0)
+    s32 = A.trace_room(tracer, room, precision=A.PREC_FP32, **kw)
+    E32 = tracer.histogram(0)
+    p64, p32 = decay_params(E64), decay_params(E32)
+    rel = np.abs(p32[..., :2] - p64[..., :2]) / p64[..., :2]
+    assert rel.max() < 0.02, rel.max()
+    assert np.abs(p32[..., 2] - p64[..., 2]).max() < 0.2
+    assert s32["lost"] / s32["e_init"] < 1e-5
