@@ -49,11 +49,26 @@ def main():
     ap.add_argument("--cutoff", type=float, default=3.0)
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--grid", type=int, default=32)
-    ap.add_argument("--cpu-rays", type=int, default=400, help="source rays of the CPU sample")
+    ap.add_argument("--cpu-rays", type=int, default=2000, help="source rays of the CPU sample")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="pach_tune setting, e.g. dist_lds=0")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
+    if args.calib_copy:
+        import torch
+
+        n = args.calib_copy * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calib_copy_bytes": n * 4}))
+        return
     if args.gpus != 1:
         print("bench.py: this build traces on one GPU only; the multi-GPU split (disjoint "
               "source-ray ranges plus one histogram all-reduce) is not implemented", file=sys.stderr)
@@ -66,6 +81,9 @@ def main():
     at, c, rho = A.uniform_medium()
     prec = A.PREC_FP64 if args.precision == "fp64" else A.PREC_FP32
     tr = A.Tracer(0)
+    for kv in args.tune:
+        k, v = kv.split("=")
+        tr.tune(k, int(v))
     sr = 1000
     sct = int(np.floor(args.cutoff * 1000 * sr / 1000))
     tr.upload_scene(room.verts, room.surf, room.alpha, room.scat, room.trans, (args.grid,) * 3)
@@ -83,6 +101,7 @@ def main():
     t0 = time.perf_counter()
     for k in range(args.steps):
         st = step(args.seed + k)
+        last = st
         tot["bounces"] += st["bounces"]
         tot["oct"] += st["bounces_late"] + st["bounces_verylate"]
         tot["broad"] += st["bounces_broad"]
@@ -123,6 +142,8 @@ def main():
                                f"{args.rays} rays, {args.cutoff} s cut-off, {args.grid}^3 grid",
                    "rays": args.rays, "triangles": room.n_tris, "receivers": 16,
                    "bounces_per_step": tot["bounces"] / args.steps,
+                   "walk": {k: last[k] for k in ("loop_steps", "voxels", "tri_tests", "shade_passes",
+                                                 "lanes_shaded", "deposits", "spills")},
                    "rays_per_s": args.rays * args.steps / dt},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,

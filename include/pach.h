@@ -110,6 +110,11 @@ typedef struct pach_stats {
     int64_t launches_phase1;
     int64_t launches_octave;
     double ms_trace;           /* wall time of the last trace (host clock) */
+    int64_t loop_steps;        /* persistent-loop steps of the octave-ray kernel (per wave) */
+    int64_t voxels;            /* voxels visited by octave-ray walks */
+    int64_t tri_tests;         /* triangle tests of octave-ray walks */
+    int64_t shade_passes;      /* shading passes (per wave) */
+    int64_t lanes_shaded;      /* octave-ray bounces shaded (sum over the passes' lanes) */
 } pach_stats;
 
 /* Context lifetime.  device: HIP device ordinal. */
@@ -133,6 +138,16 @@ int pach_scene_upload(pach_ctx *ctx, const double *verts, int T, const int *surf
                       int S, const double *alpha, const double *scat, const double *trans,
                       int grid_x, int grid_y, int grid_z);
 
+/* The uniform voxel grid pach_scene_upload builds, computed on the host (no GPU
+ * needed; for inspection and tests).  dims[3]: requested voxel counts (any
+ * <= 0: automatic), overwritten with the counts used; origin[3], cell[3]: grid
+ * corner and voxel size; ranges [ncell][2]: (first index, count) into tris;
+ * dist [ncell]: L-inf distance in voxels to the nearest occupied voxel (0 =
+ * occupied).  *n_list: capacity of tris on entry, list length on return; tris
+ * is filled only when large enough, so call once with tris = NULL to size it. */
+int pach_grid_build(const double *verts, int T, int dims[3], double origin[3], double cell[3],
+                    uint32_t *ranges, uint32_t *tris, int64_t *n_list, uint8_t *dist);
+
 /* Source origin, per-band power in W (Source.SoundPower, Classes_Sources.cs:96-104)
  * and direction mode.  dirs [n_dirs][3] only for PACH_DIR_SUPPLIED. */
 int pach_source_set(pach_ctx *ctx, const double origin[3], const double power_w[8],
@@ -145,6 +160,15 @@ int pach_source_set(pach_ctx *ctx, const double origin[3], const double power_w[
 int pach_receivers_set(pach_ctx *ctx, const double *centers, int R, double radius,
                        const double atten_db_per_m[8], double c, double rho_c,
                        int sample_rate, int sample_ct);
+
+/* Launch tuning of later traces (no counterpart in the reference; results do
+ * not depend on it).  Keys: "dist_lds" (0/1): stage the voxel distance field
+ * in LDS when it fits; "receiver_masks" (0/1): test only receivers whose
+ * voxels the walk crossed (up to 64 receivers); "walk_steps" (1-64): voxels a
+ * walk advances per step of the persistent loop (with "walk_flat" = 1: voxel
+ * steps plus triangle tests, 1-256); "shade_min" (1-64): lanes that must wait
+ * before a wave shades; "walk_flat" (0/1): flattened walk loop. */
+int pach_tune(pach_ctx *ctx, const char *key, int64_t value);
 
 /* SplitRayTracer(...).Begin() (TaskAll:68-240).  Clears the histograms first. */
 int pach_trace(pach_ctx *ctx, const pach_params *params);

@@ -507,6 +507,17 @@ static void check_rays(const rx_ctx *x, const double *o, const double *d, const 
     }
 }
 
+void or_receiver_check(const or_setup *su, double **planes, const double o[3], const double d[3],
+                       const double end[3], double I, int band, double t_sum, int ana,
+                       int64_t *deposits)
+{
+    or_counters cnt;
+    memset(&cnt, 0, sizeof(cnt));
+    rx_ctx x = {su, planes, 1.0 / su->c, 1.0 / (M_PI * su->radius * su->radius * su->radius), &cnt};
+    check_rays(&x, o, d, end, &I, band, 0, 0, t_sum, ana);
+    if (deposits) *deposits = cnt.deposits;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Tracer (TaskAll:335-536)                                                  */
 /* ------------------------------------------------------------------------ */
@@ -577,6 +588,13 @@ static void diffuse_dir(const double *N, double u_az, double u_zen, int very_lat
     for (int k = 0; k < 3; ++k) out[k] = dx[k] * x + dy[k] * y + dz[k] * z;
     normalize3(out);
 }
+
+void or_sample_diffuse(const double N[3], double u_az, double u_zen, double out[3])
+{
+    diffuse_dir(N, u_az, u_zen, 0, out);
+}
+
+void or_sincos_turn(double u, double *s, double *c) { sincos_turn(u, s, c); }
 
 static void specular(double *d, const double *N, double cos_t)
 {
@@ -680,7 +698,7 @@ static void trace_octave(const or_scene *s, const or_setup *su, const or_params 
     const int stoch = (p->mode == OR_MODE_STOCHASTIC);
     double hit[3], leg = 0;
     for (;;) {
-        if ((int64_t)r.k >= p->max_bounces) { cnt->guard_hits++; return; }
+        if ((int64_t)r.k >= p->max_bounces) { cnt->guard_hits++; cnt->remaining += r.I; return; }
         int ht = shoot(s, p, r.o, r.d, hit, &leg);
         if (r.phase == 0) cnt->bounces_late++;
         else cnt->bounces_verylate++;
@@ -699,7 +717,9 @@ static void trace_octave(const or_scene *s, const or_setup *su, const or_params 
         int sf = s->surf[ht];
         /* Basic_Material.Absorb (Classes_Materials.cs:160-164) */
         double cos_t = dot3(N, r.d);
+        double I0 = r.I;
         r.I *= (1 - s->alpha[8 * sf + r.band]);
+        cnt->absorbed += I0 - r.I;
         uint32_t kev = r.k++;
         if (stoch) {
             uint32_t w[4];
@@ -713,7 +733,7 @@ static void trace_octave(const or_scene *s, const or_setup *su, const or_params 
         if (r.phase == 0) {
             if (!(r.t < th->COTime && r.I > th->T2[r.band])) r.phase = 1; /* hand-off to End_Trace */
         } else {
-            if (!(r.t < th->COTime && r.I > th->T3[r.band])) return;
+            if (!(r.t < th->COTime && r.I > th->T3[r.band])) { cnt->remaining += r.I; return; }
         }
     }
 }
@@ -764,7 +784,11 @@ static void trace_source_ray(const or_scene *s, const or_setup *su, const or_par
         int sf = s->surf[ht];
         /* Basic_Material.Absorb for BroadRay (Classes_Materials.cs:166-178) */
         double cos_t = dot3(d, N);
-        for (int o = 0; o < 8; ++o) E[o] *= (1 - s->alpha[8 * sf + o]);
+        for (int o = 0; o < 8; ++o) {
+            double e0 = E[o];
+            E[o] *= (1 - s->alpha[8 * sf + o]);
+            cnt->absorbed += e0 - E[o];
+        }
         if (stoch) {
             /* Lambert_Scattering.Scatter_Early (Classes_Materials.cs:4498-4570) */
             double Nf[3] = {N[0], N[1], N[2]}, cf = cos_t;
@@ -814,6 +838,8 @@ static void trace_source_ray(const or_scene *s, const or_setup *su, const or_par
         if ((int)order >= p->p1_max_bounces) { cnt->p1_cap_hits++; break; }
     }
     if (split) {
+        for (int oc = 0; oc < 8; ++oc)
+            if (oc < p->oct_lo || oc > p->oct_hi) cnt->remaining += E[oc];
         /* BroadRay.SplitRay(o) per octave (TaskAll:419-423; Classes_Rays.cs:331-337) */
         for (int oc = p->oct_lo; oc <= p->oct_hi; ++oc) {
             oray R;
@@ -899,6 +925,8 @@ int or_trace_mt(const or_scene *s, const or_setup *su, const or_params *p,
         cnt->guard_hits += c->guard_hits;
         cnt->lost += c->lost;
         cnt->e_init += c->e_init;
+        cnt->absorbed += c->absorbed;
+        cnt->remaining += c->remaining;
         cnt->h_oct = c->h_oct;
     }
     free(jobs);

@@ -110,6 +110,8 @@ typedef struct {
     int64_t p1_cap_hits, guard_hits;
     double lost, e_init;
     int h_oct;
+    double absorbed;   /* energy removed by Absorb */
+    double remaining;  /* energy of rays stopped by time, thresholds or guards */
 } or_counters;
 
 /* planes: OR_NPLANES pointers (NULL to skip), each [R][8][sample_ct], accumulated (+=). */
@@ -119,6 +121,13 @@ int or_trace(const or_scene *s, const or_setup *su, const or_params *p,
 int or_trace_mt(const or_scene *s, const or_setup *su, const or_params *p,
                 double **planes, or_counters *cnt, int n_threads);
 int or_h_oct(const or_scene *s, const double power[8], int oct_lo, int oct_hi);
+/* one Spherical_Receiver.CheckRay call for every receiver (octave ray) */
+void or_receiver_check(const or_setup *su, double **planes, const double o[3], const double d[3],
+                       const double end[3], double I, int band, double t_sum, int ana,
+                       int64_t *deposits);
+/* Lambert diffuse direction about normal N for uniforms (u_az, u_zen) */
+void or_sample_diffuse(const double N[3], double u_az, double u_zen, double out[3]);
+void or_sincos_turn(double u, double *s, double *c);
 
 /* --- decay-curve integration (Classes_Math.cs) ---------------------------- */
 void or_schroeder(const double *etc, int n, double *out);

@@ -43,7 +43,8 @@ class Counters(C.Structure):
                 ("bounces_verylate", C.c_int64), ("spawned", C.c_int64),
                 ("misses", C.c_int64), ("deposits", C.c_int64), ("octave_rays", C.c_int64),
                 ("p1_cap_hits", C.c_int64), ("guard_hits", C.c_int64), ("lost", C.c_double),
-                ("e_init", C.c_double), ("h_oct", C.c_int)]
+                ("e_init", C.c_double), ("h_oct", C.c_int), ("absorbed", C.c_double),
+                ("remaining", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -97,6 +98,11 @@ def lib():
         L.or_edt.argtypes = [_dp, C.c_int, C.c_int]
         L.or_clarity.restype = C.c_double
         L.or_clarity.argtypes = [_dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int]
+        L.or_receiver_check.argtypes = [C.POINTER(Setup), C.POINTER(_dp), _dp, _dp, _dp,
+                                        C.c_double, C.c_int, C.c_double, C.c_int,
+                                        C.POINTER(C.c_int64)]
+        L.or_sample_diffuse.argtypes = [_dp, C.c_double, C.c_double, _dp]
+        L.or_sincos_turn.argtypes = [C.c_double, _dp, _dp]
         L.or_sabine.argtypes = [C.c_void_p, C.c_double, _dp, _dp]
         L.or_eyring.argtypes = [C.c_void_p, C.c_double, _dp, _dp]
         _lib = L
@@ -297,3 +303,46 @@ def log10data(x, limit_db=-60.0):
     out = np.zeros_like(a)
     lib().or_log10data(_p(a), a.size, limit_db, _p(out))
     return out
+
+
+def make_setup(receivers, radius, medium=None, sample_rate=1000, sample_ct=2000, src=(0, 0, 0),
+               power=None):
+    """Setup record for single-call helpers; keeps its arrays alive on the object."""
+    med = medium or standard_medium()
+    su = Setup()
+    su.src[:] = list(map(float, src))
+    su.power[:] = list(np.ones(8) if power is None else power)
+    su._centers = np.ascontiguousarray(receivers, dtype=np.float64).reshape(-1, 3)
+    su.R = su._centers.shape[0]
+    su.centers = _p(su._centers)
+    su.radius = radius
+    su.c = med.c
+    su.atten[:] = list(med.atten)
+    su.rho_c = med.rho_c
+    su.sample_rate = sample_rate
+    su.sample_ct = sample_ct
+    return su
+
+
+def receiver_check(su, o, d, end, I, band, t_sum=0.0, ana=0):
+    """One CheckRay over every receiver; returns (planes[11, R, 8, n], deposits)."""
+    out = np.zeros((NPLANES, su.R, 8, su.sample_ct))
+    ptrs = (_dp * NPLANES)(*[_p(out[k]) for k in range(NPLANES)])
+    o, d, end = (np.ascontiguousarray(v, dtype=np.float64) for v in (o, d, end))
+    dep = C.c_int64()
+    lib().or_receiver_check(C.byref(su), ptrs, _p(o), _p(d), _p(end), I, band, t_sum, ana,
+                            C.byref(dep))
+    return out, dep.value
+
+
+def sample_diffuse(N, u_az, u_zen):
+    n = np.ascontiguousarray(N, dtype=np.float64)
+    out = np.zeros(3)
+    lib().or_sample_diffuse(_p(n), u_az, u_zen, _p(out))
+    return out
+
+
+def sincos_turn(u):
+    s, c = C.c_double(), C.c_double()
+    lib().or_sincos_turn(u, C.byref(s), C.byref(c))
+    return s.value, c.value

@@ -46,7 +46,9 @@ class Stats(C.Structure):
                 ("resumes", C.c_int64), ("lost", C.c_double), ("e_init", C.c_double),
                 ("h_oct", C.c_int), ("rays_done", C.c_int64), ("ms_phase1", C.c_double),
                 ("ms_octave", C.c_double), ("launches_phase1", C.c_int64),
-                ("launches_octave", C.c_int64), ("ms_trace", C.c_double)]
+                ("launches_octave", C.c_int64), ("ms_trace", C.c_double),
+                ("loop_steps", C.c_int64), ("voxels", C.c_int64), ("tri_tests", C.c_int64),
+                ("shade_passes", C.c_int64), ("lanes_shaded", C.c_int64)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}
@@ -64,9 +66,12 @@ SIGNATURES = [
                                       _dp, _dp]),
     ("pach_scene_upload", C.c_int, [C.c_void_p, _dp, C.c_int, _ip, C.c_int, _dp, _dp, _dp,
                                     C.c_int, C.c_int, C.c_int]),
+    ("pach_grid_build", C.c_int, [_dp, C.c_int, C.POINTER(C.c_int), _dp, _dp, C.POINTER(C.c_uint32),
+                                  C.POINTER(C.c_uint32), C.POINTER(C.c_int64), C.POINTER(C.c_uint8)]),
     ("pach_source_set", C.c_int, [C.c_void_p, _dp, _dp, C.c_int, _dp, C.c_int64]),
     ("pach_receivers_set", C.c_int, [C.c_void_p, _dp, C.c_int, C.c_double, _dp, C.c_double,
                                      C.c_double, C.c_int, C.c_int]),
+    ("pach_tune", C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
     ("pach_trace", C.c_int, [C.c_void_p, C.POINTER(Params)]),
     ("pach_poll", C.c_int, [C.c_void_p, C.POINTER(Progress)]),
     ("pach_wait", C.c_int, [C.c_void_p]),
@@ -222,6 +227,9 @@ class Tracer:
             return rc
         self._chk(rc)
         return rc
+
+    def tune(self, key, value):
+        self._chk(lib().pach_tune(self.h, key.encode(), int(value)))
 
     def poll(self):
         pr = Progress()

@@ -32,6 +32,7 @@ HIP_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", 
 UNITS = [
     # (source, extra flags, is_hip)
     ("runtime.cpp", [], False),
+    ("grid.cpp", [], False),
     ("trace_f32.hip", [], True),
     ("trace_f64.hip", ["-ffp-contract=off"], True),
 ]

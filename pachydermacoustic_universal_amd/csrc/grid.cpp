@@ -1,0 +1,211 @@
+// grid.cpp -- host-side uniform voxel grid of a triangulated scene.
+//
+// Stands in for Polygon_Scene.partition()'s Hare Voxel_Grid (Classes_Scenes.cs:
+// 709-755): per-voxel triangle lists (exact triangle/box overlap), plus an L-inf
+// distance field that lets the GPU walk skip empty space.
+#include "grid.h"
+
+#include <algorithm>
+#include <cmath>
+#include <deque>
+
+namespace pach {
+
+// Triangle / axis-aligned box overlap by the separating-axis theorem
+// (Akenine-Moller 2001).  bc: box centre, bh: half extents, tv: 9 coordinates.
+bool tri_box_overlap(const double *bc, const double *bh, const double *tv)
+{
+    double v[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) v[i][k] = tv[3 * i + k] - bc[k];
+    const double e[3][3] = {{v[1][0] - v[0][0], v[1][1] - v[0][1], v[1][2] - v[0][2]},
+                            {v[2][0] - v[1][0], v[2][1] - v[1][1], v[2][2] - v[1][2]},
+                            {v[0][0] - v[2][0], v[0][1] - v[2][1], v[0][2] - v[2][2]}};
+    // nine axes: box axis x triangle edge
+    for (int i = 0; i < 3; ++i) {
+        for (int a = 0; a < 3; ++a) {
+            double u[3] = {0, 0, 0};
+            u[a] = 1;
+            const double ax[3] = {u[1] * e[i][2] - u[2] * e[i][1], u[2] * e[i][0] - u[0] * e[i][2],
+                                  u[0] * e[i][1] - u[1] * e[i][0]};
+            double mn = 1e300, mx = -1e300;
+            for (int j = 0; j < 3; ++j) {
+                const double p = v[j][0] * ax[0] + v[j][1] * ax[1] + v[j][2] * ax[2];
+                mn = std::min(mn, p);
+                mx = std::max(mx, p);
+            }
+            const double rad = bh[0] * std::fabs(ax[0]) + bh[1] * std::fabs(ax[1]) + bh[2] * std::fabs(ax[2]);
+            if (mn > rad || mx < -rad) return false;
+        }
+    }
+    // three box face normals
+    for (int k = 0; k < 3; ++k) {
+        const double mn = std::min({v[0][k], v[1][k], v[2][k]}), mx = std::max({v[0][k], v[1][k], v[2][k]});
+        if (mn > bh[k] || mx < -bh[k]) return false;
+    }
+    // triangle plane
+    const double n[3] = {e[0][1] * e[1][2] - e[0][2] * e[1][1], e[0][2] * e[1][0] - e[0][0] * e[1][2],
+                         e[0][0] * e[1][1] - e[0][1] * e[1][0]};
+    const double d = -(n[0] * v[0][0] + n[1] * v[0][1] + n[2] * v[0][2]);
+    const double r = bh[0] * std::fabs(n[0]) + bh[1] * std::fabs(n[1]) + bh[2] * std::fabs(n[2]);
+    return std::fabs(d) <= r;
+}
+
+// L-inf distance transform: breadth-first search from the voxels marked 0
+// over the 26-neighbourhood; every other voxel gets its distance, capped at 255.
+void linf_distance(const int G[3], std::vector<uint8_t> &dist)
+{
+    const size_t ncell = (size_t)G[0] * G[1] * G[2];
+    std::deque<uint32_t> q;
+    for (size_t i = 0; i < ncell; ++i) {
+        if (dist[i] == 0) q.push_back((uint32_t)i);
+        else dist[i] = 255;
+    }
+    while (!q.empty()) {
+        const uint32_t i = q.front();
+        q.pop_front();
+        const int x = (int)(i % G[0]), y = (int)((i / G[0]) % G[1]), z = (int)(i / ((size_t)G[0] * G[1]));
+        const int nd = dist[i] + 1;
+        if (nd > 255) continue;
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int X = x + dx, Y = y + dy, Z = z + dz;
+                    if (X < 0 || Y < 0 || Z < 0 || X >= G[0] || Y >= G[1] || Z >= G[2]) continue;
+                    const size_t jn = ((size_t)Z * G[1] + Y) * G[0] + X;
+                    if (dist[jn] > nd) {
+                        dist[jn] = (uint8_t)nd;
+                        q.push_back((uint32_t)jn);
+                    }
+                }
+    }
+}
+
+void receiver_cells(const Grid &g, const double *centers, int R, double radius, std::vector<uint64_t> &mask,
+                    std::vector<uint8_t> &dist)
+{
+    // voxels whose box comes within radius (+ 1e-3 voxel) of a centre: the walk
+    // collects their receiver bits, and they count as occupied for the jumps
+    const size_t ncell = (size_t)g.dims[0] * g.dims[1] * g.dims[2];
+    mask.assign(ncell, 0);
+    dist.assign(g.dist.begin(), g.dist.end());
+    for (size_t i = 0; i < ncell; ++i)
+        if (g.ranges[2 * i + 1] == 0) dist[i] = 255;
+    const double pad = 1e-3 * std::max({g.cell[0], g.cell[1], g.cell[2]});
+    const double rr = radius + pad;
+    for (int r = 0; r < R && r < 64; ++r) {
+        const double *C = centers + 3 * r;
+        int lo[3], hi[3];
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::max(0, (int)std::floor((C[k] - rr - g.origin[k]) / g.cell[k]));
+            hi[k] = std::min(g.dims[k] - 1, (int)std::floor((C[k] + rr - g.origin[k]) / g.cell[k]));
+        }
+        for (int z = lo[2]; z <= hi[2]; ++z)
+            for (int y = lo[1]; y <= hi[1]; ++y)
+                for (int x = lo[0]; x <= hi[0]; ++x) {
+                    const int idx[3] = {x, y, z};
+                    double d2 = 0;
+                    for (int k = 0; k < 3; ++k) {
+                        const double b0 = g.origin[k] + idx[k] * g.cell[k], b1 = b0 + g.cell[k];
+                        const double q = C[k] < b0 ? b0 - C[k] : (C[k] > b1 ? C[k] - b1 : 0.0);
+                        d2 += q * q;
+                    }
+                    if (d2 <= rr * rr) {
+                        const size_t i = ((size_t)z * g.dims[1] + y) * g.dims[0] + x;
+                        mask[i] |= 1ull << r;
+                        dist[i] = 0;
+                    }
+                }
+    }
+    linf_distance(g.dims, dist);
+}
+
+int build_grid(const double *verts, int T, const int dims_in[3], Grid &g, std::string &err)
+{
+    if (!verts || T <= 0) { err = "grid: empty scene"; return -1; }
+    double bmin[3] = {1e300, 1e300, 1e300}, bmax[3] = {-1e300, -1e300, -1e300};
+    for (int t = 0; t < T; ++t)
+        for (int j = 0; j < 3; ++j)
+            for (int k = 0; k < 3; ++k) {
+                const double x = verts[9 * t + 3 * j + k];
+                if (!std::isfinite(x)) { err = "grid: non-finite vertex"; return -1; }
+                bmin[k] = std::min(bmin[k], x);
+                bmax[k] = std::max(bmax[k], x);
+            }
+    double ext[3];
+    for (int k = 0; k < 3; ++k) ext[k] = std::max(bmax[k] - bmin[k], 1e-6);
+    int G[3] = {dims_in ? dims_in[0] : 0, dims_in ? dims_in[1] : 0, dims_in ? dims_in[2] : 0};
+    if (G[0] <= 0 || G[1] <= 0 || G[2] <= 0) {
+        // about 8 voxels per triangle, shaped like the bounding box
+        const double target = std::min(std::max(8.0 * T, 4096.0), 2.0e6);
+        const double unit = std::cbrt(ext[0] * ext[1] * ext[2] / target);
+        for (int k = 0; k < 3; ++k) G[k] = std::max(1, std::min(256, (int)std::lround(ext[k] / unit)));
+    }
+    for (int k = 0; k < 3; ++k) {
+        if (G[k] > 1024) { err = "grid: dimension above 1024"; return -1; }
+        const double pad = 1e-4 * std::max(ext[k], 1.0);
+        g.dims[k] = G[k];
+        g.origin[k] = bmin[k] - pad;
+        g.cell[k] = (ext[k] + 2 * pad) / G[k];
+    }
+    const size_t ncell = (size_t)G[0] * G[1] * G[2];
+    if (ncell > (size_t)1 << 27) { err = "grid: too many voxels"; return -1; }
+    std::vector<std::vector<uint32_t>> lists(ncell);
+    for (int t = 0; t < T; ++t) {
+        const double *p = verts + 9 * t;
+        int lo[3], hi[3];
+        for (int k = 0; k < 3; ++k) {
+            const double mn = std::min({p[k], p[3 + k], p[6 + k]}), mx = std::max({p[k], p[3 + k], p[6 + k]});
+            const double e = 1e-3 * g.cell[k];
+            lo[k] = std::max(0, (int)std::floor((mn - e - g.origin[k]) / g.cell[k]));
+            hi[k] = std::min(G[k] - 1, (int)std::floor((mx + e - g.origin[k]) / g.cell[k]));
+        }
+        // boxes are inflated by 0.1 % so a triangle grazing a voxel face is listed
+        // in both voxels and no FP32 walk can slip between them
+        const double bh[3] = {0.5 * g.cell[0] * (1 + 2e-3), 0.5 * g.cell[1] * (1 + 2e-3), 0.5 * g.cell[2] * (1 + 2e-3)};
+        for (int z = lo[2]; z <= hi[2]; ++z)
+            for (int y = lo[1]; y <= hi[1]; ++y)
+                for (int x = lo[0]; x <= hi[0]; ++x) {
+                    const double bc[3] = {g.origin[0] + (x + 0.5) * g.cell[0], g.origin[1] + (y + 0.5) * g.cell[1],
+                                          g.origin[2] + (z + 0.5) * g.cell[2]};
+                    if (tri_box_overlap(bc, bh, p)) lists[((size_t)z * G[1] + y) * G[0] + x].push_back((uint32_t)t);
+                }
+    }
+    g.ranges.assign(2 * ncell, 0);
+    g.tris.clear();
+    for (size_t i = 0; i < ncell; ++i) {
+        if (lists[i].size() > 0xFFFFFFFFu) { err = "grid: voxel list too long"; return -1; }
+        g.ranges[2 * i] = (uint32_t)g.tris.size();
+        g.ranges[2 * i + 1] = (uint32_t)lists[i].size();
+        g.tris.insert(g.tris.end(), lists[i].begin(), lists[i].end());
+    }
+    g.dist.assign(ncell, 255);
+    for (size_t i = 0; i < ncell; ++i)
+        if (g.ranges[2 * i + 1] > 0) g.dist[i] = 0;
+    linf_distance(g.dims, g.dist);
+    // a grid with no occupied voxel keeps 255 everywhere: every walk is one jump
+    g.dist.resize((ncell + 15) & ~(size_t)15, 255);
+    return 0;
+}
+
+}  // namespace pach
+
+extern "C" int pach_grid_build(const double *verts, int T, int dims[3], double origin[3], double cell[3],
+                               uint32_t *ranges, uint32_t *tris, int64_t *n_list, uint8_t *dist)
+{
+    if (!verts || !dims || !n_list) return -1;
+    pach::Grid g;
+    std::string err;
+    if (pach::build_grid(verts, T, dims, g, err) != 0) return -1;
+    for (int k = 0; k < 3; ++k) {
+        dims[k] = g.dims[k];
+        if (origin) origin[k] = g.origin[k];
+        if (cell) cell[k] = g.cell[k];
+    }
+    const size_t ncell = (size_t)g.dims[0] * g.dims[1] * g.dims[2];
+    if (ranges) std::copy(g.ranges.begin(), g.ranges.end(), ranges);
+    if (tris && *n_list >= (int64_t)g.tris.size()) std::copy(g.tris.begin(), g.tris.end(), tris);
+    if (dist) std::copy(g.dist.begin(), g.dist.begin() + ncell, dist);
+    *n_list = (int64_t)g.tris.size();
+    return 0;
+}

@@ -1,0 +1,30 @@
+// grid.h -- host-side uniform voxel grid (no HIP dependency).
+#pragma once
+
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+namespace pach {
+
+struct Grid {
+    int dims[3] = {0, 0, 0};
+    double origin[3] = {0, 0, 0};
+    double cell[3] = {1, 1, 1};
+    std::vector<uint32_t> ranges;  // [ncell][2]: first list index, triangle count
+    std::vector<uint32_t> tris;    // concatenated voxel lists
+    std::vector<uint8_t> dist;     // [ncell rounded up to 16]: L-inf distance to an occupied voxel
+};
+
+bool tri_box_overlap(const double *bc, const double *bh, const double *tv);
+// L-inf distance (voxels, capped at 255) to the nearest voxel whose dist is 0
+void linf_distance(const int dims[3], std::vector<uint8_t> &dist);
+// per-voxel bit masks of the (first 64) receiver spheres touching each voxel, and
+// the grid's distance field with those voxels counted as occupied
+void receiver_cells(const Grid &g, const double *centers, int R, double radius, std::vector<uint64_t> &mask,
+                    std::vector<uint8_t> &dist);
+// dims_in: requested voxel counts (any <= 0: chosen from the triangle count)
+int build_grid(const double *verts, int T, const int dims_in[3], Grid &g, std::string &err);
+
+}  // namespace pach

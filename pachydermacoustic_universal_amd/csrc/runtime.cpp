@@ -6,6 +6,7 @@
 // (k_octave) together with the rays the previous launch handed over, until
 // every ray of every batch has terminated.
 #include "../../include/pach.h"
+#include "grid.h"
 #include "trace_types.h"
 
 #include <algorithm>
@@ -58,7 +59,14 @@ struct pach_ctx {
     int g[3] = {0, 0, 0};
     double gmin[3] = {0, 0, 0}, cs[3] = {1, 1, 1};
     int *d_surf = nullptr;
-    uint32_t *d_cell_start = nullptr, *d_cell_tris = nullptr;
+    uint32_t *d_cell_range = nullptr, *d_cell_tris = nullptr;
+    uint8_t *d_cell_dist = nullptr;
+    uint64_t *d_cell_rmask = nullptr;
+    int ncell = 0;
+    pach::Grid grid;        // host copy (receiver voxels are marked at trace time)
+    bool rmask_dirty = true;
+    bool rmask_on = false;
+    std::vector<double> centers;
     SceneBuf<float> s32;
     SceneBuf<double> s64;
 
@@ -83,9 +91,17 @@ struct pach_ctx {
     // counters and small device scalars
     DevCounters *d_cnt = nullptr;
     unsigned *d_scal = nullptr;  // [0] p1_count [1] res_count [2] in_head [3..4] dump_count[2]
+    BandTab *d_band = nullptr;
 
     WorkBuf<float> w32;
     WorkBuf<double> w64;
+
+    // tuning (pach_tune)
+    int tune_dist_lds = 1;     // voxel distances staged in LDS when they fit
+    int tune_rmask = 1;        // per-voxel receiver masks (R <= 64)
+    int tune_walk_steps = 12;  // walk units (tests + voxel steps) per loop step
+    int tune_shade_min = 16;   // lanes waiting before shading
+    int tune_walk_flat = 1;    // flattened walk
 
     // trace state
     std::thread worker;
@@ -217,7 +233,7 @@ extern "C" int pach_create(pach_ctx **out, int device)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->n_cu = prop.multiProcessorCount;
     if (c->n_cu <= 0) c->n_cu = 256;
-    if (dalloc(c->d_cnt, 1) != hipSuccess || dalloc(c->d_scal, 8) != hipSuccess || dalloc(c->d_geo, 80 * 9) != hipSuccess) {
+    if (dalloc(c->d_cnt, 1) != hipSuccess || dalloc(c->d_scal, 8) != hipSuccess || dalloc(c->d_band, 1) != hipSuccess || dalloc(c->d_geo, 80 * 9) != hipSuccess) {
         pach_destroy(c);
         return PACH_ENOMEM;
     }
@@ -272,10 +288,11 @@ extern "C" void pach_destroy(pach_ctx *c)
     free_scene(c->s64);
     free_work(c->w32);
     free_work(c->w64);
-    dfree(c->d_surf); dfree(c->d_cell_start); dfree(c->d_cell_tris);
+    dfree(c->d_surf); dfree(c->d_cell_range); dfree(c->d_cell_tris); dfree(c->d_cell_dist);
+    dfree(c->d_cell_rmask);
     dfree(c->d_dirs); dfree(c->d_geo);
     dfree(c->d_ctr32); dfree(c->d_ctr64); dfree(c->d_hist);
-    dfree(c->d_cnt); dfree(c->d_scal);
+    dfree(c->d_cnt); dfree(c->d_scal); dfree(c->d_band);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -286,50 +303,6 @@ extern "C" const char *pach_last_error(const pach_ctx *c) { return c ? c->err.c_
 
 // ------------------------------------------------------------------ scene + grid
 namespace {
-
-// Triangle / axis-aligned box overlap by the separating-axis theorem
-// (Akenine-Moller 2001); the box is inflated by `pad` so the FP32 walk never
-// misses a triangle that grazes a voxel face.
-bool tri_box_overlap(const double *bc, const double *bh, const double *tv)
-{
-    double v[3][3];
-    for (int i = 0; i < 3; ++i)
-        for (int k = 0; k < 3; ++k) v[i][k] = tv[3 * i + k] - bc[k];
-    const double e[3][3] = {{v[1][0] - v[0][0], v[1][1] - v[0][1], v[1][2] - v[0][2]},
-                            {v[2][0] - v[1][0], v[2][1] - v[1][1], v[2][2] - v[1][2]},
-                            {v[0][0] - v[2][0], v[0][1] - v[2][1], v[0][2] - v[2][2]}};
-    // 9 edge cross axes
-    for (int i = 0; i < 3; ++i) {
-        for (int a = 0; a < 3; ++a) {
-            double ax[3] = {0, 0, 0};
-            double unit[3] = {0, 0, 0};
-            unit[a] = 1;
-            ax[0] = unit[1] * e[i][2] - unit[2] * e[i][1];
-            ax[1] = unit[2] * e[i][0] - unit[0] * e[i][2];
-            ax[2] = unit[0] * e[i][1] - unit[1] * e[i][0];
-            double mn = 1e300, mx = -1e300;
-            for (int j = 0; j < 3; ++j) {
-                const double p = v[j][0] * ax[0] + v[j][1] * ax[1] + v[j][2] * ax[2];
-                mn = std::min(mn, p);
-                mx = std::max(mx, p);
-            }
-            const double rad = bh[0] * std::fabs(ax[0]) + bh[1] * std::fabs(ax[1]) + bh[2] * std::fabs(ax[2]);
-            if (mn > rad || mx < -rad) return false;
-        }
-    }
-    // box face normals
-    for (int k = 0; k < 3; ++k) {
-        const double mn = std::min({v[0][k], v[1][k], v[2][k]}), mx = std::max({v[0][k], v[1][k], v[2][k]});
-        if (mn > bh[k] || mx < -bh[k]) return false;
-    }
-    // triangle plane
-    double n[3] = {e[0][1] * e[1][2] - e[0][2] * e[1][1], e[0][2] * e[1][0] - e[0][0] * e[1][2],
-                   e[0][0] * e[1][1] - e[0][1] * e[1][0]};
-    const double d = -(n[0] * v[0][0] + n[1] * v[0][1] + n[2] * v[0][2]);
-    const double r = bh[0] * std::fabs(n[0]) + bh[1] * std::fabs(n[1]) + bh[2] * std::fabs(n[2]);
-    if (std::fabs(d) > r) return false;
-    return true;
-}
 
 template <class R, class V>
 hipError_t upload_vec4(V *&dst, const std::vector<double> &src4)
@@ -385,8 +358,11 @@ extern "C" int pach_scene_upload(pach_ctx *c, const double *verts, int T, const 
         for (double t : c->trans) c->any_trans |= t > 0;
     }
     c->area.assign(S, 0.0);
-    std::vector<double> v0(4 * T), e1(4 * T), e2(4 * T), nr(4 * T);
-    double bmin[3] = {1e300, 1e300, 1e300}, bmax[3] = {-1e300, -1e300, -1e300};
+    // FP64 triangles: vertex 0 and edges for Moller-Trumbore.  FP32 triangles: the
+    // dual basis of (e1, e2, n): rows r0 = (e2 x n)/|n|^2, r1 = (n x e1)/|n|^2 give
+    // the barycentric u, v of a plane point x as r . (x - v0), row r2 = n/|n| its
+    // signed distance to the plane; each row stores -r . v0 in w.
+    std::vector<double> v0(4 * T), e1(4 * T), e2(4 * T), nr(4 * T), d0(4 * T), d1(4 * T), d2(4 * T);
     for (int t = 0; t < T; ++t) {
         const double *p = verts + 9 * t;
         double a[3], b[3];
@@ -396,58 +372,39 @@ extern "C" int pach_scene_upload(pach_ctx *c, const double *verts, int T, const 
             v0[4 * t + k] = p[k];
             a[k] = e1[4 * t + k] = p[3 + k] - p[k];
             b[k] = e2[4 * t + k] = p[6 + k] - p[k];
-            for (int j = 0; j < 3; ++j) {
-                bmin[k] = std::min(bmin[k], p[3 * j + k]);
-                bmax[k] = std::max(bmax[k], p[3 * j + k]);
-            }
         }
-        double n[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
-        const double l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        const double n[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+        const double nn = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+        const double l = std::sqrt(nn);
         c->area[surf[t]] += 0.5 * l;
         for (int k = 0; k < 3; ++k) nr[4 * t + k] = l > 0 ? n[k] / l : 0.0;
         v0[4 * t + 3] = e1[4 * t + 3] = e2[4 * t + 3] = nr[4 * t + 3] = 0;
-    }
-    // grid resolution: explicit, or ~8 cells per triangle shaped like the bbox
-    int G[3] = {gx, gy, gz};
-    double ext[3];
-    for (int k = 0; k < 3; ++k) ext[k] = std::max(bmax[k] - bmin[k], 1e-6);
-    if (G[0] <= 0 || G[1] <= 0 || G[2] <= 0) {
-        const double target = std::min(std::max(8.0 * T, 4096.0), 2.0e6);
-        const double unit = std::cbrt(ext[0] * ext[1] * ext[2] / target);
-        for (int k = 0; k < 3; ++k) G[k] = std::max(1, std::min(256, (int)std::lround(ext[k] / unit)));
-    }
-    for (int k = 0; k < 3; ++k) {
-        if (G[k] > 1024) return fail(c, PACH_EINVAL, "scene: grid dimension above 1024");
-        const double pad = 1e-4 * std::max(ext[k], 1.0);
-        c->g[k] = G[k];
-        c->gmin[k] = bmin[k] - pad;
-        c->cs[k] = (ext[k] + 2 * pad) / G[k];
-    }
-    const size_t ncell = (size_t)G[0] * G[1] * G[2];
-    std::vector<std::vector<uint32_t>> lists(ncell);
-    for (int t = 0; t < T; ++t) {
-        const double *p = verts + 9 * t;
-        int lo[3], hi[3];
+        const double bn[3] = {b[1] * n[2] - b[2] * n[1], b[2] * n[0] - b[0] * n[2], b[0] * n[1] - b[1] * n[0]};
+        const double na[3] = {n[1] * a[2] - n[2] * a[1], n[2] * a[0] - n[0] * a[2], n[0] * a[1] - n[1] * a[0]};
+        double w0 = 0, w1 = 0, w2 = 0;
         for (int k = 0; k < 3; ++k) {
-            const double mn = std::min({p[k], p[3 + k], p[6 + k]}), mx = std::max({p[k], p[3 + k], p[6 + k]});
-            const double e = 1e-3 * c->cs[k];
-            lo[k] = std::max(0, (int)std::floor((mn - e - c->gmin[k]) / c->cs[k]));
-            hi[k] = std::min(G[k] - 1, (int)std::floor((mx + e - c->gmin[k]) / c->cs[k]));
+            d0[4 * t + k] = nn > 0 ? bn[k] / nn : 0.0;
+            d1[4 * t + k] = nn > 0 ? na[k] / nn : 0.0;
+            d2[4 * t + k] = nr[4 * t + k];
+            w0 -= d0[4 * t + k] * p[k];
+            w1 -= d1[4 * t + k] * p[k];
+            w2 -= d2[4 * t + k] * p[k];
         }
-        for (int z = lo[2]; z <= hi[2]; ++z)
-            for (int y = lo[1]; y <= hi[1]; ++y)
-                for (int x = lo[0]; x <= hi[0]; ++x) {
-                    const double bc[3] = {c->gmin[0] + (x + 0.5) * c->cs[0], c->gmin[1] + (y + 0.5) * c->cs[1],
-                                          c->gmin[2] + (z + 0.5) * c->cs[2]};
-                    const double bh[3] = {0.5 * c->cs[0] * (1 + 2e-3), 0.5 * c->cs[1] * (1 + 2e-3),
-                                          0.5 * c->cs[2] * (1 + 2e-3)};
-                    if (tri_box_overlap(bc, bh, p)) lists[((size_t)z * G[1] + y) * G[0] + x].push_back((uint32_t)t);
-                }
+        d0[4 * t + 3] = w0;
+        d1[4 * t + 3] = w1;
+        d2[4 * t + 3] = w2;
     }
-    std::vector<uint32_t> start(ncell + 1, 0), tris;
-    for (size_t i = 0; i < ncell; ++i) start[i + 1] = start[i] + (uint32_t)lists[i].size();
-    tris.reserve(start[ncell]);
-    for (auto &l : lists) tris.insert(tris.end(), l.begin(), l.end());
+    pach::Grid grid;
+    std::string gerr;
+    const int dims[3] = {gx, gy, gz};
+    if (pach::build_grid(verts, T, dims, grid, gerr) != 0) return fail(c, PACH_EINVAL, gerr);
+    for (int k = 0; k < 3; ++k) {
+        c->g[k] = grid.dims[k];
+        c->gmin[k] = grid.origin[k];
+        c->cs[k] = grid.cell[k];
+    }
+    c->ncell = grid.dims[0] * grid.dims[1] * grid.dims[2];
+    c->rmask_dirty = true;
     // materials: (1 - alpha, s, lo, hi) per (polygon, band), Lambert_Scattering
     // split bounds (Classes_Materials.cs:4466-4484)
     std::vector<double> mat(4 * 8 * (size_t)S);
@@ -463,19 +420,22 @@ extern "C" int pach_scene_upload(pach_ctx *c, const double *verts, int T, const 
     }
     std::vector<double> surfd(surf, surf + T);
     hipError_t e;
-    if ((e = upload_vec4<float>(c->s32.v0, v0)) != hipSuccess || (e = upload_vec4<float>(c->s32.e1, e1)) != hipSuccess ||
-        (e = upload_vec4<float>(c->s32.e2, e2)) != hipSuccess || (e = upload_vec4<float>(c->s32.nrm, nr)) != hipSuccess ||
+    if ((e = upload_vec4<float>(c->s32.v0, v0)) != hipSuccess || (e = upload_vec4<float>(c->s32.e1, d0)) != hipSuccess ||
+        (e = upload_vec4<float>(c->s32.e2, d1)) != hipSuccess || (e = upload_vec4<float>(c->s32.nrm, d2)) != hipSuccess ||
         (e = upload_vec4<float>(c->s32.mat, mat)) != hipSuccess || (e = upload_scalars<float>(c->s32.trans, c->trans)) != hipSuccess ||
         (e = upload_vec4<double>(c->s64.v0, v0)) != hipSuccess || (e = upload_vec4<double>(c->s64.e1, e1)) != hipSuccess ||
         (e = upload_vec4<double>(c->s64.e2, e2)) != hipSuccess || (e = upload_vec4<double>(c->s64.nrm, nr)) != hipSuccess ||
         (e = upload_vec4<double>(c->s64.mat, mat)) != hipSuccess || (e = upload_scalars<double>(c->s64.trans, c->trans)) != hipSuccess)
         return fail(c, PACH_EHIP, std::string("scene upload: ") + hipGetErrorString(e));
     if ((e = dalloc(c->d_surf, T)) != hipSuccess || (e = hipMemcpy(c->d_surf, surf, T * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = dalloc(c->d_cell_start, start.size())) != hipSuccess ||
-        (e = hipMemcpy(c->d_cell_start, start.data(), start.size() * 4, hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = dalloc(c->d_cell_tris, tris.size())) != hipSuccess ||
-        (e = hipMemcpy(c->d_cell_tris, tris.data(), std::max<size_t>(tris.size(), 1) * 4, hipMemcpyHostToDevice)) != hipSuccess)
+        (e = dalloc(c->d_cell_range, grid.ranges.size())) != hipSuccess ||
+        (e = hipMemcpy(c->d_cell_range, grid.ranges.data(), grid.ranges.size() * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = dalloc(c->d_cell_tris, grid.tris.size())) != hipSuccess ||
+        (e = hipMemcpy(c->d_cell_tris, grid.tris.data(), std::max<size_t>(grid.tris.size(), 1) * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = dalloc(c->d_cell_dist, grid.dist.size())) != hipSuccess ||
+        (e = hipMemcpy(c->d_cell_dist, grid.dist.data(), grid.dist.size(), hipMemcpyHostToDevice)) != hipSuccess)
         return fail(c, PACH_EHIP, std::string("grid upload: ") + hipGetErrorString(e));
+    c->grid = std::move(grid);
     c->has_scene = true;
     return PACH_OK;
 }
@@ -532,6 +492,8 @@ extern "C" int pach_receivers_set(pach_ctx *c, const double *centers, int R, dou
     HIPCHK(c, dalloc(c->d_ctr64, R));
     HIPCHK(c, hipMemcpy(c->d_ctr32, c32.data(), R * sizeof(float4), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->d_ctr64, c64.data(), R * sizeof(double4), hipMemcpyHostToDevice));
+    c->centers.assign(centers, centers + 3 * (size_t)R);
+    c->rmask_dirty = true;
     c->hist_n = (size_t)R * 8 * sct;
     if (dalloc(c->d_hist, c->hist_n * kPlanes) != hipSuccess) return fail(c, PACH_ENOMEM, "histogram allocation");
     HIPCHK(c, hipMemset(c->d_hist, 0, c->hist_n * kPlanes * sizeof(double)));
@@ -600,9 +562,30 @@ template <class R> int ensure_work(pach_ctx *c, int64_t batch, int grid)
     return PACH_OK;
 }
 
+// Receiver voxel masks and the receiver-aware distance field (rebuilt when the
+// scene, the receivers or the setting change).
+int refresh_receiver_cells(pach_ctx *c)
+{
+    if (!c->rmask_dirty) return PACH_OK;
+    c->rmask_on = c->tune_rmask && c->R <= 64;
+    std::vector<uint64_t> mask;
+    std::vector<uint8_t> dist;
+    if (c->rmask_on) {
+        pach::receiver_cells(c->grid, c->centers.data(), c->R, c->radius, mask, dist);
+        HIPCHK(c, dalloc(c->d_cell_rmask, mask.size()));
+        HIPCHK(c, hipMemcpy(c->d_cell_rmask, mask.data(), mask.size() * 8, hipMemcpyHostToDevice));
+    } else {
+        dist = c->grid.dist;
+    }
+    HIPCHK(c, hipMemcpy(c->d_cell_dist, dist.data(), c->ncell, hipMemcpyHostToDevice));
+    c->rmask_dirty = false;
+    return PACH_OK;
+}
+
 template <class R> int run_trace(pach_ctx *c, const pach_params p)
 {
     (void)hipSetDevice(c->device);
+    if (int rc = refresh_receiver_cells(c)) return rc;
     auto t0 = std::chrono::steady_clock::now();
     TraceConst tc{};
     tc.mode = p.mode;
@@ -613,14 +596,21 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
     tc.dir_mode = c->dir_mode;
     tc.p1_max = p.p1_max_bounces > 0 ? p.p1_max_bounces : 128;
     tc.max_bounces = p.max_bounces > 0 ? p.max_bounces : 100000;
+    tc.walk_steps = c->tune_walk_steps;
+    tc.shade_min = c->tune_shade_min;
+    tc.walk_flat = c->tune_walk_flat;
     tc.cotime = p.cutoff_m / c->c;  // TaskAll:83
     tc.T1 = c->power[tc.h] * 1e-2;  // TaskAll:351
+    BandTab bt{};
     for (int o = 0; o < 8; ++o) {
         tc.power[o] = c->power[o];
         const bool in = o >= p.oct_lo && o <= p.oct_hi;
-        tc.T2[o] = in ? c->power[o] * std::pow(10, -4.5) : 0.0;  // TaskAll:354
-        tc.T3[o] = in ? c->power[o] * 1e-9 : 0.0;                // TaskAll:355
+        bt.T2[o] = in ? c->power[o] * std::pow(10, -4.5) : 0.0;  // TaskAll:354
+        bt.T3[o] = in ? c->power[o] * 1e-9 : 0.0;                // TaskAll:355
+        bt.atten[o] = c->atten[o];
+        bt.kexp2[o] = (float)(-0.1 * c->atten[o] * std::log2(10.0));
     }
+    tc.band = c->d_band;
     for (int k = 0; k < 3; ++k) tc.src[k] = c->origin[k];
     tc.key0 = (uint32_t)p.seed;
     tc.key1 = (uint32_t)(p.seed >> 32);
@@ -632,8 +622,13 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
     SceneBuf<R> &sb = scene_of<R>(c);
     DevScene<R> s{};
     s.v0 = sb.v0; s.e1 = sb.e1; s.e2 = sb.e2; s.nrm = sb.nrm; s.surf = c->d_surf;
-    s.cell_start = c->d_cell_start; s.cell_tris = c->d_cell_tris;
+    s.cell_range = (const uint2 *)c->d_cell_range; s.cell_tris = c->d_cell_tris;
+    s.cell_dist = c->d_cell_dist;
+    s.cell_rmask = c->rmask_on ? c->d_cell_rmask : nullptr;
     s.gx = c->g[0]; s.gy = c->g[1]; s.gz = c->g[2];
+    s.ncell = c->ncell;
+    const int lds_bytes = (c->ncell + 15) & ~15;
+    s.dist_lds = (lds_bytes <= kMaxLdsDist && c->tune_dist_lds) ? 1 : 0;
     for (int k = 0; k < 3; ++k) {
         s.gmin[k] = (R)c->gmin[k];
         s.cs[k] = (R)c->cs[k];
@@ -645,14 +640,12 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
     DevRecv<R> rv{};
     rv.ctr = ctr_of<R>(c);
     rv.n = c->R;
+    rv.masked = c->rmask_on ? 1 : 0;
     rv.r2 = (R)(c->radius * c->radius);
     rv.sizemod = 1.0 / (M_PI * c->radius * c->radius * c->radius);
     rv.c = c->c;
     rv.inv_c = 1 / c->c;
-    for (int o = 0; o < 8; ++o) {
-        rv.atten[o] = c->atten[o];
-        rv.kexp2[o] = (float)(-0.1 * c->atten[o] * std::log2(10.0));
-    }
+    rv.band = c->d_band;
     rv.rho_c = c->rho_c;
     rv.fs = (int)c->fs;
     rv.sct = c->sct;
@@ -662,13 +655,14 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
     const int64_t N = p.ray_count;
     int64_t batch = p.batch_rays > 0 ? p.batch_rays : std::min<int64_t>(N, 32768);
     batch = std::max<int64_t>(1, std::min<int64_t>(batch, 1 << 22));
-    const int grid = c->n_cu * octave_blocks_per_cu<R>();
+    const int grid = c->n_cu * octave_blocks_per_cu<R>(s.dist_lds ? lds_bytes : 0);
     int rc = ensure_work<R>(c, batch, grid);
     if (rc) return rc;
     WorkBuf<R> &w = work_of<R>(c);
     unsigned *p1_count = c->d_scal + 0, *res_count = c->d_scal + 1, *in_head = c->d_scal + 2;
     unsigned *dump_count = c->d_scal + 3;  // [2]
     hipStream_t st = c->stream;
+    HIPCHK(c, hipMemcpyAsync(c->d_band, &bt, sizeof(bt), hipMemcpyHostToDevice, st));
     HIPCHK(c, hipMemsetAsync(c->d_hist, 0, c->hist_n * kPlanes * sizeof(double), st));
     HIPCHK(c, hipMemsetAsync(c->d_cnt, 0, sizeof(DevCounters), st));
     HIPCHK(c, hipMemsetAsync(c->d_scal, 0, 8 * sizeof(unsigned), st));
@@ -742,6 +736,11 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
     S.guard_hits = (int64_t)hc.v[C_GUARD];
     S.spills = (int64_t)hc.v[C_SPILL];
     S.resumes = (int64_t)hc.v[C_RESUME];
+    S.loop_steps = (int64_t)hc.v[C_ITER];
+    S.voxels = (int64_t)hc.v[C_VOXEL];
+    S.tri_tests = (int64_t)hc.v[C_TEST];
+    S.shade_passes = (int64_t)hc.v[C_SHADE];
+    S.lanes_shaded = (int64_t)hc.v[C_SHADED];
     S.lost = hc.lost;
     double psum = 0;
     for (int o = 0; o < 8; ++o) psum += c->power[o];
@@ -760,6 +759,34 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
 }
 
 }  // namespace
+
+extern "C" int pach_tune(pach_ctx *c, const char *key, int64_t value)
+{
+    if (!c || !key) return PACH_EINVAL;
+    if (c->running) return fail(c, PACH_ESTATE, "trace running");
+    if (std::strcmp(key, "dist_lds") == 0) {
+        c->tune_dist_lds = value != 0;
+        return PACH_OK;
+    }
+    if (std::strcmp(key, "receiver_masks") == 0) {
+        c->tune_rmask = value != 0;
+        c->rmask_dirty = true;
+        return PACH_OK;
+    }
+    if (std::strcmp(key, "walk_steps") == 0 && value >= 1 && value <= 256) {
+        c->tune_walk_steps = (int)value;
+        return PACH_OK;
+    }
+    if (std::strcmp(key, "walk_flat") == 0) {
+        c->tune_walk_flat = value != 0;
+        return PACH_OK;
+    }
+    if (std::strcmp(key, "shade_min") == 0 && value >= 1 && value <= 64) {
+        c->tune_shade_min = (int)value;
+        return PACH_OK;
+    }
+    return fail(c, PACH_EINVAL, std::string("unknown tuning key ") + key);
+}
 
 extern "C" int pach_trace(pach_ctx *c, const pach_params *p)
 {

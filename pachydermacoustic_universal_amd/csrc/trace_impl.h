@@ -69,10 +69,19 @@ template <class R> __device__ __forceinline__ R rcp(R x);
 template <> __device__ __forceinline__ float rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 template <> __device__ __forceinline__ double rcp<double>(double x) { return 1.0 / x; }
 
+// FP64: sqrt and IEEE division, as the oracle; FP32: one reciprocal square root.
 template <class R> __device__ __forceinline__ void normalize3(R &x, R &y, R &z)
 {
-    R l = sqrt(x * x + y * y + z * z);
-    if (l > R(0)) { x /= l; y /= l; z /= l; }
+    if constexpr (sizeof(R) == 8) {
+        R l = sqrt(x * x + y * y + z * z);
+        if (l > R(0)) { x /= l; y /= l; z /= l; }
+    } else {
+        const float q = x * x + y * y + z * z;
+        if (q > 0.0f) {
+            const float il = __builtin_amdgcn_rsqf(q);
+            x *= il; y *= il; z *= il;
+        }
+    }
 }
 
 // Smallest accepted hit distance.  FP64 keeps Hare's t > 1e-10
@@ -83,86 +92,289 @@ template <> struct Tmin<double> { static constexpr double v = 1e-10; static cons
 template <> struct Tmin<float> { static constexpr float v = 0.0f; static constexpr bool excl = true; };
 
 // ---------------------------------------------------------------- closest hit
-// Moller-Trumbore with inclusive edges.
-template <class R>
-__device__ __forceinline__ bool tri_hit(const DevScene<R> &s, int t, R ox, R oy, R oz, R dx, R dy,
-                                        R dz, R &tout)
+// Moller-Trumbore with inclusive edges (the FP64 path: the same arithmetic as
+// the CPU oracle, so both accept and reject the same candidates).
+__device__ __forceinline__ bool tri_hit_mt(const DevScene<double> &s, int t, double ox, double oy, double oz,
+                                           double dx, double dy, double dz, double tlim, double &tout)
 {
-    const auto a = s.v0[t];
-    const auto e1 = s.e1[t];
-    const auto e2 = s.e2[t];
-    R px = dy * e2.z - dz * e2.y;
-    R py = dz * e2.x - dx * e2.z;
-    R pz = dx * e2.y - dy * e2.x;
-    R det = e1.x * px + e1.y * py + e1.z * pz;
-    if (det == R(0)) return false;
-    R inv = rcp<R>(det);
-    R sx = ox - a.x, sy = oy - a.y, sz = oz - a.z;
-    R u = (sx * px + sy * py + sz * pz) * inv;
-    if (u < R(0) || u > R(1)) return false;
-    R qx = sy * e1.z - sz * e1.y;
-    R qy = sz * e1.x - sx * e1.z;
-    R qz = sx * e1.y - sy * e1.x;
-    R v = (dx * qx + dy * qy + dz * qz) * inv;
-    if (v < R(0) || u + v > R(1)) return false;
-    R tt = (e2.x * qx + e2.y * qy + e2.z * qz) * inv;
-    if (!(tt > R(Tmin<R>::v))) return false;
+    const double4 a = s.v0[t];
+    const double4 e1 = s.e1[t];
+    const double4 e2 = s.e2[t];
+    const double px = dy * e2.z - dz * e2.y;
+    const double py = dz * e2.x - dx * e2.z;
+    const double pz = dx * e2.y - dy * e2.x;
+    const double det = e1.x * px + e1.y * py + e1.z * pz;
+    if (det == 0.0) return false;
+    const double inv = 1.0 / det;
+    const double sx = ox - a.x, sy = oy - a.y, sz = oz - a.z;
+    const double u = (sx * px + sy * py + sz * pz) * inv;
+    if (u < 0.0 || u > 1.0) return false;
+    const double qx = sy * e1.z - sz * e1.y;
+    const double qy = sz * e1.x - sx * e1.z;
+    const double qz = sx * e1.y - sy * e1.x;
+    const double v = (dx * qx + dy * qy + dz * qz) * inv;
+    if (v < 0.0 || u + v > 1.0) return false;
+    const double tt = (e2.x * qx + e2.y * qy + e2.z * qz) * inv;
+    if (!(tt > 1e-10) || !(tt <= tlim)) return false;
     tout = tt;
     return true;
 }
 
-// Amanatides-Woo walk over the uniform grid; the closest hit ends the walk once
-// it lies before the current voxel's exit.  Returns the triangle or -1.
+// FP32 test against the triangle's dual basis (host-precomputed rows r0, r1,
+// r2): r2 . x + r2.w is the signed plane distance, r0 / r1 give the barycentric
+// coordinates u, v of a point x of the plane.  The plane row decides t first,
+// so triangles crossed behind the ray or past tlim cost 8 multiply-adds.
+// Edges are widened by 4e-6 (barycentric) so no ray slips between neighbours.
+__device__ __forceinline__ bool tri_hit_dual(const DevScene<float> &s, int t, float ox, float oy, float oz,
+                                             float dx, float dy, float dz, float tlim, float &tout)
+{
+    const float4 r2 = s.nrm[t];
+    const float den = r2.x * dx + r2.y * dy + r2.z * dz;
+    const float num = -(r2.x * ox + r2.y * oy + r2.z * oz + r2.w);
+    const float tt = num * __builtin_amdgcn_rcpf(den);
+    if (!(tt > 0.0f && tt <= tlim)) return false;
+    const float px = ox + dx * tt, py = oy + dy * tt, pz = oz + dz * tt;
+    const float4 r0 = s.e1[t];
+    const float u = r0.x * px + r0.y * py + r0.z * pz + r0.w;
+    if (u < -4e-6f || u > 1.000004f) return false;
+    const float4 r1 = s.e2[t];
+    const float v = r1.x * px + r1.y * py + r1.z * pz + r1.w;
+    if (v < -4e-6f || u + v > 1.000004f) return false;
+    tout = tt;
+    return true;
+}
+
 template <class R>
-__device__ int shoot(const DevScene<R> &s, R ox, R oy, R oz, R dx, R dy, R dz, int excl, R &tbest)
+__device__ __forceinline__ bool tri_hit(const DevScene<R> &s, int t, R ox, R oy, R oz, R dx, R dy, R dz,
+                                        R tlim, R &tout)
+{
+    if constexpr (sizeof(R) == 8) return tri_hit_mt(s, t, ox, oy, oz, dx, dy, dz, tlim, tout);
+    else return tri_hit_dual(s, t, ox, oy, oz, dx, dy, dz, tlim, tout);
+}
+
+// Voxel distances staged in LDS by k_octave (dynamic shared memory).
+extern __shared__ uint8_t pach_lds_dist[];
+
+template <bool LDS>
+__device__ __forceinline__ uint8_t cell_dist(const uint8_t *g, int cell)
+{
+    if constexpr (LDS) return pach_lds_dist[cell];
+    else return g[cell];
+}
+
+// Resumable closest-hit walk over the uniform grid (Amanatides-Woo), with the
+// reference's rule that a hit ends the walk once it lies inside the current
+// voxel (Classes_VoxelGrid_PolyRefractive.cs:172).  Triangles whose hit lies
+// past the current voxel's exit are left for the voxel that contains it (every
+// voxel a triangle touches lists it), so each voxel runs the full inside test
+// on few candidates.  Empty space: a voxel at L-inf distance k >= 2 from every
+// occupied voxel lies inside a (2k-1)^3 empty block, crossed in one jump.
+template <class R> struct Walk {
+    int cx, cy, cz;
+    R tmx, tmy, tmz;  // t where the ray leaves the current voxel along each axis
+    R ix, iy, iz;     // 1 / d (infinite for a zero component)
+    int best;
+    R tbest;
+    uint64_t rmask;   // receivers whose spheres touch a voxel the walk crossed
+    uint32_t j, je;   // flattened walk: untested list entries of the current voxel
+    int k;            // flattened walk: distance value of the current voxel
+};
+
+template <class R>
+__device__ __forceinline__ R axis_exit(R o, R d, R inv, int c, const R gmin, const R cs)
 {
     const R INF = R(__builtin_huge_val());
-    int cx = (int)floor((ox - s.gmin[0]) * s.inv_cs[0]);
-    int cy = (int)floor((oy - s.gmin[1]) * s.inv_cs[1]);
-    int cz = (int)floor((oz - s.gmin[2]) * s.inv_cs[2]);
-    cx = min(max(cx, 0), s.gx - 1);
-    cy = min(max(cy, 0), s.gy - 1);
-    cz = min(max(cz, 0), s.gz - 1);
-    int sx = 0, sy = 0, sz = 0;
-    R tmx = INF, tmy = INF, tmz = INF, tdx = INF, tdy = INF, tdz = INF;
-    if (dx > R(0)) { sx = 1; tmx = (s.gmin[0] + (cx + 1) * s.cs[0] - ox) / dx; tdx = s.cs[0] / dx; }
-    else if (dx < R(0)) { sx = -1; tmx = (s.gmin[0] + cx * s.cs[0] - ox) / dx; tdx = -s.cs[0] / dx; }
-    if (dy > R(0)) { sy = 1; tmy = (s.gmin[1] + (cy + 1) * s.cs[1] - oy) / dy; tdy = s.cs[1] / dy; }
-    else if (dy < R(0)) { sy = -1; tmy = (s.gmin[1] + cy * s.cs[1] - oy) / dy; tdy = -s.cs[1] / dy; }
-    if (dz > R(0)) { sz = 1; tmz = (s.gmin[2] + (cz + 1) * s.cs[2] - oz) / dz; tdz = s.cs[2] / dz; }
-    else if (dz < R(0)) { sz = -1; tmz = (s.gmin[2] + cz * s.cs[2] - oz) / dz; tdz = -s.cs[2] / dz; }
-    int best = -1;
-    tbest = INF;
-    for (;;) {
-        const int cell = (cz * s.gy + cy) * s.gx + cx;
-        const uint32_t b = s.cell_start[cell], e = s.cell_start[cell + 1];
-        for (uint32_t j = b; j < e; ++j) {
-            const int t = (int)s.cell_tris[j];
-            if (Tmin<R>::excl && t == excl) continue;
-            R tt;
-            if (tri_hit<R>(s, t, ox, oy, oz, dx, dy, dz, tt) && (tt < tbest || (tt == tbest && t < best))) {
-                tbest = tt;
-                best = t;
+    return d > R(0) ? (gmin + (c + 1) * cs - o) * inv : (d < R(0) ? (gmin + c * cs - o) * inv : INF);
+}
+
+template <class R>
+__device__ __forceinline__ void walk_init(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R dx, R dy, R dz)
+{
+    const R INF = R(__builtin_huge_val());
+    w.ix = dx != R(0) ? rcp<R>(dx) : INF;
+    w.iy = dy != R(0) ? rcp<R>(dy) : INF;
+    w.iz = dz != R(0) ? rcp<R>(dz) : INF;
+    w.rmask = 0;
+    w.cx = min(max((int)floor((ox - s.gmin[0]) * s.inv_cs[0]), 0), s.gx - 1);
+    w.cy = min(max((int)floor((oy - s.gmin[1]) * s.inv_cs[1]), 0), s.gy - 1);
+    w.cz = min(max((int)floor((oz - s.gmin[2]) * s.inv_cs[2]), 0), s.gz - 1);
+    w.tmx = axis_exit<R>(ox, dx, w.ix, w.cx, s.gmin[0], s.cs[0]);
+    w.tmy = axis_exit<R>(oy, dy, w.iy, w.cy, s.gmin[1], s.cs[1]);
+    w.tmz = axis_exit<R>(oz, dz, w.iz, w.cz, s.gmin[2], s.cs[2]);
+    w.best = -1;
+    w.tbest = INF;
+    w.j = w.je = 0;
+    w.k = -1;  // current voxel not loaded yet
+}
+
+// Processes up to `budget` voxels; returns true once the walk has ended
+// (w.best >= 0: hit at w.tbest; w.best < 0: the ray left the grid).
+template <class R, bool LDS>
+__device__ bool walk_steps(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R dx, R dy, R dz, int excl,
+                           int budget, uint32_t &n_vox, uint32_t &n_test)
+{
+    for (int it = 0; it < budget; ++it) {
+        ++n_vox;
+        const int cell = (w.cz * s.gy + w.cy) * s.gx + w.cx;
+        const int k = cell_dist<LDS>(s.cell_dist, cell);
+        const R texit = fmin(w.tmx, fmin(w.tmy, w.tmz));
+        if (k == 0) {
+            if (s.cell_rmask) w.rmask |= s.cell_rmask[cell];
+            const uint2 rg = s.cell_range[cell];
+            // accept hits up to just past this voxel's exit; nearer voxels were done
+            const R tlim = fmin(w.tbest, texit * R(1.0000001) + R(1e-6));
+            n_test += rg.y;
+            for (uint32_t j = rg.x; j < rg.x + rg.y; ++j) {
+                const int t = (int)s.cell_tris[j];
+                R tt;
+                if (t != w.best && !(Tmin<R>::excl && t == excl) &&
+                    tri_hit<R>(s, t, ox, oy, oz, dx, dy, dz, fmin(tlim, w.tbest), tt) &&
+                    (tt < w.tbest || (tt == w.tbest && t < w.best))) {
+                    w.tbest = tt;
+                    w.best = t;
+                }
+            }
+            if (w.best >= 0 && w.tbest <= texit) return true;
+        }
+        if (k >= 2) {
+            // jump across the empty block around the current voxel
+            const int h = k - 1;
+            const int lx = max(w.cx - h, 0), hx = min(w.cx + h, s.gx - 1);
+            const int ly = max(w.cy - h, 0), hy = min(w.cy + h, s.gy - 1);
+            const int lz = max(w.cz - h, 0), hz = min(w.cz + h, s.gz - 1);
+            const R bx = axis_exit<R>(ox, dx, w.ix, dx > R(0) ? hx : lx, s.gmin[0], s.cs[0]);
+            const R by = axis_exit<R>(oy, dy, w.iy, dy > R(0) ? hy : ly, s.gmin[1], s.cs[1]);
+            const R bz = axis_exit<R>(oz, dz, w.iz, dz > R(0) ? hz : lz, s.gmin[2], s.cs[2]);
+            const int ax = (bx < by) ? (bx < bz ? 0 : 2) : (by < bz ? 1 : 2);
+            const R tb = ax == 0 ? bx : (ax == 1 ? by : bz);
+            const R px = ox + dx * tb, py = oy + dy * tb, pz = oz + dz * tb;
+            const int fx = (int)floor((px - s.gmin[0]) * s.inv_cs[0]);
+            const int fy = (int)floor((py - s.gmin[1]) * s.inv_cs[1]);
+            const int fz = (int)floor((pz - s.gmin[2]) * s.inv_cs[2]);
+            // never move backwards; stay inside the block on the other axes
+            int nx = dx > R(0) ? min(max(fx, w.cx), hx) : (dx < R(0) ? max(min(fx, w.cx), lx) : w.cx);
+            int ny = dy > R(0) ? min(max(fy, w.cy), hy) : (dy < R(0) ? max(min(fy, w.cy), ly) : w.cy);
+            int nz = dz > R(0) ? min(max(fz, w.cz), hz) : (dz < R(0) ? max(min(fz, w.cz), lz) : w.cz);
+            if (ax == 0) nx = dx > R(0) ? hx + 1 : lx - 1;
+            else if (ax == 1) ny = dy > R(0) ? hy + 1 : ly - 1;
+            else nz = dz > R(0) ? hz + 1 : lz - 1;
+            if (nx < 0 || nx >= s.gx || ny < 0 || ny >= s.gy || nz < 0 || nz >= s.gz) return true;
+            w.cx = nx; w.cy = ny; w.cz = nz;
+            w.tmx = axis_exit<R>(ox, dx, w.ix, nx, s.gmin[0], s.cs[0]);
+            w.tmy = axis_exit<R>(oy, dy, w.iy, ny, s.gmin[1], s.cs[1]);
+            w.tmz = axis_exit<R>(oz, dz, w.iz, nz, s.gmin[2], s.cs[2]);
+        } else {
+            // one voxel step along the axis whose boundary comes first
+            if (w.tmx < w.tmy && w.tmx < w.tmz) {
+                w.cx += dx > R(0) ? 1 : -1;
+                if (w.cx < 0 || w.cx >= s.gx) return true;
+                w.tmx += s.cs[0] * fabs(w.ix);
+            } else if (w.tmy < w.tmz) {
+                w.cy += dy > R(0) ? 1 : -1;
+                if (w.cy < 0 || w.cy >= s.gy) return true;
+                w.tmy += s.cs[1] * fabs(w.iy);
+            } else {
+                w.cz += dz > R(0) ? 1 : -1;
+                if (w.cz < 0 || w.cz >= s.gz) return true;
+                w.tmz += s.cs[2] * fabs(w.iz);
             }
         }
-        const int ax = (tmx < tmy) ? (tmx < tmz ? 0 : 2) : (tmy < tmz ? 1 : 2);
-        const R texit = ax == 0 ? tmx : (ax == 1 ? tmy : tmz);
-        if (best >= 0 && tbest <= texit) break;
-        if (ax == 0) {
-            cx += sx;
-            if (sx == 0 || cx < 0 || cx >= s.gx) break;
-            tmx += tdx;
-        } else if (ax == 1) {
-            cy += sy;
-            if (sy == 0 || cy < 0 || cy >= s.gy) break;
-            tmy += tdy;
-        } else {
-            cz += sz;
-            if (sz == 0 || cz < 0 || cz >= s.gz) break;
-            tmz += tdz;
-        }
     }
-    return best;
+    return false;
+}
+
+// Flattened form of walk_steps: each of the `budget` units either tests one
+// triangle of the current voxel or leaves the voxel (one step, or one jump
+// across an empty block), so lanes in triangle-dense voxels do not hold the
+// wave on a long inner loop.  Same walk, same result as walk_steps.
+template <class R, bool LDS>
+__device__ bool walk_units(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R dx, R dy, R dz, int excl,
+                           int budget, uint32_t &n_vox, uint32_t &n_test)
+{
+    for (int it = 0; it < budget; ++it) {
+        if (w.k < 0) {
+            // enter the current voxel
+            ++n_vox;
+            const int cell = (w.cz * s.gy + w.cy) * s.gx + w.cx;
+            w.k = cell_dist<LDS>(s.cell_dist, cell);
+            w.j = w.je = 0;
+            if (w.k == 0) {
+                if (s.cell_rmask) w.rmask |= s.cell_rmask[cell];
+                const uint2 rg = s.cell_range[cell];
+                w.j = rg.x;
+                w.je = rg.x + rg.y;
+            }
+        }
+        const R texit = fmin(w.tmx, fmin(w.tmy, w.tmz));
+        if (w.j < w.je) {
+            ++n_test;
+            const int t = (int)s.cell_tris[w.j++];
+            const R tlim = fmin(w.tbest, texit * R(1.0000001) + R(1e-6));
+            R tt;
+            if (t != w.best && !(Tmin<R>::excl && t == excl) && tri_hit<R>(s, t, ox, oy, oz, dx, dy, dz, tlim, tt) &&
+                (tt < w.tbest || (tt == w.tbest && t < w.best))) {
+                w.tbest = tt;
+                w.best = t;
+            }
+            continue;
+        }
+        if (w.best >= 0 && w.tbest <= texit) return true;
+        if (w.k >= 2) {
+            const int h = w.k - 1;
+            const int lx = max(w.cx - h, 0), hx = min(w.cx + h, s.gx - 1);
+            const int ly = max(w.cy - h, 0), hy = min(w.cy + h, s.gy - 1);
+            const int lz = max(w.cz - h, 0), hz = min(w.cz + h, s.gz - 1);
+            const R bx = axis_exit<R>(ox, dx, w.ix, dx > R(0) ? hx : lx, s.gmin[0], s.cs[0]);
+            const R by = axis_exit<R>(oy, dy, w.iy, dy > R(0) ? hy : ly, s.gmin[1], s.cs[1]);
+            const R bz = axis_exit<R>(oz, dz, w.iz, dz > R(0) ? hz : lz, s.gmin[2], s.cs[2]);
+            const int ax = (bx < by) ? (bx < bz ? 0 : 2) : (by < bz ? 1 : 2);
+            const R tb = ax == 0 ? bx : (ax == 1 ? by : bz);
+            const R px = ox + dx * tb, py = oy + dy * tb, pz = oz + dz * tb;
+            const int fx = (int)floor((px - s.gmin[0]) * s.inv_cs[0]);
+            const int fy = (int)floor((py - s.gmin[1]) * s.inv_cs[1]);
+            const int fz = (int)floor((pz - s.gmin[2]) * s.inv_cs[2]);
+            int nx = dx > R(0) ? min(max(fx, w.cx), hx) : (dx < R(0) ? max(min(fx, w.cx), lx) : w.cx);
+            int ny = dy > R(0) ? min(max(fy, w.cy), hy) : (dy < R(0) ? max(min(fy, w.cy), ly) : w.cy);
+            int nz = dz > R(0) ? min(max(fz, w.cz), hz) : (dz < R(0) ? max(min(fz, w.cz), lz) : w.cz);
+            if (ax == 0) nx = dx > R(0) ? hx + 1 : lx - 1;
+            else if (ax == 1) ny = dy > R(0) ? hy + 1 : ly - 1;
+            else nz = dz > R(0) ? hz + 1 : lz - 1;
+            if (nx < 0 || nx >= s.gx || ny < 0 || ny >= s.gy || nz < 0 || nz >= s.gz) return true;
+            w.cx = nx; w.cy = ny; w.cz = nz;
+            w.tmx = axis_exit<R>(ox, dx, w.ix, nx, s.gmin[0], s.cs[0]);
+            w.tmy = axis_exit<R>(oy, dy, w.iy, ny, s.gmin[1], s.cs[1]);
+            w.tmz = axis_exit<R>(oz, dz, w.iz, nz, s.gmin[2], s.cs[2]);
+        } else if (w.tmx < w.tmy && w.tmx < w.tmz) {
+            w.cx += dx > R(0) ? 1 : -1;
+            if (w.cx < 0 || w.cx >= s.gx) return true;
+            w.tmx += s.cs[0] * fabs(w.ix);
+        } else if (w.tmy < w.tmz) {
+            w.cy += dy > R(0) ? 1 : -1;
+            if (w.cy < 0 || w.cy >= s.gy) return true;
+            w.tmy += s.cs[1] * fabs(w.iy);
+        } else {
+            w.cz += dz > R(0) ? 1 : -1;
+            if (w.cz < 0 || w.cz >= s.gz) return true;
+            w.tmz += s.cs[2] * fabs(w.iz);
+        }
+        w.k = -1;
+    }
+    return false;
+}
+
+// Complete closest-hit query (the broadband phase uses it directly).
+template <class R, bool LDS>
+__device__ __forceinline__ int shoot(const DevScene<R> &s, R ox, R oy, R oz, R dx, R dy, R dz, int excl, R &tbest,
+                                     uint64_t &rmask)
+{
+    Walk<R> w;
+    walk_init<R>(s, w, ox, oy, oz, dx, dy, dz);
+    uint32_t nv = 0, nt = 0;
+    while (!walk_steps<R, LDS>(s, w, ox, oy, oz, dx, dy, dz, excl, 1 << 20, nv, nt)) {
+    }
+    tbest = w.tbest;
+    rmask = w.rmask;
+    return w.best;
 }
 
 // ---------------------------------------------------------------- receivers
@@ -188,51 +400,71 @@ __device__ __forceinline__ void hist_add(const double *base_unused, double *hist
 template <class R> __device__ __forceinline__ double band_gain(const DevRecv<R> &rv, int band, double dist);
 template <> __device__ __forceinline__ double band_gain<double>(const DevRecv<double> &rv, int band, double dist)
 {
-    return pow(10.0, -.1 * rv.atten[band] * dist);
+    return pow(10.0, -.1 * rv.band->atten[band] * dist);
 }
 template <> __device__ __forceinline__ double band_gain<float>(const DevRecv<float> &rv, int band, double dist)
 {
-    return (double)exp2f(rv.kexp2[band] * (float)dist);
+    return (double)exp2f(rv.band->kexp2[band] * (float)dist);
 }
 
 // Receiver_Bank.CheckRay / CheckBroadbandRay over every receiver.
 // sqd: squared distance to EndPt; E: intensity (octave) or energies (broadband).
 template <class R>
-__device__ void check_receivers(const DevRecv<R> &rv, R ox, R oy, R oz, R dx, R dy, R dz, R sqd,
-                                const R *E, int band, int lo, int hi, double tsum, int ana,
+__device__ __forceinline__ void receiver_test(const DevRecv<R> &rv, int r, R ox, R oy, R oz, R dx, R dy, R dz,
+                                              R sqd, const R *E, int band, int lo, int hi, double tsum, int ana,
+                                              LaneCounters &lc)
+{
+    const auto C = rv.ctr[r];
+    const R mx = ox - C.x, my = oy - C.y, mz = oz - C.z;
+    const R b = mx * dx + my * dy + mz * dz;
+    const R c = (mx * mx + my * my + mz * mz) - rv.r2;
+    if (c > R(0) && b > R(0)) return;
+    const R disc = b * b - c;
+    if (disc < R(0)) return;
+    const R sq = sqrt(disc);
+    const R t1 = -b - sq, t2 = -b + sq;
+    const R t = (t1 + t2) * R(.5);
+    if (!(t > R(0) && t * t < sqd)) return;
+    const double ts = (double)((t2 - t1) * R(0.5));
+    const double rt = (double)t * rv.inv_c + tsum;
+    const int sample = (int)(rt * rv.fs);
+    if (sample >= rv.sct || sample < 0) return;
+    const double dist = rt * rv.c;
+    const double ndx = -(double)dx, ndy = -(double)dy, ndz = -(double)dz;
+    if (band >= 0) {
+        const uint64_t idx = ((uint64_t)r * kBands + band) * rv.sct + sample;
+        const double e = (double)E[0] * band_gain<R>(rv, band, dist) * rv.sizemod * ts;
+        hist_add(nullptr, rv.hist, rv.plane_stride, idx, e, rv.rho_c, ndx, ndy, ndz, ana);
+        lc.v[C_DEP]++;
+    } else {
+        for (int o = lo; o <= hi; ++o) {
+            const uint64_t idx = ((uint64_t)r * kBands + o) * rv.sct + sample;
+            const double e = (double)E[o] * band_gain<R>(rv, o, dist) * rv.sizemod * ts;
+            hist_add(nullptr, rv.hist, rv.plane_stride, idx, e, rv.rho_c, ndx, ndy, ndz, 0);
+            lc.v[C_DEP]++;
+        }
+    }
+}
+
+// Receiver_Bank.CheckRay / CheckBroadbandRay.  sqd: squared distance to EndPt;
+// E: intensity (octave) or energies (broadband).  With per-voxel receiver
+// masks, only receivers whose spheres touch a voxel the walk crossed are
+// tested: a sphere the segment passes through contains the chord midpoint,
+// which lies in a crossed voxel, so the deposits are those of the full loop.
+template <class R>
+__device__ void check_receivers(const DevRecv<R> &rv, uint64_t cand, R ox, R oy, R oz, R dx, R dy, R dz,
+                                R sqd, const R *E, int band, int lo, int hi, double tsum, int ana,
                                 LaneCounters &lc)
 {
-    for (int r = 0; r < rv.n; ++r) {
-        const auto C = rv.ctr[r];
-        const R mx = ox - C.x, my = oy - C.y, mz = oz - C.z;
-        const R b = mx * dx + my * dy + mz * dz;
-        const R c = (mx * mx + my * my + mz * mz) - rv.r2;
-        if (c > R(0) && b > R(0)) continue;
-        const R disc = b * b - c;
-        if (disc < R(0)) continue;
-        const R sq = sqrt(disc);
-        const R t1 = -b - sq, t2 = -b + sq;
-        const R t = (t1 + t2) * R(.5);
-        if (!(t > R(0) && t * t < sqd)) continue;
-        const double ts = (double)((t2 - t1) * R(0.5));
-        const double rt = (double)t * rv.inv_c + tsum;
-        const int sample = (int)(rt * rv.fs);
-        if (sample >= rv.sct || sample < 0) continue;
-        const double dist = rt * rv.c;
-        const double ndx = -(double)dx, ndy = -(double)dy, ndz = -(double)dz;
-        if (band >= 0) {
-            const uint64_t idx = ((uint64_t)r * kBands + band) * rv.sct + sample;
-            const double e = (double)E[0] * band_gain<R>(rv, band, dist) * rv.sizemod * ts;
-            hist_add(nullptr, rv.hist, rv.plane_stride, idx, e, rv.rho_c, ndx, ndy, ndz, ana);
-            lc.v[C_DEP]++;
-        } else {
-            for (int o = lo; o <= hi; ++o) {
-                const uint64_t idx = ((uint64_t)r * kBands + o) * rv.sct + sample;
-                const double e = (double)E[o] * band_gain<R>(rv, o, dist) * rv.sizemod * ts;
-                hist_add(nullptr, rv.hist, rv.plane_stride, idx, e, rv.rho_c, ndx, ndy, ndz, 0);
-                lc.v[C_DEP]++;
-            }
+    if (rv.masked) {
+        while (cand) {
+            const int r = __builtin_ctzll(cand);
+            cand &= cand - 1;
+            receiver_test<R>(rv, r, ox, oy, oz, dx, dy, dz, sqd, E, band, lo, hi, tsum, ana, lc);
         }
+    } else {
+        for (int r = 0; r < rv.n; ++r)
+            receiver_test<R>(rv, r, ox, oy, oz, dx, dy, dz, sqd, E, band, lo, hi, tsum, ana, lc);
     }
 }
 
@@ -370,15 +602,16 @@ k_phase1(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv, int64_t 
         if (!(st.flags & F_SPLIT_PENDING)) {
             for (;;) {
                 R leg;
-                const int ht = shoot<R>(s, st.o[0], st.o[1], st.o[2], st.d[0], st.d[1], st.d[2],
-                                        excl_on ? st.tri : -1, leg);
+                uint64_t cand;
+                const int ht = shoot<R, false>(s, st.o[0], st.o[1], st.o[2], st.d[0], st.d[1], st.d[2],
+                                               excl_on ? st.tri : -1, leg, cand);
                 if (ht < 0) {
                     lc.v[C_BROAD]++;
                     if (st.order > tc.is_order) {
                         const R ex = st.o[0] + st.d[0] * R(1000000), ey = st.o[1] + st.d[1] * R(1000000),
                                 ez = st.o[2] + st.d[2] * R(1000000);
                         const R qx = ex - st.o[0], qy = ey - st.o[1], qz = ez - st.o[2];
-                        check_receivers<R>(rv, st.o[0], st.o[1], st.o[2], st.d[0], st.d[1], st.d[2],
+                        check_receivers<R>(rv, cand, st.o[0], st.o[1], st.o[2], st.d[0], st.d[1], st.d[2],
                                            qx * qx + qy * qy + qz * qz, st.E, -1, tc.oct_lo,
                                            tc.oct_hi, st.tb, 0, lc);
                     }
@@ -412,7 +645,7 @@ k_phase1(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv, int64_t 
                 const R hx = st.o[0] + st.d[0] * leg, hy = st.o[1] + st.d[1] * leg, hz = st.o[2] + st.d[2] * leg;
                 if (st.order > tc.is_order) {
                     const R qx = hx - st.o[0], qy = hy - st.o[1], qz = hz - st.o[2];
-                    check_receivers<R>(rv, st.o[0], st.o[1], st.o[2], st.d[0], st.d[1], st.d[2],
+                    check_receivers<R>(rv, cand, st.o[0], st.o[1], st.o[2], st.d[0], st.d[1], st.d[2],
                                        qx * qx + qy * qy + qz * qz, st.E, -1, tc.oct_lo, tc.oct_hi,
                                        st.tb, 0, lc);
                 }
@@ -499,27 +732,25 @@ k_phase1(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv, int64_t 
 }
 
 // ---------------------------------------------------------------- phases 2, 3
-// One bounce of an octave ray.  Returns false when the ray terminates; sets
-// has_child when Scatter_Late spawned a specular child.
+// Everything an octave-ray bounce does after its closest-hit walk: receiver
+// checks, Absorb, Scatter_Late / Scatter_VeryLate, the phase hand-off.
+// Returns false when the ray terminates; sets has_child when Scatter_Late
+// spawned a specular child.
 template <class R>
-__device__ __forceinline__ bool octave_bounce(const TraceConst &tc, const DevScene<R> &s,
-                                              const DevRecv<R> &rv, ORay<R> &r, bool &has_child,
-                                              ORay<R> &child, LaneCounters &lc)
+__device__ __forceinline__ bool octave_shade(const TraceConst &tc, const DevScene<R> &s, const DevRecv<R> &rv,
+                                             ORay<R> &r, int ht, R leg, uint64_t cand, bool &has_child,
+                                             ORay<R> &child, LaneCounters &lc)
 {
     has_child = false;
-    if ((int)r.k >= tc.max_bounces) { lc.v[C_GUARD]++; return false; }
     const int band = (int)(r.meta & M_BAND);
     const bool late3 = (r.meta & M_LATE3) != 0;
     const int ana = (r.meta & M_SCAT) ? 1 : ((r.meta & M_EVER) ? 2 : 0);
-    R leg;
-    const int ht = shoot<R>(s, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2],
-                            Tmin<R>::excl ? r.tri : -1, leg);
     lc.v[late3 ? C_VLATE : C_LATE]++;
     if (ht < 0) {
         const R ex = r.o[0] + r.d[0] * R(1000000), ey = r.o[1] + r.d[1] * R(1000000),
                 ez = r.o[2] + r.d[2] * R(1000000);
         const R qx = ex - r.o[0], qy = ey - r.o[1], qz = ez - r.o[2];
-        check_receivers<R>(rv, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2],
+        check_receivers<R>(rv, cand, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2],
                            qx * qx + qy * qy + qz * qz, &r.I, band, 0, 0, r.t, ana, lc);
         lc.lost += (double)r.I;
         lc.v[C_MISS]++;
@@ -528,7 +759,7 @@ __device__ __forceinline__ bool octave_bounce(const TraceConst &tc, const DevSce
     const R hx = r.o[0] + r.d[0] * leg, hy = r.o[1] + r.d[1] * leg, hz = r.o[2] + r.d[2] * leg;
     {
         const R qx = hx - r.o[0], qy = hy - r.o[1], qz = hz - r.o[2];
-        check_receivers<R>(rv, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2],
+        check_receivers<R>(rv, cand, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2],
                            qx * qx + qy * qy + qz * qz, &r.I, band, 0, 0, r.t, ana, lc);
     }
     r.t += (double)leg / rv.c;
@@ -593,21 +824,27 @@ __device__ __forceinline__ bool octave_bounce(const TraceConst &tc, const DevSce
         r.d[2] -= N.z * cos_t * R(2);
     }
     if (!late3) {
-        if (!(r.t < tc.cotime && (double)r.I > tc.T2[band])) r.meta |= M_LATE3;  // hand-off to End_Trace
+        if (!(r.t < tc.cotime && (double)r.I > tc.band->T2[band])) r.meta |= M_LATE3;  // hand-off to End_Trace
         return true;
     }
-    return r.t < tc.cotime && (double)r.I > tc.T3[band];
+    return r.t < tc.cotime && (double)r.I > tc.band->T3[band];
 }
 
-// Persistent, wavefront-compacting octave-ray kernel.  Each wave keeps a LIFO
-// stack of spawned rays in its own HBM slab; lanes whose ray terminated are
-// refilled from that stack first, then from the shared input (the previous
-// launch's leftovers in0, then the new broadband-phase output in1).  Once the
-// input is drained and the wave has done at least min_iters bounce steps, it
-// moves its stack and in-flight rays to the shared `dump` queue and exits, so
-// the next launch spreads the leftover work over every wave.  No wave ever
-// waits on another, so every launch ends after a bounded number of steps.
-template <class R>
+// Persistent, wavefront-compacting octave-ray kernel.
+//
+// Each lane holds one octave ray.  A loop step (a) refills idle lanes, first
+// from the wave's own LIFO stack of spawned rays (an HBM slab per wave), then
+// from the shared input (the previous launch's leftovers in0, then the new
+// broadband-phase output in1); (b) advances every walking lane's closest-hit
+// walk by up to kWalkSteps voxels; (c) shades the lanes whose walk ended, but
+// only once kShadeMin lanes are waiting or none is still walking, so the
+// shading code runs on mostly-full waves while long walks continue.
+// Once the input is drained and the wave has run min_iters steps it moves its
+// stack and in-flight rays to the shared `dump` queue and exits, so the next
+// launch spreads the leftover work over every wave.  No wave ever waits on
+// another, so every launch ends after a bounded number of steps.
+
+template <class R, bool LDS>
 __global__ void __launch_bounds__(kBlock)
 k_octave(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv,
          const ORay<R> *__restrict__ in0, const unsigned *n0_ptr, unsigned cap0,
@@ -615,6 +852,14 @@ k_octave(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv,
          ORay<R> *stacks, int stack_cap, ORay<R> *dump, unsigned *dump_count, unsigned dump_cap,
          int min_iters, DevCounters *cnt)
 {
+    if constexpr (LDS) {
+        // stage the voxel distances (ncell rounded up to 16 bytes) in LDS
+        const uint4 *src = (const uint4 *)s.cell_dist;
+        uint4 *dst = (uint4 *)pach_lds_dist;
+        const int n16 = (s.ncell + 15) >> 4;
+        for (int i = threadIdx.x; i < n16; i += kBlock) dst[i] = src[i];
+        __syncthreads();
+    }
     LaneCounters lc = {};
     lc.lost = 0;
     const int lane = (int)__lane_id();
@@ -627,10 +872,11 @@ k_octave(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv,
     int stack_n = 0;
     int iters = 0;
     bool input_done = false;
-    bool act = false;
+    bool act = false, walking = false, ready = false;
     ORay<R> r;
+    Walk<R> w;
     for (;;) {
-        // ---- refill idle lanes: own stack first, then the input queue
+        // ---- (a) refill idle lanes: own stack first, then the input queue
         const uint64_t empty = __ballot(!act);
         if (empty) {
             const int n_empty = __popcll(empty);
@@ -681,27 +927,58 @@ k_octave(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv,
             lc.v[C_SPILL] += (lane == 0) ? n_dump : 0u;
             break;
         }
-        // ---- one bounce per active lane
-        bool has_child = false;
-        ORay<R> child;
-        if (act) act = octave_bounce<R>(tc, s, rv, r, has_child, child, lc);
-        ++iters;
-        // ---- push spawned rays onto the wave's stack (overflow goes to the dump queue)
-        const uint64_t cm = __ballot(has_child);
-        if (cm) {
-            const int n = __popcll(cm);
-            if (has_child) {
-                const int pos = stack_n + __popcll(cm & lt_mask);
-                if (pos < stack_cap) {
-                    ws[pos] = child;
-                } else {
-                    const unsigned q = atomicAdd(dump_count, 1u);
-                    if (q < dump_cap) { dump[q] = child; lc.v[C_SPILL]++; }
-                    else { lc.v[C_DROP]++; lc.lost += (double)child.I; }
-                }
+        // ---- (b) start and advance closest-hit walks
+        if (act && !walking && !ready) {
+            if ((int)r.k >= tc.max_bounces) {
+                lc.v[C_GUARD]++;
+                act = false;
+            } else {
+                walk_init<R>(s, w, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2]);
+                walking = true;
             }
-            stack_n = min(stack_n + n, stack_cap);
-            __threadfence_block();
+        }
+        if (walking) {
+            const bool done =
+                tc.walk_flat
+                    ? walk_units<R, LDS>(s, w, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2],
+                                         Tmin<R>::excl ? r.tri : -1, tc.walk_steps, lc.v[C_VOXEL], lc.v[C_TEST])
+                    : walk_steps<R, LDS>(s, w, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2],
+                                         Tmin<R>::excl ? r.tri : -1, tc.walk_steps, lc.v[C_VOXEL], lc.v[C_TEST]);
+            if (done) {
+                walking = false;
+                ready = true;
+            }
+        }
+        ++iters;
+        if (lane == 0) lc.v[C_ITER]++;
+        // ---- (c) shade the lanes whose walk ended
+        const int n_ready = __popcll(__ballot(ready));
+        if (n_ready > 0 && (n_ready >= tc.shade_min || __ballot(walking) == 0)) {
+            bool has_child = false;
+            ORay<R> child;
+            if (lane == 0) lc.v[C_SHADE]++;
+            if (ready) {
+                lc.v[C_SHADED]++;
+                act = octave_shade<R>(tc, s, rv, r, w.best, w.tbest, w.rmask, has_child, child, lc);
+                ready = false;
+            }
+            // push spawned rays onto the wave's stack (overflow goes to the dump queue)
+            const uint64_t cm = __ballot(has_child);
+            if (cm) {
+                const int n = __popcll(cm);
+                if (has_child) {
+                    const int pos = stack_n + __popcll(cm & lt_mask);
+                    if (pos < stack_cap) {
+                        ws[pos] = child;
+                    } else {
+                        const unsigned q = atomicAdd(dump_count, 1u);
+                        if (q < dump_cap) { dump[q] = child; lc.v[C_SPILL]++; }
+                        else { lc.v[C_DROP]++; lc.lost += (double)child.I; }
+                    }
+                }
+                stack_n = min(stack_n + n, stack_cap);
+                __threadfence_block();
+            }
         }
     }
     for (int c = 0; c < C_N; ++c)
@@ -732,16 +1009,24 @@ hipError_t launch_octave(hipStream_t st, int grid, const TraceConst &tc, const D
                          ORay<R> *stacks, int stack_cap, ORay<R> *dump, unsigned *dump_count,
                          unsigned dump_cap, int min_iters, DevCounters *cnt)
 {
-    hipLaunchKernelGGL(k_octave<R>, dim3(grid), dim3(kBlock), 0, st, tc, s, rv, in0, n0, cap0, in1, n1,
-                       cap1, in_head, stacks, stack_cap, dump, dump_count, dump_cap, min_iters, cnt);
+    if (s.dist_lds) {
+        const size_t lds = (size_t)((s.ncell + 15) & ~15);
+        hipLaunchKernelGGL((k_octave<R, true>), dim3(grid), dim3(kBlock), lds, st, tc, s, rv, in0, n0, cap0,
+                           in1, n1, cap1, in_head, stacks, stack_cap, dump, dump_count, dump_cap, min_iters, cnt);
+    } else {
+        hipLaunchKernelGGL((k_octave<R, false>), dim3(grid), dim3(kBlock), 0, st, tc, s, rv, in0, n0, cap0,
+                           in1, n1, cap1, in_head, stacks, stack_cap, dump, dump_count, dump_cap, min_iters, cnt);
+    }
     return hipGetLastError();
 }
 
-template <class R> int octave_blocks_per_cu()
+template <class R> int octave_blocks_per_cu(int lds_bytes)
 {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_octave<R>, kBlock, 0) != hipSuccess || nb < 1)
-        nb = 1;
+    hipError_t e = lds_bytes > 0
+                       ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_octave<R, true>, kBlock, lds_bytes)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_octave<R, false>, kBlock, 0);
+    if (e != hipSuccess || nb < 1) nb = 1;
     return nb;
 }
 
@@ -755,6 +1040,6 @@ template <class R> int octave_blocks_per_cu()
                                          unsigned, const ORay<R> *, const unsigned *, unsigned,     \
                                          unsigned *, ORay<R> *, int, ORay<R> *, unsigned *,         \
                                          unsigned, int, DevCounters *);                             \
-    template int octave_blocks_per_cu<R>();
+    template int octave_blocks_per_cu<R>(int);
 
 }  // namespace pach

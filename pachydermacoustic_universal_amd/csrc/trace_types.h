@@ -9,6 +9,7 @@ namespace pach {
 constexpr int kBands = 8;
 constexpr int kPlanes = 11;
 constexpr int kBlock = 256;  // 4 wave64 per workgroup
+constexpr int kMaxLdsDist = 32768;  // voxel-distance bytes staged in LDS per workgroup
 
 template <class R> struct V4;
 template <> struct V4<float> { using T = float4; };
@@ -16,28 +17,44 @@ template <> struct V4<double> { using T = double4; };
 
 // Scene in HBM: structure-of-arrays triangle records, per-(polygon, band)
 // material records and a uniform voxel grid in CSR form.
+// FP64: e1/e2 are the edges v1 - v0, v2 - v0 (Moller-Trumbore), nrm the unit
+// normal.  FP32: e1/e2 hold the dual-basis rows r0, r1 and nrm the plane row
+// r2 (unit normal, w = -n . v0) of the triangle test in trace_impl.h.
 template <class R> struct DevScene {
     const typename V4<R>::T *v0;   // [T] vertex 0 (w unused)
-    const typename V4<R>::T *e1;   // [T] v1 - v0
-    const typename V4<R>::T *e2;   // [T] v2 - v0
-    const typename V4<R>::T *nrm;  // [T] unit normal (Topology.Normal)
+    const typename V4<R>::T *e1;   // [T] v1 - v0 (FP64) or dual row r0 (FP32)
+    const typename V4<R>::T *e2;   // [T] v2 - v0 (FP64) or dual row r1 (FP32)
+    const typename V4<R>::T *nrm;  // [T] unit normal (Topology.Normal), FP32 w = plane offset
     const int *surf;               // [T] polygon index (Surf_ID)
-    const uint32_t *cell_start;    // [ncell + 1]
+    const uint2 *cell_range;       // [ncell] (first list index, triangle count)
     const uint32_t *cell_tris;     // triangle ids per cell
+    const uint8_t *cell_dist;      // [ncell] L-inf distance (cells) to the nearest occupied cell, 0 = occupied
+    const uint64_t *cell_rmask;    // [ncell] receivers touching the voxel (null: test every receiver)
     int gx, gy, gz;
+    int ncell;
+    int dist_lds;                  // 1: k_octave stages cell_dist in LDS
     R gmin[3], cs[3], inv_cs[3];
     const typename V4<R>::T *mat;  // [S*8] (1 - alpha, s, lo, hi) of Lambert_Scattering
     const R *trans;                // [S*8] transmission
 };
 
+// Per-band constants read with a runtime band index (kept in device memory so
+// the kernels index them with plain loads instead of copying an argument array).
+struct BandTab {
+    double atten[kBands];  // air attenuation, dB/m
+    double T2[kBands];     // split-phase energy threshold (TaskAll:354)
+    double T3[kBands];     // End_Trace energy threshold (TaskAll:355)
+    float kexp2[kBands];   // -0.1 * atten * log2(10): 10^(-0.1 a x) = exp2(k x)
+};
+
 template <class R> struct DevRecv {
     const typename V4<R>::T *ctr;  // [R] centre (w unused)
     int n;
+    int masked;                    // 1: test only the receivers of the walk's voxel masks
     R r2;
     double sizemod;   // 1 / (pi r^3)
     double c, inv_c;  // speed of sound and 1/c as Spherical_Receiver holds them
-    double atten[kBands];
-    float kexp2[kBands];  // -0.1 * atten * log2(10): 10^(-0.1 a x) = exp2(k x)
+    const BandTab *band;  // per-band constants in HBM (indexed by a runtime band)
     double rho_c;
     int fs, sct;
     double *hist;         // [kPlanes][n][8][sct]
@@ -47,8 +64,11 @@ template <class R> struct DevRecv {
 struct TraceConst {
     int mode, oct_lo, oct_hi, is_order, h, dir_mode;
     int p1_max, max_bounces;
+    int walk_steps;  // voxels a lane's walk advances per loop step of k_octave
+    int shade_min;   // lanes that must be waiting before k_octave shades
+    int walk_flat;   // 1: flattened walk (walk_steps counts tests + voxels)
     double cotime, T1;
-    double T2[kBands], T3[kBands];
+    const BandTab *band;
     double power[kBands];
     double src[3];
     uint32_t key0, key1;
@@ -105,11 +125,16 @@ enum : int {
     C_SPILL,
     C_RESUME,
     C_DROP,
+    C_ITER,      // persistent-loop steps (per wave)
+    C_VOXEL,     // voxels visited by walks (per lane)
+    C_TEST,      // triangle tests started (per lane)
+    C_SHADE,     // shading passes (per wave)
+    C_SHADED,    // lanes shaded (per lane)
     C_N
 };
 
 struct DevCounters {
-    unsigned long long v[16];
+    unsigned long long v[24];
     double lost;
     double pad;
 };
@@ -129,6 +154,6 @@ hipError_t launch_octave(hipStream_t st, int grid, const TraceConst &tc, const D
                          ORay<R> *stacks, int stack_cap, ORay<R> *dump, unsigned *dump_count,
                          unsigned dump_cap, int min_iters, DevCounters *cnt);
 
-template <class R> int octave_blocks_per_cu();
+template <class R> int octave_blocks_per_cu(int lds_bytes);
 
 }  // namespace pach

@@ -1,0 +1,378 @@
+"""The reference's tracer API, backed by libpach.so on one MI355X.
+
+Names, argument meanings and call order follow Pachyderm_Acoustic_Universal:
+
+    scene = Polygon_Scene(20, 50, 1013.25, 0, False, True)      # Classes_Scenes.cs:442
+    scene.Construct(model, materials, scatterings, transmissions)  # :447-630
+    scene.partition()                                            # :709-755
+    src = GeodesicSource([120] * 8, (2, 3, 1.5), 0, False)       # Classes_Sources.cs:278
+    rec = Receiver_Bank(points, src, scene, 1000, 2000.0)        # Classes_Receivers.cs:139
+    rt = SplitRayTracer(src, rec, scene, 2.0 * scene.Sound_speed(0), [0, 7], -1, 100000, None)
+    rt.Begin()                                                   # TaskAll:209
+    while rt.ThreadState() == "Running": ...                     # TaskAll:246
+    rt.Combine_ThreadLocal_Results()                             # TaskAll:768
+    etc = rt.GetReceiver.GetEnergyHistogram(4, 0, 0)             # Classes_Receivers.cs:432
+
+Every trace runs through the C ABI; there is no CPU path.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import _abi as A
+
+Running, Stopped = "Running", "Stopped"
+
+
+# ---------------------------------------------------------------- materials
+class Basic_Material:
+    """Octave absorption coefficients (Classes_Materials.cs:60-300)."""
+
+    def __init__(self, ABS):
+        a = np.asarray(ABS, dtype=np.float64)
+        if a.size == 24:  # third octaves averaged per octave (:128-139)
+            a = a.reshape(8, 3).mean(1)
+        if a.size != 8:
+            raise ValueError("Basic_Material needs 8 octave (or 24 third-octave) coefficients")
+        self.Abs = a
+
+    def Coefficient_A_Broad(self, Octave=None):
+        return self.Abs.copy() if Octave is None else float(self.Abs[Octave])
+
+
+class Lambert_Scattering:
+    """Octave scattering coefficients (Classes_Materials.cs:4463-4485)."""
+
+    def __init__(self, Scattering):
+        s = np.asarray(Scattering, dtype=np.float64)
+        if s.size != 8:
+            raise ValueError("Lambert_Scattering needs 8 octave coefficients")
+        self.Scat = s
+
+    def Coefficient(self, octave=None):
+        return self.Scat.copy() if octave is None else float(self.Scat[octave])
+
+
+# ---------------------------------------------------------------- scene
+class Polygon_Scene:
+    """Polygon scene with a uniform medium (Classes_Scenes.cs:414-1031).
+
+    Pa is the pressure in hPa, as Uniform_Medium expects it (Classes_Medium_Props.cs:288-289)."""
+
+    def __init__(self, Temp, hr, Pa, Air_Choice=0, EdgeCorrection=False, IsAcoustic=True):
+        self.TempC, self.hr, self.Pa = float(Temp), float(hr), float(Pa)
+        self.atten, self.c, self.rho = A.uniform_medium(Temp, hr, Pa, Air_Choice, EdgeCorrection)
+        self._tris = None
+        self.grid = (0, 0, 0)
+        self.Valid = False
+
+    def Construct(self, Model, Mat, Scat, Trans=None):
+        """Model[object][polygon] = list of (x, y, z) vertices; one material,
+        scattering and transmission entry per object, copied to each polygon
+        (:484-487).  Polygons are fan-triangulated; triangles keep the polygon's
+        index as Surf_ID."""
+        if len(Mat) != len(Model) and len(Scat) != len(Model):
+            raise ValueError("The number of material codes must match the number of objects in the model")
+        tris, surf, alpha, scat, trans, areas = [], [], [], [], [], []
+        pid = 0
+        for i, obj in enumerate(Model):
+            tr = np.zeros(8) if Trans is None else np.asarray(Trans[i], dtype=np.float64)
+            for poly in obj:
+                P = np.asarray(poly, dtype=np.float64)
+                if P.shape[0] < 3:
+                    raise ValueError("polygon with fewer than 3 vertices")
+                area = 0.0
+                for k in range(1, P.shape[0] - 1):
+                    tris.append([P[0], P[k], P[k + 1]])
+                    surf.append(pid)
+                    area += 0.5 * np.linalg.norm(np.cross(P[k] - P[0], P[k + 1] - P[0]))
+                alpha.append(Mat[i].Coefficient_A_Broad())
+                scat.append(Scat[i].Coefficient())
+                trans.append(tr)
+                areas.append(area)
+                pid += 1
+        self._tris = np.asarray(tris)
+        self._surf = np.asarray(surf, dtype=np.int32)
+        self._alpha, self._scat, self._trans = map(np.asarray, (alpha, scat, trans))
+        self._area = np.asarray(areas)
+        self.Valid = True
+
+    @classmethod
+    def from_room(cls, room, Temp=20.0, hr=50.0, Pa=1013.25, Air_Choice=0):
+        """Scene of a rooms.Room (already triangulated, materials per polygon)."""
+        s = cls(Temp, hr, Pa, Air_Choice)
+        s._tris = np.asarray(room.verts, dtype=np.float64)
+        s._surf = np.asarray(room.surf, dtype=np.int32)
+        s._alpha, s._scat, s._trans = (np.asarray(x, dtype=np.float64) for x in (room.alpha, room.scat, room.trans))
+        s._area = np.zeros(room.n_surfaces)
+        for t, p in zip(s._tris, s._surf):
+            s._area[p] += 0.5 * np.linalg.norm(np.cross(t[1] - t[0], t[2] - t[0]))
+        s.Valid = True
+        return s
+
+    def partition(self, SP_Param=None, Max_Polys=None):
+        """Voxel grid of SP_Param^3 voxels (None: sized from the triangle count)."""
+        if SP_Param is not None:
+            self.grid = tuple(SP_Param) if hasattr(SP_Param, "__len__") else (SP_Param,) * 3
+
+    def Count(self):
+        return int(self._alpha.shape[0])
+
+    def SurfaceArea(self, x):
+        return float(self._area[x])
+
+    def Sound_speed(self, arg=0):
+        return self.c
+
+    def Attenuation(self, arg=0):
+        return self.atten.copy()
+
+    def Rho(self, arg=0):
+        return self.rho
+
+    def Rho_C(self, arg=0):
+        return self.rho * self.c
+
+
+# ---------------------------------------------------------------- sources
+class Source:
+    """Omni point source; power W = 1e-12 10^(0.1 SWL) per band (Classes_Sources.cs:50-104)."""
+
+    dir_mode = A.DIR_GEODESIC
+
+    def __init__(self, power_In_db, Source, ID=0, Third_Octave=False):
+        if Third_Octave:
+            raise NotImplementedError("third-octave sources are not traced by this library")
+        self.SPL = np.asarray(power_In_db, dtype=np.float64)
+        self.SourcePower = 1e-12 * 10 ** (0.1 * self.SPL)
+        self.Center = np.asarray(Source, dtype=np.float64)
+        self.S_ID = ID
+        self.dirs = None
+
+    @property
+    def Origin(self):
+        return self.Center
+
+    @property
+    def SoundPower(self):
+        return self.SourcePower
+
+    def SWL(self, octave=None):
+        if octave is None:
+            return self.SPL.copy()
+        if octave < 8:
+            return float(self.SPL[octave])
+        return 10 * np.log10(np.sum(10 ** (0.1 * self.SPL)))
+
+    def Revolution_Period(self):
+        return 80
+
+
+class GeodesicSource(Source):
+    """Stratified over the 80 faces of an order-1 geodesic sphere (Classes_Sources.cs:272-404)."""
+
+    dir_mode = A.DIR_GEODESIC
+
+
+class RandomSource(Source):
+    """theta, phi ~ U(0, 2 pi) (Classes_Sources.cs:238-266)."""
+
+    dir_mode = A.DIR_RANDOM
+
+
+class SuppliedDirectionSource(Source):
+    """Source ray i leaves along dirs[i % n] (the deterministic specular mode's seeded list)."""
+
+    dir_mode = A.DIR_SUPPLIED
+
+    def __init__(self, power_In_db, Source, dirs, ID=0):
+        super().__init__(power_In_db, Source, ID)
+        self.dirs = np.asarray(dirs, dtype=np.float64).reshape(-1, 3)
+
+
+# ---------------------------------------------------------------- receivers
+class Receiver_Bank:
+    """Spherical receivers with directional histograms (Classes_Receivers.cs:39-693).
+
+    radius: the reference fixes 1 m (:751); other radii keep the expected
+    energy unchanged (SizeMod = 1 / (pi r^3))."""
+
+    def __init__(self, Pt, SrcPT, Sc, SampleRate_in, COTime_in, Type="Stationary", third_Oct=False,
+                 radius=1.0):
+        if Type != "Stationary" or third_Oct:
+            raise NotImplementedError("only stationary octave-band receivers are traced")
+        self.Origins = np.asarray(Pt, dtype=np.float64).reshape(-1, 3)
+        self.SampleRate = int(SampleRate_in)
+        self.CutOffTime = float(COTime_in)
+        self.SampleCT = int(np.floor(COTime_in * SampleRate_in / 1000))
+        self.radius = float(radius)
+        self.SWL = SrcPT.SWL()
+        self.scene = Sc
+        self.planes = np.zeros((A.NPLANES, len(self.Origins), 8, self.SampleCT))
+        self.scale_n = 1.0
+
+    @property
+    def Count(self):
+        return len(self.Origins)
+
+    def Origin(self, i):
+        return self.Origins[i]
+
+    def Scale(self, Ray_CT):
+        """Receiver_Bank.Scale (Classes_Receivers.cs:676-679, :1774-1786)."""
+        n = float(Ray_CT)
+        self.planes /= n
+        self.planes[A.PLANES["pressure"]] *= np.sqrt(n)
+
+    def GetEnergyHistogram(self, Octave, Delay_ms, Rec_Index):
+        """Energy histogram padded by the delay (:432-438); Octave 8 sums the bands."""
+        E = self.planes[0, Rec_Index]
+        h = E.sum(0) if Octave >= 8 else E[Octave]
+        pad = int(np.floor(Delay_ms / 1000.0 * self.SampleRate))
+        return np.concatenate([np.zeros(pad), h])
+
+    def GetPressureHistogram(self, Octave, Rec_Index):
+        return self.planes[1, Rec_Index, Octave].copy()
+
+    def Directions_Pos(self, Octave, Rec_Index):
+        return self.planes[2:5, Rec_Index, Octave].copy()
+
+    def Directions_Neg(self, Octave, Rec_Index):
+        return self.planes[5:8, Rec_Index, Octave].copy()
+
+
+# ---------------------------------------------------------------- tracer
+class Simulation_Type:
+    """Abstract simulation contract (Classes_Simulation_Type.cs:27-35)."""
+
+    def Sim_Type(self):
+        raise NotImplementedError
+
+    def Begin(self):
+        raise NotImplementedError
+
+    def ProgressMsg(self):
+        raise NotImplementedError
+
+    def ThreadState(self):
+        raise NotImplementedError
+
+    def Combine_ThreadLocal_Results(self):
+        raise NotImplementedError
+
+
+class SplitRayTracer(Simulation_Type):
+    """Stochastic split ray tracer (TaskAll:33-782) on one MI355X.
+
+    cutoffTime is a path length in metres (the ctor divides it by the speed of
+    sound, TaskAll:83).  rayCountIn must be positive: the convergence mode
+    (rayCount < 1, TaskAll:288-329) is not provided.  Extra keyword arguments:
+    precision ("fp32" traversal, or "fp64"), mode ("stochastic" or "specular"),
+    seed (the reference seeds from the clock; None does the same), device."""
+
+    def __init__(self, sourceIn, receiverIn, roomIn, cutoffTime, octaveRange, isOrderIn, rayCountIn,
+                 Vis_Feedback=None, *, precision="fp32", mode="stochastic", seed=None, device=0,
+                 tracer=None, batch_rays=0):
+        if rayCountIn < 1:
+            raise NotImplementedError("convergence-driven tracing (rayCount < 1) is not provided")
+        self.Source, self.RecMain, self.Room = sourceIn, receiverIn, roomIn
+        self.COTime = cutoffTime / roomIn.Sound_speed(0)
+        self._cutoff = float(cutoffTime)
+        self._octaves = list(range(octaveRange[0], octaveRange[1] + 1))
+        self.IS_Order = int(isOrderIn)
+        self.Raycount = int(rayCountIn)
+        self.precision = A.PREC_FP64 if precision == "fp64" else A.PREC_FP32
+        self.mode = A.MODE_SPECULAR if mode == "specular" else A.MODE_STOCHASTIC
+        self.seed = int(time.time_ns() & 0xFFFFFFFFFFFF) if seed is None else int(seed)
+        self._tr = tracer or A.Tracer(device)
+        self._own = tracer is None
+        self._st = None
+        self._lost_percent = 0.0
+        self._stats = None
+        self._started = False
+        self._batch = int(batch_rays)
+
+    # -- Simulation_Type
+    def Sim_Type(self):
+        return "Stochastic Ray Tracing"
+
+    def Begin(self):
+        r, s, rb = self.Room, self.Source, self.RecMain
+        self._tr.upload_scene(r._tris, r._surf, r._alpha, r._scat, r._trans, r.grid)
+        self._tr.set_source(s.Origin, s.SoundPower, s.dir_mode, s.dirs)
+        self._tr.set_receivers(rb.Origins, rb.radius, r.atten, r.c, r.rho * r.c, rb.SampleRate,
+                               rb.SampleCT)
+        self._st = time.time()
+        self._started = True
+        self._tr.trace(rays=self.Raycount, cutoff_m=self._cutoff, seed=self.seed, mode=self.mode,
+                       precision=self.precision, oct_lo=self._octaves[0], oct_hi=self._octaves[-1],
+                       is_order=self.IS_Order, batch_rays=self._batch, async_=True)
+
+    def ThreadState(self):
+        if not self._started:
+            return Stopped
+        return Running if self._tr.poll()["running"] else Stopped
+
+    def ProgressMsg(self):
+        p = self._tr.poll()
+        if p["rays_done"] == 0 and p["running"]:
+            return "Preparing Threads"
+        return (f"Calculating Ray {p['rays_done']} of {p['rays_total']} "
+                f"(sub-ray {p['octave_rays']}).")
+
+    def Conclude_Simulation(self):
+        """Cancels the trace; the rays already traced are kept (TaskAll:254-257, :276-280)."""
+        self._tr.cancel()
+
+    def Combine_ThreadLocal_Results(self):
+        rc = self._tr.wait()
+        if rc not in (0, -5):
+            raise A.PachError(rc, A.lib().pach_last_error(self._tr.h).decode())
+        self._stats = self._tr.stats()
+        n = max(self._stats["rays_done"], 1)
+        self.RecMain.planes = self._tr.planes(scaled=False)
+        self.RecMain.Scale(n)
+        self._lost_percent = self._tr.percent_lost()
+
+    @property
+    def GetReceiver(self):
+        return self.RecMain
+
+    @property
+    def PercentLost(self):
+        return self._lost_percent
+
+    @property
+    def Stats(self):
+        return self._stats
+
+    def close(self):
+        if self._own:
+            self._tr.close()
+
+
+# ---------------------------------------------------------------- decay-curve integration
+class AcousticalMath:
+    """Classes_Math.cs:272-585, FP64 in libpach.so."""
+
+    @staticmethod
+    def SoundSpeed(Air_Temp):
+        return 331 + 0.6 * Air_Temp
+
+    @staticmethod
+    def Schroeder_Integral(etc):
+        return A.schroeder_integral(etc)
+
+    @staticmethod
+    def T_X(Schroeder_Integral, Decay_Depth, sample_f):
+        return A.t_x(Schroeder_Integral, Decay_Depth, sample_f)
+
+    @staticmethod
+    def EarlyDecayTime(Schroeder_Integral, sample_f):
+        return A.early_decay_time(Schroeder_Integral, sample_f)
+
+    @staticmethod
+    def Clarity(etc, sample_f, seconds, startTime, pressure=False):
+        return A.clarity(etc, sample_f, seconds, startTime, pressure)

@@ -1,0 +1,149 @@
+"""Host-side checks of libpach.so that need no GPU: symbol exports, argument
+validation, the medium and decay-curve entry points, and the voxel grid the
+scene upload builds."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from pachydermacoustic_universal_amd import _abi as A
+from pachydermacoustic_universal_amd import rooms
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "pach.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pach_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported():
+    lib = A.lib()
+    names = declared_symbols()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+    # and the ctypes table covers every declaration with a signature
+    assert set(n for n, _, _ in A.SIGNATURES) == set(names)
+
+
+def test_abi_version():
+    assert A.lib().pach_abi_version() == 1
+
+
+def test_null_arguments_are_rejected():
+    lib = A.lib()
+    assert lib.pach_create(None, 0) == -1
+    assert lib.pach_trace(None, None) == -1
+    assert lib.pach_poll(None, None) == -1
+    assert lib.pach_histogram(None, 0, None, 0) == -1
+    assert lib.pach_schroeder_integral(None, 3, None) == -1
+    out = np.zeros(8)
+    assert lib.pach_uniform_medium(7, 20, 50, 1013.25, 0, out.ctypes.data_as(C.POINTER(C.c_double)),
+                                   None, None) == -1
+
+
+def gpu_count():
+    try:
+        hip = C.CDLL("libamdhip64.so")
+    except OSError:
+        return 0
+    n = C.c_int(0)
+    return n.value if hip.hipGetDeviceCount(C.byref(n)) == 0 else 0
+
+
+def test_create_without_gpu_fails_cleanly():
+    if gpu_count() > 0:
+        pytest.skip("a GPU is present")
+    h = C.c_void_p()
+    assert A.lib().pach_create(C.byref(h), 0) == -2
+    assert not h.value
+
+
+def test_uniform_medium_matches_oracle():
+    for air, T, hr, pa, edge in [(0, 20, 50, 1013.25, 0), (0, 30, 70, 1000.0, 0), (1, 25, 40, 1013.25, 0),
+                                 (2, 20, 60, 1013.25, 1), (2, 10, 80, 1013.25, 0)]:
+        at, c, rho = A.uniform_medium(T, hr, pa, air, edge)
+        np.testing.assert_allclose(at, O.attenuation(air, T, hr, pa, edge), rtol=1e-12, atol=0)
+        assert c == O.sound_speed(T)
+        assert abs(rho - O.density(T, hr, pa)) < 1e-14
+
+
+def test_decay_math_matches_oracle():
+    rng = np.random.default_rng(3)
+    for _ in range(5):
+        n = 3000
+        etc = np.exp(-rng.uniform(5, 30) * np.arange(n) / 1000) * rng.uniform(0.5, 1.5, n)
+        sch = A.schroeder_integral(etc)
+        np.testing.assert_allclose(sch, O.schroeder(etc), rtol=1e-13, atol=1e-16)
+        assert abs(A.t_x(sch, 30, 1000) - O.t_x(sch, 30, 1000)) < 1e-12
+        assert abs(A.t_x(sch, 20, 1000) - O.t_x(sch, 20, 1000)) < 1e-12
+        assert abs(A.early_decay_time(sch, 1000) - O.edt(sch, 1000)) < 1e-12
+        assert abs(A.clarity(etc, 1000.0, 0.08, 0.0) - O.clarity(etc, 1000.0, 0.08, 0.0)) < 1e-12
+        assert abs(A.clarity(etc, 1000.0, 0.05, 0.013) - O.clarity(etc, 1000.0, 0.05, 0.013)) < 1e-12
+
+
+def build_grid(verts, dims=(0, 0, 0)):
+    lib = A.lib()
+    v = np.ascontiguousarray(verts, dtype=np.float64)
+    d = (C.c_int * 3)(*dims)
+    org = np.zeros(3)
+    cell = np.zeros(3)
+    nl = C.c_int64(0)
+    dp = C.POINTER(C.c_double)
+    u32 = C.POINTER(C.c_uint32)
+    assert lib.pach_grid_build(v.ctypes.data_as(dp), v.shape[0], d, org.ctypes.data_as(dp),
+                               cell.ctypes.data_as(dp), None, None, C.byref(nl), None) == 0
+    G = tuple(d)
+    ncell = G[0] * G[1] * G[2]
+    ranges = np.zeros((ncell, 2), np.uint32)
+    tris = np.zeros(max(nl.value, 1), np.uint32)
+    dist = np.zeros(ncell, np.uint8)
+    assert lib.pach_grid_build(v.ctypes.data_as(dp), v.shape[0], d, org.ctypes.data_as(dp),
+                               cell.ctypes.data_as(dp), ranges.ctypes.data_as(u32),
+                               tris.ctypes.data_as(u32), C.byref(nl),
+                               dist.ctypes.data_as(C.POINTER(C.c_uint8))) == 0
+    return G, org, cell, ranges, tris[:nl.value], dist
+
+
+@pytest.mark.parametrize("room_fn,dims", [(rooms.shoebox, (0, 0, 0)), (rooms.fan_hall, (32, 32, 32)),
+                                          (rooms.fan_hall, (0, 0, 0))])
+def test_grid_lists_cover_every_triangle_point(room_fn, dims):
+    room = room_fn()
+    G, org, cell, ranges, tris, dist = build_grid(room.verts, dims)
+    if dims[0]:
+        assert G == dims
+    lists = [set(tris[a:a + n].tolist()) for a, n in ranges]
+    rng = np.random.default_rng(0)
+    for t, tv in enumerate(room.verts):
+        # random barycentric points plus the vertices and edge midpoints
+        u = rng.random((64, 2))
+        u = np.where(u.sum(1, keepdims=True) > 1, 1 - u, u)
+        pts = tv[0] + u[:, :1] * (tv[1] - tv[0]) + u[:, 1:] * (tv[2] - tv[0])
+        pts = np.vstack([pts, tv, (tv + np.roll(tv, 1, 0)) / 2])
+        idx = np.clip(np.floor((pts - org) / cell).astype(int), 0, np.array(G) - 1)
+        for x, y, z in idx:
+            assert t in lists[(z * G[1] + y) * G[0] + x]
+
+
+def test_grid_distance_field_is_linf_distance():
+    room = rooms.fan_hall()
+    G, org, cell, ranges, tris, dist = build_grid(room.verts, (12, 14, 8))
+    occ = np.argwhere((ranges[:, 1] > 0).reshape(G[2], G[1], G[0]))
+    cells = np.argwhere(np.ones((G[2], G[1], G[0]), bool))
+    # brute-force Chebyshev distance to the nearest occupied voxel
+    d = np.abs(cells[:, None, :] - occ[None, :, :]).max(2).min(1)
+    np.testing.assert_array_equal(dist, np.minimum(d, 255))
+
+
+def test_grid_rejects_bad_input():
+    v = np.zeros((1, 3, 3))
+    v[0, 0, 0] = np.nan
+    d = (C.c_int * 3)(0, 0, 0)
+    nl = C.c_int64(0)
+    assert A.lib().pach_grid_build(v.ctypes.data_as(C.POINTER(C.c_double)), 1, d, None, None, None,
+                                   None, C.byref(nl), None) == -1

@@ -1,0 +1,76 @@
+"""The reference's SplitRayTracer call sequence, traced on the GPU and checked
+against the FP64 oracle (FP64 geometry: bins within 1e-5 relative)."""
+import time
+
+import numpy as np
+import pytest
+
+import pyoracle as O
+from pachydermacoustic_universal_amd import (AcousticalMath, Basic_Material, GeodesicSource,
+                                             Lambert_Scattering, Polygon_Scene, Receiver_Bank,
+                                             SplitRayTracer, rooms)
+
+pytestmark = pytest.mark.gpu
+
+
+def shoebox_model():
+    """The 10 x 8 x 4 m box as one object of six quad polygons."""
+    x1, y1, z1 = 10.0, 8.0, 4.0
+    P = np.array([[0, 0, 0], [x1, 0, 0], [x1, y1, 0], [0, y1, 0],
+                  [0, 0, z1], [x1, 0, z1], [x1, y1, z1], [0, y1, z1]])
+    faces = [[0, 1, 2, 3], [4, 7, 6, 5], [0, 4, 5, 1], [3, 2, 6, 7], [0, 3, 7, 4], [1, 5, 6, 2]]
+    return [[P[f] for f in faces]]
+
+
+def run(rays, seed, cancel_after=None, batch=0):
+    scene = Polygon_Scene(20, 50, 1013.25, 0, False, True)
+    scene.Construct(shoebox_model(), [Basic_Material([0.1] * 8)], [Lambert_Scattering([0.2] * 8)],
+                    [np.zeros(8)])
+    scene.partition()
+    src = GeodesicSource([120] * 8, (2, 3, 1.5), 0, False)
+    recs = rooms.shoebox().receivers
+    rb = Receiver_Bank(recs, src, scene, 1000, 600.0, radius=0.5)
+    rt = SplitRayTracer(src, rb, scene, 0.6 * scene.Sound_speed(0), [0, 7], -1, rays, None,
+                        precision="fp64", seed=seed, batch_rays=batch)
+    rt.Begin()
+    if cancel_after is not None:
+        time.sleep(cancel_after)
+        rt.Conclude_Simulation()
+    while rt.ThreadState() == "Running":
+        time.sleep(0.01)
+    rt.Combine_ThreadLocal_Results()
+    return rt, scene
+
+
+def test_split_ray_tracer_call_sequence_matches_oracle():
+    rt, scene = run(8, seed=21)
+    rb = rt.GetReceiver
+    assert rt.Sim_Type() == "Stochastic Ray Tracing"
+    ora, cnt = O.trace(rooms.shoebox(), rays=8, cutoff_s=0.6, seed=21, threads=8,
+                       medium=O.Medium(c=scene.c, atten=scene.atten, rho=scene.rho))
+    ref = O.scale(ora, 8)
+    for r in range(3):
+        for o in range(8):
+            g = rb.GetEnergyHistogram(o, 0, r)
+            np.testing.assert_allclose(g, ref[0, r, o], rtol=1e-5, atol=1e-12 * ref[0].max())
+    np.testing.assert_allclose(rb.GetEnergyHistogram(8, 5, 1)[5:], ref[0, 1].sum(0), rtol=1e-5,
+                               atol=1e-12 * ref[0].max())
+    assert abs(rt.PercentLost - 100 * cnt["lost"] / cnt["e_init"]) < 1e-9
+    etc = rb.GetEnergyHistogram(3, 0, 0)
+    sch = AcousticalMath.Schroeder_Integral(etc)
+    assert 0.2 < AcousticalMath.T_X(sch, 30, 1000) < 3.0
+    rt.close()
+
+
+def test_cancel_keeps_completed_rays_only():
+    rt, scene = run(40000, seed=5, cancel_after=0.05, batch=16)
+    st = rt.Stats
+    k = st["rays_done"]
+    assert 0 <= k < 40000
+    if k == 0:
+        pytest.skip("cancelled before the first batch")
+    ora, _ = O.trace(rooms.shoebox(), rays=k, cutoff_s=0.6, seed=5, threads=8,
+                     medium=O.Medium(c=scene.c, atten=scene.atten, rho=scene.rho))
+    ref = O.scale(ora, k)[0]
+    np.testing.assert_allclose(rt.GetReceiver.planes[0], ref, rtol=1e-5, atol=1e-12 * ref.max())
+    rt.close()

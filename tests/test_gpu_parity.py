@@ -113,3 +113,22 @@ def test_fp32_matches_fp64_statistics(tracer, medium):
     assert rel.max() < 0.02, rel.max()
     assert np.abs(p32[..., 2] - p64[..., 2]).max() < 0.2
     assert s32["lost"] / s32["e_init"] < 1e-5
+
+
+@pytest.mark.parametrize("knobs", [{"walk_flat": 0, "walk_steps": 4}, {"receiver_masks": 0, "dist_lds": 0},
+                                   {"walk_steps": 1, "shade_min": 64}])
+def test_launch_tuning_does_not_change_results(tracer, medium, knobs):
+    room = rooms.fan_hall()
+    try:
+        for k, v in knobs.items():
+            tracer.tune(k, v)
+        st, g, cnt, ora = run_both(tracer, medium, room, rays=10, cut=0.8, seed=13,
+                                   mode=A.MODE_STOCHASTIC, precision=A.PREC_FP64)
+    finally:
+        for k, v in {"walk_flat": 1, "walk_steps": 12, "receiver_masks": 1, "dist_lds": 1,
+                     "shade_min": 16}.items():
+            tracer.tune(k, v)
+    assert st["bounces"] == bounces(cnt)
+    assert st["deposits"] == cnt["deposits"]
+    for k in range(A.NPLANES):
+        assert_bins(g[k], ora[k], 1e-5, f"plane {k}")
