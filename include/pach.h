@@ -161,6 +161,17 @@ int pach_receivers_set(pach_ctx *ctx, const double *centers, int R, double radiu
                        const double atten_db_per_m[8], double c, double rho_c,
                        int sample_rate, int sample_ct);
 
+/* Batched rooms (no counterpart in the reference: several independent rooms
+ * traced in one launch).  The uploaded scene holds the rooms side by side
+ * (disjoint in space), their polygons room by room (surf_count[i] polygons for
+ * room i) and the receiver list room by room (rec_count[i] receivers).  Room i
+ * has its own source (src[i][3], power[i][8] in W) and traces rays[i] source
+ * rays; pach_trace's ray_count must then be 0 or their sum.  Histograms keep
+ * the receiver order; scaled copies divide each room by its own ray count.
+ * n = 0 returns to one room (pach_source_set's source, all receivers). */
+int pach_rooms_set(pach_ctx *ctx, int n, const double *src, const double *power, const int64_t *rays,
+                   const int *surf_count, const int *rec_count);
+
 /* Launch tuning of later traces (no counterpart in the reference; results do
  * not depend on it).  Keys: "dist_lds" (0/1): stage the voxel distance field
  * in LDS when it fits; "receiver_masks" (0/1): test only receivers whose

@@ -71,6 +71,7 @@ SIGNATURES = [
     ("pach_source_set", C.c_int, [C.c_void_p, _dp, _dp, C.c_int, _dp, C.c_int64]),
     ("pach_receivers_set", C.c_int, [C.c_void_p, _dp, C.c_int, C.c_double, _dp, C.c_double,
                                      C.c_double, C.c_int, C.c_int]),
+    ("pach_rooms_set", C.c_int, [C.c_void_p, C.c_int, _dp, _dp, C.POINTER(C.c_int64), _ip, _ip]),
     ("pach_tune", C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
     ("pach_trace", C.c_int, [C.c_void_p, C.POINTER(Params)]),
     ("pach_poll", C.c_int, [C.c_void_p, C.POINTER(Progress)]),
@@ -228,6 +229,18 @@ class Tracer:
         self._chk(rc)
         return rc
 
+    def set_rooms(self, src, power, rays, surf_count, rec_count):
+        s = np.ascontiguousarray(src, dtype=np.float64).reshape(-1, 3)
+        pw = np.ascontiguousarray(power, dtype=np.float64).reshape(-1, 8)
+        ry = np.ascontiguousarray(rays, dtype=np.int64)
+        sc = np.ascontiguousarray(surf_count, dtype=np.int32)
+        rc = np.ascontiguousarray(rec_count, dtype=np.int32)
+        self._chk(lib().pach_rooms_set(self.h, s.shape[0], _p(s), _p(pw), ry.ctypes.data_as(C.POINTER(C.c_int64)),
+                                       sc.ctypes.data_as(_ip), rc.ctypes.data_as(_ip)))
+
+    def clear_rooms(self):
+        self._chk(lib().pach_rooms_set(self.h, 0, None, None, None, None, None))
+
     def tune(self, key, value):
         self._chk(lib().pach_tune(self.h, key.encode(), int(value)))
 
@@ -272,3 +285,33 @@ def trace_room(tr: Tracer, room, *, rays, cutoff_s, seed=1, mode=MODE_STOCHASTIC
     tr.set_receivers(room.receivers, room.radius, at, c, c * rho, sample_rate, sct)
     tr.trace(rays=rays, cutoff_m=cutoff_s * c, seed=seed, mode=mode, precision=precision, **kw)
     return tr.stats()
+
+
+def trace_batch(tr: Tracer, packed, rays, *, cutoff_s, seed=1, mode=MODE_STOCHASTIC,
+                precision=PREC_FP32, sample_rate=1000, grid=(0, 0, 0), power_w=None, medium=None, **kw):
+    """Trace rooms.PackedRooms in one launch sequence; rays[i] source rays in room i.
+    Returns the context's stats; per-room histograms come from batch_histograms."""
+    at, c, rho = medium if medium is not None else uniform_medium()
+    room = packed.room
+    tr.upload_scene(room.verts, room.surf, room.alpha, room.scat, room.trans, grid)
+    n = len(rays)
+    pw = np.ones((n, 8)) if power_w is None else np.broadcast_to(power_w, (n, 8))
+    tr.set_source(packed.sources[0], pw[0], DIR_GEODESIC)
+    sct = int(np.floor(cutoff_s * 1000 * sample_rate / 1000))
+    tr.set_receivers(room.receivers, room.radius, at, c, c * rho, sample_rate, sct)
+    tr.set_rooms(packed.sources, pw, rays, packed.surf_count, packed.rec_count)
+    try:
+        tr.trace(rays=0, cutoff_m=cutoff_s * c, seed=seed, mode=mode, precision=precision, **kw)
+        return tr.stats()
+    finally:
+        tr.clear_rooms()
+
+
+def batch_histograms(tr: Tracer, packed, plane=0, scaled=True):
+    """List of per-room histograms [rec_count_i, 8, sample_ct] of the last batch."""
+    H = tr.histogram(plane, scaled)
+    out, r0 = [], 0
+    for k in packed.rec_count:
+        out.append(H[r0:r0 + int(k)])
+        r0 += int(k)
+    return out

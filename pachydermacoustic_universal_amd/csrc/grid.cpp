@@ -81,8 +81,8 @@ void linf_distance(const int G[3], std::vector<uint8_t> &dist)
     }
 }
 
-void receiver_cells(const Grid &g, const double *centers, int R, double radius, std::vector<uint64_t> &mask,
-                    std::vector<uint8_t> &dist)
+void receiver_cells(const Grid &g, const double *centers, int R, const int *bit, double radius,
+                    std::vector<uint64_t> &mask, std::vector<uint8_t> &dist)
 {
     // voxels whose box comes within radius (+ 1e-3 voxel) of a centre: the walk
     // collects their receiver bits, and they count as occupied for the jumps
@@ -93,7 +93,8 @@ void receiver_cells(const Grid &g, const double *centers, int R, double radius, 
         if (g.ranges[2 * i + 1] == 0) dist[i] = 255;
     const double pad = 1e-3 * std::max({g.cell[0], g.cell[1], g.cell[2]});
     const double rr = radius + pad;
-    for (int r = 0; r < R && r < 64; ++r) {
+    for (int r = 0; r < R; ++r) {
+        if (bit[r] < 0 || bit[r] > 63) continue;
         const double *C = centers + 3 * r;
         int lo[3], hi[3];
         for (int k = 0; k < 3; ++k) {
@@ -112,7 +113,7 @@ void receiver_cells(const Grid &g, const double *centers, int R, double radius, 
                     }
                     if (d2 <= rr * rr) {
                         const size_t i = ((size_t)z * g.dims[1] + y) * g.dims[0] + x;
-                        mask[i] |= 1ull << r;
+                        mask[i] |= 1ull << bit[r];
                         dist[i] = 0;
                     }
                 }

@@ -20,10 +20,11 @@ struct Grid {
 bool tri_box_overlap(const double *bc, const double *bh, const double *tv);
 // L-inf distance (voxels, capped at 255) to the nearest voxel whose dist is 0
 void linf_distance(const int dims[3], std::vector<uint8_t> &dist);
-// per-voxel bit masks of the (first 64) receiver spheres touching each voxel, and
-// the grid's distance field with those voxels counted as occupied
-void receiver_cells(const Grid &g, const double *centers, int R, double radius, std::vector<uint64_t> &mask,
-                    std::vector<uint8_t> &dist);
+// per-voxel bit masks of the receiver spheres touching each voxel (receiver r
+// sets bit[r], 0..63), and the grid's distance field with those voxels counted
+// as occupied
+void receiver_cells(const Grid &g, const double *centers, int R, const int *bit, double radius,
+                    std::vector<uint64_t> &mask, std::vector<uint8_t> &dist);
 // dims_in: requested voxel counts (any <= 0: chosen from the triangle count)
 int build_grid(const double *verts, int T, const int dims_in[3], Grid &g, std::string &err);
 

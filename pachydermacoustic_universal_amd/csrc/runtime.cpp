@@ -88,6 +88,17 @@ struct pach_ctx {
     double *d_hist = nullptr;
     size_t hist_n = 0;  // per plane
 
+    // rooms (pach_rooms_set); empty: one room made of the whole scene
+    struct RoomIn {
+        double src[3], power[8];
+        int64_t rays;
+        int surf_count, rec_count;
+    };
+    std::vector<RoomIn> rooms;
+    std::vector<RoomTab> room_tab;  // last trace's room table (ray ranges for scaling)
+    RoomTab *d_rooms = nullptr;
+    int d_rooms_cap = 0;
+
     // counters and small device scalars
     DevCounters *d_cnt = nullptr;
     unsigned *d_scal = nullptr;  // [0] p1_count [1] res_count [2] in_head [3..4] dump_count[2]
@@ -292,7 +303,7 @@ extern "C" void pach_destroy(pach_ctx *c)
     dfree(c->d_cell_rmask);
     dfree(c->d_dirs); dfree(c->d_geo);
     dfree(c->d_ctr32); dfree(c->d_ctr64); dfree(c->d_hist);
-    dfree(c->d_cnt); dfree(c->d_scal); dfree(c->d_band);
+    dfree(c->d_cnt); dfree(c->d_scal); dfree(c->d_band); dfree(c->d_rooms);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -504,21 +515,61 @@ extern "C" int pach_receivers_set(pach_ctx *c, const double *centers, int R, dou
 // ------------------------------------------------------------------ trace
 namespace {
 
-int h_oct(const pach_ctx *c, int lo, int hi)
+int h_oct(const pach_ctx *c, const double *power, int s0, int s1, int lo, int hi)
 {
     // SplitRayTracer ctor (TaskAll:88-109): band with the least total
-    // absorption among bands carrying >= 1e-6 of the strongest band's power.
+    // absorption (over the room's polygons s0..s1-1) among bands carrying
+    // >= 1e-6 of the strongest band's power.
     double total[8] = {0};
-    for (int s = 0; s < c->S; ++s)
+    for (int s = s0; s < s1; ++s)
         for (int o = lo; o <= hi; ++o) total[o] += c->area[s] * c->alpha[8 * s + o];
     double mn = HUGE_VAL, op = 0;
     int h = 0;
-    for (int o = lo; o <= hi; ++o) op = std::max(op, c->power[o]);
+    for (int o = lo; o <= hi; ++o) op = std::max(op, power[o]);
     for (int o = lo; o <= hi; ++o) {
-        if (c->power[o] < op * 1e-6) continue;
+        if (power[o] < op * 1e-6) continue;
         if (mn > total[o]) { mn = total[o]; h = o; }
     }
     return h;
+}
+
+// Room table of a trace: the user's rooms, or one room of the whole scene.
+int build_rooms(pach_ctx *c, const pach_params &p, std::vector<RoomTab> &tab)
+{
+    std::vector<pach_ctx::RoomIn> in = c->rooms;
+    if (in.empty()) {
+        pach_ctx::RoomIn one{};
+        std::memcpy(one.src, c->origin, sizeof(one.src));
+        std::memcpy(one.power, c->power, sizeof(one.power));
+        one.rays = p.ray_count;
+        one.surf_count = c->S;
+        one.rec_count = c->R;
+        in.push_back(one);
+    }
+    tab.assign(in.size(), RoomTab{});
+    int64_t rb = 0;
+    int sb = 0, cb = 0;
+    for (size_t i = 0; i < in.size(); ++i) {
+        RoomTab &t = tab[i];
+        std::memcpy(t.src, in[i].src, sizeof(t.src));
+        std::memcpy(t.power, in[i].power, sizeof(t.power));
+        t.h = h_oct(c, in[i].power, sb, sb + in[i].surf_count, p.oct_lo, p.oct_hi);
+        t.T1 = in[i].power[t.h] * 1e-2;  // TaskAll:351
+        for (int o = 0; o < 8; ++o) {
+            const bool on = o >= p.oct_lo && o <= p.oct_hi;
+            t.T2[o] = on ? in[i].power[o] * std::pow(10, -4.5) : 0.0;  // TaskAll:354
+            t.T3[o] = on ? in[i].power[o] * 1e-9 : 0.0;                // TaskAll:355
+        }
+        t.ray_base = rb;
+        t.ray_count = in[i].rays;
+        t.rec_base = cb;
+        t.rec_count = in[i].rec_count;
+        rb += in[i].rays;
+        sb += in[i].surf_count;
+        cb += in[i].rec_count;
+    }
+    if (sb != c->S || cb != c->R) return fail(c, PACH_EINVAL, "rooms: surface or receiver counts do not match the scene");
+    return PACH_OK;
 }
 
 template <class R> SceneBuf<R> &scene_of(pach_ctx *c);
@@ -567,11 +618,25 @@ template <class R> int ensure_work(pach_ctx *c, int64_t batch, int grid)
 int refresh_receiver_cells(pach_ctx *c)
 {
     if (!c->rmask_dirty) return PACH_OK;
-    c->rmask_on = c->tune_rmask && c->R <= 64;
+    // mask bits are receiver indices relative to each room's first receiver
+    std::vector<int> bit(c->R);
+    bool fits = true;
+    if (c->rooms.empty()) {
+        for (int r = 0; r < c->R; ++r) bit[r] = r;
+        fits = c->R <= 64;
+    } else {
+        int base = 0;
+        for (const auto &rm : c->rooms) {
+            fits = fits && rm.rec_count <= 64;
+            for (int k = 0; k < rm.rec_count && base + k < c->R; ++k) bit[base + k] = k;
+            base += rm.rec_count;
+        }
+    }
+    c->rmask_on = c->tune_rmask && fits;
     std::vector<uint64_t> mask;
     std::vector<uint8_t> dist;
     if (c->rmask_on) {
-        pach::receiver_cells(c->grid, c->centers.data(), c->R, c->radius, mask, dist);
+        pach::receiver_cells(c->grid, c->centers.data(), c->R, bit.data(), c->radius, mask, dist);
         HIPCHK(c, dalloc(c->d_cell_rmask, mask.size()));
         HIPCHK(c, hipMemcpy(c->d_cell_rmask, mask.data(), mask.size() * 8, hipMemcpyHostToDevice));
     } else {
@@ -592,7 +657,6 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
     tc.oct_lo = p.oct_lo;
     tc.oct_hi = p.oct_hi;
     tc.is_order = p.is_order;
-    tc.h = h_oct(c, p.oct_lo, p.oct_hi);
     tc.dir_mode = c->dir_mode;
     tc.p1_max = p.p1_max_bounces > 0 ? p.p1_max_bounces : 128;
     tc.max_bounces = p.max_bounces > 0 ? p.max_bounces : 100000;
@@ -600,18 +664,22 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
     tc.shade_min = c->tune_shade_min;
     tc.walk_flat = c->tune_walk_flat;
     tc.cotime = p.cutoff_m / c->c;  // TaskAll:83
-    tc.T1 = c->power[tc.h] * 1e-2;  // TaskAll:351
     BandTab bt{};
     for (int o = 0; o < 8; ++o) {
-        tc.power[o] = c->power[o];
-        const bool in = o >= p.oct_lo && o <= p.oct_hi;
-        bt.T2[o] = in ? c->power[o] * std::pow(10, -4.5) : 0.0;  // TaskAll:354
-        bt.T3[o] = in ? c->power[o] * 1e-9 : 0.0;                // TaskAll:355
         bt.atten[o] = c->atten[o];
         bt.kexp2[o] = (float)(-0.1 * c->atten[o] * std::log2(10.0));
     }
     tc.band = c->d_band;
-    for (int k = 0; k < 3; ++k) tc.src[k] = c->origin[k];
+    std::vector<RoomTab> rooms;
+    if (int rc = build_rooms(c, p, rooms)) return rc;
+    if ((int)rooms.size() > c->d_rooms_cap) {
+        HIPCHK(c, dalloc(c->d_rooms, rooms.size()));
+        c->d_rooms_cap = (int)rooms.size();
+    }
+    HIPCHK(c, hipMemcpy(c->d_rooms, rooms.data(), rooms.size() * sizeof(RoomTab), hipMemcpyHostToDevice));
+    tc.rooms = c->d_rooms;
+    tc.n_rooms = (int)rooms.size();
+    c->room_tab = rooms;
     tc.key0 = (uint32_t)p.seed;
     tc.key1 = (uint32_t)(p.seed >> 32);
     tc.dirs = c->d_dirs;
@@ -652,7 +720,7 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
     rv.hist = c->d_hist;
     rv.plane_stride = c->hist_n;
 
-    const int64_t N = p.ray_count;
+    const int64_t N = rooms.back().ray_base + rooms.back().ray_count;
     int64_t batch = p.batch_rays > 0 ? p.batch_rays : std::min<int64_t>(N, 32768);
     batch = std::max<int64_t>(1, std::min<int64_t>(batch, 1 << 22));
     const int grid = c->n_cu * octave_blocks_per_cu<R>(s.dist_lds ? lds_bytes : 0);
@@ -742,10 +810,13 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
     S.shade_passes = (int64_t)hc.v[C_SHADE];
     S.lanes_shaded = (int64_t)hc.v[C_SHADED];
     S.lost = hc.lost;
-    double psum = 0;
-    for (int o = 0; o < 8; ++o) psum += c->power[o];
-    S.e_init = psum * (double)started;
-    S.h_oct = tc.h;
+    S.e_init = 0;
+    for (const RoomTab &t : rooms) {
+        double psum = 0;
+        for (int o = 0; o < 8; ++o) psum += t.power[o];
+        S.e_init += psum * (double)std::min(std::max<int64_t>(started - t.ray_base, 0), t.ray_count);
+    }
+    S.h_oct = rooms[0].h;
     S.rays_done = started;
     S.ms_phase1 = ms_p1;
     S.ms_octave = ms_oct;
@@ -759,6 +830,34 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
 }
 
 }  // namespace
+
+extern "C" int pach_rooms_set(pach_ctx *c, int n, const double *src, const double *power, const int64_t *rays,
+                              const int *surf_count, const int *rec_count)
+{
+    if (!c) return PACH_EINVAL;
+    if (c->running) return fail(c, PACH_ESTATE, "trace running");
+    if (n == 0) {
+        c->rooms.clear();
+        c->rmask_dirty = true;
+        return PACH_OK;
+    }
+    if (n < 0 || n > (1 << 24) || !src || !power || !rays || !surf_count || !rec_count)
+        return fail(c, PACH_EINVAL, "rooms: bad count or null array");
+    std::vector<pach_ctx::RoomIn> v(n);
+    for (int i = 0; i < n; ++i) {
+        std::memcpy(v[i].src, src + 3 * i, sizeof(v[i].src));
+        std::memcpy(v[i].power, power + 8 * i, sizeof(v[i].power));
+        v[i].rays = rays[i];
+        v[i].surf_count = surf_count[i];
+        v[i].rec_count = rec_count[i];
+        if (rays[i] < 0 || surf_count[i] < 0 || rec_count[i] < 0) return fail(c, PACH_EINVAL, "rooms: negative count");
+        for (int o = 0; o < 8; ++o)
+            if (!(v[i].power[o] >= 0)) return fail(c, PACH_EINVAL, "rooms: bad power");
+    }
+    c->rooms = std::move(v);
+    c->rmask_dirty = true;
+    return PACH_OK;
+}
 
 extern "C" int pach_tune(pach_ctx *c, const char *key, int64_t value)
 {
@@ -795,7 +894,14 @@ extern "C" int pach_trace(pach_ctx *c, const pach_params *p)
     if (c->worker.joinable()) c->worker.join();
     if (!c->has_scene || !c->has_src || !c->has_recv)
         return fail(c, PACH_ESTATE, "scene, source and receivers must be set before tracing");
-    if (p->ray_count <= 0 || p->ray_count > ((int64_t)1 << 32))
+    int64_t total = p->ray_count;
+    if (!c->rooms.empty()) {
+        total = 0;
+        for (const auto &rm : c->rooms) total += rm.rays;
+        if (p->ray_count != 0 && p->ray_count != total)
+            return fail(c, PACH_EINVAL, "ray_count must be 0 or the sum of the rooms' ray counts");
+    }
+    if (total <= 0 || total > ((int64_t)1 << 32))
         return fail(c, PACH_EINVAL, "ray_count must lie in [1, 2^32]");
     if (p->oct_lo < 0 || p->oct_hi > 7 || p->oct_lo > p->oct_hi) return fail(c, PACH_EINVAL, "bad octave range");
     if (!(p->cutoff_m > 0)) return fail(c, PACH_EINVAL, "cutoff must be positive");
@@ -803,7 +909,7 @@ extern "C" int pach_trace(pach_ctx *c, const pach_params *p)
     if (p->precision != PACH_PREC_FP32 && p->precision != PACH_PREC_FP64) return fail(c, PACH_EINVAL, "bad precision");
     c->cancel = false;
     c->rays_done = 0;
-    c->rays_total = p->ray_count;
+    c->rays_total = total;
     c->status = 0;
     c->running = true;
     const pach_params pp = *p;
@@ -852,10 +958,15 @@ extern "C" int pach_histogram(pach_ctx *c, int plane, double *out, int scaled)
     (void)hipSetDevice(c->device);
     HIPCHK(c, hipMemcpy(out, c->d_hist + (size_t)plane * c->hist_n, c->hist_n * sizeof(double), hipMemcpyDeviceToHost));
     if (scaled) {
-        const double n = (double)c->rays_done.load();
-        if (n > 0) {
+        // Receiver_Bank.Scale with each room's completed source rays
+        const int64_t done = c->rays_done.load();
+        const size_t per_rec = (size_t)8 * c->sct;
+        for (const RoomTab &t : c->room_tab) {
+            const double n = (double)std::min(std::max<int64_t>(done - t.ray_base, 0), t.ray_count);
+            if (!(n > 0)) continue;
             const double f = plane == PACH_PLANE_PRESSURE ? 1.0 / std::sqrt(n) : 1.0 / n;
-            for (size_t i = 0; i < c->hist_n; ++i) out[i] *= f;
+            for (size_t i = (size_t)t.rec_base * per_rec; i < (size_t)(t.rec_base + t.rec_count) * per_rec; ++i)
+                out[i] *= f;
         }
     }
     return PACH_OK;

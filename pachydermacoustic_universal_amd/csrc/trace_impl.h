@@ -452,18 +452,20 @@ __device__ __forceinline__ void receiver_test(const DevRecv<R> &rv, int r, R ox,
 // tested: a sphere the segment passes through contains the chord midpoint,
 // which lies in a crossed voxel, so the deposits are those of the full loop.
 template <class R>
-__device__ void check_receivers(const DevRecv<R> &rv, uint64_t cand, R ox, R oy, R oz, R dx, R dy, R dz,
-                                R sqd, const R *E, int band, int lo, int hi, double tsum, int ana,
+__device__ void check_receivers(const DevRecv<R> &rv, const RoomTab &rm, uint64_t cand, R ox, R oy, R oz, R dx,
+                                R dy, R dz, R sqd, const R *E, int band, int lo, int hi, double tsum, int ana,
                                 LaneCounters &lc)
 {
+    // voxel mask bits are receiver indices relative to the room's first receiver
     if (rv.masked) {
+        cand &= rm.rec_count >= 64 ? ~0ull : ((1ull << rm.rec_count) - 1);
         while (cand) {
             const int r = __builtin_ctzll(cand);
             cand &= cand - 1;
-            receiver_test<R>(rv, r, ox, oy, oz, dx, dy, dz, sqd, E, band, lo, hi, tsum, ana, lc);
+            receiver_test<R>(rv, rm.rec_base + r, ox, oy, oz, dx, dy, dz, sqd, E, band, lo, hi, tsum, ana, lc);
         }
     } else {
-        for (int r = 0; r < rv.n; ++r)
+        for (int r = rm.rec_base; r < rm.rec_base + rm.rec_count; ++r)
             receiver_test<R>(rv, r, ox, oy, oz, dx, dy, dz, sqd, E, band, lo, hi, tsum, ana, lc);
     }
 }
@@ -528,6 +530,7 @@ __device__ __forceinline__ void diffuse_dir(R nx, R ny, R nz, R uaz, R uzen, R &
 }
 
 // ---------------------------------------------------------------- phase 1
+// src: the global source-ray index (Philox stream and geodesic face k % 80)
 template <class R>
 __device__ __forceinline__ void source_dir(const TraceConst &tc, uint32_t src, R &dx, R &dy, R &dz)
 {
@@ -556,14 +559,14 @@ __device__ __forceinline__ void source_dir(const TraceConst &tc, uint32_t src, R
 
 template <class R>
 __device__ __forceinline__ void make_oray(ORay<R> &r, R ox, R oy, R oz, R dx, R dy, R dz, R I, double t,
-                                          int band, int tri, uint32_t src, uint64_t uid, uint32_t mask)
+                                          int band, int tri, uint32_t src, uint64_t uid, uint32_t mask, int room)
 {
     r.o[0] = ox; r.o[1] = oy; r.o[2] = oz; r.I = I;
     r.d[0] = dx; r.d[1] = dy; r.d[2] = dz; r.tri = tri;
     r.src = src; r.t = t;
     r.uid_lo = (uint32_t)uid; r.uid_hi = (uint32_t)(uid >> 32);
     r.k = 0;
-    r.meta = (uint32_t)band | mask;
+    r.meta = (uint32_t)band | mask | ((uint32_t)room << M_ROOM_SHIFT);
 }
 
 template <class R>
@@ -590,12 +593,23 @@ k_phase1(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv, int64_t 
         if (resume_in) {
             st = resume_in[i];
         } else {
-            st.src = (uint32_t)(src_begin + i);
-            st.o[0] = (R)tc.src[0]; st.o[1] = (R)tc.src[1]; st.o[2] = (R)tc.src[2];
+            const int64_t g = src_begin + i;
+            // room of source ray g: last room whose ray_base <= g
+            int lo = 0, hi = tc.n_rooms - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (tc.rooms[mid].ray_base <= g) lo = mid;
+                else hi = mid - 1;
+            }
+            const RoomTab &rm = tc.rooms[lo];
+            st.room = lo;
+            st.src = (uint32_t)g;
+            st.o[0] = (R)rm.src[0]; st.o[1] = (R)rm.src[1]; st.o[2] = (R)rm.src[2];
             source_dir<R>(tc, st.src, st.d[0], st.d[1], st.d[2]);
-            for (int o = 0; o < kBands; ++o) st.E[o] = (R)tc.power[o];
+            for (int o = 0; o < kBands; ++o) st.E[o] = (R)rm.power[o];
             st.tb = 0; st.order = 0; st.tri = -1; st.k = 0; st.flags = 0;
         }
+        const RoomTab &room = tc.rooms[st.room];
         const bool stoch = tc.mode == 0;
         const int excl_on = Tmin<R>::excl ? 1 : 0;
         bool split = true, suspended = false;
@@ -611,7 +625,7 @@ k_phase1(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv, int64_t 
                         const R ex = st.o[0] + st.d[0] * R(1000000), ey = st.o[1] + st.d[1] * R(1000000),
                                 ez = st.o[2] + st.d[2] * R(1000000);
                         const R qx = ex - st.o[0], qy = ey - st.o[1], qz = ez - st.o[2];
-                        check_receivers<R>(rv, cand, st.o[0], st.o[1], st.o[2], st.d[0], st.d[1], st.d[2],
+                        check_receivers<R>(rv, room, cand, st.o[0], st.o[1], st.o[2], st.d[0], st.d[1], st.d[2],
                                            qx * qx + qy * qy + qz * qz, st.E, -1, tc.oct_lo,
                                            tc.oct_hi, st.tb, 0, lc);
                     }
@@ -645,7 +659,7 @@ k_phase1(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv, int64_t 
                 const R hx = st.o[0] + st.d[0] * leg, hy = st.o[1] + st.d[1] * leg, hz = st.o[2] + st.d[2] * leg;
                 if (st.order > tc.is_order) {
                     const R qx = hx - st.o[0], qy = hy - st.o[1], qz = hz - st.o[2];
-                    check_receivers<R>(rv, cand, st.o[0], st.o[1], st.o[2], st.d[0], st.d[1], st.d[2],
+                    check_receivers<R>(rv, room, cand, st.o[0], st.o[1], st.o[2], st.d[0], st.d[1], st.d[2],
                                        qx * qx + qy * qy + qz * qz, st.E, -1, tc.oct_lo, tc.oct_hi,
                                        st.tb, 0, lc);
                 }
@@ -675,12 +689,12 @@ k_phase1(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv, int64_t 
                         if (tau > R(0)) {
                             ORay<R> tr;
                             make_oray<R>(tr, hx, hy, hz, st.d[0], st.d[1], st.d[2], st.E[oc] * tau, st.tb,
-                                         oc, ht, st.src, child_uid(0, PACH_TAG_EARLY_TR(st.k, oc)), 0u);
+                                         oc, ht, st.src, child_uid(0, PACH_TAG_EARLY_TR(st.k, oc)), 0u, st.room);
                             st.E[oc] *= (R(1) - tau);
                             out[q++] = tr;
                             ORay<R> td;
                             make_oray<R>(td, hx, hy, hz, -ddx, -ddy, -ddz, childI * tau, st.tb, oc, ht,
-                                         st.src, child_uid(0, PACH_TAG_EARLY_TD(st.k, oc)), M_SCAT | M_EVER);
+                                         st.src, child_uid(0, PACH_TAG_EARLY_TD(st.k, oc)), M_SCAT | M_EVER, st.room);
                             childI *= (R(1) - tau);
                             out[q++] = td;
                             lc.v[C_SPAWN] += 2;
@@ -688,7 +702,7 @@ k_phase1(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv, int64_t 
                         }
                         ORay<R> ch;
                         make_oray<R>(ch, hx, hy, hz, ddx, ddy, ddz, childI, st.tb, oc, ht, st.src,
-                                     child_uid(0, PACH_TAG_EARLY(st.k, oc)), M_SCAT | M_EVER);
+                                     child_uid(0, PACH_TAG_EARLY(st.k, oc)), M_SCAT | M_EVER, st.room);
                         out[q++] = ch;
                         lc.v[C_SPAWN]++;
                         lc.v[C_OCT]++;
@@ -702,7 +716,7 @@ k_phase1(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv, int64_t 
                     st.d[2] -= N.z * cos_t * R(2);
                 }
                 st.k++;
-                if (!((double)st.E[tc.h] > tc.T1)) break;
+                if (!((double)st.E[room.h] > room.T1)) break;
                 if (st.order >= tc.p1_max) { lc.v[C_P1CAP]++; break; }
             }
         }
@@ -719,7 +733,7 @@ k_phase1(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv, int64_t 
                 for (int oc = tc.oct_lo; oc <= tc.oct_hi; ++oc) {
                     ORay<R> r;
                     make_oray<R>(r, st.o[0], st.o[1], st.o[2], st.d[0], st.d[1], st.d[2], st.E[oc], st.tb,
-                                 oc, st.tri, st.src, child_uid(0, PACH_TAG_SPLIT(oc)), 0u);
+                                 oc, st.tri, st.src, child_uid(0, PACH_TAG_SPLIT(oc)), 0u, st.room);
                     out[base + (oc - tc.oct_lo)] = r;
                     lc.v[C_OCT]++;
                 }
@@ -744,13 +758,14 @@ __device__ __forceinline__ bool octave_shade(const TraceConst &tc, const DevScen
     has_child = false;
     const int band = (int)(r.meta & M_BAND);
     const bool late3 = (r.meta & M_LATE3) != 0;
+    const RoomTab &room = tc.rooms[r.meta >> M_ROOM_SHIFT];
     const int ana = (r.meta & M_SCAT) ? 1 : ((r.meta & M_EVER) ? 2 : 0);
     lc.v[late3 ? C_VLATE : C_LATE]++;
     if (ht < 0) {
         const R ex = r.o[0] + r.d[0] * R(1000000), ey = r.o[1] + r.d[1] * R(1000000),
                 ez = r.o[2] + r.d[2] * R(1000000);
         const R qx = ex - r.o[0], qy = ey - r.o[1], qz = ez - r.o[2];
-        check_receivers<R>(rv, cand, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2],
+        check_receivers<R>(rv, room, cand, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2],
                            qx * qx + qy * qy + qz * qz, &r.I, band, 0, 0, r.t, ana, lc);
         lc.lost += (double)r.I;
         lc.v[C_MISS]++;
@@ -759,7 +774,7 @@ __device__ __forceinline__ bool octave_shade(const TraceConst &tc, const DevScen
     const R hx = r.o[0] + r.d[0] * leg, hy = r.o[1] + r.d[1] * leg, hz = r.o[2] + r.d[2] * leg;
     {
         const R qx = hx - r.o[0], qy = hy - r.o[1], qz = hz - r.o[2];
-        check_receivers<R>(rv, cand, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2],
+        check_receivers<R>(rv, room, cand, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2],
                            qx * qx + qy * qy + qz * qz, &r.I, band, 0, 0, r.t, ana, lc);
     }
     r.t += (double)leg / rv.c;
@@ -824,10 +839,10 @@ __device__ __forceinline__ bool octave_shade(const TraceConst &tc, const DevScen
         r.d[2] -= N.z * cos_t * R(2);
     }
     if (!late3) {
-        if (!(r.t < tc.cotime && (double)r.I > tc.band->T2[band])) r.meta |= M_LATE3;  // hand-off to End_Trace
+        if (!(r.t < tc.cotime && (double)r.I > room.T2[band])) r.meta |= M_LATE3;  // hand-off to End_Trace
         return true;
     }
-    return r.t < tc.cotime && (double)r.I > tc.band->T3[band];
+    return r.t < tc.cotime && (double)r.I > room.T3[band];
 }
 
 // Persistent, wavefront-compacting octave-ray kernel.

@@ -42,9 +42,22 @@ template <class R> struct DevScene {
 // the kernels index them with plain loads instead of copying an argument array).
 struct BandTab {
     double atten[kBands];  // air attenuation, dB/m
+    float kexp2[kBands];   // -0.1 * atten * log2(10): 10^(-0.1 a x) = exp2(k x)
+};
+
+// One room of a trace (a single-scene trace is one room).  Source rays
+// [ray_base, ray_base + ray_count) leave this room's source; its receivers are
+// [rec_base, rec_base + rec_count) of the receiver list.
+struct RoomTab {
+    double src[3];
+    double T1;             // broadband-phase threshold: power[h] 1e-2 (TaskAll:351)
+    double power[kBands];  // source power per band, W
     double T2[kBands];     // split-phase energy threshold (TaskAll:354)
     double T3[kBands];     // End_Trace energy threshold (TaskAll:355)
-    float kexp2[kBands];   // -0.1 * atten * log2(10): 10^(-0.1 a x) = exp2(k x)
+    int64_t ray_base, ray_count;
+    int h;                 // steering band (TaskAll:88-109)
+    int rec_base, rec_count;
+    int pad;
 };
 
 template <class R> struct DevRecv {
@@ -62,15 +75,15 @@ template <class R> struct DevRecv {
 };
 
 struct TraceConst {
-    int mode, oct_lo, oct_hi, is_order, h, dir_mode;
+    int mode, oct_lo, oct_hi, is_order, dir_mode;
     int p1_max, max_bounces;
     int walk_steps;  // voxels a lane's walk advances per loop step of k_octave
     int shade_min;   // lanes that must be waiting before k_octave shades
     int walk_flat;   // 1: flattened walk (walk_steps counts tests + voxels)
-    double cotime, T1;
+    double cotime;
     const BandTab *band;
-    double power[kBands];
-    double src[3];
+    const RoomTab *rooms;  // [n_rooms], ray_base ascending
+    int n_rooms;
     uint32_t key0, key1;
     const double *dirs;   // supplied directions [n_dirs][3]
     int64_t n_dirs;
@@ -79,7 +92,8 @@ struct TraceConst {
 };
 
 // Octave ray (OctaveRay, Classes_Rays.cs:30-181).  meta bits: 0-2 band,
-// 3 currently scattered, 4 ever scattered, 5 End_Trace phase, 6 dead slot.
+// 3 currently scattered, 4 ever scattered, 5 End_Trace phase, 6 dead slot,
+// 8-31 room.
 template <class R> struct alignas(16) ORay {
     R o[3];
     R I;
@@ -99,6 +113,7 @@ template <class R> struct alignas(16) P1State {
     int order, tri;
     uint32_t k, src;
     int flags;
+    int room;
 };
 
 enum : uint32_t {
@@ -107,6 +122,7 @@ enum : uint32_t {
     M_EVER = 1u << 4,
     M_LATE3 = 1u << 5,
     M_DEAD = 1u << 6,
+    M_ROOM_SHIFT = 8,
 };
 
 enum : int { F_SPLIT_PENDING = 1 };

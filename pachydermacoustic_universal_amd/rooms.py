@@ -178,3 +178,117 @@ def small_workload() -> Room:
 def hall_workload() -> Room:
     """Seeded polyhedral hall (~2,000 triangles, 16 receivers)."""
     return fan_hall()
+
+
+def convex_room(seed, n_tris, size=(3.0, 30.0), n_receivers=4, radius=0.5, name="convex"):
+    """Random convex polyhedron with ~n_tris triangles (hull of n_tris/2 + 2
+    points on a random ellipsoid, sides in `size`), one polygon per triangle,
+    normals into the room; source and receivers at seeded interior points."""
+    from scipy.spatial import ConvexHull
+
+    rng = np.random.default_rng(seed)
+    n_pts = max(n_tris // 2 + 2, 8)
+    axes = rng.uniform(size[0], size[1], 3) / 2
+    p = rng.normal(size=(n_pts, 3))
+    p /= np.linalg.norm(p, axis=1, keepdims=True)
+    p *= axes
+    hull = ConvexHull(p)
+    tris = p[hull.simplices]
+    c = p.mean(0)
+    # orient inward: normal must point toward the centroid
+    n = np.cross(tris[:, 1] - tris[:, 0], tris[:, 2] - tris[:, 0])
+    flip = np.einsum("ij,ij->i", n, c - tris[:, 0]) < 0
+    tris[flip] = tris[flip][:, ::-1]
+    T = tris.shape[0]
+    alpha = rng.uniform(0.02, 0.40, (T, NBANDS))
+    scat = rng.uniform(0.05, 0.70, (T, NBANDS))
+
+    def interior():
+        # a vertex pulled 30-60 % of the way toward the centroid stays inside
+        v = p[rng.integers(n_pts)]
+        return c + (v - c) * rng.uniform(0.3, 0.6)
+
+    src = interior()
+    recs = np.array([interior() for _ in range(n_receivers)])
+    return Room(name=name, verts=tris, surf=np.arange(T, dtype=np.int32), alpha=alpha, scat=scat,
+                trans=np.zeros((T, NBANDS)), source=src, receivers=recs, radius=float(radius),
+                volume=float(hull.volume), meta={"axes": axes})
+
+
+def batch_rooms(seed=3, n=256, n_receivers=4, radius=0.5):
+    """The batched workload: n seeded rooms, each a shoebox with sides U(3, 30)
+    m or a random convex polyhedron with 12-500 triangles, random per-band
+    absorption U(0.02, 0.40) and scattering U(0.05, 0.70) per polygon, one source
+    and 4 receivers; ray counts log-uniform in [1e4, 1e6]."""
+    rng = np.random.default_rng(seed)
+    out, rays = [], []
+    for i in range(n):
+        if rng.random() < 0.5:
+            L = rng.uniform(3, 30, 3)
+            a = rng.uniform(0.02, 0.40, (6, NBANDS))
+            s = rng.uniform(0.05, 0.70, (6, NBANDS))
+            lo = np.minimum(0.6, L * 0.2)
+            src = lo + (L - 2 * lo) * rng.random(3)
+            recs = lo + (L - 2 * lo) * rng.random((n_receivers, 3))
+            r = shoebox(*L, alpha=a, scat=s, source=src, receivers=recs, radius=min(radius, float(lo.min()) * 0.5),
+                        name=f"batch{i}_box")
+            r.radius = radius
+        else:
+            r = convex_room(int(rng.integers(1 << 31)), int(rng.integers(12, 501)), n_receivers=n_receivers,
+                            radius=radius, name=f"batch{i}_convex")
+        out.append(r)
+        rays.append(int(round(10 ** rng.uniform(4, 6))))
+    return out, rays
+
+
+@dataclass
+class PackedRooms:
+    room: Room                 # all rooms side by side in one scene
+    offsets: np.ndarray        # [n, 3] translation of each room
+    tri_count: np.ndarray      # [n]
+    surf_count: np.ndarray     # [n]
+    rec_count: np.ndarray      # [n]
+    sources: np.ndarray        # [n, 3] (translated)
+
+    def member(self, i):
+        """Room i in packed coordinates (the geometry the batch traces)."""
+        t0, t1 = int(self.tri_count[:i].sum()), int(self.tri_count[:i + 1].sum())
+        s0, s1 = int(self.surf_count[:i].sum()), int(self.surf_count[:i + 1].sum())
+        r0, r1 = int(self.rec_count[:i].sum()), int(self.rec_count[:i + 1].sum())
+        R = self.room
+        return Room(name=f"member{i}", verts=R.verts[t0:t1], surf=R.surf[t0:t1] - s0, alpha=R.alpha[s0:s1],
+                    scat=R.scat[s0:s1], trans=R.trans[s0:s1], source=self.sources[i],
+                    receivers=R.receivers[r0:r1], radius=R.radius)
+
+
+def pack_rooms(rooms_list, gap=4.0):
+    """Places rooms on a square lattice with `gap` metres between bounding
+    boxes, so no ray can pass from one room into another; polygons, receivers
+    and sources are concatenated room by room."""
+    n = len(rooms_list)
+    side = int(np.ceil(np.sqrt(n)))
+    ext = np.array([r.bbox()[1] - r.bbox()[0] for r in rooms_list])
+    cell = ext.max(0) + gap
+    offs, verts, surf, al, sc, tr, recs, srcs = [], [], [], [], [], [], [], []
+    sbase = 0
+    for i, r in enumerate(rooms_list):
+        lo = r.bbox()[0]
+        o = np.array([(i % side) * cell[0], (i // side) * cell[1], 0.0]) - lo
+        offs.append(o)
+        verts.append(r.verts + o)
+        surf.append(r.surf + sbase)
+        sbase += r.n_surfaces
+        al.append(r.alpha)
+        sc.append(r.scat)
+        tr.append(r.trans)
+        recs.append(r.receivers + o)
+        srcs.append(r.source + o)
+    radius = rooms_list[0].radius
+    packed = Room(name=f"packed{n}", verts=np.concatenate(verts), surf=np.concatenate(surf).astype(np.int32),
+                  alpha=np.concatenate(al), scat=np.concatenate(sc), trans=np.concatenate(tr),
+                  source=srcs[0], receivers=np.concatenate(recs), radius=radius)
+    return PackedRooms(room=packed, offsets=np.array(offs),
+                       tri_count=np.array([r.n_tris for r in rooms_list]),
+                       surf_count=np.array([r.n_surfaces for r in rooms_list]),
+                       rec_count=np.array([len(r.receivers) for r in rooms_list]),
+                       sources=np.array(srcs))
