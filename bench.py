@@ -48,13 +48,20 @@ def main():
     ap.add_argument("--rays", type=int, default=1_000_000)
     ap.add_argument("--cutoff", type=float, default=3.0)
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
-    ap.add_argument("--grid", type=int, default=32)
+    ap.add_argument("--grid", type=int, default=None,
+                    help="voxels per axis (default: 32 for the hall, automatic for the batch)")
     ap.add_argument("--cpu-rays", type=int, default=2000, help="source rays of the CPU sample")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="pach_tune setting, e.g. dist_lds=0")
+    ap.add_argument("--autotune", type=int, default=0, metavar="RAYS",
+                    help="run pach_autotune with RAYS sample source rays before the warm-up")
+    ap.add_argument("--workload", choices=["hall", "batch"], default="hall",
+                    help="hall: the 2k-triangle hall; batch: 256 packed rooms in one launch sequence")
+    ap.add_argument("--batch-ray-scale", type=float, default=0.01,
+                    help="batch workload: scale of each room's log-uniform [1e4, 1e6] ray count")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -77,7 +84,6 @@ def main():
     from pachydermacoustic_universal_amd import _abi as A
     from pachydermacoustic_universal_amd import rooms
 
-    room = rooms.fan_hall()
     at, c, rho = A.uniform_medium()
     prec = A.PREC_FP64 if args.precision == "fp64" else A.PREC_FP32
     tr = A.Tracer(0)
@@ -86,12 +92,33 @@ def main():
         tr.tune(k, int(v))
     sr = 1000
     sct = int(np.floor(args.cutoff * 1000 * sr / 1000))
-    tr.upload_scene(room.verts, room.surf, room.alpha, room.scat, room.trans, (args.grid,) * 3)
+    if args.workload == "batch":
+        members, base_rays = rooms.batch_rooms(seed=3, n=256)
+        packed = rooms.pack_rooms(members)
+        room = packed.room
+        room_rays = np.maximum(1, np.round(np.array(base_rays) * args.batch_ray_scale)).astype(np.int64)
+        n_rays = int(room_rays.sum())
+        grid = (0, 0, 0) if not args.grid else (args.grid,) * 3
+    else:
+        room = rooms.fan_hall()
+        n_rays = args.rays
+        grid = (args.grid or 32,) * 3
+    tr.upload_scene(room.verts, room.surf, room.alpha, room.scat, room.trans, grid)
     tr.set_source(room.source, np.ones(8), A.DIR_GEODESIC)
     tr.set_receivers(room.receivers, room.radius, at, c, c * rho, sr, sct)
+    if args.workload == "batch":
+        tr.set_rooms(packed.sources, np.ones((len(members), 8)), room_rays, packed.surf_count, packed.rec_count)
+    trace_rays = 0 if args.workload == "batch" else n_rays
+    tuning = None
+    if args.autotune:
+        t_t = time.perf_counter()
+        tuning = tr.autotune(rays=trace_rays, cutoff_m=args.cutoff * c, sample_rays=args.autotune,
+                             seed=args.seed + 7, precision=prec)
+        tuning["seconds"] = time.perf_counter() - t_t
+        print(f"[bench] autotune: {tuning}", file=sys.stderr, flush=True)
 
     def step(seed):
-        tr.trace(rays=args.rays, cutoff_m=args.cutoff * c, seed=seed, precision=prec)
+        tr.trace(rays=trace_rays, cutoff_m=args.cutoff * c, seed=seed, precision=prec)
         return tr.stats()
 
     for w in range(args.warmup):
@@ -125,26 +152,46 @@ def main():
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         med = O.Medium(c=c, atten=at, rho=rho)
         t1 = time.perf_counter()
-        _, cnt = O.trace(room, rays=args.cpu_rays, cutoff_s=args.cutoff, seed=args.seed,
-                         medium=med, grid=(args.grid,) * 3, threads=threads)
+        if args.workload == "batch":
+            # every room traced alone from its own first ray index, its share of the sample
+            cb, nr = 0, 0
+            base = np.concatenate([[0], np.cumsum(room_rays)])
+            for i in range(len(members)):
+                k = max(1, int(round(room_rays[i] * args.cpu_rays / n_rays)))
+                _, cnt = O.trace(packed.member(i), rays=k, cutoff_s=args.cutoff, seed=args.seed, medium=med,
+                                 threads=threads, ray_begin=int(base[i]))
+                cb += cnt["bounces_broad"] + cnt["bounces_late"] + cnt["bounces_verylate"]
+                nr += k
+            what = f"{nr} source rays spread over the 256 rooms in proportion"
+        else:
+            _, cnt = O.trace(room, rays=args.cpu_rays, cutoff_s=args.cutoff, seed=args.seed,
+                             medium=med, grid=grid, threads=threads)
+            cb = cnt["bounces_broad"] + cnt["bounces_late"] + cnt["bounces_verylate"]
+            what = f"{args.cpu_rays} source rays of the same hall workload"
         cdt = time.perf_counter() - t1
-        cb = cnt["bounces_broad"] + cnt["bounces_late"] + cnt["bounces_verylate"]
         cpu = {"value": cb / cdt, "unit": "ray-bounces/s", "cores": threads, "kind": "port",
-               "sample": f"{args.cpu_rays} source rays of the same hall workload "
-                         f"({cb:.3e} bounces, {cdt:.1f} s), C FP64 restatement, "
+               "sample": f"{what} ({cb:.3e} bounces, {cdt:.1f} s), C FP64 restatement, "
                          f"{threads} threads on {cpu_model()} (nproc={os.cpu_count()})"}
+    tr_grid = list(tuning["grid"]) if tuning else list(grid)
+    if args.workload == "batch":
+        workload_name = (f"256 packed rooms (shoeboxes 3-30 m and convex hulls of 12-500 tris), "
+                         f"4 receivers each, {n_rays} rays (log-uniform [1e4, 1e6] x {args.batch_ray_scale}), "
+                         f"{args.cutoff} s cut-off")
+    else:
+        workload_name = (f"seeded polyhedral hall, ~2000 tris, 16 receivers, {n_rays} rays, "
+                         f"{args.cutoff} s cut-off, {tr_grid[0]}x{tr_grid[1]}x{tr_grid[2]} grid")
     out = {
         "metric": "ray-bounces/s", "value": value, "unit": "ray-bounces/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": args.precision, "data": "synthetic",
-        "config": {"workload": "seeded polyhedral hall, ~2000 tris, 16 receivers, "
-                               f"{args.rays} rays, {args.cutoff} s cut-off, {args.grid}^3 grid",
-                   "rays": args.rays, "triangles": room.n_tris, "receivers": 16,
+        "config": {"workload": workload_name,
+                   "rays": n_rays, "triangles": room.n_tris, "receivers": int(room.receivers.shape[0]),
+                   "grid": tr_grid, "tuning": tuning,
                    "bounces_per_step": tot["bounces"] / args.steps,
                    "walk": {k: last[k] for k in ("loop_steps", "voxels", "tri_tests", "shade_passes",
                                                  "lanes_shaded", "deposits", "spills")},
-                   "rays_per_s": args.rays * args.steps / dt},
+                   "rays_per_s": n_rays * args.steps / dt},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_octave", "avg_launch_ms": tot["ms_oct"] / max(tot["n_oct"], 1),

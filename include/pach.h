@@ -178,8 +178,35 @@ int pach_rooms_set(pach_ctx *ctx, int n, const double *src, const double *power,
  * voxels the walk crossed (up to 64 receivers); "walk_steps" (1-64): voxels a
  * walk advances per step of the persistent loop (with "walk_flat" = 1: voxel
  * steps plus triangle tests, 1-256); "shade_min" (1-64): lanes that must wait
- * before a wave shades; "walk_flat" (0/1): flattened walk loop. */
+ * before a wave shades; "walk_flat" (0/1): flattened walk loop; "refill_min"
+ * (1-64): idle lanes a wave gathers before refilling them (the refill
+ * threshold); "dump_min" (>= 1): loop steps a drained wave runs before handing
+ * its leftover rays to the next launch; "block" (256/512): workgroup size of
+ * the octave kernel; "occupancy" (0/4): 4 caps registers at 128 VGPRs so four
+ * waves share each SIMD. */
 int pach_tune(pach_ctx *ctx, const char *key, int64_t value);
+
+/* Autotuner result (no counterpart in the reference). */
+typedef struct pach_tuning {
+    int grid[3];           /* voxel grid dimensions chosen (now uploaded) */
+    int block, occupancy;  /* octave-kernel workgroup size and register cap */
+    int walk_steps, shade_min, refill_min, dump_min;
+    int trials;            /* sample traces run */
+    double rate_start;     /* ray-bounces/s of the sample under the settings on entry */
+    double rate_best;      /* ... under the chosen settings */
+} pach_tuning;
+
+/* Picks voxel resolution, workgroup size, register cap, walk budget, shading
+ * and refill thresholds for this scene, receivers and trace parameters by
+ * timing sample traces (coordinate descent over each knob's candidates,
+ * starting from the current settings), then applies the best: the grid is
+ * rebuilt at the chosen resolution and the knobs are set as by pach_tune.
+ * sample_rays: source rays per trial (<= 0: min(ray count, 8192)); with rooms
+ * set, every room's ray count is scaled down to the sample in proportion.
+ * Overwrites the histograms and statistics (tune before the real trace).
+ * No counterpart in the reference (its Voxel_Grid resolution is a user input,
+ * Classes_Scenes.cs:709-755). */
+int pach_autotune(pach_ctx *ctx, const pach_params *p, int64_t sample_rays, pach_tuning *out);
 
 /* SplitRayTracer(...).Begin() (TaskAll:68-240).  Clears the histograms first. */
 int pach_trace(pach_ctx *ctx, const pach_params *params);

@@ -56,6 +56,22 @@ class Stats(C.Structure):
         return d
 
 
+class Tuning(C.Structure):
+    _fields_ = [("grid", C.c_int * 3), ("block", C.c_int), ("occupancy", C.c_int),
+                ("walk_steps", C.c_int), ("shade_min", C.c_int), ("refill_min", C.c_int),
+                ("dump_min", C.c_int), ("trials", C.c_int), ("rate_start", C.c_double),
+                ("rate_best", C.c_double)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["grid"] = list(self.grid)
+        return d
+
+
+# knobs of pach_tune that an autotuning result sets
+TUNED_KNOBS = ("block", "occupancy", "walk_steps", "shade_min", "refill_min", "dump_min")
+
+
 # (name, restype, argtypes) of every entry point declared in include/pach.h
 SIGNATURES = [
     ("pach_abi_version", C.c_int, []),
@@ -73,6 +89,7 @@ SIGNATURES = [
                                      C.c_double, C.c_int, C.c_int]),
     ("pach_rooms_set", C.c_int, [C.c_void_p, C.c_int, _dp, _dp, C.POINTER(C.c_int64), _ip, _ip]),
     ("pach_tune", C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
+    ("pach_autotune", C.c_int, [C.c_void_p, C.POINTER(Params), C.c_int64, C.POINTER(Tuning)]),
     ("pach_trace", C.c_int, [C.c_void_p, C.POINTER(Params)]),
     ("pach_poll", C.c_int, [C.c_void_p, C.POINTER(Progress)]),
     ("pach_wait", C.c_int, [C.c_void_p]),
@@ -214,15 +231,31 @@ class Tracer:
         self.R = ctr.shape[0]
         self.sample_ct = sample_ct
 
-    def trace(self, *, rays, cutoff_m, seed=1, mode=MODE_STOCHASTIC, precision=PREC_FP32,
-              oct_lo=0, oct_hi=7, is_order=-1, p1_max_bounces=128, max_bounces=100000,
-              batch_rays=0, async_=False):
+    @staticmethod
+    def _params(rays, cutoff_m, seed=1, mode=MODE_STOCHASTIC, precision=PREC_FP32, oct_lo=0, oct_hi=7,
+                is_order=-1, p1_max_bounces=128, max_bounces=100000, batch_rays=0, async_=False):
         p = Params()
         p.cutoff_m = cutoff_m
         p.oct_lo, p.oct_hi, p.is_order = oct_lo, oct_hi, is_order
         p.mode, p.precision, p.async_ = mode, precision, int(async_)
         p.ray_count, p.seed = rays, seed
         p.p1_max_bounces, p.max_bounces, p.batch_rays = p1_max_bounces, max_bounces, batch_rays
+        return p
+
+    def autotune(self, *, rays, cutoff_m, sample_rays=0, **kw):
+        """pach_autotune: times sample traces and applies the fastest settings."""
+        p = self._params(rays, cutoff_m, **kw)
+        t = Tuning()
+        self._chk(lib().pach_autotune(self.h, C.byref(p), int(sample_rays), C.byref(t)))
+        return t.as_dict()
+
+    def apply_tuning(self, tuning):
+        """Re-applies an autotune result (the grid through the next scene upload)."""
+        for k in TUNED_KNOBS:
+            self.tune(k, tuning[k])
+
+    def trace(self, *, rays, cutoff_m, **kw):
+        p = self._params(rays, cutoff_m, **kw)
         rc = lib().pach_trace(self.h, C.byref(p))
         if rc == -5:
             return rc

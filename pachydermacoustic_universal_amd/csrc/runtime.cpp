@@ -64,6 +64,7 @@ struct pach_ctx {
     uint64_t *d_cell_rmask = nullptr;
     int ncell = 0;
     pach::Grid grid;        // host copy (receiver voxels are marked at trace time)
+    std::vector<double> verts;  // host copy of the triangles (grid rebuilds)
     bool rmask_dirty = true;
     bool rmask_on = false;
     std::vector<double> centers;
@@ -113,6 +114,9 @@ struct pach_ctx {
     int tune_walk_steps = 12;  // walk units (tests + voxel steps) per loop step
     int tune_shade_min = 16;   // lanes waiting before shading
     int tune_walk_flat = 1;    // flattened walk
+    int tune_refill_min = 1;   // idle lanes gathered before a wave refills
+    int tune_dump_min = 256;   // loop steps before a drained wave hands over its work
+    OctShape tune_shape;       // k_octave workgroup size and register cap
 
     // trace state
     std::thread worker;
@@ -339,6 +343,33 @@ template <class R> hipError_t upload_scalars(R *&dst, const std::vector<double> 
     return hipMemcpy(dst, tmp.data(), tmp.size() * sizeof(R), hipMemcpyHostToDevice);
 }
 
+// Builds the voxel grid of the stored triangles (dims 0: automatic) and
+// uploads its voxel lists and distance field.
+int upload_grid(pach_ctx *c, const int dims[3])
+{
+    pach::Grid grid;
+    std::string gerr;
+    if (pach::build_grid(c->verts.data(), (int)(c->verts.size() / 9), dims, grid, gerr) != 0)
+        return fail(c, PACH_EINVAL, gerr);
+    hipError_t e;
+    if ((e = dalloc(c->d_cell_range, grid.ranges.size())) != hipSuccess ||
+        (e = hipMemcpy(c->d_cell_range, grid.ranges.data(), grid.ranges.size() * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = dalloc(c->d_cell_tris, std::max<size_t>(grid.tris.size(), 1))) != hipSuccess ||
+        (e = hipMemcpy(c->d_cell_tris, grid.tris.data(), grid.tris.size() * 4, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = dalloc(c->d_cell_dist, grid.dist.size())) != hipSuccess ||
+        (e = hipMemcpy(c->d_cell_dist, grid.dist.data(), grid.dist.size(), hipMemcpyHostToDevice)) != hipSuccess)
+        return fail(c, PACH_EHIP, std::string("grid upload: ") + hipGetErrorString(e));
+    for (int k = 0; k < 3; ++k) {
+        c->g[k] = grid.dims[k];
+        c->gmin[k] = grid.origin[k];
+        c->cs[k] = grid.cell[k];
+    }
+    c->ncell = grid.dims[0] * grid.dims[1] * grid.dims[2];
+    c->grid = std::move(grid);
+    c->rmask_dirty = true;
+    return PACH_OK;
+}
+
 }  // namespace
 
 extern "C" int pach_scene_upload(pach_ctx *c, const double *verts, int T, const int *surf, int S,
@@ -405,17 +436,9 @@ extern "C" int pach_scene_upload(pach_ctx *c, const double *verts, int T, const 
         d1[4 * t + 3] = w1;
         d2[4 * t + 3] = w2;
     }
-    pach::Grid grid;
-    std::string gerr;
+    c->verts.assign(verts, verts + 9 * (size_t)T);
     const int dims[3] = {gx, gy, gz};
-    if (pach::build_grid(verts, T, dims, grid, gerr) != 0) return fail(c, PACH_EINVAL, gerr);
-    for (int k = 0; k < 3; ++k) {
-        c->g[k] = grid.dims[k];
-        c->gmin[k] = grid.origin[k];
-        c->cs[k] = grid.cell[k];
-    }
-    c->ncell = grid.dims[0] * grid.dims[1] * grid.dims[2];
-    c->rmask_dirty = true;
+    if (int rc = upload_grid(c, dims)) return rc;
     // materials: (1 - alpha, s, lo, hi) per (polygon, band), Lambert_Scattering
     // split bounds (Classes_Materials.cs:4466-4484)
     std::vector<double> mat(4 * 8 * (size_t)S);
@@ -438,15 +461,8 @@ extern "C" int pach_scene_upload(pach_ctx *c, const double *verts, int T, const 
         (e = upload_vec4<double>(c->s64.e2, e2)) != hipSuccess || (e = upload_vec4<double>(c->s64.nrm, nr)) != hipSuccess ||
         (e = upload_vec4<double>(c->s64.mat, mat)) != hipSuccess || (e = upload_scalars<double>(c->s64.trans, c->trans)) != hipSuccess)
         return fail(c, PACH_EHIP, std::string("scene upload: ") + hipGetErrorString(e));
-    if ((e = dalloc(c->d_surf, T)) != hipSuccess || (e = hipMemcpy(c->d_surf, surf, T * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = dalloc(c->d_cell_range, grid.ranges.size())) != hipSuccess ||
-        (e = hipMemcpy(c->d_cell_range, grid.ranges.data(), grid.ranges.size() * 4, hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = dalloc(c->d_cell_tris, grid.tris.size())) != hipSuccess ||
-        (e = hipMemcpy(c->d_cell_tris, grid.tris.data(), std::max<size_t>(grid.tris.size(), 1) * 4, hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = dalloc(c->d_cell_dist, grid.dist.size())) != hipSuccess ||
-        (e = hipMemcpy(c->d_cell_dist, grid.dist.data(), grid.dist.size(), hipMemcpyHostToDevice)) != hipSuccess)
-        return fail(c, PACH_EHIP, std::string("grid upload: ") + hipGetErrorString(e));
-    c->grid = std::move(grid);
+    if ((e = dalloc(c->d_surf, T)) != hipSuccess || (e = hipMemcpy(c->d_surf, surf, T * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess)
+        return fail(c, PACH_EHIP, std::string("scene upload: ") + hipGetErrorString(e));
     c->has_scene = true;
     return PACH_OK;
 }
@@ -582,12 +598,11 @@ template <class R> const typename V4<R>::T *ctr_of(pach_ctx *c);
 template <> const float4 *ctr_of<float>(pach_ctx *c) { return c->d_ctr32; }
 template <> const double4 *ctr_of<double>(pach_ctx *c) { return c->d_ctr64; }
 
-template <class R> int ensure_work(pach_ctx *c, int64_t batch, int grid)
+template <class R> int ensure_work(pach_ctx *c, int64_t batch, size_t waves)
 {
     WorkBuf<R> &w = work_of<R>(c);
     const unsigned p1_cap = (unsigned)std::min<int64_t>(batch * 200 + 4096, (int64_t)1 << 31);
     const unsigned res_cap = (unsigned)batch;
-    const size_t waves = (size_t)grid * (kBlock / 64);
     const int stack_cap = 4096;
     const unsigned dump_cap = (unsigned)std::min<size_t>(waves * (stack_cap + 64) + (1u << 20), (size_t)1 << 31);
     hipError_t e = hipSuccess;
@@ -663,6 +678,7 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
     tc.walk_steps = c->tune_walk_steps;
     tc.shade_min = c->tune_shade_min;
     tc.walk_flat = c->tune_walk_flat;
+    tc.refill_min = c->tune_refill_min;
     tc.cotime = p.cutoff_m / c->c;  // TaskAll:83
     BandTab bt{};
     for (int o = 0; o < 8; ++o) {
@@ -723,8 +739,9 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
     const int64_t N = rooms.back().ray_base + rooms.back().ray_count;
     int64_t batch = p.batch_rays > 0 ? p.batch_rays : std::min<int64_t>(N, 32768);
     batch = std::max<int64_t>(1, std::min<int64_t>(batch, 1 << 22));
-    const int grid = c->n_cu * octave_blocks_per_cu<R>(s.dist_lds ? lds_bytes : 0);
-    int rc = ensure_work<R>(c, batch, grid);
+    const OctShape shape = c->tune_shape;
+    const int grid = c->n_cu * octave_blocks_per_cu<R>(shape, s.dist_lds ? lds_bytes : 0);
+    int rc = ensure_work<R>(c, batch, (size_t)grid * (shape.block / 64));
     if (rc) return rc;
     WorkBuf<R> &w = work_of<R>(c);
     unsigned *p1_count = c->d_scal + 0, *res_count = c->d_scal + 1, *in_head = c->d_scal + 2;
@@ -743,7 +760,7 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
     int res_cur = 0;
     int64_t next = 0;       // next fresh source ray
     int64_t started = 0;
-    const int min_iters = 256;
+    const int min_iters = c->tune_dump_min;
     for (;;) {
         const bool stop_new = c->cancel.load();
         bool p1_launched = false;
@@ -770,7 +787,7 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
         HIPCHK(c, hipMemsetAsync(in_head, 0, sizeof(unsigned), st));
         HIPCHK(c, hipMemsetAsync(dump_count + (1 - cur), 0, sizeof(unsigned), st));
         HIPCHK(c, hipEventRecord(c->ev[2], st));
-        HIPCHK(c, launch_octave<R>(st, grid, tc, s, rv, pending ? w.dump[cur] : nullptr, dump_count + cur,
+        HIPCHK(c, launch_octave<R>(st, grid, shape, tc, s, rv, pending ? w.dump[cur] : nullptr, dump_count + cur,
                                    w.dump_cap, p1_launched ? w.p1_out : nullptr, p1_count, w.p1_cap, in_head,
                                    w.stacks, w.stack_cap, w.dump[1 - cur], dump_count + (1 - cur), w.dump_cap,
                                    min_iters, c->d_cnt));
@@ -785,6 +802,8 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
         cur = 1 - cur;
         pending = std::min(h[3 + cur], w.dump_cap);
         if (n_resume > w.res_cap) return fail(c, PACH_EHIP, "resume buffer overflow");
+        // every launch completes the bounces its waves start, so this only trips on a defect
+        if (n_oct > ((int64_t)1 << 20)) return fail(c, PACH_EHIP, "octave launches exceed 2^20: no progress");
         // a source ray is complete once its broadband phase and every ray it spawned have ended
         if (n_resume == 0 && pending == 0) c->rays_done = started;
     }
@@ -884,6 +903,22 @@ extern "C" int pach_tune(pach_ctx *c, const char *key, int64_t value)
         c->tune_shade_min = (int)value;
         return PACH_OK;
     }
+    if (std::strcmp(key, "refill_min") == 0 && value >= 1 && value <= 64) {
+        c->tune_refill_min = (int)value;
+        return PACH_OK;
+    }
+    if (std::strcmp(key, "dump_min") == 0 && value >= 1 && value <= (1 << 20)) {
+        c->tune_dump_min = (int)value;
+        return PACH_OK;
+    }
+    if (std::strcmp(key, "block") == 0 && (value == 256 || value == 512)) {
+        c->tune_shape.block = (int)value;
+        return PACH_OK;
+    }
+    if (std::strcmp(key, "occupancy") == 0 && (value == 0 || value == 4)) {
+        c->tune_shape.occ = (int)value;
+        return PACH_OK;
+    }
     return fail(c, PACH_EINVAL, std::string("unknown tuning key ") + key);
 }
 
@@ -921,6 +956,119 @@ extern "C" int pach_trace(pach_ctx *c, const pach_params *p)
     };
     if (!p->async) return body();
     c->worker = std::thread(body);
+    return PACH_OK;
+}
+
+extern "C" int pach_autotune(pach_ctx *c, const pach_params *p, int64_t sample_rays, pach_tuning *out)
+{
+    if (!c || !p || !out) return PACH_EINVAL;
+    if (c->running) return fail(c, PACH_ESTATE, "trace running");
+    if (c->worker.joinable()) c->worker.join();
+    if (!c->has_scene || !c->has_src || !c->has_recv)
+        return fail(c, PACH_ESTATE, "scene, source and receivers must be set before tuning");
+    pach_params pp = *p;
+    pp.async = 0;
+    const std::vector<pach_ctx::RoomIn> saved = c->rooms;
+    int64_t total = p->ray_count;
+    if (!saved.empty()) {
+        total = 0;
+        for (const auto &rm : saved) total += rm.rays;
+    }
+    const int64_t sample = sample_rays > 0 ? std::min(sample_rays, total) : std::min<int64_t>(total, 8192);
+    if (sample <= 0) return fail(c, PACH_EINVAL, "autotune: no rays to sample");
+    if (!saved.empty()) {
+        // every room keeps its share of the sample (at least one ray if it has any)
+        int64_t n = 0;
+        for (auto &rm : c->rooms) {
+            if (rm.rays > 0)
+                rm.rays = std::max<int64_t>(1, (int64_t)std::llround((double)rm.rays * (double)sample / (double)total));
+            n += rm.rays;
+        }
+        pp.ray_count = n;
+    } else {
+        pp.ray_count = sample;
+    }
+    struct Knobs {
+        int grid[3], block, occ, walk_steps, shade_min, refill_min, dump_min;
+    };
+    auto current = [c]() {
+        Knobs k{{c->g[0], c->g[1], c->g[2]}, c->tune_shape.block, c->tune_shape.occ, c->tune_walk_steps,
+                c->tune_shade_min, c->tune_refill_min, c->tune_dump_min};
+        return k;
+    };
+    auto apply = [c](const Knobs &k) -> int {
+        if (k.grid[0] != c->g[0] || k.grid[1] != c->g[1] || k.grid[2] != c->g[2])
+            if (int rc = upload_grid(c, k.grid)) return rc;
+        c->tune_shape.block = k.block;
+        c->tune_shape.occ = k.occ;
+        c->tune_walk_steps = k.walk_steps;
+        c->tune_shade_min = k.shade_min;
+        c->tune_refill_min = k.refill_min;
+        c->tune_dump_min = k.dump_min;
+        return PACH_OK;
+    };
+    int trials = 0;
+    auto rate = [&](double &r) -> int {
+        if (int rc = pach_trace(c, &pp)) return rc;
+        ++trials;
+        const pach_stats &S = c->stats;
+        r = (double)(S.bounces_broad + S.bounces_late + S.bounces_verylate) / (S.ms_trace * 1e-3);
+        return PACH_OK;
+    };
+    const Knobs start = current();
+    int rc = PACH_OK;
+    double r0 = 0, best_r = 0;
+    Knobs best = start;
+    do {
+        double warm;
+        if ((rc = rate(warm))) break;  // first trial allocates the work buffers
+        if ((rc = rate(r0))) break;
+        best_r = r0;
+        auto try_knob = [&](auto set, const std::vector<int> &vals) -> int {
+            const Knobs base = best;
+            for (int v : vals) {
+                Knobs k = base;
+                set(k, v);
+                if (std::memcmp(&k, &base, sizeof(Knobs)) == 0) continue;
+                if (int e = apply(k)) return e;
+                double r;
+                if (int e = rate(r)) return e;
+                if (r > best_r * 1.01) { best_r = r; best = k; }
+            }
+            return apply(best);
+        };
+        // voxel resolution: linear scale of the starting grid (the dominant knob first)
+        const std::vector<int> pct = {50, 71, 141, 200};
+        if ((rc = try_knob([&](Knobs &k, int q) {
+                 for (int a = 0; a < 3; ++a)
+                     k.grid[a] = std::max(1, std::min(1024, (int)std::lround(start.grid[a] * q / 100.0)));
+             }, pct)))
+            break;
+        if ((rc = try_knob([](Knobs &k, int v) { k.occ = v; }, {0, 4}))) break;
+        if ((rc = try_knob([](Knobs &k, int v) { k.block = v; }, {256, 512}))) break;
+        if ((rc = try_knob([](Knobs &k, int v) { k.walk_steps = v; }, {6, 8, 12, 16, 24, 32}))) break;
+        if ((rc = try_knob([](Knobs &k, int v) { k.shade_min = v; }, {8, 12, 16, 24, 32, 48}))) break;
+        if ((rc = try_knob([](Knobs &k, int v) { k.refill_min = v; }, {1, 4, 8, 16, 32}))) break;
+        if ((rc = try_knob([](Knobs &k, int v) { k.dump_min = v; }, {64, 256, 1024}))) break;
+    } while (false);
+    c->rooms = saved;
+    if (rc) {
+        const std::string msg = c->err;
+        apply(start);
+        c->err = msg;
+        return rc;
+    }
+    if ((rc = apply(best))) return rc;
+    for (int a = 0; a < 3; ++a) out->grid[a] = best.grid[a];
+    out->block = best.block;
+    out->occupancy = best.occ;
+    out->walk_steps = best.walk_steps;
+    out->shade_min = best.shade_min;
+    out->refill_min = best.refill_min;
+    out->dump_min = best.dump_min;
+    out->trials = trials;
+    out->rate_start = r0;
+    out->rate_best = best_r;
     return PACH_OK;
 }
 

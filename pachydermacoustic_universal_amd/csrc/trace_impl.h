@@ -847,20 +847,27 @@ __device__ __forceinline__ bool octave_shade(const TraceConst &tc, const DevScen
 
 // Persistent, wavefront-compacting octave-ray kernel.
 //
-// Each lane holds one octave ray.  A loop step (a) refills idle lanes, first
-// from the wave's own LIFO stack of spawned rays (an HBM slab per wave), then
-// from the shared input (the previous launch's leftovers in0, then the new
-// broadband-phase output in1); (b) advances every walking lane's closest-hit
-// walk by up to kWalkSteps voxels; (c) shades the lanes whose walk ended, but
-// only once kShadeMin lanes are waiting or none is still walking, so the
-// shading code runs on mostly-full waves while long walks continue.
-// Once the input is drained and the wave has run min_iters steps it moves its
-// stack and in-flight rays to the shared `dump` queue and exits, so the next
-// launch spreads the leftover work over every wave.  No wave ever waits on
-// another, so every launch ends after a bounded number of steps.
+// Each lane holds one octave ray.  A loop step (a) refills idle lanes (once
+// refill_min of them are idle), first from the wave's own LIFO stack of
+// spawned rays (an HBM slab per wave), then from the shared input (the
+// previous launch's leftovers in0, then the new broadband-phase output in1);
+// (b) advances every walking lane's closest-hit walk by walk_steps units;
+// (c) shades the lanes whose walk ended, but only once shade_min lanes are
+// waiting or none is still walking, so the shading code runs on mostly-full
+// waves while long walks continue.
+// Once the input is drained and the wave has run min_iters steps it drains:
+// every step it moves its stack and the lanes that are between bounces to the
+// shared `dump` queue, while walks already under way run to their hit and are
+// shaded (a walk is never cut short, so every launch completes at least the
+// bounces its waves started and the relaunch sequence always advances).  The
+// wave exits when no lane is left; the next launch spreads the leftover work
+// over every wave.  No wave ever waits on another.
 
-template <class R, bool LDS>
-__global__ void __launch_bounds__(kBlock)
+// BS: workgroup size (waves share the LDS copy of the distance field);
+// OCC: 0 = compiler's register allocation, else waves per SIMD to fit
+// (a register cap of 512 / OCC VGPRs).
+template <class R, bool LDS, int BS, int OCC>
+__global__ void __launch_bounds__(BS, OCC > 0 ? OCC : 1)
 k_octave(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv,
          const ORay<R> *__restrict__ in0, const unsigned *n0_ptr, unsigned cap0,
          const ORay<R> *__restrict__ in1, const unsigned *n1_ptr, unsigned cap1, unsigned *in_head,
@@ -872,29 +879,30 @@ k_octave(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv,
         const uint4 *src = (const uint4 *)s.cell_dist;
         uint4 *dst = (uint4 *)pach_lds_dist;
         const int n16 = (s.ncell + 15) >> 4;
-        for (int i = threadIdx.x; i < n16; i += kBlock) dst[i] = src[i];
+        for (int i = threadIdx.x; i < n16; i += BS) dst[i] = src[i];
         __syncthreads();
     }
     LaneCounters lc = {};
     lc.lost = 0;
     const int lane = (int)__lane_id();
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const uint64_t wave = (uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    const uint64_t wave = (uint64_t)blockIdx.x * (BS / 64) + (threadIdx.x >> 6);
     ORay<R> *ws = stacks + wave * (uint64_t)stack_cap;
     const unsigned n0 = in0 ? min(*n0_ptr, cap0) : 0u;
     const unsigned n1 = in1 ? min(*n1_ptr, cap1) : 0u;
     const unsigned n_in = n0 + n1;
     int stack_n = 0;
     int iters = 0;
-    bool input_done = false;
+    bool input_done = false, draining = false;
     bool act = false, walking = false, ready = false;
     ORay<R> r;
     Walk<R> w;
     for (;;) {
-        // ---- (a) refill idle lanes: own stack first, then the input queue
+        // ---- (a) refill idle lanes once refill_min of them are idle (or all
+        // are): own stack first, then the input queue
         const uint64_t empty = __ballot(!act);
-        if (empty) {
-            const int n_empty = __popcll(empty);
+        const int n_empty = __popcll(empty);
+        if (!draining && n_empty > 0 && (n_empty >= tc.refill_min || n_empty == 64)) {
             const int rank = __popcll(empty & lt_mask);
             const int from_stack = min(n_empty, stack_n);
             if (!act && rank < from_stack) {
@@ -917,30 +925,35 @@ k_octave(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv,
                 }
             }
         }
+        if (input_done && iters >= min_iters) draining = true;
+        if (draining) {
+            // ---- hand the stack and the lanes between bounces to the next launch
+            const bool give = act && !walking && !ready;
+            const uint64_t gm = __ballot(give);
+            const unsigned n_dump = (unsigned)stack_n + (unsigned)__popcll(gm);
+            if (n_dump > 0) {
+                unsigned base = 0;
+                if (lane == 0) base = atomicAdd(dump_count, n_dump);
+                base = __shfl(base, 0);
+                for (int i = lane; i < stack_n; i += 64) {
+                    const unsigned q = base + (unsigned)i;
+                    if (q < dump_cap) dump[q] = ws[i];
+                    else { lc.v[C_DROP]++; lc.lost += (double)ws[i].I; }
+                }
+                if (give) {
+                    const unsigned q = base + (unsigned)stack_n + (unsigned)__popcll(gm & lt_mask);
+                    if (q < dump_cap) dump[q] = r;
+                    else { lc.v[C_DROP]++; lc.lost += (double)r.I; }
+                    act = false;
+                }
+                lc.v[C_SPILL] += (lane == 0) ? n_dump : 0u;
+                stack_n = 0;
+            }
+        }
         const uint64_t am = __ballot(act);
         if (am == 0) {
             if (input_done) break;
             continue;  // the claim returned only dead slots: claim again
-        }
-        if (input_done && iters >= min_iters) {
-            // ---- hand the wave's remaining work to the next launch
-            const int n_act = __popcll(am);
-            const unsigned n_dump = (unsigned)(stack_n + n_act);
-            unsigned base = 0;
-            if (lane == 0) base = atomicAdd(dump_count, n_dump);
-            base = __shfl(base, 0);
-            for (int i = lane; i < stack_n; i += 64) {
-                const unsigned q = base + (unsigned)i;
-                if (q < dump_cap) dump[q] = ws[i];
-                else { lc.v[C_DROP]++; lc.lost += (double)ws[i].I; }
-            }
-            if (act) {
-                const unsigned q = base + (unsigned)stack_n + (unsigned)__popcll(am & lt_mask);
-                if (q < dump_cap) dump[q] = r;
-                else { lc.v[C_DROP]++; lc.lost += (double)r.I; }
-            }
-            lc.v[C_SPILL] += (lane == 0) ? n_dump : 0u;
-            break;
         }
         // ---- (b) start and advance closest-hit walks
         if (act && !walking && !ready) {
@@ -1017,32 +1030,55 @@ hipError_t launch_phase1(hipStream_t st, const TraceConst &tc, const DevScene<R>
     return hipGetLastError();
 }
 
+template <class R, int BS, int OCC>
+static void launch_octave_bs(hipStream_t st, int grid, size_t lds, const TraceConst &tc, const DevScene<R> &s,
+                             const DevRecv<R> &rv, const ORay<R> *in0, const unsigned *n0, unsigned cap0,
+                             const ORay<R> *in1, const unsigned *n1, unsigned cap1, unsigned *in_head,
+                             ORay<R> *stacks, int stack_cap, ORay<R> *dump, unsigned *dump_count,
+                             unsigned dump_cap, int min_iters, DevCounters *cnt)
+{
+    if (lds)
+        hipLaunchKernelGGL((k_octave<R, true, BS, OCC>), dim3(grid), dim3(BS), lds, st, tc, s, rv, in0, n0, cap0,
+                           in1, n1, cap1, in_head, stacks, stack_cap, dump, dump_count, dump_cap, min_iters, cnt);
+    else
+        hipLaunchKernelGGL((k_octave<R, false, BS, OCC>), dim3(grid), dim3(BS), 0, st, tc, s, rv, in0, n0, cap0,
+                           in1, n1, cap1, in_head, stacks, stack_cap, dump, dump_count, dump_cap, min_iters, cnt);
+}
+
 template <class R>
-hipError_t launch_octave(hipStream_t st, int grid, const TraceConst &tc, const DevScene<R> &s,
+hipError_t launch_octave(hipStream_t st, int grid, const OctShape &sh, const TraceConst &tc, const DevScene<R> &s,
                          const DevRecv<R> &rv, const ORay<R> *in0, const unsigned *n0, unsigned cap0,
                          const ORay<R> *in1, const unsigned *n1, unsigned cap1, unsigned *in_head,
                          ORay<R> *stacks, int stack_cap, ORay<R> *dump, unsigned *dump_count,
                          unsigned dump_cap, int min_iters, DevCounters *cnt)
 {
-    if (s.dist_lds) {
-        const size_t lds = (size_t)((s.ncell + 15) & ~15);
-        hipLaunchKernelGGL((k_octave<R, true>), dim3(grid), dim3(kBlock), lds, st, tc, s, rv, in0, n0, cap0,
-                           in1, n1, cap1, in_head, stacks, stack_cap, dump, dump_count, dump_cap, min_iters, cnt);
+    const size_t lds = s.dist_lds ? (size_t)((s.ncell + 15) & ~15) : 0;
+#define PACH_OCT(BS_, OCC_)                                                                                 \
+    launch_octave_bs<R, BS_, OCC_>(st, grid, lds, tc, s, rv, in0, n0, cap0, in1, n1, cap1, in_head, stacks, \
+                                   stack_cap, dump, dump_count, dump_cap, min_iters, cnt)
+    if (sh.block == 512) {
+        if (sh.occ == 4) PACH_OCT(512, 4); else PACH_OCT(512, 0);
     } else {
-        hipLaunchKernelGGL((k_octave<R, false>), dim3(grid), dim3(kBlock), 0, st, tc, s, rv, in0, n0, cap0,
-                           in1, n1, cap1, in_head, stacks, stack_cap, dump, dump_count, dump_cap, min_iters, cnt);
+        if (sh.occ == 4) PACH_OCT(256, 4); else PACH_OCT(256, 0);
     }
+#undef PACH_OCT
     return hipGetLastError();
 }
 
-template <class R> int octave_blocks_per_cu(int lds_bytes)
+template <class R, int BS, int OCC> static int blocks_per_cu_bs(int lds_bytes)
 {
     int nb = 0;
     hipError_t e = lds_bytes > 0
-                       ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_octave<R, true>, kBlock, lds_bytes)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_octave<R, false>, kBlock, 0);
+                       ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_octave<R, true, BS, OCC>, BS, lds_bytes)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_octave<R, false, BS, OCC>, BS, 0);
     if (e != hipSuccess || nb < 1) nb = 1;
     return nb;
+}
+
+template <class R> int octave_blocks_per_cu(const OctShape &sh, int lds_bytes)
+{
+    if (sh.block == 512) return sh.occ == 4 ? blocks_per_cu_bs<R, 512, 4>(lds_bytes) : blocks_per_cu_bs<R, 512, 0>(lds_bytes);
+    return sh.occ == 4 ? blocks_per_cu_bs<R, 256, 4>(lds_bytes) : blocks_per_cu_bs<R, 256, 0>(lds_bytes);
 }
 
 #define PACH_INSTANTIATE(R)                                                                        \
@@ -1050,11 +1086,11 @@ template <class R> int octave_blocks_per_cu(int lds_bytes)
                                          const DevRecv<R> &, int64_t, int, const P1State<R> *, int, \
                                          ORay<R> *, unsigned *, unsigned, P1State<R> *, unsigned *, \
                                          DevCounters *);                                           \
-    template hipError_t launch_octave<R>(hipStream_t, int, const TraceConst &, const DevScene<R> &, \
-                                         const DevRecv<R> &, const ORay<R> *, const unsigned *,     \
-                                         unsigned, const ORay<R> *, const unsigned *, unsigned,     \
-                                         unsigned *, ORay<R> *, int, ORay<R> *, unsigned *,         \
-                                         unsigned, int, DevCounters *);                             \
-    template int octave_blocks_per_cu<R>(int);
+    template hipError_t launch_octave<R>(hipStream_t, int, const OctShape &, const TraceConst &,   \
+                                         const DevScene<R> &, const DevRecv<R> &, const ORay<R> *,  \
+                                         const unsigned *, unsigned, const ORay<R> *,               \
+                                         const unsigned *, unsigned, unsigned *, ORay<R> *, int,    \
+                                         ORay<R> *, unsigned *, unsigned, int, DevCounters *);      \
+    template int octave_blocks_per_cu<R>(const OctShape &, int);
 
 }  // namespace pach

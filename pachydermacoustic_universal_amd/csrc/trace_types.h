@@ -80,6 +80,7 @@ struct TraceConst {
     int walk_steps;  // voxels a lane's walk advances per loop step of k_octave
     int shade_min;   // lanes that must be waiting before k_octave shades
     int walk_flat;   // 1: flattened walk (walk_steps counts tests + voxels)
+    int refill_min;  // idle lanes a wave gathers before it refills them
     double cotime;
     const BandTab *band;
     const RoomTab *rooms;  // [n_rooms], ray_base ascending
@@ -163,13 +164,20 @@ hipError_t launch_phase1(hipStream_t st, const TraceConst &tc, const DevScene<R>
                          unsigned *out_count, unsigned out_cap, P1State<R> *resume_out,
                          unsigned *resume_count, DevCounters *cnt);
 
+// Launch shape of k_octave: workgroup size (256 or 512) and register cap
+// (occ 0: the compiler's choice; 4: at most 128 VGPRs, 4 waves per SIMD).
+struct OctShape {
+    int block = 256;
+    int occ = 0;
+};
+
 template <class R>
-hipError_t launch_octave(hipStream_t st, int grid, const TraceConst &tc, const DevScene<R> &s,
+hipError_t launch_octave(hipStream_t st, int grid, const OctShape &sh, const TraceConst &tc, const DevScene<R> &s,
                          const DevRecv<R> &rv, const ORay<R> *in0, const unsigned *n0, unsigned cap0,
                          const ORay<R> *in1, const unsigned *n1, unsigned cap1, unsigned *in_head,
                          ORay<R> *stacks, int stack_cap, ORay<R> *dump, unsigned *dump_count,
                          unsigned dump_cap, int min_iters, DevCounters *cnt);
 
-template <class R> int octave_blocks_per_cu(int lds_bytes);
+template <class R> int octave_blocks_per_cu(const OctShape &sh, int lds_bytes);
 
 }  // namespace pach

@@ -115,8 +115,14 @@ def test_fp32_matches_fp64_statistics(tracer, medium):
     assert s32["lost"] / s32["e_init"] < 1e-5
 
 
+DEFAULT_KNOBS = {"walk_flat": 1, "walk_steps": 12, "receiver_masks": 1, "dist_lds": 1, "shade_min": 16,
+                 "refill_min": 1, "dump_min": 256, "block": 256, "occupancy": 0}
+
+
 @pytest.mark.parametrize("knobs", [{"walk_flat": 0, "walk_steps": 4}, {"receiver_masks": 0, "dist_lds": 0},
-                                   {"walk_steps": 1, "shade_min": 64}])
+                                   {"walk_steps": 1, "shade_min": 64},
+                                   {"refill_min": 24, "block": 512, "dump_min": 16},
+                                   {"occupancy": 4, "refill_min": 8, "shade_min": 8}])
 def test_launch_tuning_does_not_change_results(tracer, medium, knobs):
     room = rooms.fan_hall()
     try:
@@ -125,8 +131,7 @@ def test_launch_tuning_does_not_change_results(tracer, medium, knobs):
         st, g, cnt, ora = run_both(tracer, medium, room, rays=10, cut=0.8, seed=13,
                                    mode=A.MODE_STOCHASTIC, precision=A.PREC_FP64)
     finally:
-        for k, v in {"walk_flat": 1, "walk_steps": 12, "receiver_masks": 1, "dist_lds": 1,
-                     "shade_min": 16}.items():
+        for k, v in DEFAULT_KNOBS.items():
             tracer.tune(k, v)
     assert st["bounces"] == bounces(cnt)
     assert st["deposits"] == cnt["deposits"]
