@@ -103,6 +103,30 @@ SIGNATURES = [
     ("pach_clarity", C.c_int, [_dp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, _dp]),
 ]
 
+class DnConfig(C.Structure):
+    _fields_ = [("bins", C.c_int), ("patch", C.c_int), ("width", C.c_int), ("blocks", C.c_int),
+                ("n_desc", C.c_int), ("max_batch", C.c_int), ("beta1", C.c_float), ("beta2", C.c_float),
+                ("eps", C.c_float), ("weight_decay", C.c_float)]
+
+
+_fp = C.POINTER(C.c_float)
+_u16p = C.POINTER(C.c_uint16)
+
+# (name, restype, argtypes) of every entry point declared in include/pach_denoise.h
+DN_SIGNATURES = [
+    ("pach_dn_create", C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(DnConfig)]),
+    ("pach_dn_destroy", None, [C.c_void_p]),
+    ("pach_dn_last_error", C.c_char_p, [C.c_void_p]),
+    ("pach_dn_param_count", C.c_int64, [C.c_void_p]),
+    ("pach_dn_set_params", C.c_int, [C.c_void_p, _fp]),
+    ("pach_dn_get_params", C.c_int, [C.c_void_p, _fp]),
+    ("pach_dn_forward", C.c_int, [C.c_void_p, _fp, _fp, C.c_int, _fp]),
+    ("pach_dn_loss_grad", C.c_int, [C.c_void_p, _fp, _fp, _fp, C.c_int, _dp, _fp]),
+    ("pach_dn_dataset", C.c_int, [C.c_void_p, _fp, _fp, _fp, C.c_int64]),
+    ("pach_dn_train_step", C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_int, C.c_float, _dp]),
+    ("pach_dn_gemm", C.c_int, [C.c_void_p, _u16p, _u16p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _fp]),
+]
+
 _lib = None
 
 
@@ -119,7 +143,7 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"libpach.so not built at {LIB_PATH}; run __graft_entry__.build()")
         L = C.CDLL(LIB_PATH)
-        for name, res, args in SIGNATURES:
+        for name, res, args in SIGNATURES + DN_SIGNATURES:
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args

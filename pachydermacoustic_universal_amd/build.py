@@ -35,6 +35,7 @@ UNITS = [
     ("grid.cpp", [], False),
     ("trace_f32.hip", [], True),
     ("trace_f64.hip", ["-ffp-contract=off"], True),
+    ("denoise.hip", [], True),
 ]
 
 
