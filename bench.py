@@ -40,6 +40,30 @@ def cpu_model():
     return platform.processor()
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_octave_hall_traffic.json")
+
+
+def measured_traffic(args, tot, n_rays):
+    """HBM bytes per k_octave launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes (scaled by a plain copy's counters).  The profiled
+    command's per-launch figure is used as is when this run is the same
+    workload, else its bytes per octave bounce times this run's bounces per
+    launch.  None when no profile is committed."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            prof = json.load(f)
+    except (OSError, ValueError):
+        return None, None
+    same = (args.workload == "hall" and prof.get("rays") == n_rays and prof.get("cutoff") == args.cutoff
+            and prof.get("grid") == (args.grid or 32) and prof.get("precision") == args.precision
+            and not args.tune and not args.autotune)
+    if same:
+        return prof["hbm_bytes_per_launch"], f"measured on this workload ({prof['command']})"
+    per_bounce = prof["hbm_bytes_per_launch"] / prof["oct_bounces_per_launch"]
+    return (per_bounce * tot["oct"] / max(tot["n_oct"], 1),
+            f"{per_bounce:.2f} B per octave bounce measured on '{prof['command']}', x this run's bounces per launch")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -143,6 +167,7 @@ def main():
     # dominant kernel: the persistent octave-ray kernel
     oct_bytes = tot["oct"] * BYTES_OCT_BOUNCE + tot["spawn"] * BYTES_SPAWN
     achieved = oct_bytes / (tot["ms_oct"] / 1e3) / 1e9 if tot["ms_oct"] > 0 else 0.0
+    traffic, traffic_note = measured_traffic(args, tot, n_rays)
 
     cpu = None
     if not args.no_cpu:
@@ -193,7 +218,7 @@ def main():
                                                  "lanes_shaded", "deposits", "spills")},
                    "rays_per_s": n_rays * args.steps / dt},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_note": traffic_note,
                      "kernel": "k_octave", "avg_launch_ms": tot["ms_oct"] / max(tot["n_oct"], 1),
                      "launches": tot["n_oct"]},
         "cpu_baseline": cpu,

@@ -17,7 +17,8 @@
  *                H = H + V W2 + b2
  *   head       : Y = H Wh + bh                                  [P, 8 * patch]
  *   output     : out = x + unpatchify(Y)       (residual on the input)
- *   loss       : mean over samples, bands and bins of (out - target)^2
+ *   loss       : mean over samples, bands and bins of w[bin] (out - target)^2
+ *                (w = 1 until pach_dn_set_loss_weights)
  *
  * Parameter vector (FP32, this order): We [KE, width], be [width],
  * Pos [P, width], per block { Wc [3 width, width], bc [width], W1 [width,
@@ -66,6 +67,10 @@ int pach_dn_forward(pach_dn *dn, const float *x, const float *desc, int batch, f
  * an update; grad may be NULL. */
 int pach_dn_loss_grad(pach_dn *dn, const float *x, const float *desc, const float *target, int batch,
                       double *loss, float *grad);
+
+/* Per-bin loss weights w[bins] (>= 0) for later loss / gradient / training
+ * calls; NULL restores w = 1. */
+int pach_dn_set_loss_weights(pach_dn *dn, const float *w);
 
 /* Training set kept in device memory: x, target [n, 8, bins], desc [n, n_desc]. */
 int pach_dn_dataset(pach_dn *dn, const float *x, const float *desc, const float *target, int64_t n);

@@ -128,15 +128,16 @@ def forward(L, params, x, desc, bf16=False):
     return out, cache
 
 
-def loss_grad(L, params, x, desc, target, bf16=False):
-    """Mean squared error and its gradient (parameter-vector layout)."""
+def loss_grad(L, params, x, desc, target, bf16=False, w=None):
+    """Mean of w[bin] (out - target)^2 and its gradient (parameter-vector layout)."""
     r = bf16_round if bf16 else (lambda a: a)
     W = L.unpack(params)
     out, c = forward(L, params, x, desc, bf16)
     B = c["B"]
     diff = out - target
-    loss = float(np.mean(diff ** 2))
-    dout = 2 * diff / diff.size
+    w = np.ones(L.bins) if w is None else np.asarray(w, dtype=np.float64)
+    loss = float(np.mean(w * diff ** 2))
+    dout = 2 * w * diff / diff.size
     dY = dout.reshape(B, 8, L.P, L.patch).transpose(0, 2, 1, 3).reshape(B * L.P, L.F)
     dYb = r(dY)
     G = {}

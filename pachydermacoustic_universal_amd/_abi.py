@@ -122,6 +122,7 @@ DN_SIGNATURES = [
     ("pach_dn_get_params", C.c_int, [C.c_void_p, _fp]),
     ("pach_dn_forward", C.c_int, [C.c_void_p, _fp, _fp, C.c_int, _fp]),
     ("pach_dn_loss_grad", C.c_int, [C.c_void_p, _fp, _fp, _fp, C.c_int, _dp, _fp]),
+    ("pach_dn_set_loss_weights", C.c_int, [C.c_void_p, _fp]),
     ("pach_dn_dataset", C.c_int, [C.c_void_p, _fp, _fp, _fp, C.c_int64]),
     ("pach_dn_train_step", C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_int, C.c_float, _dp]),
     ("pach_dn_gemm", C.c_int, [C.c_void_p, _u16p, _u16p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _fp]),

@@ -261,10 +261,11 @@ __global__ void k_col2im_add(const float *__restrict__ dC, int B, int P, int D, 
     dHb[i] = f2bf(s);
 }
 
-// out = x + unpatchify(Y); with a target also the loss terms and dL/dY.
+// out = x + unpatchify(Y); with a target also the loss terms w[t] (out - tgt)^2
+// and dL/dY (w: per-bin loss weights, null for 1).
 __global__ void k_output(const float *__restrict__ Y, const float *__restrict__ x, const float *__restrict__ tgt,
-                         int B, int P, int patch, int bins, float *__restrict__ out, float *__restrict__ dY,
-                         uint16_t *__restrict__ dYb, double *__restrict__ loss_part)
+                         const float *__restrict__ w, int B, int P, int patch, int bins, float *__restrict__ out,
+                         float *__restrict__ dY, uint16_t *__restrict__ dYb, double *__restrict__ loss_part)
 {
     __shared__ double red[256];
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -280,8 +281,9 @@ __global__ void k_output(const float *__restrict__ Y, const float *__restrict__ 
         if (out) out[i] = o;
         if (tgt) {
             const float d = o - tgt[i];
-            l = (double)d * d;
-            const float g = 2.0f * d / (float)total;
+            const float wt = w ? w[t] : 1.0f;
+            l = (double)wt * d * d;
+            const float g = 2.0f * wt * d / (float)total;
             dY[yi] = g;
             dYb[yi] = f2bf(g);
         }
@@ -356,6 +358,7 @@ struct pach_dn {
     double *lossp = nullptr;
     size_t lossp_n = 0;
     float *x = nullptr, *desc = nullptr, *tgt = nullptr, *out = nullptr;
+    float *lw = nullptr;  // per-bin loss weights, null until set
     int64_t *idx = nullptr;
     float *ds_x = nullptr, *ds_desc = nullptr, *ds_t = nullptr;
     int64_t ds_n = 0;
@@ -546,7 +549,7 @@ int loss_and_grad(pach_dn *d, int B, bool grad, double *loss)
     const size_t total = (size_t)B * 8 * d->cfg.bins;
     const unsigned nb = nblk(total);
     hipLaunchKernelGGL(k_output, dim3(nb), dim3(256), 0, d->st, (const float *)d->Y, (const float *)d->x,
-                       (const float *)d->tgt, B, d->P, d->cfg.patch, d->cfg.bins, (float *)nullptr, d->dY, d->dYb,
+                       (const float *)d->tgt, (const float *)d->lw, B, d->P, d->cfg.patch, d->cfg.bins, (float *)nullptr, d->dY, d->dYb,
                        d->lossp);
     DNCHK(d, hipGetLastError());
     if (grad)
@@ -665,6 +668,7 @@ extern "C" void pach_dn_destroy(pach_dn *d)
     if (d->ds_x) (void)hipFree(d->ds_x);
     if (d->ds_t) (void)hipFree(d->ds_t);
     if (d->ds_desc) (void)hipFree(d->ds_desc);
+    if (d->lw) (void)hipFree(d->lw);
     if (d->st) (void)hipStreamDestroy(d->st);
     delete d;
 }
@@ -718,7 +722,7 @@ extern "C" int pach_dn_forward(pach_dn *d, const float *x, const float *desc, in
     if (int rc = forward(d, batch)) return rc;
     const size_t total = (size_t)batch * 8 * d->cfg.bins;
     hipLaunchKernelGGL(k_output, dim3(nblk(total)), dim3(256), 0, d->st, (const float *)d->Y, (const float *)d->x,
-                       (const float *)nullptr, batch, d->P, d->cfg.patch, d->cfg.bins, d->out, (float *)nullptr,
+                       (const float *)nullptr, (const float *)nullptr, batch, d->P, d->cfg.patch, d->cfg.bins, d->out, (float *)nullptr,
                        (uint16_t *)nullptr, (double *)nullptr);
     DNCHK(d, hipGetLastError());
     DNCHK(d, hipMemcpyAsync(out, d->out, total * 4, hipMemcpyDeviceToHost, d->st));
@@ -738,6 +742,24 @@ extern "C" int pach_dn_loss_grad(pach_dn *d, const float *x, const float *desc, 
         DNCHK(d, hipMemcpyAsync(grad, d->g, (size_t)d->n_params * 4, hipMemcpyDeviceToHost, d->st));
         DNCHK(d, hipStreamSynchronize(d->st));
     }
+    return PACH_OK;
+}
+
+extern "C" int pach_dn_set_loss_weights(pach_dn *d, const float *w)
+{
+    if (!d) return PACH_EINVAL;
+    (void)hipSetDevice(d->device);
+    DNCHK(d, hipStreamSynchronize(d->st));
+    if (!w) {
+        if (d->lw) (void)hipFree(d->lw);
+        d->lw = nullptr;
+        return PACH_OK;
+    }
+    for (int t = 0; t < d->cfg.bins; ++t)
+        if (!(w[t] >= 0) || !std::isfinite(w[t])) return dn_fail(d, PACH_EINVAL, "loss weights must be finite and >= 0");
+    if (!d->lw && hipMalloc((void **)&d->lw, (size_t)d->cfg.bins * 4) != hipSuccess)
+        return dn_fail(d, PACH_ENOMEM, "loss weights");
+    DNCHK(d, hipMemcpy(d->lw, w, (size_t)d->cfg.bins * 4, hipMemcpyHostToDevice));
     return PACH_OK;
 }
 

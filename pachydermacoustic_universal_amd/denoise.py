@@ -7,11 +7,16 @@ weights): parameter initialisation, the histogram normalisation, room
 descriptors, low/high ray-count training pairs traced by the tracer itself,
 the training loop, and inference that returns denoised energy histograms.
 
-Normalisation: each band of a histogram E[8, bins] (energy per bin, already
-divided by the ray count as Receiver_Bank.Scale does) becomes
-x = (log10(max(E / S, 1e-10)) + 5) / 2.5 with S the band's total energy in the
-low-ray input; the network's output y maps back to S * 10^(2.5 y - 5), and
-bins within 0.05 of the floor are returned as zero.
+Representation: each band of a histogram E[8, bins] (energy per bin, already
+divided by the ray count as Receiver_Bank.Scale does) enters as its
+normalised log backward-integrated (Schroeder) curve,
+x(t) = (log10(max(Sch(t) / S, 1e-10)) + 5) / 2.5 with Sch(t) = sum over
+tau >= t of E(tau) and S = Sch(0) of the low-ray input: T30 and EDT are line
+fits to exactly this curve (in dB) and C80 is a ratio of two of its values,
+so a mean-squared error on it weighs the bins by what the room-acoustic
+parameters see.  The network's output y maps back to a curve
+S * 10^(2.5 y - 5) (zero within 0.05 of the floor), made non-increasing, and
+its backward differences are the denoised histogram.
 """
 from __future__ import annotations
 
@@ -28,7 +33,7 @@ from . import rooms as RM
 FLOOR = 1e-10
 WEIGHTS_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights")
 DEFAULT_WEIGHTS = os.path.join(WEIGHTS_DIR, "denoiser.npz")
-DEFAULTS = dict(bins=3000, patch=24, width=256, blocks=4, n_desc=10, max_batch=256)
+DEFAULTS = dict(bins=3000, patch=24, width=256, blocks=4, n_desc=12, max_batch=256)
 
 
 def layout(bins, patch, width, blocks, n_desc, **_):
@@ -77,27 +82,42 @@ def init_params(cfg, seed=0):
 def normalize(E):
     """E [..., 8, bins] -> (x float32 [..., 8, bins], S [..., 8, 1])."""
     E = np.asarray(E, dtype=np.float64)
-    S = np.maximum(E.sum(-1, keepdims=True), 1e-300)
-    x = (np.log10(np.maximum(E / S, FLOOR)) + 5.0) / 2.5
+    sch = np.cumsum(E[..., ::-1], axis=-1)[..., ::-1]
+    S = np.maximum(sch[..., :1], 1e-300)
+    x = (np.log10(np.maximum(sch / S, FLOOR)) + 5.0) / 2.5
     return x.astype(np.float32), S
 
 
 def denormalize(y, S):
+    """Network output (normalised log Schroeder curve) -> energy histogram."""
     y = np.asarray(y, dtype=np.float64)
-    E = S * 10.0 ** (2.5 * y - 5.0)
-    E[y <= (np.log10(FLOOR) + 5.0) / 2.5 + 0.05] = 0.0
+    sch = S * 10.0 ** (2.5 * y - 5.0)
+    sch[y <= (np.log10(FLOOR) + 5.0) / 2.5 + 0.05] = 0.0
+    sch = np.minimum.accumulate(sch, axis=-1)
+    E = sch.copy()
+    E[..., :-1] -= sch[..., 1:]
     return E
 
 
-def descriptors(room: RM.Room):
-    """log10 volume / 4, log10 surface area / 4, area-weighted mean absorption per band."""
+def descriptors(room: RM.Room, receivers=None):
+    """Per-receiver descriptors [R, 12]: log10 volume / 4, log10 surface area
+    / 4, area-weighted mean absorption per band (8), log10 source-receiver
+    distance / 2, area-weighted mean scattering."""
     v = room.verts
     area_t = 0.5 * np.linalg.norm(np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0]), axis=1)
     area_s = np.bincount(room.surf, weights=area_t, minlength=room.n_surfaces)
     total = area_s.sum()
     vol = room.volume if room.volume > 0 else RM._tri_volume(room.verts)
     abs_mean = (area_s[:, None] * room.alpha).sum(0) / total
-    return np.concatenate([[np.log10(vol) / 4, np.log10(total) / 4], abs_mean]).astype(np.float32)
+    scat_mean = float((area_s[:, None] * room.scat).sum() / (8 * total))
+    recs = np.asarray(room.receivers if receivers is None else receivers, dtype=np.float64).reshape(-1, 3)
+    dist = np.maximum(np.linalg.norm(recs - room.source, axis=1), 1e-3)
+    head = np.concatenate([[np.log10(vol) / 4, np.log10(total) / 4], abs_mean])
+    out = np.empty((recs.shape[0], 12))
+    out[:, :10] = head
+    out[:, 10] = np.log10(dist) / 2
+    out[:, 11] = scat_mean
+    return out.astype(np.float32)
 
 
 class Denoiser:
@@ -178,6 +198,15 @@ class Denoiser:
                                             x.shape[0], C.byref(loss), None if g is None else self._f(g)))
         return loss.value, g
 
+    def set_loss_weights(self, w):
+        """Per-bin loss weights [bins] (None: uniform)."""
+        if w is None:
+            self._chk(A.lib().pach_dn_set_loss_weights(self.h, None))
+            return
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == self.cfg["bins"]
+        self._chk(A.lib().pach_dn_set_loss_weights(self.h, self._f(w)))
+
     def set_dataset(self, x, desc, target):
         x = np.ascontiguousarray(x, dtype=np.float32)
         t = np.ascontiguousarray(target, dtype=np.float32)
@@ -220,7 +249,7 @@ class Denoiser:
     def denoise(self, E, desc):
         """Energy histograms [B, 8, bins] (scaled) -> denoised energy histograms."""
         x, S = normalize(E)
-        y = self.forward(x, np.broadcast_to(desc, (x.shape[0], self.cfg["n_desc"])))
+        y = self.forward(x, np.broadcast_to(desc, (x.shape[0], self.cfg["n_desc"])))  # desc: [B or 1, n_desc]
         return denormalize(y, S)
 
 
@@ -235,8 +264,9 @@ def trace_energy(tr, packed, rays, cutoff_s, seed, precision=A.PREC_FP32):
 def make_pairs(tr, room_list, lo_rays, hi_rays, cutoff_s=3.0, lo_repeats=1, seed=0, pack=16, log=None):
     """Traces every room at lo_rays (lo_repeats independent seeds) and hi_rays.
     Returns dict(x [N, 8, bins] normalised low-ray input, t [N, 8, bins]
-    normalised high-ray target (same S), desc [N, n_desc], room [N], rec [N])."""
-    xs, ts, ds, ri, rr = [], [], [], [], []
+    normalised high-ray target (same S), S [N, 8, 1] the input's band
+    energies, desc [N, n_desc], room [N], rec [N])."""
+    xs, ts, ss, ds, ri, rr = [], [], [], [], [], []
     for g0 in range(0, len(room_list), pack):
         grp = room_list[g0:g0 + pack]
         pk = RM.pack_rooms(grp)
@@ -251,34 +281,58 @@ def make_pairs(tr, room_list, lo_rays, hi_rays, cutoff_s=3.0, lo_repeats=1, seed
             d = descriptors(room)
             for E in Elo:
                 x, S = normalize(E[r0:r0 + k])
-                t = (np.log10(np.maximum(Ehi[r0:r0 + k] / S, FLOOR)) + 5.0) / 2.5
+                sch_hi = np.cumsum(Ehi[r0:r0 + k][..., ::-1], axis=-1)[..., ::-1]
+                t = (np.log10(np.maximum(sch_hi / S, FLOOR)) + 5.0) / 2.5
                 xs.append(x)
+                ss.append(S)
                 ts.append(t.astype(np.float32))
-                ds.append(np.repeat(d[None], k, 0))
+                ds.append(d)
                 ri.append(np.full(k, g0 + i))
                 rr.append(np.arange(k))
             r0 += k
         if log:
             log(f"pairs: rooms {g0}-{g0 + n - 1} traced in {time.perf_counter() - t0:.1f} s")
-    return dict(x=np.concatenate(xs), t=np.concatenate(ts), desc=np.concatenate(ds), room=np.concatenate(ri),
-                rec=np.concatenate(rr))
+    return dict(x=np.concatenate(xs), t=np.concatenate(ts), S=np.concatenate(ss), desc=np.concatenate(ds),
+                room=np.concatenate(ri), rec=np.concatenate(rr))
 
 
-def train(dn, data, steps, batch=256, lr=1e-3, warmup=100, seed=0, log=None, log_every=100):
-    """AdamW with linear warm-up and cosine decay; returns the loss history."""
+def early_weights(bins, fs=1000, gain=20.0, tau_s=0.15):
+    """Loss weights 1 + gain exp(-t / tau): the first few hundred ms (EDT,
+    C80) weigh as much as the rest of the decay."""
+    t = np.arange(bins) / fs
+    return (1.0 + gain * np.exp(-t / tau_s)).astype(np.float32)
+
+
+def train(dn, data, steps, batch=256, lr=1e-3, warmup=100, seed=0, log=None, log_every=100, val=None,
+          eval_every=250, weights=None):
+    """AdamW with linear warm-up and cosine decay.  With validation pairs the
+    parameters with the lowest validation loss (checked every eval_every
+    steps) are kept.  Returns (loss history, validation history)."""
     dn.set_dataset(data["x"], data["desc"], data["t"])
+    vw = 1.0 if weights is None else weights
+    dn.set_loss_weights(weights)
     rng = np.random.default_rng(seed)
     n = data["x"].shape[0]
-    hist = []
+    hist, vhist = [], []
+    best = (np.inf, None, -1)
     t0 = time.perf_counter()
     for s in range(steps):
         a = min(1.0, (s + 1) / warmup) * 0.5 * (1 + np.cos(np.pi * s / steps))
         idx = rng.choice(n, size=min(batch, n), replace=False)
         loss = dn.train_step(idx, lr * a)
         hist.append(loss)
+        if val is not None and ((s + 1) % eval_every == 0 or s == steps - 1):
+            vl = float(np.mean(vw * (dn.forward(val["x"], val["desc"]) - val["t"]) ** 2))
+            vhist.append((s + 1, vl))
+            if vl < best[0]:
+                best = (vl, dn.get_params(), s + 1)
+            if log:
+                log(f"step {s + 1}: validation loss {vl:.3e} (best {best[0]:.3e} at {best[2]})")
         if log and (s % log_every == 0 or s == steps - 1):
             log(f"train step {s}: loss {loss:.5f} ({time.perf_counter() - t0:.1f} s)")
-    return np.array(hist)
+    if best[1] is not None:
+        dn.set_params(best[1])
+    return np.array(hist), vhist
 
 
 def decay_params(E, fs=1000):
@@ -295,6 +349,37 @@ def decay_params(E, fs=1000):
 
 def errors(P, Pref):
     """Relative T30 / EDT errors and absolute C80 error (dB) against a reference."""
-    return dict(t30=np.abs(P[..., 0] - Pref[..., 0]) / Pref[..., 0],
-                edt=np.abs(P[..., 1] - Pref[..., 1]) / Pref[..., 1],
-                c80=np.abs(P[..., 2] - Pref[..., 2]))
+    with np.errstate(invalid="ignore"):
+        return dict(t30=np.abs(P[..., 0] - Pref[..., 0]) / Pref[..., 0],
+                    edt=np.abs(P[..., 1] - Pref[..., 1]) / Pref[..., 1],
+                    c80=np.abs(P[..., 2] - Pref[..., 2]))
+
+
+def summarize(err, t30=0.05, edt=0.05, c80=1.0):
+    """Mean / median / 90th percentile of each error and the share of
+    (receiver, band) pairs inside the just-noticeable differences."""
+    lim = dict(t30=t30, edt=edt, c80=c80)
+    out = {}
+    for k, e in err.items():
+        e = e[np.isfinite(e)]
+        out[k] = dict(mean=float(e.mean()), median=float(np.median(e)), p90=float(np.percentile(e, 90)),
+                      within=float((e <= lim[k]).mean()))
+    return out
+
+
+def trace_with_progress(tr, log, every_s=20.0, **kw):
+    """pach_trace on the context's worker thread, logging progress; returns wall seconds."""
+    t0 = time.perf_counter()
+    tr.trace(async_=True, **kw)
+    last = t0
+    while tr.poll()["running"]:
+        time.sleep(0.05)
+        if time.perf_counter() - last > every_s:
+            p = tr.poll()
+            log(f"trace: {p['rays_done']} of {p['rays_total']} source rays "
+                f"({time.perf_counter() - t0:.0f} s)")
+            last = time.perf_counter()
+    rc = tr.wait()
+    if rc != 0:
+        raise A.PachError(rc, A.lib().pach_last_error(tr.h).decode())
+    return time.perf_counter() - t0

@@ -170,6 +170,100 @@ def fan_hall(seed=2, nu=14, nv=18, nw=8, height=12.0, front_half=8.0, back_half=
                 volume=_tri_volume(tris), meta={"grid": (nu, nv, nw), "jitter": jitter})
 
 
+def hall_variant(seed, n_receivers=16, radius=0.5) -> Room:
+    """The polyhedral-hall generator with seeded proportions: height U(7, 16) m,
+    stage half-width U(5, 10) m splaying to a back half-width U(+2, 22) m over a
+    depth U(18, 40) m, source on the stage; jitter, materials and receivers as
+    fan_hall (the denoiser's training rooms)."""
+    rng = np.random.default_rng(1_000_003 + seed)
+    height = rng.uniform(7, 16)
+    front = rng.uniform(5, 10)
+    back = rng.uniform(front + 2, 22)
+    depth = rng.uniform(18, 40)
+    src = (rng.uniform(-0.5, 0.5) * front, rng.uniform(2.0, 0.2 * depth), 1.5)
+    return fan_hall(seed=seed, height=height, front_half=front, back_half=back, depth=depth,
+                    n_receivers=n_receivers, radius=radius, source=src, name=f"hall_variant{seed}")
+
+
+def _box_tris(lo, hi, bottom=True):
+    """Axis-aligned box [lo, hi] as triangles with outward normals (the room
+    side of an obstacle); without its bottom face when bottom is False."""
+    x0, y0, z0 = lo
+    x1, y1, z1 = hi
+    P = np.array([[x0, y0, z0], [x1, y0, z0], [x1, y1, z0], [x0, y1, z0],
+                  [x0, y0, z1], [x1, y0, z1], [x1, y1, z1], [x0, y1, z1]])
+    faces = [[0, 4, 5, 1], [3, 2, 6, 7], [0, 3, 7, 4], [1, 5, 6, 2], [4, 7, 6, 5]]
+    if bottom:
+        faces.append([0, 1, 2, 3])
+    # the shoebox order above gives inward normals; reversed: outward
+    return _quads_to_tris(P[np.array(faces)][:, ::-1])[0]
+
+
+def concert_hall(seed=5, n_receivers=64, radius=0.5, name="concert_hall") -> Room:
+    """Procedural concert hall of ~50,000 triangles: a 28 x 44 x 18 m shell
+    subdivided into 0.5 m quads (walls and ceiling vertex-jittered +-5 cm, the
+    floor flat), 180 seat-block boxes (4 x 0.8 x 0.45 m, open to the floor) in
+    six columns over the audience area, and four balcony slabs (3 x 30 x
+    0.8 m, subdivided) along the side walls at 5 and 9 m.  Each quad or box
+    face is a polygon with absorption U(0.02, 0.40) and scattering
+    U(0.05, 0.70) per band; the source is on the stage, and 64 receivers sit
+    on a jittered 8 x 8 grid 1.25 m over the floor of the audience area."""
+    rng = np.random.default_rng(seed)
+    W, Lh, H = 28.0, 44.0, 18.0
+
+    def shell_map(u, v, w):
+        return (-W / 2 + W * u, Lh * v, H * w)
+
+    quads = _deformed_box(shell_map, 56, 88, 36, 0.05, rng)
+    floor = np.abs(quads[..., 2]) < 0.08
+    quads[..., 2][floor] = 0.0
+    parts = [quads]
+    # seat blocks: 6 columns x 30 rows, 1 m pitch, aisles between columns
+    boxes = []
+    for col in range(6):
+        x0 = -12.5 + col * 4.2
+        for row in range(30):
+            y0 = 12.0 + row * 0.95
+            boxes.append(_box_tris((x0, y0, 0.0), (x0 + 4.0, y0 + 0.8, 0.45), bottom=False))
+    # balcony slabs, 0.2 m clear of the side walls, as subdivided closed boxes
+    slabs = []
+    for side in (-1, 1):
+        for z in (5.0, 9.0):
+            xa = side * (W / 2 - 0.2)
+            xb = side * (W / 2 - 3.2)
+            lo = (min(xa, xb), 10.0, z)
+            hi = (max(xa, xb), 40.0, z + 0.8)
+
+            def slab_map(u, v, w, lo=lo, hi=hi):
+                return (lo[0] + (hi[0] - lo[0]) * u, lo[1] + (hi[1] - lo[1]) * v, lo[2] + (hi[2] - lo[2]) * w)
+
+            q = _deformed_box(slab_map, 6, 60, 2, 0.0, rng)
+            slabs.append(q[:, ::-1])  # outward normals: the room side of the slab
+    shell_tris, shell_surf = _quads_to_tris(parts[0])
+    tris = [shell_tris]
+    surf = [shell_surf]
+    nsurf = int(shell_surf.max()) + 1
+    for q in slabs:
+        t, sf = _quads_to_tris(q)
+        tris.append(t)
+        surf.append(sf + nsurf)
+        nsurf += q.shape[0]
+    for b in boxes:
+        tris.append(b)
+        surf.append(nsurf + np.arange(b.shape[0], dtype=np.int32) // 2)
+        nsurf += b.shape[0] // 2
+    tris = np.concatenate(tris)
+    surf = np.concatenate(surf).astype(np.int32)
+    alpha = rng.uniform(0.02, 0.40, (nsurf, NBANDS))
+    scat = rng.uniform(0.05, 0.70, (nsurf, NBANDS))
+    gx, gy = np.meshgrid(np.linspace(-10.5, 10.5, 8), np.linspace(14.0, 38.0, 8))
+    recs = np.stack([gx.ravel() + rng.uniform(-0.4, 0.4, 64), gy.ravel() + rng.uniform(-0.4, 0.4, 64),
+                     np.full(64, 1.25)], axis=1)[:n_receivers]
+    return Room(name=name, verts=tris, surf=surf, alpha=alpha, scat=scat, trans=np.zeros((nsurf, NBANDS)),
+                source=np.array([0.0, 4.0, 1.5]), receivers=recs, radius=float(radius),
+                volume=_tri_volume(tris), meta={"seed": seed})
+
+
 def small_workload() -> Room:
     """Plain method, small: the 12-triangle shoebox."""
     return shoebox()

@@ -22,7 +22,8 @@ def tiny(seed=0, bins=48, patch=8, width=64, blocks=2, n_desc=3, B=2):
 
 def test_backward_matches_finite_differences():
     L, params, x, desc, tgt = tiny()
-    loss, g = R.loss_grad(L, params, x, desc, tgt)
+    w = np.random.default_rng(7).uniform(0.5, 2.0, L.bins)
+    loss, g = R.loss_grad(L, params, x, desc, tgt, w=w)
     rng = np.random.default_rng(1)
     # every tensor gets a few probes
     probes = []
@@ -33,7 +34,7 @@ def test_backward_matches_finite_differences():
     for i in probes:
         p1 = params.copy(); p1[i] += h
         p2 = params.copy(); p2[i] -= h
-        fd = (R.loss_grad(L, p1, x, desc, tgt)[0] - R.loss_grad(L, p2, x, desc, tgt)[0]) / (2 * h)
+        fd = (R.loss_grad(L, p1, x, desc, tgt, w=w)[0] - R.loss_grad(L, p2, x, desc, tgt, w=w)[0]) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)) + 1e-8, (i, fd, g[i])
 
 
@@ -92,7 +93,14 @@ def test_normalisation_round_trip():
     E = rng.exponential(1.0, (3, 8, 300)) * np.exp(-np.arange(300) / 40.0)
     E[:, :, :5] = 0
     x, S = DN.normalize(E)
+    # the input is the normalised log Schroeder curve: non-increasing, 2 at t = 0
+    assert np.all(np.diff(x, axis=-1) <= 1e-6) and np.allclose(x[..., 0], 2.0)
     back = DN.denormalize(x, S)
-    ok = E / S > 1e-9
-    np.testing.assert_allclose(back[ok], E[ok], rtol=1e-5)
-    assert np.all(back[:, :, :5] == 0)
+    sch = np.cumsum(E[..., ::-1], axis=-1)[..., ::-1]
+    sch_back = np.cumsum(back[..., ::-1], axis=-1)[..., ::-1]
+    ok = sch / S > 1e-8
+    np.testing.assert_allclose(sch_back[ok], sch[ok], rtol=1e-5)
+    # bins are differences of the FP32-coded curve: exact to ~1e-6 of the curve
+    assert np.all(np.abs(back - E)[ok] <= 1e-5 * sch[ok])
+    assert np.all(back[:, :, :5] == 0) and np.all(back >= 0)
+    np.testing.assert_allclose(back.sum(-1), E.sum(-1), rtol=1e-6)

@@ -4,8 +4,9 @@
     product of the same BF16 operands (FP32 accumulation: 1e-5 of the scale).
   * Forward pass: the oracle with BF16 rounding at the same points; 2e-3
     absolute on outputs of order one.
-  * Loss and gradient: relative 2-norm error per parameter tensor below 2e-2
-    (BF16 operands; the rounding of an intermediate can flip one BF16 step).
+  * Loss (with per-bin weights) and gradient: relative 2-norm error per
+    parameter tensor below 2e-2 (BF16 operands; the rounding of an
+    intermediate can flip one BF16 step).
   * AdamW training on a fixed batch lowers the loss.
 """
 import numpy as np
@@ -72,9 +73,14 @@ def test_forward_matches_oracle(net):
 def test_gradient_matches_oracle(net):
     L, params, x, desc, tgt = random_problem(2)
     print("[case] denoiser loss/grad B=3", flush=True)
+    w = np.random.default_rng(5).uniform(0.5, 3.0, L.bins).astype(np.float32)
     net.set_params(params)
-    loss, g = net.loss_grad(x, desc, tgt)
-    rloss, rg = R.loss_grad(L, params, x, desc, tgt, bf16=True)
+    net.set_loss_weights(w)
+    try:
+        loss, g = net.loss_grad(x, desc, tgt)
+    finally:
+        net.set_loss_weights(None)
+    rloss, rg = R.loss_grad(L, params, x, desc, tgt, bf16=True, w=w.astype(np.float64))
     assert abs(loss - rloss) <= 1e-3 * rloss
     for name, o in L.offsets.items():
         n = int(np.prod(L.shapes[name]))
