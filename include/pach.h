@@ -183,20 +183,22 @@ int pach_rooms_set(pach_ctx *ctx, int n, const double *src, const double *power,
  * threshold); "dump_min" (>= 1): loop steps a drained wave runs before handing
  * its leftover rays to the next launch; "block" (256/512): workgroup size of
  * the octave kernel; "occupancy" (0/4): 4 caps registers at 128 VGPRs so four
- * waves share each SIMD. */
+ * waves share each SIMD; "shade_queue" (0/1): a lane whose walk ended queues
+ * its hit and walks on, and the wave shades 64 queued hits at a time. */
 int pach_tune(pach_ctx *ctx, const char *key, int64_t value);
 
 /* Autotuner result (no counterpart in the reference). */
 typedef struct pach_tuning {
     int grid[3];           /* voxel grid dimensions chosen (now uploaded) */
     int block, occupancy;  /* octave-kernel workgroup size and register cap */
+    int shade_queue;       /* 1: finished walks are shaded 64 at a time from a per-wave queue */
     int walk_steps, shade_min, refill_min, dump_min;
     int trials;            /* sample traces run */
     double rate_start;     /* ray-bounces/s of the sample under the settings on entry */
     double rate_best;      /* ... under the chosen settings */
 } pach_tuning;
 
-/* Picks voxel resolution, workgroup size, register cap, walk budget, shading
+/* Picks voxel resolution, shade queue, workgroup size, register cap, walk budget, shading
  * and refill thresholds for this scene, receivers and trace parameters by
  * timing sample traces (coordinate descent over each knob's candidates,
  * starting from the current settings), then applies the best: the grid is

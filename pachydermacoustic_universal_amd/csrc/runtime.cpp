@@ -37,6 +37,8 @@ template <class R> struct WorkBuf {
     ORay<R> *stacks = nullptr;
     int stack_cap = 0;
     size_t n_waves = 0;
+    ShadeItem<R> *shq = nullptr;  // per-wave shade queues (shade_queue on)
+    size_t shq_waves = 0;
     ORay<R> *dump[2] = {nullptr, nullptr};
     unsigned dump_cap = 0;
 };
@@ -598,7 +600,7 @@ template <class R> const typename V4<R>::T *ctr_of(pach_ctx *c);
 template <> const float4 *ctr_of<float>(pach_ctx *c) { return c->d_ctr32; }
 template <> const double4 *ctr_of<double>(pach_ctx *c) { return c->d_ctr64; }
 
-template <class R> int ensure_work(pach_ctx *c, int64_t batch, size_t waves)
+template <class R> int ensure_work(pach_ctx *c, int64_t batch, size_t waves, bool queue)
 {
     WorkBuf<R> &w = work_of<R>(c);
     const unsigned p1_cap = (unsigned)std::min<int64_t>(batch * 200 + 4096, (int64_t)1 << 31);
@@ -619,6 +621,10 @@ template <class R> int ensure_work(pach_ctx *c, int64_t batch, size_t waves)
         if ((e = dalloc(w.stacks, waves * (size_t)stack_cap)) != hipSuccess) return fail(c, PACH_ENOMEM, "wave stacks");
         w.n_waves = waves;
         w.stack_cap = stack_cap;
+    }
+    if (queue && w.shq_waves < waves) {
+        if ((e = dalloc(w.shq, waves * (size_t)kShadeQ)) != hipSuccess) return fail(c, PACH_ENOMEM, "shade queues");
+        w.shq_waves = waves;
     }
     if (w.dump_cap < dump_cap) {
         if ((e = dalloc(w.dump[0], dump_cap)) != hipSuccess || (e = dalloc(w.dump[1], dump_cap)) != hipSuccess)
@@ -741,7 +747,7 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
     batch = std::max<int64_t>(1, std::min<int64_t>(batch, 1 << 22));
     const OctShape shape = c->tune_shape;
     const int grid = c->n_cu * octave_blocks_per_cu<R>(shape, s.dist_lds ? lds_bytes : 0);
-    int rc = ensure_work<R>(c, batch, (size_t)grid * (shape.block / 64));
+    int rc = ensure_work<R>(c, batch, (size_t)grid * (shape.block / 64), shape.queue != 0);
     if (rc) return rc;
     WorkBuf<R> &w = work_of<R>(c);
     unsigned *p1_count = c->d_scal + 0, *res_count = c->d_scal + 1, *in_head = c->d_scal + 2;
@@ -789,7 +795,8 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
         HIPCHK(c, hipEventRecord(c->ev[2], st));
         HIPCHK(c, launch_octave<R>(st, grid, shape, tc, s, rv, pending ? w.dump[cur] : nullptr, dump_count + cur,
                                    w.dump_cap, p1_launched ? w.p1_out : nullptr, p1_count, w.p1_cap, in_head,
-                                   w.stacks, w.stack_cap, w.dump[1 - cur], dump_count + (1 - cur), w.dump_cap,
+                                   w.stacks, w.stack_cap, shape.queue ? w.shq : nullptr, w.dump[1 - cur],
+                                   dump_count + (1 - cur), w.dump_cap,
                                    min_iters, c->d_cnt));
         HIPCHK(c, hipEventRecord(c->ev[3], st));
         unsigned h[8];
@@ -919,6 +926,10 @@ extern "C" int pach_tune(pach_ctx *c, const char *key, int64_t value)
         c->tune_shape.occ = (int)value;
         return PACH_OK;
     }
+    if (std::strcmp(key, "shade_queue") == 0 && (value == 0 || value == 1)) {
+        c->tune_shape.queue = (int)value;
+        return PACH_OK;
+    }
     return fail(c, PACH_EINVAL, std::string("unknown tuning key ") + key);
 }
 
@@ -989,11 +1000,11 @@ extern "C" int pach_autotune(pach_ctx *c, const pach_params *p, int64_t sample_r
         pp.ray_count = sample;
     }
     struct Knobs {
-        int grid[3], block, occ, walk_steps, shade_min, refill_min, dump_min;
+        int grid[3], block, occ, queue, walk_steps, shade_min, refill_min, dump_min;
     };
     auto current = [c]() {
-        Knobs k{{c->g[0], c->g[1], c->g[2]}, c->tune_shape.block, c->tune_shape.occ, c->tune_walk_steps,
-                c->tune_shade_min, c->tune_refill_min, c->tune_dump_min};
+        Knobs k{{c->g[0], c->g[1], c->g[2]}, c->tune_shape.block, c->tune_shape.occ, c->tune_shape.queue,
+                c->tune_walk_steps, c->tune_shade_min, c->tune_refill_min, c->tune_dump_min};
         return k;
     };
     auto apply = [c](const Knobs &k) -> int {
@@ -1001,6 +1012,7 @@ extern "C" int pach_autotune(pach_ctx *c, const pach_params *p, int64_t sample_r
             if (int rc = upload_grid(c, k.grid)) return rc;
         c->tune_shape.block = k.block;
         c->tune_shape.occ = k.occ;
+        c->tune_shape.queue = k.queue;
         c->tune_walk_steps = k.walk_steps;
         c->tune_shade_min = k.shade_min;
         c->tune_refill_min = k.refill_min;
@@ -1044,6 +1056,7 @@ extern "C" int pach_autotune(pach_ctx *c, const pach_params *p, int64_t sample_r
                      k.grid[a] = std::max(1, std::min(1024, (int)std::lround(start.grid[a] * q / 100.0)));
              }, pct)))
             break;
+        if ((rc = try_knob([](Knobs &k, int v) { k.queue = v; }, {0, 1}))) break;
         if ((rc = try_knob([](Knobs &k, int v) { k.occ = v; }, {0, 4}))) break;
         if ((rc = try_knob([](Knobs &k, int v) { k.block = v; }, {256, 512}))) break;
         if ((rc = try_knob([](Knobs &k, int v) { k.walk_steps = v; }, {6, 8, 12, 16, 24, 32}))) break;
@@ -1062,6 +1075,7 @@ extern "C" int pach_autotune(pach_ctx *c, const pach_params *p, int64_t sample_r
     for (int a = 0; a < 3; ++a) out->grid[a] = best.grid[a];
     out->block = best.block;
     out->occupancy = best.occ;
+    out->shade_queue = best.queue;
     out->walk_steps = best.walk_steps;
     out->shade_min = best.shade_min;
     out->refill_min = best.refill_min;

@@ -866,13 +866,17 @@ __device__ __forceinline__ bool octave_shade(const TraceConst &tc, const DevScen
 // BS: workgroup size (waves share the LDS copy of the distance field);
 // OCC: 0 = compiler's register allocation, else waves per SIMD to fit
 // (a register cap of 512 / OCC VGPRs).
-template <class R, bool LDS, int BS, int OCC>
+// SQ: shade queue.  A lane whose walk ended writes (ray, hit) to its wave's
+// queue (kShadeQ items in HBM/L2) and refills at once; whenever 64 items wait
+// (or no lane is walking) every lane shades one, so the shading code runs on
+// full waves, and continuations and spawned rays go onto the wave's stack.
+template <class R, bool LDS, int BS, int OCC, bool SQ>
 __global__ void __launch_bounds__(BS, OCC > 0 ? OCC : 1)
 k_octave(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv,
          const ORay<R> *__restrict__ in0, const unsigned *n0_ptr, unsigned cap0,
          const ORay<R> *__restrict__ in1, const unsigned *n1_ptr, unsigned cap1, unsigned *in_head,
-         ORay<R> *stacks, int stack_cap, ORay<R> *dump, unsigned *dump_count, unsigned dump_cap,
-         int min_iters, DevCounters *cnt)
+         ORay<R> *stacks, int stack_cap, ShadeItem<R> *shq, ORay<R> *dump, unsigned *dump_count,
+         unsigned dump_cap, int min_iters, DevCounters *cnt)
 {
     if constexpr (LDS) {
         // stage the voxel distances (ncell rounded up to 16 bytes) in LDS
@@ -888,6 +892,8 @@ k_octave(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv,
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const uint64_t wave = (uint64_t)blockIdx.x * (BS / 64) + (threadIdx.x >> 6);
     ORay<R> *ws = stacks + wave * (uint64_t)stack_cap;
+    ShadeItem<R> *wq = SQ ? shq + wave * (uint64_t)kShadeQ : nullptr;
+    int qn = 0;  // items in the wave's shade queue
     const unsigned n0 = in0 ? min(*n0_ptr, cap0) : 0u;
     const unsigned n1 = in1 ? min(*n1_ptr, cap1) : 0u;
     const unsigned n_in = n0 + n1;
@@ -951,7 +957,7 @@ k_octave(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv,
             }
         }
         const uint64_t am = __ballot(act);
-        if (am == 0) {
+        if (am == 0 && qn == 0) {
             if (input_done) break;
             continue;  // the claim returned only dead slots: claim again
         }
@@ -979,33 +985,71 @@ k_octave(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv,
         }
         ++iters;
         if (lane == 0) lc.v[C_ITER]++;
-        // ---- (c) shade the lanes whose walk ended
-        const int n_ready = __popcll(__ballot(ready));
-        if (n_ready > 0 && (n_ready >= tc.shade_min || __ballot(walking) == 0)) {
-            bool has_child = false;
-            ORay<R> child;
-            if (lane == 0) lc.v[C_SHADE]++;
-            if (ready) {
-                lc.v[C_SHADED]++;
-                act = octave_shade<R>(tc, s, rv, r, w.best, w.tbest, w.rmask, has_child, child, lc);
-                ready = false;
-            }
-            // push spawned rays onto the wave's stack (overflow goes to the dump queue)
-            const uint64_t cm = __ballot(has_child);
-            if (cm) {
-                const int n = __popcll(cm);
-                if (has_child) {
-                    const int pos = stack_n + __popcll(cm & lt_mask);
-                    if (pos < stack_cap) {
-                        ws[pos] = child;
-                    } else {
-                        const unsigned q = atomicAdd(dump_count, 1u);
-                        if (q < dump_cap) { dump[q] = child; lc.v[C_SPILL]++; }
-                        else { lc.v[C_DROP]++; lc.lost += (double)child.I; }
-                    }
+        // push a ray onto the wave's stack (overflow goes to the dump queue)
+        auto push = [&](bool has, const ORay<R> &ray) {
+            const uint64_t cm = __ballot(has);
+            if (!cm) return;
+            if (has) {
+                const int pos = stack_n + __popcll(cm & lt_mask);
+                if (pos < stack_cap) {
+                    ws[pos] = ray;
+                } else {
+                    const unsigned q = atomicAdd(dump_count, 1u);
+                    if (q < dump_cap) { dump[q] = ray; lc.v[C_SPILL]++; }
+                    else { lc.v[C_DROP]++; lc.lost += (double)ray.I; }
                 }
-                stack_n = min(stack_n + n, stack_cap);
+            }
+            stack_n = min(stack_n + __popcll(cm), stack_cap);
+        };
+        if constexpr (SQ) {
+            // ---- (c) queue the finished walks, shade 64 at a time
+            const uint64_t fm = __ballot(ready);
+            if (fm) {
+                if (ready) {
+                    ShadeItem<R> it;
+                    it.r = r;
+                    it.best = w.best;
+                    it.tbest = w.tbest;
+                    it.rmask = w.rmask;
+                    wq[qn + __popcll(fm & lt_mask)] = it;
+                    act = false;
+                    ready = false;
+                }
+                qn += __popcll(fm);
                 __threadfence_block();
+            }
+            if (qn >= 64 || (qn > 0 && __ballot(walking) == 0)) {
+                const int k = min(qn, 64);
+                bool alive = false, has_child = false;
+                ORay<R> sr, child;
+                if (lane == 0) lc.v[C_SHADE]++;
+                if (lane < k) {
+                    const ShadeItem<R> it = wq[qn - k + lane];
+                    sr = it.r;
+                    lc.v[C_SHADED]++;
+                    alive = octave_shade<R>(tc, s, rv, sr, it.best, it.tbest, it.rmask, has_child, child, lc);
+                }
+                qn -= k;
+                push(has_child, child);
+                push(alive, sr);  // on top: the continuation is walked next
+                __threadfence_block();
+            }
+        } else {
+            // ---- (c) shade the lanes whose walk ended
+            const int n_ready = __popcll(__ballot(ready));
+            if (n_ready > 0 && (n_ready >= tc.shade_min || __ballot(walking) == 0)) {
+                bool has_child = false;
+                ORay<R> child;
+                if (lane == 0) lc.v[C_SHADE]++;
+                if (ready) {
+                    lc.v[C_SHADED]++;
+                    act = octave_shade<R>(tc, s, rv, r, w.best, w.tbest, w.rmask, has_child, child, lc);
+                    ready = false;
+                }
+                if (__ballot(has_child)) {
+                    push(has_child, child);
+                    __threadfence_block();
+                }
             }
         }
     }
@@ -1030,55 +1074,66 @@ hipError_t launch_phase1(hipStream_t st, const TraceConst &tc, const DevScene<R>
     return hipGetLastError();
 }
 
-template <class R, int BS, int OCC>
+template <class R, int BS, int OCC, bool SQ>
 static void launch_octave_bs(hipStream_t st, int grid, size_t lds, const TraceConst &tc, const DevScene<R> &s,
                              const DevRecv<R> &rv, const ORay<R> *in0, const unsigned *n0, unsigned cap0,
                              const ORay<R> *in1, const unsigned *n1, unsigned cap1, unsigned *in_head,
-                             ORay<R> *stacks, int stack_cap, ORay<R> *dump, unsigned *dump_count,
-                             unsigned dump_cap, int min_iters, DevCounters *cnt)
+                             ORay<R> *stacks, int stack_cap, ShadeItem<R> *shq, ORay<R> *dump,
+                             unsigned *dump_count, unsigned dump_cap, int min_iters, DevCounters *cnt)
 {
     if (lds)
-        hipLaunchKernelGGL((k_octave<R, true, BS, OCC>), dim3(grid), dim3(BS), lds, st, tc, s, rv, in0, n0, cap0,
-                           in1, n1, cap1, in_head, stacks, stack_cap, dump, dump_count, dump_cap, min_iters, cnt);
+        hipLaunchKernelGGL((k_octave<R, true, BS, OCC, SQ>), dim3(grid), dim3(BS), lds, st, tc, s, rv, in0, n0,
+                           cap0, in1, n1, cap1, in_head, stacks, stack_cap, shq, dump, dump_count, dump_cap,
+                           min_iters, cnt);
     else
-        hipLaunchKernelGGL((k_octave<R, false, BS, OCC>), dim3(grid), dim3(BS), 0, st, tc, s, rv, in0, n0, cap0,
-                           in1, n1, cap1, in_head, stacks, stack_cap, dump, dump_count, dump_cap, min_iters, cnt);
+        hipLaunchKernelGGL((k_octave<R, false, BS, OCC, SQ>), dim3(grid), dim3(BS), 0, st, tc, s, rv, in0, n0,
+                           cap0, in1, n1, cap1, in_head, stacks, stack_cap, shq, dump, dump_count, dump_cap,
+                           min_iters, cnt);
 }
 
 template <class R>
 hipError_t launch_octave(hipStream_t st, int grid, const OctShape &sh, const TraceConst &tc, const DevScene<R> &s,
                          const DevRecv<R> &rv, const ORay<R> *in0, const unsigned *n0, unsigned cap0,
                          const ORay<R> *in1, const unsigned *n1, unsigned cap1, unsigned *in_head,
-                         ORay<R> *stacks, int stack_cap, ORay<R> *dump, unsigned *dump_count,
+                         ORay<R> *stacks, int stack_cap, ShadeItem<R> *shq, ORay<R> *dump, unsigned *dump_count,
                          unsigned dump_cap, int min_iters, DevCounters *cnt)
 {
     const size_t lds = s.dist_lds ? (size_t)((s.ncell + 15) & ~15) : 0;
-#define PACH_OCT(BS_, OCC_)                                                                                 \
-    launch_octave_bs<R, BS_, OCC_>(st, grid, lds, tc, s, rv, in0, n0, cap0, in1, n1, cap1, in_head, stacks, \
-                                   stack_cap, dump, dump_count, dump_cap, min_iters, cnt)
+#define PACH_OCT(BS_, OCC_, SQ_)                                                                            \
+    launch_octave_bs<R, BS_, OCC_, SQ_>(st, grid, lds, tc, s, rv, in0, n0, cap0, in1, n1, cap1, in_head,     \
+                                        stacks, stack_cap, shq, dump, dump_count, dump_cap, min_iters, cnt)
+    if (sh.queue && !shq) return hipErrorInvalidValue;
     if (sh.block == 512) {
-        if (sh.occ == 4) PACH_OCT(512, 4); else PACH_OCT(512, 0);
+        if (sh.occ == 4) { if (sh.queue) PACH_OCT(512, 4, true); else PACH_OCT(512, 4, false); }
+        else { if (sh.queue) PACH_OCT(512, 0, true); else PACH_OCT(512, 0, false); }
     } else {
-        if (sh.occ == 4) PACH_OCT(256, 4); else PACH_OCT(256, 0);
+        if (sh.occ == 4) { if (sh.queue) PACH_OCT(256, 4, true); else PACH_OCT(256, 4, false); }
+        else { if (sh.queue) PACH_OCT(256, 0, true); else PACH_OCT(256, 0, false); }
     }
 #undef PACH_OCT
     return hipGetLastError();
 }
 
-template <class R, int BS, int OCC> static int blocks_per_cu_bs(int lds_bytes)
+template <class R, int BS, int OCC, bool SQ> static int blocks_per_cu_bs(int lds_bytes)
 {
     int nb = 0;
-    hipError_t e = lds_bytes > 0
-                       ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_octave<R, true, BS, OCC>, BS, lds_bytes)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_octave<R, false, BS, OCC>, BS, 0);
+    hipError_t e =
+        lds_bytes > 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_octave<R, true, BS, OCC, SQ>, BS, lds_bytes)
+                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_octave<R, false, BS, OCC, SQ>, BS, 0);
     if (e != hipSuccess || nb < 1) nb = 1;
     return nb;
 }
 
+template <class R, bool SQ> static int blocks_per_cu_q(const OctShape &sh, int lds_bytes)
+{
+    if (sh.block == 512)
+        return sh.occ == 4 ? blocks_per_cu_bs<R, 512, 4, SQ>(lds_bytes) : blocks_per_cu_bs<R, 512, 0, SQ>(lds_bytes);
+    return sh.occ == 4 ? blocks_per_cu_bs<R, 256, 4, SQ>(lds_bytes) : blocks_per_cu_bs<R, 256, 0, SQ>(lds_bytes);
+}
+
 template <class R> int octave_blocks_per_cu(const OctShape &sh, int lds_bytes)
 {
-    if (sh.block == 512) return sh.occ == 4 ? blocks_per_cu_bs<R, 512, 4>(lds_bytes) : blocks_per_cu_bs<R, 512, 0>(lds_bytes);
-    return sh.occ == 4 ? blocks_per_cu_bs<R, 256, 4>(lds_bytes) : blocks_per_cu_bs<R, 256, 0>(lds_bytes);
+    return sh.queue ? blocks_per_cu_q<R, true>(sh, lds_bytes) : blocks_per_cu_q<R, false>(sh, lds_bytes);
 }
 
 #define PACH_INSTANTIATE(R)                                                                        \
@@ -1090,7 +1145,8 @@ template <class R> int octave_blocks_per_cu(const OctShape &sh, int lds_bytes)
                                          const DevScene<R> &, const DevRecv<R> &, const ORay<R> *,  \
                                          const unsigned *, unsigned, const ORay<R> *,               \
                                          const unsigned *, unsigned, unsigned *, ORay<R> *, int,    \
-                                         ORay<R> *, unsigned *, unsigned, int, DevCounters *);      \
+                                         ShadeItem<R> *, ORay<R> *, unsigned *, unsigned, int,      \
+                                         DevCounters *);                                            \
     template int octave_blocks_per_cu<R>(const OctShape &, int);
 
 }  // namespace pach

@@ -169,13 +169,25 @@ hipError_t launch_phase1(hipStream_t st, const TraceConst &tc, const DevScene<R>
 struct OctShape {
     int block = 256;
     int occ = 0;
+    int queue = 0;  // 1: finished walks go through a per-wave shade queue
 };
+
+// A finished walk waiting in a wave's shade queue (k_octave with a queue):
+// the ray, the triangle hit (-1: miss), the hit distance and the receiver
+// candidates the walk collected.
+template <class R> struct alignas(16) ShadeItem {
+    ORay<R> r;
+    int best;
+    R tbest;
+    uint64_t rmask;
+};
+constexpr int kShadeQ = 128;  // per-wave queue capacity (at most 63 wait + 64 pushed)
 
 template <class R>
 hipError_t launch_octave(hipStream_t st, int grid, const OctShape &sh, const TraceConst &tc, const DevScene<R> &s,
                          const DevRecv<R> &rv, const ORay<R> *in0, const unsigned *n0, unsigned cap0,
                          const ORay<R> *in1, const unsigned *n1, unsigned cap1, unsigned *in_head,
-                         ORay<R> *stacks, int stack_cap, ORay<R> *dump, unsigned *dump_count,
+                         ORay<R> *stacks, int stack_cap, ShadeItem<R> *shq, ORay<R> *dump, unsigned *dump_count,
                          unsigned dump_cap, int min_iters, DevCounters *cnt);
 
 template <class R> int octave_blocks_per_cu(const OctShape &sh, int lds_bytes);
