@@ -62,6 +62,8 @@ struct pach_ctx {
     double gmin[3] = {0, 0, 0}, cs[3] = {1, 1, 1};
     int *d_surf = nullptr;
     uint32_t *d_cell_range = nullptr, *d_cell_tris = nullptr;
+    float4 *d_cell_rows = nullptr;  // FP32 test rows (r2, r0, r1) of every voxel-list entry
+    std::vector<float> tri_rows32;  // [T][12] FP32 rows r2, r0, r1 per triangle
     uint8_t *d_cell_dist = nullptr;
     uint64_t *d_cell_rmask = nullptr;
     int ncell = 0;
@@ -305,7 +307,7 @@ extern "C" void pach_destroy(pach_ctx *c)
     free_scene(c->s64);
     free_work(c->w32);
     free_work(c->w64);
-    dfree(c->d_surf); dfree(c->d_cell_range); dfree(c->d_cell_tris); dfree(c->d_cell_dist);
+    dfree(c->d_surf); dfree(c->d_cell_range); dfree(c->d_cell_tris); dfree(c->d_cell_dist); dfree(c->d_cell_rows);
     dfree(c->d_cell_rmask);
     dfree(c->d_dirs); dfree(c->d_geo);
     dfree(c->d_ctr32); dfree(c->d_ctr64); dfree(c->d_hist);
@@ -353,8 +355,17 @@ int upload_grid(pach_ctx *c, const int dims[3])
     std::string gerr;
     if (pach::build_grid(c->verts.data(), (int)(c->verts.size() / 9), dims, grid, gerr) != 0)
         return fail(c, PACH_EINVAL, gerr);
+    // FP32 rows copied into list order: a triangle test reads its id and its
+    // rows at the same list index, so the two loads do not wait on each other
+    std::vector<float4> rows(std::max<size_t>(grid.tris.size(), 1) * 3);
+    for (size_t j = 0; j < grid.tris.size(); ++j) {
+        const float *q = c->tri_rows32.data() + 12 * (size_t)grid.tris[j];
+        for (int k = 0; k < 3; ++k) rows[3 * j + k] = make_float4(q[4 * k], q[4 * k + 1], q[4 * k + 2], q[4 * k + 3]);
+    }
     hipError_t e;
-    if ((e = dalloc(c->d_cell_range, grid.ranges.size())) != hipSuccess ||
+    if ((e = dalloc(c->d_cell_rows, rows.size())) != hipSuccess ||
+        (e = hipMemcpy(c->d_cell_rows, rows.data(), rows.size() * sizeof(float4), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = dalloc(c->d_cell_range, grid.ranges.size())) != hipSuccess ||
         (e = hipMemcpy(c->d_cell_range, grid.ranges.data(), grid.ranges.size() * 4, hipMemcpyHostToDevice)) != hipSuccess ||
         (e = dalloc(c->d_cell_tris, std::max<size_t>(grid.tris.size(), 1))) != hipSuccess ||
         (e = hipMemcpy(c->d_cell_tris, grid.tris.data(), grid.tris.size() * 4, hipMemcpyHostToDevice)) != hipSuccess ||
@@ -439,6 +450,13 @@ extern "C" int pach_scene_upload(pach_ctx *c, const double *verts, int T, const 
         d2[4 * t + 3] = w2;
     }
     c->verts.assign(verts, verts + 9 * (size_t)T);
+    c->tri_rows32.resize(12 * (size_t)T);
+    for (size_t t = 0; t < (size_t)T; ++t)
+        for (int k = 0; k < 4; ++k) {
+            c->tri_rows32[12 * t + k] = (float)d2[4 * t + k];
+            c->tri_rows32[12 * t + 4 + k] = (float)d0[4 * t + k];
+            c->tri_rows32[12 * t + 8 + k] = (float)d1[4 * t + k];
+        }
     const int dims[3] = {gx, gy, gz};
     if (int rc = upload_grid(c, dims)) return rc;
     // materials: (1 - alpha, s, lo, hi) per (polygon, band), Lambert_Scattering
@@ -713,6 +731,7 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
     DevScene<R> s{};
     s.v0 = sb.v0; s.e1 = sb.e1; s.e2 = sb.e2; s.nrm = sb.nrm; s.surf = c->d_surf;
     s.cell_range = (const uint2 *)c->d_cell_range; s.cell_tris = c->d_cell_tris;
+    s.cell_rows = c->d_cell_rows;
     s.cell_dist = c->d_cell_dist;
     s.cell_rmask = c->rmask_on ? c->d_cell_rmask : nullptr;
     s.gx = c->g[0]; s.gy = c->g[1]; s.gz = c->g[2];

@@ -125,31 +125,34 @@ __device__ __forceinline__ bool tri_hit_mt(const DevScene<double> &s, int t, dou
 // coordinates u, v of a point x of the plane.  The plane row decides t first,
 // so triangles crossed behind the ray or past tlim cost 8 multiply-adds.
 // Edges are widened by 4e-6 (barycentric) so no ray slips between neighbours.
-__device__ __forceinline__ bool tri_hit_dual(const DevScene<float> &s, int t, float ox, float oy, float oz,
+// The rows are read from the voxel list's copy at list index j.
+__device__ __forceinline__ bool tri_hit_dual(const DevScene<float> &s, uint32_t j, float ox, float oy, float oz,
                                              float dx, float dy, float dz, float tlim, float &tout)
 {
-    const float4 r2 = s.nrm[t];
+    const float4 *rows = s.cell_rows + 3 * (size_t)j;
+    const float4 r2 = rows[0];
     const float den = r2.x * dx + r2.y * dy + r2.z * dz;
     const float num = -(r2.x * ox + r2.y * oy + r2.z * oz + r2.w);
     const float tt = num * __builtin_amdgcn_rcpf(den);
     if (!(tt > 0.0f && tt <= tlim)) return false;
     const float px = ox + dx * tt, py = oy + dy * tt, pz = oz + dz * tt;
-    const float4 r0 = s.e1[t];
+    const float4 r0 = rows[1];
     const float u = r0.x * px + r0.y * py + r0.z * pz + r0.w;
     if (u < -4e-6f || u > 1.000004f) return false;
-    const float4 r1 = s.e2[t];
+    const float4 r1 = rows[2];
     const float v = r1.x * px + r1.y * py + r1.z * pz + r1.w;
     if (v < -4e-6f || u + v > 1.000004f) return false;
     tout = tt;
     return true;
 }
 
+// Triangle t, entry j of the voxel lists.
 template <class R>
-__device__ __forceinline__ bool tri_hit(const DevScene<R> &s, int t, R ox, R oy, R oz, R dx, R dy, R dz,
-                                        R tlim, R &tout)
+__device__ __forceinline__ bool tri_hit(const DevScene<R> &s, int t, uint32_t j, R ox, R oy, R oz, R dx, R dy,
+                                        R dz, R tlim, R &tout)
 {
     if constexpr (sizeof(R) == 8) return tri_hit_mt(s, t, ox, oy, oz, dx, dy, dz, tlim, tout);
-    else return tri_hit_dual(s, t, ox, oy, oz, dx, dy, dz, tlim, tout);
+    else return tri_hit_dual(s, j, ox, oy, oz, dx, dy, dz, tlim, tout);
 }
 
 // Voxel distances staged in LDS by k_octave (dynamic shared memory).
@@ -228,7 +231,7 @@ __device__ bool walk_steps(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R
                 const int t = (int)s.cell_tris[j];
                 R tt;
                 if (t != w.best && !(Tmin<R>::excl && t == excl) &&
-                    tri_hit<R>(s, t, ox, oy, oz, dx, dy, dz, fmin(tlim, w.tbest), tt) &&
+                    tri_hit<R>(s, t, j, ox, oy, oz, dx, dy, dz, fmin(tlim, w.tbest), tt) &&
                     (tt < w.tbest || (tt == w.tbest && t < w.best))) {
                     w.tbest = tt;
                     w.best = t;
@@ -308,10 +311,11 @@ __device__ bool walk_units(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R
         const R texit = fmin(w.tmx, fmin(w.tmy, w.tmz));
         if (w.j < w.je) {
             ++n_test;
-            const int t = (int)s.cell_tris[w.j++];
+            const uint32_t jj = w.j++;
+            const int t = (int)s.cell_tris[jj];
             const R tlim = fmin(w.tbest, texit * R(1.0000001) + R(1e-6));
             R tt;
-            if (t != w.best && !(Tmin<R>::excl && t == excl) && tri_hit<R>(s, t, ox, oy, oz, dx, dy, dz, tlim, tt) &&
+            if (t != w.best && !(Tmin<R>::excl && t == excl) && tri_hit<R>(s, t, jj, ox, oy, oz, dx, dy, dz, tlim, tt) &&
                 (tt < w.tbest || (tt == w.tbest && t < w.best))) {
                 w.tbest = tt;
                 w.best = t;
