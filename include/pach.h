@@ -182,8 +182,9 @@ int pach_rooms_set(pach_ctx *ctx, int n, const double *src, const double *power,
  * (1-64): idle lanes a wave gathers before refilling them (the refill
  * threshold); "dump_min" (>= 1): loop steps a drained wave runs before handing
  * its leftover rays to the next launch; "block" (256/512): workgroup size of
- * the octave kernel; "occupancy" (0/4): 4 caps registers at 128 VGPRs so four
- * waves share each SIMD; "shade_queue" (0/1): a lane whose walk ended queues
+ * the octave kernel; "occupancy" (-1/0/4/5): 4 and 5 cap registers at 128 and
+ * 102 VGPRs so four or five waves share each SIMD, 0 leaves the compiler's
+ * allocation, -1 (default) takes 4 for FP32 and 0 for FP64; "shade_queue" (0/1): a lane whose walk ended queues
  * its hit and walks on, and the wave shades 64 queued hits at a time. */
 int pach_tune(pach_ctx *ctx, const char *key, int64_t value);
 

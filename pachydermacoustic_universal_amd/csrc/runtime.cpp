@@ -764,7 +764,8 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
     const int64_t N = rooms.back().ray_base + rooms.back().ray_count;
     int64_t batch = p.batch_rays > 0 ? p.batch_rays : std::min<int64_t>(N, 32768);
     batch = std::max<int64_t>(1, std::min<int64_t>(batch, 1 << 22));
-    const OctShape shape = c->tune_shape;
+    OctShape shape = c->tune_shape;
+    if (shape.occ < 0) shape.occ = sizeof(R) == 4 ? 4 : 0;
     const int grid = c->n_cu * octave_blocks_per_cu<R>(shape, s.dist_lds ? lds_bytes : 0);
     int rc = ensure_work<R>(c, batch, (size_t)grid * (shape.block / 64), shape.queue != 0);
     if (rc) return rc;
@@ -941,7 +942,7 @@ extern "C" int pach_tune(pach_ctx *c, const char *key, int64_t value)
         c->tune_shape.block = (int)value;
         return PACH_OK;
     }
-    if (std::strcmp(key, "occupancy") == 0 && (value == 0 || value == 4)) {
+    if (std::strcmp(key, "occupancy") == 0 && (value == -1 || value == 0 || value == 4 || value == 5)) {
         c->tune_shape.occ = (int)value;
         return PACH_OK;
     }
@@ -1076,7 +1077,7 @@ extern "C" int pach_autotune(pach_ctx *c, const pach_params *p, int64_t sample_r
              }, pct)))
             break;
         if ((rc = try_knob([](Knobs &k, int v) { k.queue = v; }, {0, 1}))) break;
-        if ((rc = try_knob([](Knobs &k, int v) { k.occ = v; }, {0, 4}))) break;
+        if ((rc = try_knob([](Knobs &k, int v) { k.occ = v; }, {0, 4, 5}))) break;
         if ((rc = try_knob([](Knobs &k, int v) { k.block = v; }, {256, 512}))) break;
         if ((rc = try_knob([](Knobs &k, int v) { k.walk_steps = v; }, {6, 8, 12, 16, 24, 32}))) break;
         if ((rc = try_knob([](Knobs &k, int v) { k.shade_min = v; }, {8, 12, 16, 24, 32, 48}))) break;

@@ -1106,14 +1106,19 @@ hipError_t launch_octave(hipStream_t st, int grid, const OctShape &sh, const Tra
 #define PACH_OCT(BS_, OCC_, SQ_)                                                                            \
     launch_octave_bs<R, BS_, OCC_, SQ_>(st, grid, lds, tc, s, rv, in0, n0, cap0, in1, n1, cap1, in_head,     \
                                         stacks, stack_cap, shq, dump, dump_count, dump_cap, min_iters, cnt)
-    if (sh.queue && !shq) return hipErrorInvalidValue;
-    if (sh.block == 512) {
-        if (sh.occ == 4) { if (sh.queue) PACH_OCT(512, 4, true); else PACH_OCT(512, 4, false); }
-        else { if (sh.queue) PACH_OCT(512, 0, true); else PACH_OCT(512, 0, false); }
-    } else {
-        if (sh.occ == 4) { if (sh.queue) PACH_OCT(256, 4, true); else PACH_OCT(256, 4, false); }
-        else { if (sh.queue) PACH_OCT(256, 0, true); else PACH_OCT(256, 0, false); }
+#define PACH_OCT_Q(BS_, OCC_) \
+    { if (sh.queue) PACH_OCT(BS_, OCC_, true); else PACH_OCT(BS_, OCC_, false); }
+#define PACH_OCT_O(BS_)                                  \
+    {                                                    \
+        if (sh.occ == 4) PACH_OCT_Q(BS_, 4)              \
+        else if (sh.occ == 5) PACH_OCT_Q(BS_, 5)         \
+        else PACH_OCT_Q(BS_, 0)                          \
     }
+    if (sh.queue && !shq) return hipErrorInvalidValue;
+    if (sh.block == 512) PACH_OCT_O(512)
+    else PACH_OCT_O(256)
+#undef PACH_OCT_O
+#undef PACH_OCT_Q
 #undef PACH_OCT
     return hipGetLastError();
 }
@@ -1128,11 +1133,16 @@ template <class R, int BS, int OCC, bool SQ> static int blocks_per_cu_bs(int lds
     return nb;
 }
 
+template <class R, int BS, bool SQ> static int blocks_per_cu_o(const OctShape &sh, int lds_bytes)
+{
+    if (sh.occ == 4) return blocks_per_cu_bs<R, BS, 4, SQ>(lds_bytes);
+    if (sh.occ == 5) return blocks_per_cu_bs<R, BS, 5, SQ>(lds_bytes);
+    return blocks_per_cu_bs<R, BS, 0, SQ>(lds_bytes);
+}
+
 template <class R, bool SQ> static int blocks_per_cu_q(const OctShape &sh, int lds_bytes)
 {
-    if (sh.block == 512)
-        return sh.occ == 4 ? blocks_per_cu_bs<R, 512, 4, SQ>(lds_bytes) : blocks_per_cu_bs<R, 512, 0, SQ>(lds_bytes);
-    return sh.occ == 4 ? blocks_per_cu_bs<R, 256, 4, SQ>(lds_bytes) : blocks_per_cu_bs<R, 256, 0, SQ>(lds_bytes);
+    return sh.block == 512 ? blocks_per_cu_o<R, 512, SQ>(sh, lds_bytes) : blocks_per_cu_o<R, 256, SQ>(sh, lds_bytes);
 }
 
 template <class R> int octave_blocks_per_cu(const OctShape &sh, int lds_bytes)

@@ -166,10 +166,12 @@ hipError_t launch_phase1(hipStream_t st, const TraceConst &tc, const DevScene<R>
                          unsigned *resume_count, DevCounters *cnt);
 
 // Launch shape of k_octave: workgroup size (256 or 512) and register cap
-// (occ 0: the compiler's choice; 4: at most 128 VGPRs, 4 waves per SIMD).
+// (occ 0: the compiler's choice; 4 / 5: at most 128 / 102 VGPRs, so 4 / 5
+// waves share a SIMD; -1: 4 for FP32, 0 for FP64, whose 4-wave build spills
+// ~370 bytes per lane).
 struct OctShape {
     int block = 256;
-    int occ = 0;
+    int occ = -1;
     int queue = 0;  // 1: finished walks go through a per-wave shade queue
 };
 

@@ -22,7 +22,7 @@ def test_autotune_hall_then_parity():
         print("[tuning]", t, flush=True)
         assert t["trials"] >= 10
         assert t["rate_best"] >= t["rate_start"] > 0
-        assert t["block"] in (256, 512) and t["occupancy"] in (0, 4)
+        assert t["block"] in (256, 512) and t["occupancy"] in (-1, 0, 4, 5) and t["shade_queue"] in (0, 1)
         assert 1 <= t["walk_steps"] <= 256 and 1 <= t["shade_min"] <= 64 and 1 <= t["refill_min"] <= 64
         assert all(1 <= g <= 1024 for g in t["grid"])
         # trace under the tuned settings (same context, no re-upload) against the oracle

@@ -116,14 +116,14 @@ def test_fp32_matches_fp64_statistics(tracer, medium):
 
 
 DEFAULT_KNOBS = {"walk_flat": 1, "walk_steps": 12, "receiver_masks": 1, "dist_lds": 1, "shade_min": 16,
-                 "refill_min": 1, "dump_min": 256, "block": 256, "occupancy": 0, "shade_queue": 0}
+                 "refill_min": 1, "dump_min": 256, "block": 256, "occupancy": -1, "shade_queue": 0}
 
 
 @pytest.mark.parametrize("knobs", [{"walk_flat": 0, "walk_steps": 4}, {"receiver_masks": 0, "dist_lds": 0},
                                    {"walk_steps": 1, "shade_min": 64},
                                    {"refill_min": 24, "block": 512, "dump_min": 16},
                                    {"occupancy": 4, "refill_min": 8, "shade_min": 8},
-                                   {"shade_queue": 1},
+                                   {"shade_queue": 1}, {"occupancy": 5}, {"occupancy": 0},
                                    {"shade_queue": 1, "block": 512, "refill_min": 16, "dump_min": 8,
                                     "walk_steps": 3}])
 def test_launch_tuning_does_not_change_results(tracer, medium, knobs):
