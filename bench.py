@@ -45,10 +45,10 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_octave_hall_traffic.json")
 
 def measured_traffic(args, tot, n_rays):
     """HBM bytes per k_octave launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE passes (scaled by a plain copy's counters).  The profiled
-    command's per-launch figure is used as is when this run is the same
-    workload, else its bytes per octave bounce times this run's bounces per
-    launch.  None when no profile is committed."""
+    WRITE_SIZE passes (each scaled by what the counter reads for a plain copy
+    of known size): the profiled command's bytes per octave bounce times this
+    run's octave bounces per launch (launch counts vary from run to run; the
+    bounces of a workload do not).  None when no profile is committed."""
     try:
         with open(TRAFFIC_FILE) as f:
             prof = json.load(f)
@@ -57,11 +57,11 @@ def measured_traffic(args, tot, n_rays):
     same = (args.workload == "hall" and prof.get("rays") == n_rays and prof.get("cutoff") == args.cutoff
             and prof.get("grid") == (args.grid or 32) and prof.get("precision") == args.precision
             and not args.tune and not args.autotune)
-    if same:
-        return prof["hbm_bytes_per_launch"], f"measured on this workload ({prof['command']})"
-    per_bounce = prof["hbm_bytes_per_launch"] / prof["oct_bounces_per_launch"]
+    per_bounce = prof["hbm_bytes_per_bounce"]
+    where = "this workload" if same else "the 1e6-ray hall"
     return (per_bounce * tot["oct"] / max(tot["n_oct"], 1),
-            f"{per_bounce:.2f} B per octave bounce measured on '{prof['command']}', x this run's bounces per launch")
+            f"{per_bounce:.2f} HBM B per octave bounce measured on {where} ('{prof['command']}', "
+            f"profiles/k_octave_fp32_hall_1e6_rays.md) x this run's octave bounces per launch")
 
 
 def main():
@@ -74,7 +74,7 @@ def main():
     ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--grid", type=int, default=None,
                     help="voxels per axis (default: 32 for the hall, automatic for the batch)")
-    ap.add_argument("--cpu-rays", type=int, default=2000, help="source rays of the CPU sample")
+    ap.add_argument("--cpu-rays", type=int, default=5000, help="source rays of the CPU sample")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
