@@ -74,7 +74,7 @@ typedef struct pach_params {
     int mode;             /* PACH_MODE_* */
     int precision;        /* PACH_PREC_* */
     int async;            /* 0: return when done; 1: return at once, use pach_poll/pach_wait */
-    int64_t ray_count;    /* Raycount (> 0) */
+    int64_t ray_count;    /* Raycount: > 0 source rays; < 1 convergence mode (PACH_CONV_*) */
     uint64_t seed;        /* Philox key of every ray stream */
     int p1_max_bounces;   /* broadband-phase bounce cap (0 = 128) */
     int max_bounces;      /* per octave ray bounce guard, phases 2+3 (0 = 100000) */
@@ -115,7 +115,16 @@ typedef struct pach_stats {
     int64_t tri_tests;         /* triangle tests of octave-ray walks */
     int64_t shade_passes;      /* shading passes (per wave) */
     int64_t lanes_shaded;      /* octave-ray bounces shaded (sum over the passes' lanes) */
+    /* convergence mode (ray_count < 1) */
+    int64_t revolutions;          /* revolutions of 80 source rays traced */
+    int conv_receiver_visible;    /* farthest receiver the source sees (-1: none), checked */
+    int conv_receiver_occluded;   /* farthest receiver it does not see (-1: none), checked */
+    int converged;                /* 1: every check passed */
 } pach_stats;
+
+/* Convergence checks of a trace with ray_count < 1 (TaskAll:111-176). */
+#define PACH_CONV_SPECMIN 0  /* ray_count == 0: SpecMin_Convergence_Check */
+#define PACH_CONV_MINIMUM 1  /* ray_count < 0: Minimum_Convergence_Check */
 
 /* Context lifetime.  device: HIP device ordinal. */
 int pach_create(pach_ctx **out, int device);
@@ -188,6 +197,18 @@ int pach_rooms_set(pach_ctx *ctx, int n, const double *src, const double *power,
  * its hit and walks on, and the wave shades 64 queued hits at a time. */
 int pach_tune(pach_ctx *ctx, const char *key, int64_t value);
 
+/* One revolution of a convergence check on host arrays, the function the
+ * convergence mode applies to each checked receiver (TaskAll:870-956 for
+ * PACH_CONV_SPECMIN, :1063-1112 for PACH_CONV_MINIMUM).  energy, spec, diff,
+ * rogue: the receiver's running (unscaled) energy histogram in the steering
+ * band and its specular / diffuse / specular-scattered analysis arrays (only
+ * energy for PACH_CONV_MINIMUM), n bins at fs; direct_s: source-receiver
+ * distance / c.  state: pach_convergence_state_size() doubles, zero before the
+ * first revolution. */
+int64_t pach_convergence_state_size(int kind, int n, int fs);
+int pach_convergence_step(int kind, const double *energy, const double *spec, const double *diff,
+                          const double *rogue, int n, int fs, double direct_s, double *state, int *converged);
+
 /* Autotuner result (no counterpart in the reference). */
 typedef struct pach_tuning {
     int grid[3];           /* voxel grid dimensions chosen (now uploaded) */
@@ -211,7 +232,14 @@ typedef struct pach_tuning {
  * Classes_Scenes.cs:709-755). */
 int pach_autotune(pach_ctx *ctx, const pach_params *p, int64_t sample_rays, pach_tuning *out);
 
-/* SplitRayTracer(...).Begin() (TaskAll:68-240).  Clears the histograms first. */
+/* SplitRayTracer(...).Begin() (TaskAll:68-240).  Clears the histograms first.
+ * With ray_count < 1 (and no rooms set) the trace runs in convergence mode
+ * (TaskAll:111-176, :288-329): revolutions of 80 source rays, each followed by
+ * the convergence checks of the farthest receiver the source sees and the
+ * farthest it does not (ray_count < 0: Minimum_Convergence_Check, 0:
+ * SpecMin_Convergence_Check, both on the steering band h_oct), until every
+ * check passes (at most 2^20 revolutions); rays_done then counts the
+ * revolutions' rays. */
 int pach_trace(pach_ctx *ctx, const pach_params *params);
 /* ThreadState / ProgressMsg analogue (TaskAll:246-252, :742-763). */
 int pach_poll(pach_ctx *ctx, pach_progress *out);

@@ -48,7 +48,8 @@ class Stats(C.Structure):
                 ("ms_octave", C.c_double), ("launches_phase1", C.c_int64),
                 ("launches_octave", C.c_int64), ("ms_trace", C.c_double),
                 ("loop_steps", C.c_int64), ("voxels", C.c_int64), ("tri_tests", C.c_int64),
-                ("shade_passes", C.c_int64), ("lanes_shaded", C.c_int64)]
+                ("shade_passes", C.c_int64), ("lanes_shaded", C.c_int64), ("revolutions", C.c_int64),
+                ("conv_receiver_visible", C.c_int), ("conv_receiver_occluded", C.c_int), ("converged", C.c_int)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}
@@ -89,6 +90,9 @@ SIGNATURES = [
                                      C.c_double, C.c_int, C.c_int]),
     ("pach_rooms_set", C.c_int, [C.c_void_p, C.c_int, _dp, _dp, C.POINTER(C.c_int64), _ip, _ip]),
     ("pach_tune", C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
+    ("pach_convergence_state_size", C.c_int64, [C.c_int, C.c_int, C.c_int]),
+    ("pach_convergence_step", C.c_int, [C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_double, _dp,
+                                        C.POINTER(C.c_int)]),
     ("pach_autotune", C.c_int, [C.c_void_p, C.POINTER(Params), C.c_int64, C.POINTER(Tuning)]),
     ("pach_trace", C.c_int, [C.c_void_p, C.POINTER(Params)]),
     ("pach_poll", C.c_int, [C.c_void_p, C.POINTER(Progress)]),
@@ -170,6 +174,28 @@ def uniform_medium(temp_c=20.0, hr=50.0, pa_hpa=1013.25, air_choice=0, edge_corr
     _chk(lib().pach_uniform_medium(air_choice, temp_c, hr, pa_hpa, int(edge_correction), _p(at),
                                    C.byref(c), C.byref(rho)))
     return at, c.value, rho.value
+
+
+CONV_SPECMIN, CONV_MINIMUM = 0, 1
+
+
+class ConvergenceCheck:
+    """One receiver's convergence check (pach_convergence_step) driven from the host."""
+
+    def __init__(self, kind, n, fs=1000, direct_s=0.0):
+        self.kind, self.n, self.fs, self.direct_s = kind, n, fs, direct_s
+        size = lib().pach_convergence_state_size(kind, n, fs)
+        if size < 0:
+            raise PachError(-1, "bad convergence check")
+        self.state = np.zeros(size)
+
+    def step(self, energy, spec=None, diff=None, rogue=None):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) if a is not None else None
+                for a in (energy, spec, diff, rogue)]
+        out = C.c_int()
+        _chk(lib().pach_convergence_step(self.kind, *[None if a is None else _p(a) for a in arrs], self.n, self.fs,
+                                         self.direct_s, _p(self.state), C.byref(out)))
+        return bool(out.value)
 
 
 def schroeder_integral(etc):

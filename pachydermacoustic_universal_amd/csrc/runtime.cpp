@@ -686,6 +686,172 @@ int refresh_receiver_cells(pach_ctx *c)
     return PACH_OK;
 }
 
+// ------------------------------------------------------------------ convergence mode
+// TaskAll:111-176 and :288-329: with rayCount < 1 the tracer runs revolutions
+// of Source.Revolution_Period() = 80 source rays (Classes_Sources.cs:197-200)
+// until the convergence checks of two receivers pass.
+
+// Closest hit of the stored triangles (FP64 Moller-Trumbore, t > 1e-10):
+// Check_Validity's Room.shoot (TaskAll:178-199).
+bool host_shoot(const std::vector<double> &v, const double o[3], const double d[3], double &tbest)
+{
+    tbest = 1e300;
+    bool hit = false;
+    const size_t T = v.size() / 9;
+    for (size_t t = 0; t < T; ++t) {
+        const double *a = &v[9 * t];
+        const double e1[3] = {a[3] - a[0], a[4] - a[1], a[5] - a[2]};
+        const double e2[3] = {a[6] - a[0], a[7] - a[1], a[8] - a[2]};
+        const double pv[3] = {d[1] * e2[2] - d[2] * e2[1], d[2] * e2[0] - d[0] * e2[2], d[0] * e2[1] - d[1] * e2[0]};
+        const double det = e1[0] * pv[0] + e1[1] * pv[1] + e1[2] * pv[2];
+        if (det == 0.0) continue;
+        const double inv = 1.0 / det;
+        const double sv[3] = {o[0] - a[0], o[1] - a[1], o[2] - a[2]};
+        const double u = (sv[0] * pv[0] + sv[1] * pv[1] + sv[2] * pv[2]) * inv;
+        if (u < 0.0 || u > 1.0) continue;
+        const double q[3] = {sv[1] * e1[2] - sv[2] * e1[1], sv[2] * e1[0] - sv[0] * e1[2], sv[0] * e1[1] - sv[1] * e1[0]};
+        const double w = (d[0] * q[0] + d[1] * q[1] + d[2] * q[2]) * inv;
+        if (w < 0.0 || u + w > 1.0) continue;
+        const double tt = (e2[0] * q[0] + e2[1] * q[1] + e2[2] * q[2]) * inv;
+        if (tt > 1e-10 && tt < tbest) {
+            tbest = tt;
+            hit = true;
+        }
+    }
+    return hit;
+}
+
+// AcousticalMath.Envelope (Classes_Math.cs:327-383): attack / release
+// smoothing clipped to a sliding maximum over localMaxWindow seconds.
+std::vector<double> envelope(const double *x, int n, double fs, double attack, double release, double window)
+{
+    std::vector<double> y(n);
+    if (n == 0) return y;
+    const int w = std::max(1, (int)std::lround(window * fs));
+    const double a = 1.0 - std::exp(-1.0 / (attack * fs));
+    const double r = 1.0 - std::exp(-1.0 / (release * fs));
+    auto lmax = [&](int i) {
+        double m = x[i];
+        for (int k = std::max(0, i - w + 1); k < i; ++k) m = std::max(m, x[k]);
+        return m;
+    };
+    double prev = x[0];
+    y[0] = std::min(prev, lmax(0));
+    prev = y[0];
+    for (int i = 1; i < n; ++i) {
+        const double coeff = x[i] > prev ? a : r;
+        const double yn = std::min(prev + coeff * (x[i] - prev), lmax(i));
+        y[i] = yn;
+        prev = yn;
+    }
+    return y;
+}
+
+// AcousticalMath.SPL_Intensity (Classes_Math.cs:97-102), floored at 0 dB.
+double spl_intensity(double I)
+{
+    const double v = 10 * std::log10(I / 1e-12);
+    return v < 0 ? 0 : v;
+}
+
+int64_t conv_state_size(int kind, int n, int fs)
+{
+    if (kind == PACH_CONV_MINIMUM) return 4;
+    const int step = std::max(1, fs / 20);
+    return 2 + n / step;
+}
+
+// One revolution of Minimum_Convergence_Check.Check (TaskAll:1063-1112) or
+// SpecMin_Convergence_Check.Check (TaskAll:870-956) on the running (unscaled)
+// energy histogram of one receiver in the steering band.
+bool conv_step(int kind, const double *E, const double *spec, const double *diff, const double *rogue, int n, int fs,
+               double direct_s, double *st)
+{
+    if (kind == PACH_CONV_MINIMUM) {
+        // state: snapshot50, snapshot80, snapshotinf, count
+        const int s0 = (int)std::floor(direct_s * fs);
+        const int s50 = (int)std::floor(50.0 * fs / 1000) + s0;
+        const int s80 = (int)std::floor(80.0 * fs / 1000) + s0;
+        double sn50 = 0, sn80 = 0, sninf = 0;
+        for (int i = s0; i < s50 && i < n; ++i) sn50 += E[i];
+        sn80 += sn50;
+        for (int i = s50; i < s80 && i < n; ++i) sn80 += E[i];
+        for (int i = s80; i < n; ++i) sninf += E[i];
+        const double conv1 = std::fabs(sn50 - st[0]) / st[0];
+        const double conv2 = std::fabs(sn80 - st[1]) / st[1];
+        const double convinf = std::fabs(sninf - st[2]) / st[2];
+        if (sn50 == 0 || conv1 > 0.02 || sn80 == 0 || conv2 > 0.02 || sninf == 0 || convinf > 0.1) st[3] = 0;
+        else st[3] += 1;
+        st[0] = sn50;
+        st[1] = sn80;
+        st[2] = sninf;
+        return st[3] > 5;
+    }
+    // state: ray count, count, snapshot[binct]
+    const int step = std::max(1, fs / 20);
+    const int binct = n / step;
+    double *snap = st + 2;
+    st[0] += 80;
+    std::vector<double> sn(binct, 0.0), sn_spec(binct, 0.0), sn_diff(binct, 0.0);
+    for (int i = 0; i < binct; ++i)
+        for (int j = 0; j < step; ++j) {
+            sn[i] += E[i * step + j];
+            sn_spec[i] += spec[i * step + j];
+            sn_diff[i] += diff[i * step + j];
+        }
+    std::vector<double> es = envelope(spec, n, fs, .001, 0.01, 0.02);
+    std::vector<double> ed = envelope(diff, n, fs, .001, 0.01, 0.02);
+    std::vector<double> er = envelope(rogue, n, fs, .001, 0.01, 0.02);
+    std::vector<double> si_spec(n), si_comb(n);
+    for (int i = 0; i < n; ++i) {
+        si_comb[i] = spl_intensity(ed[i] + er[i]) - 110;
+        si_spec[i] = spl_intensity(es[i]) - 110;
+    }
+    const double inf60diff = binct > 0 ? sn_diff[0] * 1e-6 : 0.0;
+    int stop = binct - 1;
+    while (stop > 1 && sn_spec[stop] < inf60diff) --stop;
+    double conv = 0;
+    for (int i = 0; i < binct; ++i) {
+        if (sn[i] == 0 || snap[i] == 0) continue;
+        const double d = std::fabs(sn[i] - snap[i]) / snap[i];
+        if (i < stop) conv = std::max(conv, d);
+    }
+    int spec_cross = 0;
+    for (int i = n - 1; i > 0; --i) {
+        spec_cross = i;
+        if (si_spec[i] > si_comb[i] + 4) break;
+    }
+    const double nm = 343.0 * spec_cross / n;
+    const double nmin = std::max(5000.0, std::ceil(4 * nm * nm));
+    for (int i = 0; i < binct; ++i) snap[i] = sn[i];
+    if (conv == 0) return false;
+    if (conv > 0.1) st[1] = 0;
+    st[1] += 1;
+    return st[1] > 10 && st[0] > nmin;
+}
+
+// Check_Validity over every receiver (TaskAll:132-162): the farthest receiver
+// the source sees and the farthest one it does not.
+void conv_receivers(const pach_ctx *c, int &vis, int &occ)
+{
+    vis = occ = -1;
+    double max_v = 0, max_o = 0;
+    for (int i = 0; i < c->R; ++i) {
+        const double *C = &c->centers[3 * i];
+        double d[3] = {C[0] - c->origin[0], C[1] - c->origin[1], C[2] - c->origin[2]};
+        const double dist = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        for (double &x : d) x /= dist;
+        double t;
+        const bool visible = host_shoot(c->verts, c->origin, d, t) ? t >= dist : true;
+        if (visible) {
+            if (max_v < dist) { vis = i; max_v = dist; }
+        } else if (max_o < dist) {
+            occ = i;
+            max_o = dist;
+        }
+    }
+}
+
 template <class R> int run_trace(pach_ctx *c, const pach_params p)
 {
     (void)hipSetDevice(c->device);
@@ -710,8 +876,12 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
         bt.kexp2[o] = (float)(-0.1 * c->atten[o] * std::log2(10.0));
     }
     tc.band = c->d_band;
+    // convergence mode (rayCount < 1): revolutions of 80 source rays until the checks pass
+    const bool conv_mode = p.ray_count <= 0 && c->rooms.empty();
+    pach_params pr = p;
+    if (conv_mode) pr.ray_count = (int64_t)1 << 40;
     std::vector<RoomTab> rooms;
-    if (int rc = build_rooms(c, p, rooms)) return rc;
+    if (int rc = build_rooms(c, pr, rooms)) return rc;
     if ((int)rooms.size() > c->d_rooms_cap) {
         HIPCHK(c, dalloc(c->d_rooms, rooms.size()));
         c->d_rooms_cap = (int)rooms.size();
@@ -780,59 +950,107 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
 
     double ms_p1 = 0, ms_oct = 0;
     int64_t n_p1 = 0, n_oct = 0;
-    int cur = 0;            // dump buffer holding the previous launch's leftovers
-    unsigned pending = 0;   // rays in dump[cur]
-    unsigned n_resume = 0;  // broadband rays waiting in res[res_cur]
-    int res_cur = 0;
-    int64_t next = 0;       // next fresh source ray
     int64_t started = 0;
     const int min_iters = c->tune_dump_min;
-    for (;;) {
-        const bool stop_new = c->cancel.load();
-        bool p1_launched = false;
-        HIPCHK(c, hipMemsetAsync(p1_count, 0, 2 * sizeof(unsigned), st));  // p1_count, res_count
-        if (n_resume > 0) {
-            HIPCHK(c, hipEventRecord(c->ev[0], st));
-            HIPCHK(c, launch_phase1<R>(st, tc, s, rv, 0, 0, w.res[res_cur], (int)n_resume, w.p1_out, p1_count,
-                                       w.p1_cap, w.res[1 - res_cur], res_count, c->d_cnt));
-            HIPCHK(c, hipEventRecord(c->ev[1], st));
-            res_cur = 1 - res_cur;
-            p1_launched = true;
-        } else if (next < N && !stop_new) {
-            const int nb = (int)std::min<int64_t>(batch, N - next);
-            HIPCHK(c, hipEventRecord(c->ev[0], st));
-            HIPCHK(c, launch_phase1<R>(st, tc, s, rv, next, nb, nullptr, 0, w.p1_out, p1_count, w.p1_cap,
-                                       w.res[1 - res_cur], res_count, c->d_cnt));
-            HIPCHK(c, hipEventRecord(c->ev[1], st));
-            res_cur = 1 - res_cur;
-            next += nb;
-            started += nb;
-            p1_launched = true;
+    // Source rays [first, first + count): broadband batches feed the octave
+    // kernel together with the previous launch's leftovers until every ray
+    // they spawned has ended.
+    auto trace_range = [&](int64_t first, int64_t count) -> int {
+        int cur = 0;            // dump buffer holding the previous launch's leftovers
+        unsigned pending = 0;   // rays in dump[cur]
+        unsigned n_resume = 0;  // broadband rays waiting in res[res_cur]
+        int res_cur = 0;
+        int64_t next = first;   // next fresh source ray
+        const int64_t end = first + count;
+        HIPCHK(c, hipMemsetAsync(c->d_scal, 0, 8 * sizeof(unsigned), st));
+        for (;;) {
+            const bool stop_new = c->cancel.load();
+            bool p1_launched = false;
+            HIPCHK(c, hipMemsetAsync(p1_count, 0, 2 * sizeof(unsigned), st));  // p1_count, res_count
+            if (n_resume > 0) {
+                HIPCHK(c, hipEventRecord(c->ev[0], st));
+                HIPCHK(c, launch_phase1<R>(st, tc, s, rv, 0, 0, w.res[res_cur], (int)n_resume, w.p1_out, p1_count,
+                                           w.p1_cap, w.res[1 - res_cur], res_count, c->d_cnt));
+                HIPCHK(c, hipEventRecord(c->ev[1], st));
+                res_cur = 1 - res_cur;
+                p1_launched = true;
+            } else if (next < end && !stop_new) {
+                const int nb = (int)std::min<int64_t>(batch, end - next);
+                HIPCHK(c, hipEventRecord(c->ev[0], st));
+                HIPCHK(c, launch_phase1<R>(st, tc, s, rv, next, nb, nullptr, 0, w.p1_out, p1_count, w.p1_cap,
+                                           w.res[1 - res_cur], res_count, c->d_cnt));
+                HIPCHK(c, hipEventRecord(c->ev[1], st));
+                res_cur = 1 - res_cur;
+                next += nb;
+                started += nb;
+                p1_launched = true;
+            }
+            if (!p1_launched && pending == 0) break;
+            HIPCHK(c, hipMemsetAsync(in_head, 0, sizeof(unsigned), st));
+            HIPCHK(c, hipMemsetAsync(dump_count + (1 - cur), 0, sizeof(unsigned), st));
+            HIPCHK(c, hipEventRecord(c->ev[2], st));
+            HIPCHK(c, launch_octave<R>(st, grid, shape, tc, s, rv, pending ? w.dump[cur] : nullptr, dump_count + cur,
+                                       w.dump_cap, p1_launched ? w.p1_out : nullptr, p1_count, w.p1_cap, in_head,
+                                       w.stacks, w.stack_cap, shape.queue ? w.shq : nullptr, w.dump[1 - cur],
+                                       dump_count + (1 - cur), w.dump_cap, min_iters, c->d_cnt));
+            HIPCHK(c, hipEventRecord(c->ev[3], st));
+            unsigned h[8];
+            HIPCHK(c, hipMemcpyAsync(h, c->d_scal, sizeof(h), hipMemcpyDeviceToHost, st));
+            HIPCHK(c, hipStreamSynchronize(st));
+            float ms = 0;
+            if (p1_launched && hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) { ms_p1 += ms; ++n_p1; }
+            if (hipEventElapsedTime(&ms, c->ev[2], c->ev[3]) == hipSuccess) { ms_oct += ms; ++n_oct; }
+            n_resume = h[1];
+            cur = 1 - cur;
+            pending = std::min(h[3 + cur], w.dump_cap);
+            if (n_resume > w.res_cap) return fail(c, PACH_EHIP, "resume buffer overflow");
+            // every launch completes the bounces its waves start, so this only trips on a defect
+            if (n_oct > ((int64_t)1 << 20)) return fail(c, PACH_EHIP, "octave launches exceed 2^20: no progress");
+            // a source ray is complete once its broadband phase and every ray it spawned have ended
+            if (n_resume == 0 && pending == 0) c->rays_done = started;
         }
-        if (!p1_launched && pending == 0) break;
-        HIPCHK(c, hipMemsetAsync(in_head, 0, sizeof(unsigned), st));
-        HIPCHK(c, hipMemsetAsync(dump_count + (1 - cur), 0, sizeof(unsigned), st));
-        HIPCHK(c, hipEventRecord(c->ev[2], st));
-        HIPCHK(c, launch_octave<R>(st, grid, shape, tc, s, rv, pending ? w.dump[cur] : nullptr, dump_count + cur,
-                                   w.dump_cap, p1_launched ? w.p1_out : nullptr, p1_count, w.p1_cap, in_head,
-                                   w.stacks, w.stack_cap, shape.queue ? w.shq : nullptr, w.dump[1 - cur],
-                                   dump_count + (1 - cur), w.dump_cap,
-                                   min_iters, c->d_cnt));
-        HIPCHK(c, hipEventRecord(c->ev[3], st));
-        unsigned h[8];
-        HIPCHK(c, hipMemcpyAsync(h, c->d_scal, sizeof(h), hipMemcpyDeviceToHost, st));
-        HIPCHK(c, hipStreamSynchronize(st));
-        float ms = 0;
-        if (p1_launched && hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) { ms_p1 += ms; ++n_p1; }
-        if (hipEventElapsedTime(&ms, c->ev[2], c->ev[3]) == hipSuccess) { ms_oct += ms; ++n_oct; }
-        n_resume = h[1];
-        cur = 1 - cur;
-        pending = std::min(h[3 + cur], w.dump_cap);
-        if (n_resume > w.res_cap) return fail(c, PACH_EHIP, "resume buffer overflow");
-        // every launch completes the bounces its waves start, so this only trips on a defect
-        if (n_oct > ((int64_t)1 << 20)) return fail(c, PACH_EHIP, "octave launches exceed 2^20: no progress");
-        // a source ray is complete once its broadband phase and every ray it spawned have ended
-        if (n_resume == 0 && pending == 0) c->rays_done = started;
+        return PACH_OK;
+    };
+    int64_t revolutions = 0;
+    int conv_vis = -1, conv_occ = -1;
+    bool converged = false;
+    if (!conv_mode) {
+        if (int e = trace_range(0, N)) return e;
+    } else {
+        const int kind = p.ray_count < 0 ? PACH_CONV_MINIMUM : PACH_CONV_SPECMIN;
+        const int hb = rooms[0].h;
+        conv_receivers(c, conv_vis, conv_occ);
+        const int recs[2] = {conv_vis, conv_occ};
+        const int64_t ss = conv_state_size(kind, c->sct, (int)c->fs);
+        std::vector<double> state(2 * ss, 0.0), rows(4 * (size_t)c->sct);
+        const int period = 80;  // Source.Revolution_Period()
+        for (;;) {
+            if (int e = trace_range(revolutions * period, period)) return e;
+            ++revolutions;
+            if (c->cancel.load()) break;
+            bool conv = true;
+            for (int k = 0; k < 2; ++k) {
+                if (recs[k] < 0) continue;
+                const size_t off = ((size_t)recs[k] * kBands + hb) * c->sct;
+                const int planes[4] = {0, 8, 9, 10};  // energy; specular, diffuse, specular-scattered
+                for (int q = 0; q < 4; ++q)
+                    HIPCHK(c, hipMemcpyAsync(rows.data() + (size_t)q * c->sct, c->d_hist + planes[q] * c->hist_n + off,
+                                             c->sct * sizeof(double), hipMemcpyDeviceToHost, st));
+                HIPCHK(c, hipStreamSynchronize(st));
+                const double *C = &c->centers[3 * recs[k]];
+                const double dist = std::sqrt((C[0] - c->origin[0]) * (C[0] - c->origin[0]) +
+                                              (C[1] - c->origin[1]) * (C[1] - c->origin[1]) +
+                                              (C[2] - c->origin[2]) * (C[2] - c->origin[2]));
+                conv &= conv_step(kind, rows.data(), rows.data() + c->sct, rows.data() + 2 * (size_t)c->sct,
+                                  rows.data() + 3 * (size_t)c->sct, c->sct, (int)c->fs, dist / c->c,
+                                  state.data() + k * ss);
+            }
+            if (conv) { converged = true; break; }
+            if (revolutions >= ((int64_t)1 << 20)) break;  // 84 M source rays without convergence
+        }
+        // the room's ray range ends with the last revolution (scaling, e_init)
+        rooms[0].ray_count = started;
+        c->room_tab = rooms;
     }
     c->rays_done = started;
     DevCounters hc{};
@@ -869,8 +1087,13 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
     S.launches_phase1 = n_p1;
     S.launches_octave = n_oct;
     S.ms_trace = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    S.revolutions = revolutions;
+    S.conv_receiver_visible = conv_vis;
+    S.conv_receiver_occluded = conv_occ;
+    S.converged = converged ? 1 : 0;
     c->oct_rays = S.octave_rays;
     if (hc.v[C_DROP]) return fail(c, PACH_ENOMEM, "octave-ray queue overflow: rays dropped");
+    if (conv_mode) return c->cancel.load() && !converged ? PACH_ECANCEL : PACH_OK;
     if (c->cancel.load() && started < N) return PACH_ECANCEL;
     return PACH_OK;
 }
@@ -967,8 +1190,10 @@ extern "C" int pach_trace(pach_ctx *c, const pach_params *p)
         if (p->ray_count != 0 && p->ray_count != total)
             return fail(c, PACH_EINVAL, "ray_count must be 0 or the sum of the rooms' ray counts");
     }
-    if (total <= 0 || total > ((int64_t)1 << 32))
-        return fail(c, PACH_EINVAL, "ray_count must lie in [1, 2^32]");
+    const bool conv_mode = c->rooms.empty() && p->ray_count <= 0;  // TaskAll:111, :236-239
+    if (!conv_mode && (total <= 0 || total > ((int64_t)1 << 32)))
+        return fail(c, PACH_EINVAL, "ray_count must lie in [1, 2^32] (or be < 1 for convergence mode)");
+    if (conv_mode) total = 0;
     if (p->oct_lo < 0 || p->oct_hi > 7 || p->oct_lo > p->oct_hi) return fail(c, PACH_EINVAL, "bad octave range");
     if (!(p->cutoff_m > 0)) return fail(c, PACH_EINVAL, "cutoff must be positive");
     if (p->mode != PACH_MODE_STOCHASTIC && p->mode != PACH_MODE_SPECULAR) return fail(c, PACH_EINVAL, "bad mode");
@@ -1103,6 +1328,24 @@ extern "C" int pach_autotune(pach_ctx *c, const pach_params *p, int64_t sample_r
     out->trials = trials;
     out->rate_start = r0;
     out->rate_best = best_r;
+    return PACH_OK;
+}
+
+extern "C" int64_t pach_convergence_state_size(int kind, int n, int fs)
+{
+    if ((kind != PACH_CONV_MINIMUM && kind != PACH_CONV_SPECMIN) || n <= 0 || fs <= 0) return -1;
+    return conv_state_size(kind, n, fs);
+}
+
+extern "C" int pach_convergence_step(int kind, const double *energy, const double *spec, const double *diff,
+                                     const double *rogue, int n, int fs, double direct_s, double *state,
+                                     int *converged)
+{
+    if ((kind != PACH_CONV_MINIMUM && kind != PACH_CONV_SPECMIN) || !energy || !state || !converged || n <= 0 ||
+        fs <= 0)
+        return PACH_EINVAL;
+    if (kind == PACH_CONV_SPECMIN && (!spec || !diff || !rogue)) return PACH_EINVAL;
+    *converged = conv_step(kind, energy, spec, diff, rogue, n, fs, direct_s, state) ? 1 : 0;
     return PACH_OK;
 }
 

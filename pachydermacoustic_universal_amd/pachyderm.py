@@ -267,16 +267,17 @@ class SplitRayTracer(Simulation_Type):
     """Stochastic split ray tracer (TaskAll:33-782) on one MI355X.
 
     cutoffTime is a path length in metres (the ctor divides it by the speed of
-    sound, TaskAll:83).  rayCountIn must be positive: the convergence mode
-    (rayCount < 1, TaskAll:288-329) is not provided.  Extra keyword arguments:
+    sound, TaskAll:83).  rayCountIn < 1 selects the convergence mode
+    (TaskAll:111-176, :288-329): revolutions of 80 source rays until the
+    checks of the farthest visible and the farthest occluded receiver pass
+    (rayCount < 0: Minimum_Convergence_Check, 0: SpecMin_Convergence_Check).
+    Extra keyword arguments:
     precision ("fp32" traversal, or "fp64"), mode ("stochastic" or "specular"),
     seed (the reference seeds from the clock; None does the same), device."""
 
     def __init__(self, sourceIn, receiverIn, roomIn, cutoffTime, octaveRange, isOrderIn, rayCountIn,
                  Vis_Feedback=None, *, precision="fp32", mode="stochastic", seed=None, device=0,
                  tracer=None, batch_rays=0):
-        if rayCountIn < 1:
-            raise NotImplementedError("convergence-driven tracing (rayCount < 1) is not provided")
         self.Source, self.RecMain, self.Room = sourceIn, receiverIn, roomIn
         self.COTime = cutoffTime / roomIn.Sound_speed(0)
         self._cutoff = float(cutoffTime)
@@ -317,6 +318,9 @@ class SplitRayTracer(Simulation_Type):
 
     def ProgressMsg(self):
         p = self._tr.poll()
+        if self.Raycount < 1:
+            return (f"Revolution {-(-p['rays_done'] // 80)} - Ray {p['rays_done']} : sub {p['octave_rays']} - "
+                    f"Convergence not yet reached...")
         if p["rays_done"] == 0 and p["running"]:
             return "Preparing Threads"
         return (f"Calculating Ray {p['rays_done']} of {p['rays_total']} "
@@ -343,6 +347,15 @@ class SplitRayTracer(Simulation_Type):
     @property
     def PercentLost(self):
         return self._lost_percent
+
+    @property
+    def Convergence_Report(self):
+        """Convergence mode: receivers checked, revolutions and outcome (TaskAll:724-727)."""
+        if self.Raycount >= 1 or self._stats is None:
+            return None
+        st = self._stats
+        return {"visible_receiver": st["conv_receiver_visible"], "occluded_receiver": st["conv_receiver_occluded"],
+                "revolutions": st["revolutions"], "converged": bool(st["converged"])}
 
     @property
     def Stats(self):
