@@ -197,6 +197,20 @@ int pach_rooms_set(pach_ctx *ctx, int n, const double *src, const double *power,
  * its hit and walks on, and the wave shades 64 queued hits at a time. */
 int pach_tune(pach_ctx *ctx, const char *key, int64_t value);
 
+/* Deterministic direct sound of a point source (Direct_Sound,
+ * Classes_Direct.cs:654-840), for the energy-time curve that
+ * IR_Construction.ETCurve (Classes_Math.cs:1417-1474) assembles.  Host
+ * arithmetic, no context needed.  verts [T][3][3], surf_of_tri [T], trans
+ * [S][8] (may be NULL); src, power [8] (W), centers [R][3], atten [8] (dB/m),
+ * c (m/s).  Out per receiver: energy [R][8] = power 10^(-0.1 a d) transmod /
+ * (4 pi d^2) (the reference computes it whether or not the path is blocked),
+ * time_s [R] = d / c, audible [R] (Check_Validity: no surface nearer than the
+ * receiver, continuing through transmissive ones, whose coefficients make
+ * transmod). */
+int pach_direct_sound(const double *verts, int T, const int *surf_of_tri, const double *trans, int S,
+                      const double src[3], const double power[8], const double *centers, int R,
+                      const double atten[8], double c, double *energy, double *time_s, int *audible);
+
 /* One revolution of a convergence check on host arrays, the function the
  * convergence mode applies to each checked receiver (TaskAll:870-956 for
  * PACH_CONV_SPECMIN, :1063-1112 for PACH_CONV_MINIMUM).  energy, spec, diff,

@@ -10,7 +10,9 @@ from . import rooms  # noqa: F401
 from .pachyderm import (  # noqa: F401
     AcousticalMath,
     Basic_Material,
+    Direct_Sound,
     GeodesicSource,
+    IR_Construction,
     Lambert_Scattering,
     Polygon_Scene,
     RandomSource,
@@ -22,6 +24,7 @@ from .pachyderm import (  # noqa: F401
     SuppliedDirectionSource,
 )
 
-__all__ = ["AcousticalMath", "Basic_Material", "GeodesicSource", "Lambert_Scattering",
+__all__ = ["AcousticalMath", "Basic_Material", "Direct_Sound", "GeodesicSource", "IR_Construction",
+           "Lambert_Scattering",
            "Polygon_Scene", "RandomSource", "Receiver_Bank", "Running", "Simulation_Type",
            "SplitRayTracer", "Stopped", "SuppliedDirectionSource", "rooms"]

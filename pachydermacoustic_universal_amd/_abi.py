@@ -90,6 +90,8 @@ SIGNATURES = [
                                      C.c_double, C.c_int, C.c_int]),
     ("pach_rooms_set", C.c_int, [C.c_void_p, C.c_int, _dp, _dp, C.POINTER(C.c_int64), _ip, _ip]),
     ("pach_tune", C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
+    ("pach_direct_sound", C.c_int, [_dp, C.c_int, _ip, _dp, C.c_int, _dp, _dp, _dp, C.c_int, _dp, C.c_double,
+                                    _dp, _dp, _ip]),
     ("pach_convergence_state_size", C.c_int64, [C.c_int, C.c_int, C.c_int]),
     ("pach_convergence_step", C.c_int, [C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_double, _dp,
                                         C.POINTER(C.c_int)]),
@@ -174,6 +176,28 @@ def uniform_medium(temp_c=20.0, hr=50.0, pa_hpa=1013.25, air_choice=0, edge_corr
     _chk(lib().pach_uniform_medium(air_choice, temp_c, hr, pa_hpa, int(edge_correction), _p(at),
                                    C.byref(c), C.byref(rho)))
     return at, c.value, rho.value
+
+
+def direct_sound(room, power=None, atten=None, c=None):
+    """pach_direct_sound for a rooms.Room: (energy [R, 8], time_s [R], audible [R])."""
+    if atten is None or c is None:
+        at, cc, _ = uniform_medium()
+        atten = at if atten is None else atten
+        c = cc if c is None else c
+    v = np.ascontiguousarray(room.verts, dtype=np.float64)
+    sf = np.ascontiguousarray(room.surf, dtype=np.int32)
+    tr = np.ascontiguousarray(room.trans, dtype=np.float64)
+    src = np.ascontiguousarray(room.source, dtype=np.float64)
+    pw = np.ascontiguousarray(np.ones(8) if power is None else power, dtype=np.float64)
+    ctr = np.ascontiguousarray(room.receivers, dtype=np.float64).reshape(-1, 3)
+    R = ctr.shape[0]
+    at = np.ascontiguousarray(atten, dtype=np.float64)
+    energy = np.zeros((R, 8))
+    time_s = np.zeros(R)
+    aud = np.zeros(R, dtype=np.int32)
+    _chk(lib().pach_direct_sound(_p(v), v.shape[0], sf.ctypes.data_as(_ip), _p(tr), tr.shape[0], _p(src), _p(pw),
+                                 _p(ctr), R, _p(at), float(c), _p(energy), _p(time_s), aud.ctypes.data_as(_ip)))
+    return energy, time_s, aud.astype(bool)
 
 
 CONV_SPECMIN, CONV_MINIMUM = 0, 1

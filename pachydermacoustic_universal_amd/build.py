@@ -33,6 +33,7 @@ UNITS = [
     # (source, extra flags, is_hip)
     ("runtime.cpp", [], False),
     ("grid.cpp", [], False),
+    ("direct.cpp", [], False),
     ("trace_f32.hip", [], True),
     ("trace_f64.hip", ["-ffp-contract=off"], True),
     ("denoise.hip", [], True),

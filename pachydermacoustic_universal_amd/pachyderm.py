@@ -366,6 +366,105 @@ class SplitRayTracer(Simulation_Type):
             self._tr.close()
 
 
+# ---------------------------------------------------------------- deterministic direct sound
+class Direct_Sound(Simulation_Type):
+    """Deterministic direct sound of a point source (Classes_Direct.cs:31-1010).
+
+    Begin computes, for every receiver, whether the path from the source is
+    open (Check_Validity, continuing through transmissive polygons), the
+    arrival time d / c and the energy per band P 10^(-0.1 a d) transmod /
+    (4 pi d^2) at sample 0 of a 100-sample direct-sound window
+    (pach_direct_sound).  As in the reference the energy is computed for
+    blocked paths too; IsOccluded (sic) returns True when the path is OPEN.
+    Screen (edge-diffraction) attenuation of blocked paths is not provided."""
+
+    def __init__(self, Src_in, Rec_in, Room_in, Octaves, ScreenCalc=False, SPL_Only=False):
+        if ScreenCalc:
+            raise NotImplementedError("screen attenuation of blocked direct paths is not provided")
+        self.Src, self.Rec, self.Room = Src_in, Rec_in, Room_in
+        self.Oct_choice = Octaves
+        self.SPL_Only = bool(SPL_Only)
+        self.Delay_ms = 0.0
+        self.SampleFreq = Rec_in.SampleRate
+        self._done = False
+
+    def Sim_Type(self):
+        return "Deterministic Direct Sound"
+
+    def Begin(self):
+        r = self.Room
+        room = _RoomView(r._tris, r._surf, r._trans, self.Src.Origin, self.Rec.Origins)
+        E, t, aud = A.direct_sound(room, self.Src.SoundPower, r.atten, r.c)
+        n = 1 if self.SPL_Only else 100  # samples = ceil((max - min) fs / c) + 100 for one point
+        self.Io = np.zeros((len(self.Rec.Origins), 8, n))
+        self.Io[:, :, 0] = E
+        self.Time_Pt = t + self.Delay_ms
+        self.Validity = aud
+        self._done = True
+
+    def ThreadState(self):
+        return Stopped
+
+    def ProgressMsg(self):
+        return "Calculating direct sound: complete" if self._done else "Calculating direct sound..."
+
+    def Combine_ThreadLocal_Results(self):
+        pass
+
+    def IsOccluded(self, id):
+        return bool(self.Validity[id])
+
+    def Time(self, Rec_ID):
+        return float(self.Time_Pt[Rec_ID])
+
+    def Min_Time(self, Rec_ID):
+        return float(self.Time_Pt[Rec_ID])
+
+    def EnergyValue(self, octave, Rec_ID):
+        return self.Io[Rec_ID, octave].copy()
+
+    def EnergySum(self, Rec_ID, t):
+        return float(self.Io[Rec_ID, :, t].sum()) if t < self.Io.shape[2] else 0.0
+
+
+class _RoomView:
+    """The fields of rooms.Room that pach_direct_sound reads."""
+
+    def __init__(self, verts, surf, trans, source, receivers):
+        self.verts, self.surf, self.trans, self.source, self.receivers = verts, surf, trans, source, receivers
+
+
+class IR_Construction:
+    """Classes_Math.cs, IR_Construction."""
+
+    @staticmethod
+    def ETCurve(Direct, ISData, RTData, CO_Time_ms, Sampling_Frequency, Octave, Rec_ID, Src_ID=0,
+                Start_at_Zero=False):
+        """Energy-time curve of one receiver and band (Octave 8: all bands) from
+        the direct sound, image sources and the ray tracer's scaled histograms
+        (Classes_Math.cs:1417-1474).  Image-source paths are not provided here
+        (ISData entries must be None).  Where the reference would index past
+        the end of the histogram, the direct sound's bins are dropped."""
+        Direct = list(Direct) if Direct is not None else [None]
+        ISData = list(ISData) if ISData is not None else [None]
+        delay = Direct[Src_ID].Delay_ms if Direct[0] is not None else 0.0
+        if RTData[Src_ID] is not None:
+            H = RTData[Src_ID].GetEnergyHistogram(Octave, delay, Rec_ID).copy()
+        else:
+            H = np.zeros(int((CO_Time_ms + int(delay)) * 0.001 * Sampling_Frequency))
+        D = Direct[Src_ID]
+        if D is not None and D.IsOccluded(Rec_ID):
+            start = 0 if Start_at_Zero else int(np.ceil((delay * 0.001 + D.Time(Rec_ID)) * Sampling_Frequency))
+            n = D.Io.shape[2]
+            for i in range(n):
+                v = D.EnergySum(Rec_ID, i) if Octave == 8 else D.EnergyValue(Octave, Rec_ID)[i]
+                if start + i < H.size:
+                    H[start + i] += v
+        if ISData[Src_ID] is not None:
+            raise NotImplementedError("image-source paths are not provided")
+        return H
+
+
 # ---------------------------------------------------------------- decay-curve integration
 class AcousticalMath:
     """Classes_Math.cs:272-585, FP64 in libpach.so."""

@@ -140,3 +140,18 @@ def test_launch_tuning_does_not_change_results(tracer, medium, knobs):
     assert st["deposits"] == cnt["deposits"]
     for k in range(A.NPLANES):
         assert_bins(g[k], ora[k], 1e-5, f"plane {k}")
+
+
+@pytest.mark.parametrize("is_order", [0, 1])
+def test_is_order_gates_the_broadband_receiver_checks(tracer, medium, is_order):
+    """IS_Order >= 0 leaves the first orders to deterministic methods: the
+    broadband phase records only bounces of higher order (TaskAll:368, :378)."""
+    room = rooms.shoebox()
+    st, g, cnt, ora = run_both(tracer, medium, room, rays=8, cut=0.6, seed=31, mode=A.MODE_STOCHASTIC,
+                               precision=A.PREC_FP64, is_order=is_order)
+    assert st["deposits"] == cnt["deposits"]
+    for k in range(A.NPLANES):
+        assert_bins(g[k], ora[k], 1e-5, f"plane {k}")
+    # the direct sound (order 0) is gone: nothing arrives before the first reflection
+    d = np.linalg.norm(room.receivers - room.source, axis=1).min()
+    assert g[0][:, :, : int(d / medium[1] * 1000)].sum() == 0
