@@ -255,30 +255,50 @@ class Denoiser:
 
 # ---------------------------------------------------------------- training pairs
 
-def trace_energy(tr, packed, rays, cutoff_s, seed, precision=A.PREC_FP32):
+def trace_energy(tr, packed, rays, cutoff_s, seed, precision=A.PREC_FP32, is_order=-1):
     """Scaled energy histograms of every receiver of a packed batch, [R, 8, bins]."""
-    A.trace_batch(tr, packed, rays, cutoff_s=cutoff_s, seed=seed, precision=precision)
+    A.trace_batch(tr, packed, rays, cutoff_s=cutoff_s, seed=seed, precision=precision, is_order=is_order)
     return tr.histogram(0, scaled=True)
 
 
-def make_pairs(tr, room_list, lo_rays, hi_rays, cutoff_s=3.0, lo_repeats=1, seed=0, pack=16, log=None):
+def direct_part(room, fs=1000):
+    """Deterministic direct sound of a room's receivers: (energy [R, 8] with
+    zeros where the path is blocked, arrival bin [R] = ceil(d / c * fs)), what
+    IR_Construction.ETCurve adds to a trace made with IS_Order = 0."""
+    E, t, aud = A.direct_sound(room)
+    return E * aud[:, None], np.ceil(t * fs).astype(np.int64)
+
+
+def with_direct(E, dE, dbin):
+    """Energy histograms [N, 8, bins] plus their direct sound (ETCurve)."""
+    out = np.array(E, dtype=np.float64, copy=True)
+    for i in range(out.shape[0]):
+        if dbin[i] < out.shape[-1]:
+            out[i, :, dbin[i]] += dE[i]
+    return out
+
+
+def make_pairs(tr, room_list, lo_rays, hi_rays, cutoff_s=3.0, lo_repeats=1, seed=0, pack=16, log=None, is_order=-1):
     """Traces every room at lo_rays (lo_repeats independent seeds) and hi_rays.
     Returns dict(x [N, 8, bins] normalised low-ray input, t [N, 8, bins]
     normalised high-ray target (same S), S [N, 8, 1] the input's band
-    energies, desc [N, n_desc], room [N], rec [N])."""
-    xs, ts, ss, ds, ri, rr = [], [], [], [], [], []
+    energies, desc [N, n_desc], room [N], rec [N], the direct sound dE [N, 8]
+    and its bin dbin [N] for the ETCurve)."""
+    xs, ts, ss, ds, ri, rr, des, dbs = [], [], [], [], [], [], [], []
     for g0 in range(0, len(room_list), pack):
         grp = room_list[g0:g0 + pack]
         pk = RM.pack_rooms(grp)
         n = len(grp)
         t0 = time.perf_counter()
-        Ehi = trace_energy(tr, pk, np.full(n, hi_rays), cutoff_s, seed + 7919 * (g0 + 1))
-        Elo = [trace_energy(tr, pk, np.full(n, lo_rays), cutoff_s, seed + 7919 * (g0 + 1) + 104729 * (k + 1))
+        Ehi = trace_energy(tr, pk, np.full(n, hi_rays), cutoff_s, seed + 7919 * (g0 + 1), is_order=is_order)
+        Elo = [trace_energy(tr, pk, np.full(n, lo_rays), cutoff_s, seed + 7919 * (g0 + 1) + 104729 * (k + 1),
+                            is_order=is_order)
                for k in range(lo_repeats)]
         r0 = 0
         for i, room in enumerate(grp):
             k = int(pk.rec_count[i])
             d = descriptors(room)
+            dE, db = direct_part(room)
             for E in Elo:
                 x, S = normalize(E[r0:r0 + k])
                 sch_hi = np.cumsum(Ehi[r0:r0 + k][..., ::-1], axis=-1)[..., ::-1]
@@ -287,13 +307,15 @@ def make_pairs(tr, room_list, lo_rays, hi_rays, cutoff_s=3.0, lo_repeats=1, seed
                 ss.append(S)
                 ts.append(t.astype(np.float32))
                 ds.append(d)
+                des.append(dE)
+                dbs.append(db)
                 ri.append(np.full(k, g0 + i))
                 rr.append(np.arange(k))
             r0 += k
         if log:
             log(f"pairs: rooms {g0}-{g0 + n - 1} traced in {time.perf_counter() - t0:.1f} s")
     return dict(x=np.concatenate(xs), t=np.concatenate(ts), S=np.concatenate(ss), desc=np.concatenate(ds),
-                room=np.concatenate(ri), rec=np.concatenate(rr))
+                room=np.concatenate(ri), rec=np.concatenate(rr), dE=np.concatenate(des), dbin=np.concatenate(dbs))
 
 
 def early_weights(bins, fs=1000, gain=20.0, tau_s=0.15):

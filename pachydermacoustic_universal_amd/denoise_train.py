@@ -7,7 +7,9 @@ hall generator with random proportions, ~2,000 triangles, 16 receivers);
 validation rooms use seeds the training never sees.  Every room is traced in
 packed batches of 16 rooms (one launch sequence per batch) at lo_rays
 (`repeats` independent seeds, each a separate input) and once at hi_rays
-(the target), 3 s cut-off, 1 ms bins, FP32 traversal.  The network trains
+(the target), 3 s cut-off, 1 ms bins, FP32 traversal, IS_Order 0 (the
+direct sound is left to Direct_Sound and added back by the ETCurve wherever
+decay parameters are compared).  The network trains
 with AdamW (batch 256, linear warm-up, cosine decay, per-bin loss weights
 1 + 20 exp(-t / 0.15 s)) through libpach.so's BF16 MFMA kernels, once per
 learning rate in --lrs, each run keeping the parameters of its lowest
@@ -42,6 +44,8 @@ def main(argv=None):
     ap.add_argument("--hi-rays", type=int, default=40000)
     ap.add_argument("--repeats", type=int, default=2)
     ap.add_argument("--cutoff", type=float, default=3.0)
+    ap.add_argument("--is-order", type=int, default=0,
+                    help="IS_Order of the traces: 0 leaves the direct sound to Direct_Sound, added by ETCurve")
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--lrs", default="3e-4,1e-3", help="learning rates tried; the best on validation is kept")
@@ -61,8 +65,9 @@ def main(argv=None):
     log(f"{len(train_rooms)} training rooms, {len(val_rooms)} validation rooms")
     t0 = time.perf_counter()
     data = DN.make_pairs(tr, train_rooms, args.lo_rays, args.hi_rays, args.cutoff, args.repeats, seed=11,
-                         log=log)
-    val = DN.make_pairs(tr, val_rooms, args.lo_rays, args.hi_rays, args.cutoff, 1, seed=977, log=log)
+                         log=log, is_order=args.is_order)
+    val = DN.make_pairs(tr, val_rooms, args.lo_rays, args.hi_rays, args.cutoff, 1, seed=977, log=log,
+                        is_order=args.is_order)
     t_gen = time.perf_counter() - t0
     log(f"pairs: {data['x'].shape[0]} training, {val['x'].shape[0]} validation samples in {t_gen:.0f} s")
     tr.close()
@@ -70,8 +75,13 @@ def main(argv=None):
     dn = DN.Denoiser(max_batch=args.batch, weight_decay=args.weight_decay)
     log(f"network: {dn.n_params} parameters")
     w = DN.early_weights(dn.cfg["bins"], gain=args.early_gain, tau_s=args.early_tau)
-    P_hi = DN.decay_params(DN.denormalize(val["t"], val["S"]))
-    plain = DN.summarize(DN.errors(DN.decay_params(DN.denormalize(val["x"], val["S"])), P_hi))
+    # decay parameters of energy-time curves: the traces plus, with IS_Order 0,
+    # the deterministic direct sound (ETCurve)
+    def etc(E):
+        return DN.with_direct(E, val["dE"], val["dbin"]) if args.is_order >= 0 else E
+
+    P_hi = DN.decay_params(etc(DN.denormalize(val["t"], val["S"])))
+    plain = DN.summarize(DN.errors(DN.decay_params(etc(DN.denormalize(val["x"], val["S"]))), P_hi))
 
     def score(s):
         # mean errors in units of their just-noticeable differences
@@ -84,7 +94,7 @@ def main(argv=None):
         hist, vhist = DN.train(dn, data, args.steps, batch=args.batch, lr=lr, seed=args.seed, log=log, val=val,
                                eval_every=args.eval_every, weights=w)
         out = dn.forward(val["x"], val["desc"])
-        den = DN.summarize(DN.errors(DN.decay_params(DN.denormalize(out, val["S"])), P_hi))
+        den = DN.summarize(DN.errors(DN.decay_params(etc(DN.denormalize(out, val["S"]))), P_hi))
         v = {"lr": lr, "score": score(den), "loss_first": float(hist[0]), "loss_last100": float(np.mean(hist[-100:])),
              "val_history": vhist, "val_network": float(np.mean(w * (out - val["t"]) ** 2)), "denoised": den}
         variants.append(v)
@@ -95,7 +105,8 @@ def main(argv=None):
     dn.set_params(best[1])
     rep = {
         "rooms": args.rooms, "val_rooms": args.val_rooms, "lo_rays": args.lo_rays, "hi_rays": args.hi_rays,
-        "repeats": args.repeats, "cutoff_s": args.cutoff, "steps": args.steps, "batch": args.batch,
+        "repeats": args.repeats, "cutoff_s": args.cutoff, "is_order": args.is_order, "steps": args.steps,
+        "batch": args.batch,
         "weight_decay": args.weight_decay, "early_gain": args.early_gain, "early_tau": args.early_tau,
         "params": dn.n_params, "train_samples": int(data["x"].shape[0]), "val_samples": int(val["x"].shape[0]),
         "seconds": {"pairs": t_gen, "train": t_train, "total": time.perf_counter() - t_all},
@@ -107,7 +118,7 @@ def main(argv=None):
     log(json.dumps(rep))
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     dn.save(args.out, meta={k: rep[k] for k in ("rooms", "lo_rays", "hi_rays", "repeats", "steps", "chosen_lr",
-                                                "early_gain", "early_tau")})
+                                                "early_gain", "early_tau", "is_order")})
     if args.report:
         with open(args.report, "w") as f:
             json.dump(rep, f, indent=1)

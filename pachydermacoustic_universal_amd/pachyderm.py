@@ -141,6 +141,7 @@ class Source:
     """Omni point source; power W = 1e-12 10^(0.1 SWL) per band (Classes_Sources.cs:50-104)."""
 
     dir_mode = A.DIR_GEODESIC
+    type_name = "Geodesic"  # Source.Type() (Classes_Sources.cs:220-223, :282)
 
     def __init__(self, power_In_db, Source, ID=0, Third_Octave=False):
         if Third_Octave:
@@ -180,12 +181,14 @@ class RandomSource(Source):
     """theta, phi ~ U(0, 2 pi) (Classes_Sources.cs:238-266)."""
 
     dir_mode = A.DIR_RANDOM
+    type_name = "PseudoRandom"
 
 
 class SuppliedDirectionSource(Source):
     """Source ray i leaves along dirs[i % n] (the deterministic specular mode's seeded list)."""
 
     dir_mode = A.DIR_SUPPLIED
+    type_name = "Supplied"
 
     def __init__(self, power_In_db, Source, dirs, ID=0):
         super().__init__(power_In_db, Source, ID)
