@@ -187,7 +187,9 @@ int pach_rooms_set(pach_ctx *ctx, int n, const double *src, const double *power,
  * voxels the walk crossed (up to 64 receivers); "walk_steps" (1-64): voxels a
  * walk advances per step of the persistent loop (with "walk_flat" = 1: voxel
  * steps plus triangle tests, 1-256); "shade_min" (1-64): lanes that must wait
- * before a wave shades; "walk_flat" (0/1): flattened walk loop; "refill_min"
+ * before a wave shades; "walk_flat" (0/1/2): nested walk loop, flattened walk
+ * loop, or flattened with one kind of unit (tests or voxel moves) per wave
+ * iteration; "refill_min"
  * (1-64): idle lanes a wave gathers before refilling them (the refill
  * threshold); "dump_min" (>= 1): loop steps a drained wave runs before handing
  * its leftover rays to the next launch; "block" (256/512): workgroup size of
@@ -228,6 +230,7 @@ typedef struct pach_tuning {
     int grid[3];           /* voxel grid dimensions chosen (now uploaded) */
     int block, occupancy;  /* octave-kernel workgroup size and register cap */
     int shade_queue;       /* 1: finished walks are shaded 64 at a time from a per-wave queue */
+    int walk_flat;         /* walk loop form (pach_tune "walk_flat") */
     int walk_steps, shade_min, refill_min, dump_min;
     int trials;            /* sample traces run */
     double rate_start;     /* ray-bounces/s of the sample under the settings on entry */

@@ -59,7 +59,7 @@ class Stats(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("grid", C.c_int * 3), ("block", C.c_int), ("occupancy", C.c_int), ("shade_queue", C.c_int),
-                ("walk_steps", C.c_int), ("shade_min", C.c_int), ("refill_min", C.c_int),
+                ("walk_flat", C.c_int), ("walk_steps", C.c_int), ("shade_min", C.c_int), ("refill_min", C.c_int),
                 ("dump_min", C.c_int), ("trials", C.c_int), ("rate_start", C.c_double),
                 ("rate_best", C.c_double)]
 
@@ -70,7 +70,8 @@ class Tuning(C.Structure):
 
 
 # knobs of pach_tune that an autotuning result sets
-TUNED_KNOBS = ("block", "occupancy", "shade_queue", "walk_steps", "shade_min", "refill_min", "dump_min")
+TUNED_KNOBS = ("block", "occupancy", "shade_queue", "walk_flat", "walk_steps", "shade_min", "refill_min",
+               "dump_min")
 
 
 # (name, restype, argtypes) of every entry point declared in include/pach.h

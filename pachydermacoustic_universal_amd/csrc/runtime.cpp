@@ -117,7 +117,7 @@ struct pach_ctx {
     int tune_rmask = 1;        // per-voxel receiver masks (R <= 64)
     int tune_walk_steps = 12;  // walk units (tests + voxel steps) per loop step
     int tune_shade_min = 16;   // lanes waiting before shading
-    int tune_walk_flat = 1;    // flattened walk
+    int tune_walk_flat = 2;    // flattened walk, one kind of unit per wave iteration
     int tune_refill_min = 1;   // idle lanes gathered before a wave refills
     int tune_dump_min = 256;   // loop steps before a drained wave hands over its work
     OctShape tune_shape;       // k_octave workgroup size and register cap
@@ -1145,8 +1145,8 @@ extern "C" int pach_tune(pach_ctx *c, const char *key, int64_t value)
         c->tune_walk_steps = (int)value;
         return PACH_OK;
     }
-    if (std::strcmp(key, "walk_flat") == 0) {
-        c->tune_walk_flat = value != 0;
+    if (std::strcmp(key, "walk_flat") == 0 && value >= 0 && value <= 2) {
+        c->tune_walk_flat = (int)value;
         return PACH_OK;
     }
     if (std::strcmp(key, "shade_min") == 0 && value >= 1 && value <= 64) {
@@ -1245,11 +1245,11 @@ extern "C" int pach_autotune(pach_ctx *c, const pach_params *p, int64_t sample_r
         pp.ray_count = sample;
     }
     struct Knobs {
-        int grid[3], block, occ, queue, walk_steps, shade_min, refill_min, dump_min;
+        int grid[3], block, occ, queue, walk_flat, walk_steps, shade_min, refill_min, dump_min;
     };
     auto current = [c]() {
         Knobs k{{c->g[0], c->g[1], c->g[2]}, c->tune_shape.block, c->tune_shape.occ, c->tune_shape.queue,
-                c->tune_walk_steps, c->tune_shade_min, c->tune_refill_min, c->tune_dump_min};
+                c->tune_walk_flat, c->tune_walk_steps, c->tune_shade_min, c->tune_refill_min, c->tune_dump_min};
         return k;
     };
     auto apply = [c](const Knobs &k) -> int {
@@ -1258,6 +1258,7 @@ extern "C" int pach_autotune(pach_ctx *c, const pach_params *p, int64_t sample_r
         c->tune_shape.block = k.block;
         c->tune_shape.occ = k.occ;
         c->tune_shape.queue = k.queue;
+        c->tune_walk_flat = k.walk_flat;
         c->tune_walk_steps = k.walk_steps;
         c->tune_shade_min = k.shade_min;
         c->tune_refill_min = k.refill_min;
@@ -1304,6 +1305,7 @@ extern "C" int pach_autotune(pach_ctx *c, const pach_params *p, int64_t sample_r
         if ((rc = try_knob([](Knobs &k, int v) { k.queue = v; }, {0, 1}))) break;
         if ((rc = try_knob([](Knobs &k, int v) { k.occ = v; }, {0, 4, 5}))) break;
         if ((rc = try_knob([](Knobs &k, int v) { k.block = v; }, {256, 512}))) break;
+        if ((rc = try_knob([](Knobs &k, int v) { k.walk_flat = v; }, {1, 2}))) break;
         if ((rc = try_knob([](Knobs &k, int v) { k.walk_steps = v; }, {6, 8, 12, 16, 24, 32}))) break;
         if ((rc = try_knob([](Knobs &k, int v) { k.shade_min = v; }, {8, 12, 16, 24, 32, 48}))) break;
         if ((rc = try_knob([](Knobs &k, int v) { k.refill_min = v; }, {1, 4, 8, 16, 32}))) break;
@@ -1321,6 +1323,7 @@ extern "C" int pach_autotune(pach_ctx *c, const pach_params *p, int64_t sample_r
     out->block = best.block;
     out->occupancy = best.occ;
     out->shade_queue = best.queue;
+    out->walk_flat = best.walk_flat;
     out->walk_steps = best.walk_steps;
     out->shade_min = best.shade_min;
     out->refill_min = best.refill_min;

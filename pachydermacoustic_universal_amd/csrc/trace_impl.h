@@ -290,7 +290,11 @@ __device__ bool walk_steps(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R
 // triangle of the current voxel or leaves the voxel (one step, or one jump
 // across an empty block), so lanes in triangle-dense voxels do not hold the
 // wave on a long inner loop.  Same walk, same result as walk_steps.
-template <class R, bool LDS>
+// SCHED: the wave takes one kind of unit per iteration -- triangle tests when
+// at least half of its walking lanes have one pending, else voxel moves -- so
+// an iteration runs one of the two code paths instead of both; lanes of the
+// other kind sit that unit out.
+template <class R, bool LDS, bool SCHED = false>
 __device__ bool walk_units(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R dx, R dy, R dz, int excl,
                            int budget, uint32_t &n_vox, uint32_t &n_test)
 {
@@ -309,7 +313,14 @@ __device__ bool walk_units(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R
             }
         }
         const R texit = fmin(w.tmx, fmin(w.tmy, w.tmz));
-        if (w.j < w.je) {
+        const bool pending = w.j < w.je;
+        if constexpr (SCHED) {
+            const int n_pending = __popcll(__ballot(pending));
+            const int n_walking = __popcll(__ballot(true));
+            const bool tests_now = 2 * n_pending >= n_walking;
+            if (pending != tests_now) continue;  // this lane's kind of unit waits
+        }
+        if (pending) {
             ++n_test;
             const uint32_t jj = w.j++;
             const int t = (int)s.cell_tris[jj];
@@ -976,12 +987,16 @@ k_octave(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv,
             }
         }
         if (walking) {
+            const int ex = Tmin<R>::excl ? r.tri : -1;
             const bool done =
-                tc.walk_flat
-                    ? walk_units<R, LDS>(s, w, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2],
-                                         Tmin<R>::excl ? r.tri : -1, tc.walk_steps, lc.v[C_VOXEL], lc.v[C_TEST])
-                    : walk_steps<R, LDS>(s, w, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2],
-                                         Tmin<R>::excl ? r.tri : -1, tc.walk_steps, lc.v[C_VOXEL], lc.v[C_TEST]);
+                tc.walk_flat == 2
+                    ? walk_units<R, LDS, true>(s, w, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2], ex, tc.walk_steps,
+                                               lc.v[C_VOXEL], lc.v[C_TEST])
+                : tc.walk_flat
+                    ? walk_units<R, LDS>(s, w, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2], ex, tc.walk_steps,
+                                         lc.v[C_VOXEL], lc.v[C_TEST])
+                    : walk_steps<R, LDS>(s, w, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2], ex, tc.walk_steps,
+                                         lc.v[C_VOXEL], lc.v[C_TEST]);
             if (done) {
                 walking = false;
                 ready = true;

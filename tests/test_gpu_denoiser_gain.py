@@ -1,7 +1,9 @@
 """What the trained denoiser gains over the plain trace it starts from.
 
 Two hall variants the training never saw (16 receivers each) are traced
-plainly at 1e4 source rays and, as the reference, at 3e5 rays.  The shipped
+plainly at 1e4 source rays and, as the reference, at 3e5 rays, with
+IS_Order 0 as the network was trained: the direct sound comes from
+Direct_Sound and is added to every energy-time curve (ETCurve).  The shipped
 network (pachydermacoustic_universal_amd/weights/denoiser.npz) denoises the
 1e4-ray histograms; its mean EDT error and mean C80 error against the
 reference must be lower than the plain 1e4-ray trace's, and its T30 error no
@@ -28,14 +30,16 @@ def test_denoiser_beats_the_plain_trace_it_starts_from():
         for seed in (200_001, 200_002):
             room = rooms.hall_variant(seed)
             print(f"[case] denoiser gain {room.name}", flush=True)
-            A.trace_room(tr, room, rays=300_000, cutoff_s=3.0, seed=1)
+            dE, dbin = DN.direct_part(room)
+            etc = lambda E: DN.with_direct(E, dE, dbin)  # noqa: E731
+            A.trace_room(tr, room, rays=300_000, cutoff_s=3.0, seed=1, is_order=0)
             E_ref = tr.histogram(0, scaled=True)
-            A.trace_room(tr, room, rays=10_000, cutoff_s=3.0, seed=2)
+            A.trace_room(tr, room, rays=10_000, cutoff_s=3.0, seed=2, is_order=0)
             E_lo = tr.histogram(0, scaled=True)
             E_dn = dn.denoise(E_lo, DN.descriptors(room))
-            P_ref = DN.decay_params(E_ref)
-            err_plain.append(DN.errors(DN.decay_params(E_lo), P_ref))
-            err_dn.append(DN.errors(DN.decay_params(E_dn), P_ref))
+            P_ref = DN.decay_params(etc(E_ref))
+            err_plain.append(DN.errors(DN.decay_params(etc(E_lo)), P_ref))
+            err_dn.append(DN.errors(DN.decay_params(etc(E_dn)), P_ref))
     finally:
         tr.close()
         dn.close()

@@ -115,7 +115,7 @@ def test_fp32_matches_fp64_statistics(tracer, medium):
     assert s32["lost"] / s32["e_init"] < 1e-5
 
 
-DEFAULT_KNOBS = {"walk_flat": 1, "walk_steps": 12, "receiver_masks": 1, "dist_lds": 1, "shade_min": 16,
+DEFAULT_KNOBS = {"walk_flat": 2, "walk_steps": 12, "receiver_masks": 1, "dist_lds": 1, "shade_min": 16,
                  "refill_min": 1, "dump_min": 256, "block": 256, "occupancy": -1, "shade_queue": 0}
 
 
@@ -124,6 +124,7 @@ DEFAULT_KNOBS = {"walk_flat": 1, "walk_steps": 12, "receiver_masks": 1, "dist_ld
                                    {"refill_min": 24, "block": 512, "dump_min": 16},
                                    {"occupancy": 4, "refill_min": 8, "shade_min": 8},
                                    {"shade_queue": 1}, {"occupancy": 5}, {"occupancy": 0},
+                                   {"walk_flat": 1}, {"walk_flat": 2, "walk_steps": 5, "shade_queue": 1},
                                    {"shade_queue": 1, "block": 512, "refill_min": 16, "dump_min": 8,
                                     "walk_steps": 3}])
 def test_launch_tuning_does_not_change_results(tracer, medium, knobs):
