@@ -5,10 +5,11 @@
 //
 // GEMM tiling: a 256-thread workgroup computes a 128 x 128 tile of C; each of
 // its four wave64s owns a 64 x 64 quarter as 2 x 2 MFMA 32x32 blocks.  K is
-// consumed 32 at a time through LDS, where both operands are kept k-contiguous
-// (rows padded to 40 elements) so each lane fetches its 8-element MFMA operand
-// with one 16-byte read.  Transposed operands are transposed on the way into
-// LDS.  Weight gradients (K = every patch of the batch) split K over the grid
+// consumed 64 at a time through LDS, where both operands are kept k-contiguous
+// (rows padded to 72 elements) so each lane fetches its 8-element MFMA operand
+// with one 16-byte read; the global loads of the next K tile are issued before
+// the MFMAs of the current one, so they are in flight while it computes.
+// Transposed operands are transposed on the way into LDS.  Weight gradients (K = every patch of the batch) split K over the grid
 // and sum the partial tiles in a second kernel, so results are deterministic.
 #include <hip/hip_runtime.h>
 
@@ -26,7 +27,8 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int kBM = 128, kBN = 128, kBK = 32, kLdsK = kBK + 8;
+constexpr int kBM = 128, kBN = 128, kBK = 64, kLdsK = kBK + 8;
+constexpr int kChunks = kBM * kBK / 8 / 256;  // 16-byte chunks per operand tile per thread
 constexpr int kThreads = 256;
 
 __device__ __forceinline__ uint16_t f2bf(float f)
@@ -88,35 +90,58 @@ k_gemm(const uint16_t *__restrict__ A, int lda, const uint16_t *__restrict__ B, 
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
 
-    for (int k0 = kb; k0 < ke; k0 += kBK) {
+    // chunk c of a tile: non-transposed operands read rows of kBK (8 chunks a
+    // row), transposed ones read k-rows of 128 (16 chunks a row)
+    uint4 ra[kChunks], rb[kChunks];
+    auto load = [&](int k0) {
 #pragma unroll
-        for (int pass = 0; pass < 2; ++pass) {
-            const int c = tid + pass * kThreads;  // 512 chunks of 8 elements per operand tile
-            uint4 va = make_uint4(0, 0, 0, 0), vb = make_uint4(0, 0, 0, 0);
+        for (int pass = 0; pass < kChunks; ++pass) {
+            const int c = tid + pass * kThreads;
+            ra[pass] = make_uint4(0, 0, 0, 0);
+            rb[pass] = make_uint4(0, 0, 0, 0);
             if (!TA) {
-                const int r = c >> 2, kc = (c & 3) * 8;
-                if (m0 + r < M && k0 + kc < ke) va = *(const uint4 *)(A + (size_t)(m0 + r) * lda + k0 + kc);
-                *(uint4 *)(As + r * kLdsK + kc) = va;
+                const int r = c >> 3, kc = (c & 7) * 8;
+                if (m0 + r < M && k0 + kc < ke) ra[pass] = *(const uint4 *)(A + (size_t)(m0 + r) * lda + k0 + kc);
             } else {
                 const int kr = c >> 4, mc = (c & 15) * 8;
-                if (k0 + kr < ke && m0 + mc < M) va = *(const uint4 *)(A + (size_t)(k0 + kr) * lda + m0 + mc);
-                const uint16_t *e = (const uint16_t *)&va;
+                if (k0 + kr < ke && m0 + mc < M) ra[pass] = *(const uint4 *)(A + (size_t)(k0 + kr) * lda + m0 + mc);
+            }
+            if (TB) {
+                const int r = c >> 3, kc = (c & 7) * 8;
+                if (n0 + r < N && k0 + kc < ke) rb[pass] = *(const uint4 *)(B + (size_t)(n0 + r) * ldb + k0 + kc);
+            } else {
+                const int kr = c >> 4, nc = (c & 15) * 8;
+                if (k0 + kr < ke && n0 + nc < N) rb[pass] = *(const uint4 *)(B + (size_t)(k0 + kr) * ldb + n0 + nc);
+            }
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int pass = 0; pass < kChunks; ++pass) {
+            const int c = tid + pass * kThreads;
+            if (!TA) {
+                *(uint4 *)(As + (c >> 3) * kLdsK + (c & 7) * 8) = ra[pass];
+            } else {
+                const int kr = c >> 4, mc = (c & 15) * 8;
+                const uint16_t *e = (const uint16_t *)&ra[pass];
 #pragma unroll
                 for (int i = 0; i < 8; ++i) As[(mc + i) * kLdsK + kr] = e[i];
             }
             if (TB) {
-                const int r = c >> 2, kc = (c & 3) * 8;
-                if (n0 + r < N && k0 + kc < ke) vb = *(const uint4 *)(B + (size_t)(n0 + r) * ldb + k0 + kc);
-                *(uint4 *)(Bs + r * kLdsK + kc) = vb;
+                *(uint4 *)(Bs + (c >> 3) * kLdsK + (c & 7) * 8) = rb[pass];
             } else {
                 const int kr = c >> 4, nc = (c & 15) * 8;
-                if (k0 + kr < ke && n0 + nc < N) vb = *(const uint4 *)(B + (size_t)(k0 + kr) * ldb + n0 + nc);
-                const uint16_t *e = (const uint16_t *)&vb;
+                const uint16_t *e = (const uint16_t *)&rb[pass];
 #pragma unroll
                 for (int i = 0; i < 8; ++i) Bs[(nc + i) * kLdsK + kr] = e[i];
             }
         }
+    };
+    if (kb < ke) load(kb);
+    for (int k0 = kb; k0 < ke; k0 += kBK) {
+        store();
         __syncthreads();
+        if (k0 + kBK < ke) load(k0 + kBK);  // next tile in flight during the MFMAs
 #pragma unroll
         for (int kk = 0; kk < kBK; kk += 16) {
             // operand layout of the 32x32x16 MFMA: lane l supplies row (l % 32)
