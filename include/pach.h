@@ -36,6 +36,7 @@ extern "C" {
 #define PACH_ESTATE (-3)   /* call out of order (no scene, trace running, ...) */
 #define PACH_ENOMEM (-4)   /* device allocation failed */
 #define PACH_ECANCEL (-5)  /* trace was cancelled (results hold completed rays) */
+#define PACH_ERANGE (-6)   /* output buffer too small (the count needed is returned) */
 
 /* trace modes */
 #define PACH_MODE_STOCHASTIC 0  /* Lambert split scheme of Lambert_Scattering */
@@ -212,6 +213,35 @@ int pach_tune(pach_ctx *ctx, const char *key, int64_t value);
 int pach_direct_sound(const double *verts, int T, const int *surf_of_tri, const double *trans, int S,
                       const double src[3], const double power[8], const double *centers, int R,
                       const double atten[8], double c, double *energy, double *time_s, int *audible);
+
+/* Image sources: one valid specular path of a reflection sequence to one
+ * receiver (Specular_Path, Classes_ImageSource.cs:1891-1982). */
+#define PACH_IS_MAX_ORDER 8
+typedef struct pach_is_path {
+    int32_t receiver;
+    int32_t order;                         /* reflections */
+    int32_t seq[PACH_IS_MAX_ORDER];        /* object reflected from, source side first */
+    int32_t poly[PACH_IS_MAX_ORDER];       /* polygon hit at each reflection */
+    double pts[PACH_IS_MAX_ORDER + 2][3];  /* source, reflection points, receiver */
+    double length;                         /* m */
+    double time_s;                         /* length / c */
+    double energy[8];                      /* PathEnergy per octave band */
+} pach_is_path;
+
+/* ImageSourceData for planar objects without edge diffraction
+ * (Classes_ImageSource.cs: TraverseOrders :266-348, ProcessImages :350-476,
+ * OcclusionIntersect :1350-1392, FinalOcclusion :1245-1281, Power_Recursion
+ * :535-803, B_Function_NoInterp / H_Function :859-1030, Specular_Path
+ * :1914-1982), on the uploaded scene, source and receiver centres.  Every
+ * sequence of 1..max_order objects (repeats allowed) is validated for every
+ * receiver on the GPU in FP64.  object_of_surface [S] groups polygons into the
+ * reference's objects (Polygon_Scene.Construct's Model[i]); an object reflects
+ * in the plane of its lowest polygon.  NULL: each polygon is its own object
+ * (n_objects = S).  The valid paths are sorted by receiver, then in the
+ * reference's depth-first sequence order; min(total, cap) are written to out
+ * and *n_paths = total.  Returns PACH_ERANGE when total > cap. */
+int pach_image_sources(pach_ctx *ctx, const int *object_of_surface, int n_objects, int max_order,
+                       pach_is_path *out, int64_t cap, int64_t *n_paths);
 
 /* One revolution of a convergence check on host arrays, the function the
  * convergence mode applies to each checked receiver (TaskAll:870-956 for

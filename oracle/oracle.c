@@ -17,9 +17,15 @@
  *                   bins and energies), energy budget, sampler KS tests,
  *                   geodesic stratification, Sabine/Eyring diffuse-field trend.
  *   decay math      exponential decays with closed-form T30/EDT/C80.
+ *   image sources   shoebox image lattice (path count 4n^2+2 and lengths per
+ *                   order), mirror law at every reflection point, spherical
+ *                   spreading at zero losses, per-wall material products,
+ *                   segment-box occlusion and transmissive crossings
+ *                   (tests/test_image_source_oracle.py).
  * Parity unpinned (no executable reference exists anywhere offline):
  *   Hare GetRandomPoint's exact (u,v) mapping, Hare Ray.Reverse (taken as
- *   d -> -d), Hare tie-breaking at shared edges, and .NET System.Random
+ *   d -> -d), Hare tie-breaking at shared edges, Hare's DistToPlane /
+ *   Normalize rounding and its X_Event on a miss (taken as t = 0), and .NET System.Random
  *   streams (replaced by per-ray Philox streams, matched only in distribution).
  */
 #define _GNU_SOURCE
@@ -1025,4 +1031,270 @@ void or_eyring(const or_scene *s, double volume, const double atten[8], double o
         for (int q = 0; q < s->S; ++q) ta += s->area[q] * (-log(1 - s->alpha[8 * q + t]));
         out[t] = 0.161 * volume / (ta + 4 * atten[t] * volume);
     }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Image sources: ImageSourceData for planar objects without edge            */
+/* diffraction (Classes_ImageSource.cs).                                     */
+/*   TraverseOrders :266-348   every sequence of 1..MaxOrder objects,         */
+/*                              repeats allowed, depth-first                  */
+/*   ProcessImages  :350-476   backward validation, final checks, energy     */
+/*   OcclusionIntersect :1350-1392, FinalOcclusion :1245-1281                 */
+/*   Power_Recursion :535-803 (planar branch), project_wavefront :805-857     */
+/*   B_Function_NoInterp :859-890, H_Function :892-1030 (planar branch)       */
+/*   Specular_Path :1914-1982                                                */
+/*   Polygon_Scene.Image :701-707                                            */
+/* Hare's X_Event of a miss is taken as t = 0 (the library is not part of the */
+/* reference): OcclusionIntersect then fails and FinalOcclusion reports open. */
+/* ------------------------------------------------------------------------ */
+/* transmissive surfaces a check passes before it gives up (as a miss) */
+#define IS_PASS_LIMIT (1 << 16)
+
+typedef struct {
+    const or_scene *s;
+    const int *obj;    /* [S] */
+    const double *pl;  /* [n_obj][6]: unit normal, point */
+} is_ctx;
+
+static int is_transmissive(const or_scene *s, int p)
+{
+    for (int b = 0; b < 8; ++b)
+        if (s->trans[8 * p + b] > 0) return 1;
+    return 0;
+}
+
+static void is_normalize(double *v)
+{
+    double l = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    v[0] /= l; v[1] /= l; v[2] /= l;
+}
+
+static double is_len(const double *a, const double *b)
+{
+    double x = a[0] - b[0], y = a[1] - b[1], z = a[2] - b[2];
+    return sqrt(x * x + y * y + z * z);
+}
+
+static int is_sign(double x) { return x > 0 ? 1 : (x < 0 ? -1 : 0); }
+
+/* closest hit; X = o + d t */
+static int is_shoot(const is_ctx *k, const double o[3], const double d[3], double *t, double X[3])
+{
+    int tri = or_shoot_grid(k->s, o, d, t);
+    if (tri >= 0)
+        for (int j = 0; j < 3; ++j) X[j] = o[j] + d[j] * *t;
+    return tri;
+}
+
+/* OcclusionIntersect (:1350-1392): from O towards the image E, passing
+ * transmissive polygons of other objects, the first surface must belong to
+ * object polyX. */
+static int is_occl_intersect(const is_ctx *k, const double O[3], const double E[3], int polyX, double tm[8],
+                             double Xout[3], int *poly_out)
+{
+    const or_scene *s = k->s;
+    double D[3] = {E[0] - O[0], E[1] - O[1], E[2] - O[2]};
+    double L = sqrt(D[0] * D[0] + D[1] * D[1] + D[2] * D[2]);
+    D[0] /= L; D[1] /= L; D[2] /= L;
+    double o[3] = {O[0], O[1], O[2]}, X[3], t = 0;
+    int tri = -1;
+    for (int guard = 0; guard < IS_PASS_LIMIT; ++guard) {
+        tri = is_shoot(k, o, D, &t, X);
+        if (tri >= 0) {
+            int p = s->surf[tri];
+            if (is_transmissive(s, p) && polyX != k->obj[p]) {
+                for (int b = 0; b < 8; ++b) tm[b] *= (1 - s->alpha[8 * p + b]) * s->trans[8 * p + b];
+                memcpy(o, X, sizeof(o));
+                continue;
+            }
+        }
+        break;
+    }
+    if (tri < 0) return 0;
+    if (t < 0.000001) return 0;
+    if (polyX != k->obj[s->surf[tri]]) return 0;
+    *poly_out = s->surf[tri];
+    memcpy(Xout, X, sizeof(X));
+    return 1;
+}
+
+/* FinalOcclusion (:1245-1281): from the first reflection point back to the
+ * source; 1 = occluded. */
+static int is_final_occlusion(const is_ctx *k, const double S[3], const double P1[3], int polyX, double tm[8])
+{
+    const or_scene *s = k->s;
+    double D[3] = {S[0] - P1[0], S[1] - P1[1], S[2] - P1[2]};
+    double L = sqrt(D[0] * D[0] + D[1] * D[1] + D[2] * D[2]);
+    double L2 = 0;
+    is_normalize(D);
+    double o[3] = {P1[0], P1[1], P1[2]}, X[3], t = 0;
+    int tri = -1;
+    for (int guard = 0; guard < IS_PASS_LIMIT; ++guard) {
+        tri = is_shoot(k, o, D, &t, X);
+        if (tri >= 0) {
+            int p = s->surf[tri];
+            if (is_transmissive(s, p) && polyX != k->obj[p] && t < L) {
+                for (int b = 0; b < 8; ++b) tm[b] *= (1 - s->alpha[8 * p + b]) * s->trans[8 * p + b];
+                memcpy(o, X, sizeof(o));
+                L2 += t;
+                continue;
+            }
+        }
+        break;
+    }
+    if (tri < 0) return 0;
+    L2 += t;
+    return L2 < L;
+}
+
+/* ProcessImages (:350-476) for one sequence and one receiver, planar objects:
+ * one image per reflection, one path.  Returns 1 and fills *out when valid. */
+static int is_process(const is_ctx *k, const int *seq, int n, const double (*img)[3], const double src[3],
+                      const double power[8], const double E[3], const double atten[8], double c, double rho_c,
+                      or_is_path *out)
+{
+    const or_scene *s = k->s;
+    double P[OR_IS_MAX_ORDER + 2][3];
+    int polys[OR_IS_MAX_ORDER];
+    double tm[8] = {1, 1, 1, 1, 1, 1, 1, 1};
+    memcpy(P[0], src, sizeof(P[0]));
+    memcpy(P[n + 1], E, sizeof(P[0]));
+    for (int q = n - 1; q >= 0; --q)
+        if (!is_occl_intersect(k, P[q + 2], img[q], seq[q], tm, P[q + 1], &polys[q])) return 0;
+    /* final checks (:423-441): both legs on one side of each reflecting plane,
+     * then the first leg unobstructed */
+    for (int i = 0; i < n; ++i) {
+        double d1[3], d2[3];
+        for (int j = 0; j < 3; ++j) {
+            d1[j] = P[i][j] - P[i + 1][j];
+            d2[j] = P[i + 2][j] - P[i + 1][j];
+        }
+        const double *nrm = k->pl + 6 * seq[i];
+        is_normalize(d1);
+        is_normalize(d2);
+        if (is_sign(nrm[0] * d1[0] + nrm[1] * d1[1] + nrm[2] * d1[2]) !=
+            is_sign(nrm[0] * d2[0] + nrm[1] * d2[1] + nrm[2] * d2[2]))
+            return 0;
+        if (i == n - 1 && is_final_occlusion(k, P[0], P[1], seq[0], tm)) return 0;
+    }
+    /* source power along the first leg (Source.DirPower: no directivity) */
+    for (int b = 0; b < 8; ++b) tm[b] *= power[b];
+    /* Power_Recursion, planar branch: wavefront curvature 1 / length of the
+     * first leg, projected over every later leg */
+    double kv = 1.0 / is_len(P[1], P[0]);
+    double l0 = is_len(P[0], P[1]);
+    double t0 = l0 / c, t1 = l0 / c;
+    for (int o = 0; o < n; ++o) {
+        double l = is_len(P[o + 1], P[o + 2]);
+        kv = (kv == INFINITY) ? 0 : 1 / kv;
+        kv += l;
+        kv = (kv == INFINITY) ? 0 : 1 / kv;
+    }
+    double h = sqrt(fabs((1 / (4 * M_PI)) * (kv * kv - 0.0 * 0.0)));
+    double p = fabs(h);
+    /* Specular_Path */
+    double length = 0;
+    for (int q = 1; q < n + 2; ++q) {
+        double x = P[q][0] - P[q - 1][0], y = P[q][1] - P[q - 1][1], z = P[q][2] - P[q - 1][2];
+        length += sqrt(x * x + y * y + z * z);
+    }
+    for (int b = 0; b < 8; ++b) {
+        /* B_Function_NoInterp: amplitude with the air attenuation of the mean
+         * of the time limits (here the first leg's travel time) */
+        double mod = sqrt(tm[b] * pow(10, -.05 * atten[b] * c * 0.5 * (t0 + t1)));
+        double B = p * mod;
+        /* H_Function: pressure of the source's DirPressure (Source.Rho_C 411.6) */
+        double H = B * sqrt(power[b] * 411.6);
+        double Hrc = H * H / rho_c;
+        double e = pow(10, -.1 * atten[b] * length) * Hrc;
+        e *= tm[b];
+        out->energy[b] = e;
+    }
+    for (int q = 0; q < n; ++q) {
+        int pq = polys[q];
+        for (int b = 0; b < 8; ++b) {
+            out->energy[b] *= (1 - s->alpha[8 * pq + b]) * (1 - s->scat[8 * pq + b]) * (1 - s->trans[8 * pq + b]);
+            if (out->energy[b] == 0) out->energy[b] = 1E-22;
+        }
+    }
+    out->order = n;
+    for (int q = 0; q < n; ++q) {
+        out->seq[q] = seq[q];
+        out->poly[q] = polys[q];
+    }
+    memcpy(out->pts, P, sizeof(double) * 3 * (n + 2));
+    out->length = length;
+    out->time = length / c;
+    return 1;
+}
+
+typedef struct {
+    const is_ctx *k;
+    int n_obj, max_order, R;
+    const double *src, *power, *recs, *atten;
+    double c, rho_c;
+    or_is_path *out;
+    int64_t cap, n;
+} is_run;
+
+static void is_traverse(is_run *r, int *seq, int order, double (*img)[3])
+{
+    const double *prev = order > 0 ? img[order - 1] : r->src;
+    for (int q = 0; q < r->n_obj; ++q) {
+        if (order == 0 && q > 0) break; /* caller sets seq[0] */
+        if (order > 0) seq[order] = q;
+        /* Polygon_Scene.Image: mirror over the plane of the object's first polygon */
+        const double *pl = r->k->pl + 6 * seq[order];
+        double dist = pl[0] * (prev[0] - pl[3]) + pl[1] * (prev[1] - pl[4]) + pl[2] * (prev[2] - pl[5]);
+        for (int j = 0; j < 3; ++j) img[order][j] = prev[j] - 2 * pl[j] * dist;
+        for (int rec = 0; rec < r->R; ++rec) {
+            or_is_path tmp;
+            memset(&tmp, 0, sizeof(tmp));
+            if (is_process(r->k, seq, order + 1, (const double (*)[3])img, r->src, r->power, r->recs + 3 * rec,
+                           r->atten, r->c, r->rho_c, &tmp)) {
+                tmp.rec = rec;
+                if (r->n < r->cap) r->out[r->n] = tmp;
+                r->n++;
+            }
+        }
+        if (order + 1 < r->max_order) is_traverse(r, seq, order + 1, img);
+    }
+}
+
+int64_t or_image_sources(const or_scene *s, const int *obj_of_surf, int n_obj, int max_order,
+                         const double src[3], const double power[8], const double *recs, int R,
+                         const double atten[8], double c, double rho_c, or_is_path *out, int64_t cap)
+{
+    if (max_order < 1 || max_order > OR_IS_MAX_ORDER || n_obj < 1) return -1;
+    int *obj = malloc(sizeof(int) * s->S);
+    double *pl = calloc((size_t)n_obj * 6, sizeof(double));
+    int *first_tri = malloc(sizeof(int) * n_obj);
+    for (int o = 0; o < n_obj; ++o) first_tri[o] = -1;
+    for (int p = 0; p < s->S; ++p) obj[p] = obj_of_surf ? obj_of_surf[p] : p;
+    /* ObjectMembers[o][0]: the object's lowest polygon; its first triangle */
+    int *first_poly = malloc(sizeof(int) * n_obj);
+    for (int o = 0; o < n_obj; ++o) first_poly[o] = -1;
+    for (int p = 0; p < s->S; ++p)
+        if (obj[p] >= 0 && obj[p] < n_obj && first_poly[obj[p]] < 0) first_poly[obj[p]] = p;
+    for (int t = 0; t < s->T; ++t) {
+        int p = s->surf[t], o = obj[p];
+        if (o >= 0 && o < n_obj && first_poly[o] == p && first_tri[o] < 0) first_tri[o] = t;
+    }
+    for (int o = 0; o < n_obj; ++o) {
+        int t = first_tri[o];
+        if (t < 0) { pl[6 * o] = 0; continue; } /* object without triangles: never hit */
+        memcpy(pl + 6 * o, s->n + 3 * t, sizeof(double) * 3);
+        memcpy(pl + 6 * o + 3, s->v0 + 3 * t, sizeof(double) * 3);
+    }
+    is_ctx k = {s, obj, pl};
+    is_run r = {&k, n_obj, max_order, R, src, power, recs, atten, c, rho_c, out, cap, 0};
+    int seq[OR_IS_MAX_ORDER];
+    double img[OR_IS_MAX_ORDER][3];
+    /* Begin / Calculate (:117-170): one depth-first traversal per first object */
+    for (int q0 = 0; q0 < n_obj; ++q0) {
+        seq[0] = q0;
+        is_traverse(&r, seq, 0, img);
+    }
+    free(obj); free(pl); free(first_tri); free(first_poly);
+    return r.n;
 }

@@ -140,6 +140,26 @@ double or_clarity(const double *etc, int n, double fs, double seconds, double st
 void or_sabine(const or_scene *s, double volume, const double atten[8], double out[8]);
 void or_eyring(const or_scene *s, double volume, const double atten[8], double out[8]);
 
+/* --- image sources (ImageSourceData, planar objects, no edge diffraction) -- */
+#define OR_IS_MAX_ORDER 8
+typedef struct {
+    int rec, order;
+    int seq[OR_IS_MAX_ORDER];    /* object of each reflection, source side first */
+    int poly[OR_IS_MAX_ORDER];   /* polygon hit at each reflection */
+    double pts[OR_IS_MAX_ORDER + 2][3];  /* source, reflection points, receiver */
+    double length, time;
+    double energy[8];
+} or_is_path;
+
+/* obj_of_surf: [S] object of each polygon (NULL: every polygon its own object);
+ * planes come from each object's first polygon (lowest index) through its
+ * first triangle.  Appends the valid paths of every sequence of 1..max_order
+ * objects, sequences in depth-first order, receivers inner; returns the number
+ * of valid paths (written up to cap). */
+int64_t or_image_sources(const or_scene *s, const int *obj_of_surf, int n_obj, int max_order,
+                         const double src[3], const double power[8], const double *recs, int R,
+                         const double atten[8], double c, double rho_c, or_is_path *out, int64_t cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -105,6 +105,9 @@ def lib():
         L.or_sincos_turn.argtypes = [C.c_double, _dp, _dp]
         L.or_sabine.argtypes = [C.c_void_p, C.c_double, _dp, _dp]
         L.or_eyring.argtypes = [C.c_void_p, C.c_double, _dp, _dp]
+        L.or_image_sources.restype = C.c_int64
+        L.or_image_sources.argtypes = [C.c_void_p, _ip, C.c_int, C.c_int, _dp, _dp, _dp, C.c_int, _dp,
+                                       C.c_double, C.c_double, C.c_void_p, C.c_int64]
         _lib = L
     return _lib
 
@@ -151,6 +154,12 @@ def standard_medium(temp_c=20.0, hr=50.0, pa_hpa=1013.25, air_choice=0):
                   rho=density(temp_c, hr, pa_hpa))
 
 
+IS_MAX_ORDER = 8
+IS_PATH = np.dtype([("rec", "<i4"), ("order", "<i4"), ("seq", "<i4", (IS_MAX_ORDER,)),
+                    ("poly", "<i4", (IS_MAX_ORDER,)), ("pts", "<f8", (IS_MAX_ORDER + 2, 3)),
+                    ("length", "<f8"), ("time", "<f8"), ("energy", "<f8", (8,))])
+
+
 class Scene:
     """Oracle scene (triangles + per-polygon materials + its own voxel grid)."""
 
@@ -186,6 +195,29 @@ class Scene:
     def h_oct(self, power, oct_lo=0, oct_hi=7):
         pw = np.ascontiguousarray(power, dtype=np.float64)
         return lib().or_h_oct(self.h, _p(pw), oct_lo, oct_hi)
+
+    def image_sources(self, max_order, src, power, receivers, medium, obj_of_surf=None, n_obj=None):
+        """Valid specular paths of every sequence of 1..max_order objects
+        (ImageSourceData, planar objects): a structured array of IS_PATH, in
+        the reference's depth-first sequence order, receivers inner."""
+        src = np.ascontiguousarray(src, dtype=np.float64)
+        pw = np.ascontiguousarray(power, dtype=np.float64)
+        rec = np.ascontiguousarray(receivers, dtype=np.float64).reshape(-1, 3)
+        at = np.ascontiguousarray(medium.atten, dtype=np.float64)
+        S = self._a.shape[0]
+        if obj_of_surf is None:
+            obj_ptr, n_obj = None, S
+        else:
+            obj = np.ascontiguousarray(obj_of_surf, dtype=np.int32)
+            obj_ptr, n_obj = obj.ctypes.data_as(_ip), int(n_obj if n_obj is not None else obj.max() + 1)
+        args = (self.h, obj_ptr, n_obj, int(max_order), _p(src), _p(pw), _p(rec), rec.shape[0], _p(at),
+                float(medium.c), float(medium.rho_c))
+        n = lib().or_image_sources(*args, None, 0)
+        if n < 0:
+            raise ValueError("image sources: bad arguments")
+        out = np.zeros(n, dtype=IS_PATH)
+        lib().or_image_sources(*args, out.ctypes.data, n)
+        return out
 
     def sabine(self, volume, atten):
         out = np.zeros(8)

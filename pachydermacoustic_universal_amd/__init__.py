@@ -13,18 +13,20 @@ from .pachyderm import (  # noqa: F401
     Direct_Sound,
     GeodesicSource,
     IR_Construction,
+    ImageSourceData,
     Lambert_Scattering,
     Polygon_Scene,
     RandomSource,
     Receiver_Bank,
     Running,
     Simulation_Type,
+    Specular_Path,
     SplitRayTracer,
     Stopped,
     SuppliedDirectionSource,
 )
 
 __all__ = ["AcousticalMath", "Basic_Material", "Direct_Sound", "GeodesicSource", "IR_Construction",
-           "Lambert_Scattering",
+           "ImageSourceData", "Lambert_Scattering", "Specular_Path",
            "Polygon_Scene", "RandomSource", "Receiver_Bank", "Running", "Simulation_Type",
            "SplitRayTracer", "Stopped", "SuppliedDirectionSource", "rooms"]

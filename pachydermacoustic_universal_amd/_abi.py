@@ -14,7 +14,8 @@ _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "libpach.so")
 
 PACH_OK = 0
-ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ESTATE", -4: "ENOMEM", -5: "ECANCEL"}
+ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ESTATE", -4: "ENOMEM", -5: "ECANCEL", -6: "ERANGE"}
+PACH_ERANGE = -6
 MODE_STOCHASTIC, MODE_SPECULAR = 0, 1
 PREC_FP32, PREC_FP64 = 0, 1
 DIR_GEODESIC, DIR_RANDOM, DIR_SUPPLIED = 0, 1, 2
@@ -69,6 +70,14 @@ class Tuning(C.Structure):
         return d
 
 
+# pach_is_path (one valid image-source path), as a numpy record
+IS_MAX_ORDER = 8
+IS_PATH = np.dtype([("receiver", "<i4"), ("order", "<i4"), ("seq", "<i4", (IS_MAX_ORDER,)),
+                    ("poly", "<i4", (IS_MAX_ORDER,)), ("pts", "<f8", (IS_MAX_ORDER + 2, 3)),
+                    ("length", "<f8"), ("time_s", "<f8"), ("energy", "<f8", (8,))])
+assert IS_PATH.itemsize == 392
+
+
 # knobs of pach_tune that an autotuning result sets
 TUNED_KNOBS = ("block", "occupancy", "shade_queue", "walk_flat", "walk_steps", "shade_min", "refill_min",
                "dump_min")
@@ -96,6 +105,8 @@ SIGNATURES = [
     ("pach_convergence_state_size", C.c_int64, [C.c_int, C.c_int, C.c_int]),
     ("pach_convergence_step", C.c_int, [C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_int, C.c_double, _dp,
                                         C.POINTER(C.c_int)]),
+    ("pach_image_sources", C.c_int, [C.c_void_p, _ip, C.c_int, C.c_int, C.c_void_p, C.c_int64,
+                                     C.POINTER(C.c_int64)]),
     ("pach_autotune", C.c_int, [C.c_void_p, C.POINTER(Params), C.c_int64, C.POINTER(Tuning)]),
     ("pach_trace", C.c_int, [C.c_void_p, C.POINTER(Params)]),
     ("pach_poll", C.c_int, [C.c_void_p, C.POINTER(Progress)]),
@@ -364,6 +375,28 @@ class Tracer:
     def cancel(self):
         self._chk(lib().pach_cancel(self.h))
 
+    def image_sources(self, max_order, object_of_surface=None, n_objects=0):
+        """Valid image-source paths (pach_image_sources) of the uploaded scene,
+        source and receiver centres: an IS_PATH array sorted by receiver, then
+        in the reference's depth-first sequence order."""
+        if object_of_surface is not None:
+            obj = np.ascontiguousarray(object_of_surface, dtype=np.int32)
+            optr = obj.ctypes.data_as(_ip)
+            n_objects = int(n_objects) if n_objects else int(obj.max()) + 1
+        else:
+            optr = None
+        n = C.c_int64()
+        cap = 4096
+        while True:
+            out = np.zeros(cap, dtype=IS_PATH)
+            rc = lib().pach_image_sources(self.h, optr, n_objects, int(max_order), out.ctypes.data, cap,
+                                          C.byref(n))
+            if rc == PACH_ERANGE:
+                cap = int(n.value)
+                continue
+            self._chk(rc)
+            return out[:n.value].copy()
+
     def histogram(self, plane=0, scaled=True):
         out = np.zeros((self.R, 8, self.sample_ct))
         self._chk(lib().pach_histogram(self.h, plane, _p(out), int(scaled)))
@@ -394,6 +427,16 @@ def trace_room(tr: Tracer, room, *, rays, cutoff_s, seed=1, mode=MODE_STOCHASTIC
     tr.set_receivers(room.receivers, room.radius, at, c, c * rho, sample_rate, sct)
     tr.trace(rays=rays, cutoff_m=cutoff_s * c, seed=seed, mode=mode, precision=precision, **kw)
     return tr.stats()
+
+
+def image_sources_room(tr: Tracer, room, max_order, *, medium=None, power_w=None, grid=(0, 0, 0),
+                       object_of_surface=None, n_objects=0):
+    """Upload a rooms.Room and return its valid image-source paths (IS_PATH array)."""
+    at, c, rho = medium if medium is not None else uniform_medium()
+    tr.upload_scene(room.verts, room.surf, room.alpha, room.scat, room.trans, grid)
+    tr.set_source(room.source, np.ones(8) if power_w is None else power_w, DIR_GEODESIC)
+    tr.set_receivers(room.receivers, room.radius, at, c, c * rho, 1000, 1000)
+    return tr.image_sources(max_order, object_of_surface, n_objects)
 
 
 def trace_batch(tr: Tracer, packed, rays, *, cutoff_s, seed=1, mode=MODE_STOCHASTIC,

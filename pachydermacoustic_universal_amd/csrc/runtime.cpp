@@ -7,6 +7,7 @@
 // every ray of every batch has terminated.
 #include "../../include/pach.h"
 #include "grid.h"
+#include "image_source.h"
 #include "trace_types.h"
 
 #include <algorithm>
@@ -69,6 +70,7 @@ struct pach_ctx {
     int ncell = 0;
     pach::Grid grid;        // host copy (receiver voxels are marked at trace time)
     std::vector<double> verts;  // host copy of the triangles (grid rebuilds)
+    std::vector<int> surf;      // host copy of surf_of_tri
     bool rmask_dirty = true;
     bool rmask_on = false;
     std::vector<double> centers;
@@ -450,6 +452,7 @@ extern "C" int pach_scene_upload(pach_ctx *c, const double *verts, int T, const 
         d2[4 * t + 3] = w2;
     }
     c->verts.assign(verts, verts + 9 * (size_t)T);
+    c->surf.assign(surf, surf + T);
     c->tri_rows32.resize(12 * (size_t)T);
     for (size_t t = 0; t < (size_t)T; ++t)
         for (int k = 0; k < 4; ++k) {
@@ -852,6 +855,29 @@ void conv_receivers(const pach_ctx *c, int &vis, int &occ)
     }
 }
 
+// Device view of the uploaded scene (no receiver masks, distances read from HBM).
+template <class R> DevScene<R> dev_scene(pach_ctx *c)
+{
+    SceneBuf<R> &sb = scene_of<R>(c);
+    DevScene<R> s{};
+    s.v0 = sb.v0; s.e1 = sb.e1; s.e2 = sb.e2; s.nrm = sb.nrm; s.surf = c->d_surf;
+    s.cell_range = (const uint2 *)c->d_cell_range; s.cell_tris = c->d_cell_tris;
+    s.cell_rows = c->d_cell_rows;
+    s.cell_dist = c->d_cell_dist;
+    s.cell_rmask = nullptr;
+    s.gx = c->g[0]; s.gy = c->g[1]; s.gz = c->g[2];
+    s.ncell = c->ncell;
+    s.dist_lds = 0;
+    for (int k = 0; k < 3; ++k) {
+        s.gmin[k] = (R)c->gmin[k];
+        s.cs[k] = (R)c->cs[k];
+        s.inv_cs[k] = (R)(1.0 / c->cs[k]);
+    }
+    s.mat = sb.mat;
+    s.trans = sb.trans;
+    return s;
+}
+
 template <class R> int run_trace(pach_ctx *c, const pach_params p)
 {
     (void)hipSetDevice(c->device);
@@ -897,24 +923,10 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
     tc.geo = c->d_geo;
     tc.has_trans = c->any_trans ? 1 : 0;
 
-    SceneBuf<R> &sb = scene_of<R>(c);
-    DevScene<R> s{};
-    s.v0 = sb.v0; s.e1 = sb.e1; s.e2 = sb.e2; s.nrm = sb.nrm; s.surf = c->d_surf;
-    s.cell_range = (const uint2 *)c->d_cell_range; s.cell_tris = c->d_cell_tris;
-    s.cell_rows = c->d_cell_rows;
-    s.cell_dist = c->d_cell_dist;
+    DevScene<R> s = dev_scene<R>(c);
     s.cell_rmask = c->rmask_on ? c->d_cell_rmask : nullptr;
-    s.gx = c->g[0]; s.gy = c->g[1]; s.gz = c->g[2];
-    s.ncell = c->ncell;
     const int lds_bytes = (c->ncell + 15) & ~15;
     s.dist_lds = (lds_bytes <= kMaxLdsDist && c->tune_dist_lds) ? 1 : 0;
-    for (int k = 0; k < 3; ++k) {
-        s.gmin[k] = (R)c->gmin[k];
-        s.cs[k] = (R)c->cs[k];
-        s.inv_cs[k] = (R)(1.0 / c->cs[k]);
-    }
-    s.mat = sb.mat;
-    s.trans = sb.trans;
 
     DevRecv<R> rv{};
     rv.ctr = ctr_of<R>(c);
@@ -1213,6 +1225,146 @@ extern "C" int pach_trace(pach_ctx *c, const pach_params *p)
     if (!p->async) return body();
     c->worker = std::thread(body);
     return PACH_OK;
+}
+
+// ------------------------------------------------------------------ image sources
+namespace {
+
+// Reference order of the valid paths: by receiver, then the depth-first order
+// of TraverseOrders (a sequence precedes its extensions).
+bool is_path_before(const pach_is_path &a, const pach_is_path &b)
+{
+    if (a.receiver != b.receiver) return a.receiver < b.receiver;
+    const int n = std::min(a.order, b.order);
+    for (int q = 0; q < n; ++q)
+        if (a.seq[q] != b.seq[q]) return a.seq[q] < b.seq[q];
+    return a.order < b.order;
+}
+
+}  // namespace
+
+extern "C" int pach_image_sources(pach_ctx *c, const int *object_of_surface, int n_objects, int max_order,
+                                  pach_is_path *out, int64_t cap, int64_t *n_paths)
+{
+    if (!c || !n_paths || cap < 0 || (cap > 0 && !out)) return PACH_EINVAL;
+    if (c->running) return fail(c, PACH_ESTATE, "trace running");
+    if (c->worker.joinable()) c->worker.join();
+    if (!c->has_scene || !c->has_src || !c->has_recv)
+        return fail(c, PACH_ESTATE, "scene, source and receivers must be set before image sources");
+    if (max_order < 1 || max_order > PACH_IS_MAX_ORDER)
+        return fail(c, PACH_EINVAL, "max_order must lie in [1, PACH_IS_MAX_ORDER]");
+    const int S = c->S, T = c->T;
+    std::vector<int> obj(S);
+    if (!object_of_surface) {
+        if (n_objects != 0 && n_objects != S) return fail(c, PACH_EINVAL, "n_objects must be 0 or S without a map");
+        n_objects = S;
+        for (int p = 0; p < S; ++p) obj[p] = p;
+    } else {
+        if (n_objects < 1) return fail(c, PACH_EINVAL, "n_objects must be positive");
+        for (int p = 0; p < S; ++p) {
+            if (object_of_surface[p] < 0 || object_of_surface[p] >= n_objects)
+                return fail(c, PACH_EINVAL, "object_of_surface out of range");
+            obj[p] = object_of_surface[p];
+        }
+    }
+    // threads per order: n_objects^order x R, kept below 2^62
+    {
+        double tot = c->R;
+        for (int o = 0; o < max_order; ++o) tot *= n_objects;
+        if (tot > 4.0e18) return fail(c, PACH_EINVAL, "too many reflection sequences (n_objects^max_order x R)");
+    }
+    // each object's plane: its lowest polygon (ObjectMembers[o][0]), that
+    // polygon's first triangle (unit normal as uploaded, vertex 0)
+    std::vector<int> first_poly(n_objects, -1), first_tri(n_objects, -1);
+    for (int p = 0; p < S; ++p)
+        if (first_poly[obj[p]] < 0) first_poly[obj[p]] = p;
+    for (int t = 0; t < T; ++t) {
+        const int o = obj[c->surf[t]];
+        if (first_poly[o] == c->surf[t] && first_tri[o] < 0) first_tri[o] = t;
+    }
+    std::vector<IsPlane> planes(n_objects);
+    for (int o = 0; o < n_objects; ++o) {
+        IsPlane &pl = planes[o];
+        std::memset(&pl, 0, sizeof(pl));
+        const int t = first_tri[o];
+        if (t < 0) continue;  // an object without triangles is never hit
+        const double *v = &c->verts[9 * (size_t)t];
+        const double a[3] = {v[3] - v[0], v[4] - v[1], v[5] - v[2]};
+        const double b[3] = {v[6] - v[0], v[7] - v[1], v[8] - v[2]};
+        const double n[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+        const double l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        for (int k = 0; k < 3; ++k) {
+            pl.n[k] = l > 0 ? n[k] / l : 0.0;
+            pl.p[k] = v[k];
+        }
+    }
+    (void)hipSetDevice(c->device);
+    constexpr unsigned long long kChunk = 1ull << 16;  // device result records per launch
+    int *d_obj = nullptr;
+    IsPlane *d_plane = nullptr;
+    pach_is_path *d_out = nullptr;
+    unsigned long long *d_count = nullptr;
+    std::vector<pach_is_path> found;
+    auto body = [&]() -> int {
+        HIPCHK(c, dalloc(d_obj, S));
+        HIPCHK(c, dalloc(d_plane, n_objects));
+        HIPCHK(c, dalloc(d_out, kChunk));
+        HIPCHK(c, dalloc(d_count, 1));
+        HIPCHK(c, hipMemcpy(d_obj, obj.data(), S * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(d_plane, planes.data(), n_objects * sizeof(IsPlane), hipMemcpyHostToDevice));
+        IsJob j{};
+        j.obj = d_obj;
+        j.plane = d_plane;
+        j.rec = c->d_ctr64;
+        j.n_obj = n_objects;
+        j.R = c->R;
+        for (int k = 0; k < 3; ++k) j.src[k] = c->origin[k];
+        for (int b = 0; b < 8; ++b) {
+            j.power[b] = c->power[b];
+            j.atten[b] = c->atten[b];
+        }
+        j.c = c->c;
+        j.rho_c = c->rho_c;
+        j.out = d_out;
+        j.count = d_count;
+        j.cap = kChunk;
+        const DevScene<double> s = dev_scene<double>(c);
+        const int grid = std::max(1, c->n_cu) * 8;
+        hipStream_t st = c->stream;
+        for (int order = 1; order <= max_order; ++order) {
+            j.order = order;
+            int64_t total = c->R;
+            for (int o = 0; o < order; ++o) total *= n_objects;
+            int64_t span = std::min<int64_t>(total, (int64_t)1 << 26);
+            for (int64_t lo = 0; lo < total;) {
+                const int64_t hi = std::min(total, lo + span);
+                unsigned long long n = 0;
+                HIPCHK(c, hipMemsetAsync(d_count, 0, sizeof(unsigned long long), st));
+                HIPCHK(c, launch_image_sources(st, grid, s, j, lo, hi));
+                HIPCHK(c, hipMemcpyAsync(&n, d_count, sizeof(n), hipMemcpyDeviceToHost, st));
+                HIPCHK(c, hipStreamSynchronize(st));
+                if (n > kChunk) {
+                    // more valid paths than the buffer holds: redo the range in smaller pieces
+                    span = std::max<int64_t>(1, (hi - lo) * (int64_t)kChunk / (int64_t)(2 * n));
+                    continue;
+                }
+                const size_t old = found.size();
+                found.resize(old + n);
+                if (n) HIPCHK(c, hipMemcpy(found.data() + old, d_out, n * sizeof(pach_is_path), hipMemcpyDeviceToHost));
+                lo = hi;
+                if (n < kChunk / 4) span = std::min<int64_t>(span * 2, (int64_t)1 << 26);
+            }
+        }
+        return PACH_OK;
+    };
+    const int rc = body();
+    dfree(d_obj); dfree(d_plane); dfree(d_out); dfree(d_count);
+    if (rc) return rc;
+    std::stable_sort(found.begin(), found.end(), is_path_before);
+    *n_paths = (int64_t)found.size();
+    const size_t w = std::min<size_t>(found.size(), (size_t)cap);
+    if (w) std::memcpy(out, found.data(), w * sizeof(pach_is_path));
+    return (int64_t)found.size() > cap ? fail(c, PACH_ERANGE, "image sources: output buffer too small") : PACH_OK;
 }
 
 extern "C" int pach_autotune(pach_ctx *c, const pach_params *p, int64_t sample_rays, pach_tuning *out)

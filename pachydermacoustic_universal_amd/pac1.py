@@ -16,15 +16,20 @@ byte length followed by UTF-8; bool 1 byte, int 4, float 4, double 8):
         double delay (ms); per receiver: int window length n, bool validity,
         double arrival time (s); per window sample: double energy[8], then for
         each band and axis float positive, float negative direction sums
-      Receiver_Bank.Write_Data (Classes_Receivers.cs:220-338):
+      ImageSourceData.Write_Data (Classes_ImageSource.cs:1443-1583), when given:
+        string "Image-Source_Data"; per receiver: int receiver, int path count;
+        per Specular_Path: short 0, int point count, points (double x, y, z),
+        double energy[8], bool special filter (the reference writes true and
+        16,384 complex ones: int length, then double real, imaginary per
+        entry), double prms[8], double travel time, int sequence[points - 2]
         string "Ray-Traced_Data", int receiver type (0 stationary), int sample
         rate, int sample count, double cut-off time (ms); per receiver: double
         speed of sound; per sample and band: double energy, double pressure,
         double positive x, y, z, double negative x, y, z
     string "End"
 
-Image-source data (IS_Data) is not produced by this package and is neither
-written nor read.
+Compound paths (edge diffraction, curved surfaces) are not produced by this
+package and are neither written nor read.
 """
 from __future__ import annotations
 
@@ -66,6 +71,9 @@ class _W:
     def f32(self, v):
         self.f.write(struct.pack("<f", float(v)))
 
+    def i16(self, v):
+        self.f.write(struct.pack("<h", int(v)))
+
     def boolean(self, v):
         self.f.write(b"\x01" if v else b"\x00")
 
@@ -104,6 +112,9 @@ class _R:
     def f32(self):
         return struct.unpack("<f", self._take(4))[0]
 
+    def i16(self):
+        return struct.unpack("<h", self._take(2))[0]
+
     def boolean(self):
         return self._take(1)[0] != 0
 
@@ -127,10 +138,40 @@ def _direction_sums(direct, rec):
     return pos, neg
 
 
-def write_pac1(filename, direct, receiver, version=VERSION, date=None):
-    """Write one source's Direct_Sound (pachyderm.Direct_Sound, after Begin) and
+_FILTER_LEN = 16384
+
+
+def _write_image_sources(w, isd, special_filter):
+    """ImageSourceData.Write_Data (Classes_ImageSource.cs:1443-1583), specular paths."""
+    w.string("Image-Source_Data")
+    ones = np.zeros(2 * _FILTER_LEN)
+    ones[0::2] = 1.0
+    for q, paths in enumerate(isd.Paths):
+        w.i32(q)
+        w.i32(len(paths))
+        for p in paths:
+            w.i16(0)
+            pts = p.Path[0]
+            w.i32(len(pts))
+            w.f64s(pts)
+            w.f64s(p.PathEnergy)
+            w.boolean(special_filter)
+            if special_filter:
+                w.i32(_FILTER_LEN)
+                w.f64s(ones)
+            w.f64s(p.prms)
+            w.f64(p.TravelTime)
+            for d in p.Reflection_Sequence:
+                w.i32(d)
+
+
+def write_pac1(filename, direct, receiver, version=VERSION, date=None, image_sources=None, special_filter=True):
+    """Write one source's Direct_Sound (pachyderm.Direct_Sound, after Begin),
+    ImageSourceData (pachyderm.ImageSourceData after Begin; optional) and
     ray-traced Receiver_Bank (pachyderm.Receiver_Bank after Combine, scaled).
-    Either may be None, but not both."""
+    Direct and receiver may not both be None.  special_filter=False writes
+    each path's materials filter as absent (a form the reference's reader
+    accepts) instead of the 16,384 unit entries the reference writes."""
     if direct is None and receiver is None:
         raise ValueError("There is no simulated data to save.")
     rb = receiver if receiver is not None else direct.Rec
@@ -172,6 +213,8 @@ def write_pac1(filename, direct, receiver, version=VERSION, date=None):
                         for a in range(3):
                             w.f32(pos[oct, t, a])
                             w.f32(neg[oct, t, a])
+        if image_sources is not None:
+            _write_image_sources(w, image_sources, special_filter)
         if receiver is not None:
             w.string("Ray-Traced_Data")
             w.i32(0)  # Stationary
@@ -215,7 +258,7 @@ def read_pac1(filename):
         rho_c.append(r.f64() if v3 >= 2.0 else 400.0)
     out["receivers"] = np.array(recs)
     out["rho_c"] = np.array(rho_c)
-    out["direct"], out["ray_traced"] = [], []
+    out["direct"], out["ray_traced"], out["image_sources"] = [], [], []
     while True:
         tag = r.string()
         if tag in ("End", ""):
@@ -237,6 +280,28 @@ def read_pac1(filename):
                             rec["neg"][oct, t, a] = r.f32()
                 d["receivers"].append(rec)
             out["direct"].append(d)
+        elif tag == "Image-Source_Data":
+            per_rec = []
+            for _ in range(R):
+                r.i32()
+                paths = []
+                for _ in range(r.i32()):
+                    kind = r.i16()
+                    if kind != 0:
+                        raise ValueError("compound image-source paths are not supported")
+                    n = r.i32()
+                    p = {"pts": r.f64s(3 * n).reshape(n, 3), "energy": r.f64s(8)}
+                    p["filter"] = None
+                    if r.boolean():
+                        m = r.i32()
+                        f = r.f64s(2 * m)
+                        p["filter"] = f[0::2] + 1j * f[1::2]
+                    p["prms"] = r.f64s(8)
+                    p["time"] = r.f64()
+                    p["seq"] = [r.i32() for _ in range(n - 2)]
+                    paths.append(p)
+                per_rec.append(paths)
+            out["image_sources"].append(per_rec)
         elif tag == "Ray-Traced_Data":
             d = {"type": r.i32(), "sample_rate": r.i32()}
             n = r.i32()

@@ -75,7 +75,7 @@ class Polygon_Scene:
         index as Surf_ID."""
         if len(Mat) != len(Model) and len(Scat) != len(Model):
             raise ValueError("The number of material codes must match the number of objects in the model")
-        tris, surf, alpha, scat, trans, areas = [], [], [], [], [], []
+        tris, surf, alpha, scat, trans, areas, objects = [], [], [], [], [], [], []
         pid = 0
         for i, obj in enumerate(Model):
             tr = np.zeros(8) if Trans is None else np.asarray(Trans[i], dtype=np.float64)
@@ -92,11 +92,13 @@ class Polygon_Scene:
                 scat.append(Scat[i].Coefficient())
                 trans.append(tr)
                 areas.append(area)
+                objects.append(i)
                 pid += 1
         self._tris = np.asarray(tris)
         self._surf = np.asarray(surf, dtype=np.int32)
         self._alpha, self._scat, self._trans = map(np.asarray, (alpha, scat, trans))
         self._area = np.asarray(areas)
+        self._object = np.asarray(objects, dtype=np.int32)  # Object_ID (:498-502)
         self.Valid = True
 
     @classmethod
@@ -107,6 +109,7 @@ class Polygon_Scene:
         s._surf = np.asarray(room.surf, dtype=np.int32)
         s._alpha, s._scat, s._trans = (np.asarray(x, dtype=np.float64) for x in (room.alpha, room.scat, room.trans))
         s._area = np.zeros(room.n_surfaces)
+        s._object = np.arange(room.n_surfaces, dtype=np.int32)
         for t, p in zip(s._tris, s._surf):
             s._area[p] += 0.5 * np.linalg.norm(np.cross(t[1] - t[0], t[2] - t[0]))
         s.Valid = True
@@ -119,6 +122,14 @@ class Polygon_Scene:
 
     def Count(self):
         return int(self._alpha.shape[0])
+
+    @property
+    def ObjectCount(self):
+        """Objects (Model[i] of Construct), not polygons (:967-973)."""
+        return int(self._object.max()) + 1
+
+    def ObjectID(self, i):
+        return int(self._object[i])
 
     def SurfaceArea(self, x):
         return float(self._area[x])
@@ -430,6 +441,122 @@ class Direct_Sound(Simulation_Type):
         return float(self.Io[Rec_ID, :, t].sum()) if t < self.Io.shape[2] else 0.0
 
 
+class Specular_Path:
+    """One valid specular reflection path (Classes_ImageSource.cs:1891-2100).
+
+    Path[0]: source, reflection points, receiver; TravelTime = length / c;
+    Energy(Octave) = [PathEnergy[Octave]] (Octave 8: the band sum);
+    prms = sqrt(PathEnergy rho c)."""
+
+    def __init__(self, rec, order, seq, polys, pts, length, time_s, energy, rho_c, src_id=0, direct_time=0.0):
+        self.Receiver = int(rec)
+        self.Sequence = [int(x) for x in seq[:order]]
+        self.Polygons = [int(x) for x in polys[:order]]
+        self.ValidPath = np.array(pts[: order + 2], dtype=np.float64)
+        self.Length = float(length)
+        self.Time = float(time_s)
+        self.PathEnergy = np.array(energy, dtype=np.float64)
+        self.prms = np.sqrt(self.PathEnergy * rho_c)
+        self.Identifier = self._identify(src_id, direct_time)
+
+    def _identify(self, src_id, direct_time):
+        # Identify (:1999-2034)
+        return (f"S{src_id:03d}-Order {len(self.Sequence):03d}: {round((self.Time - direct_time) * 1000)} ms. "
+                + "".join(f"{d} " for d in self.Sequence))
+
+    def __str__(self):
+        return self.Identifier
+
+    @property
+    def Path(self):
+        return [self.ValidPath]
+
+    @property
+    def TravelTime(self):
+        return self.Time
+
+    @property
+    def Reflection_Sequence(self):
+        return list(self.Sequence)
+
+    def Energy(self, Octave, SampleFrequency=44100):
+        if Octave < 8:
+            return np.array([self.PathEnergy[Octave]])
+        return np.array([self.PathEnergy.sum()])
+
+    def Set_Power(self, factor):
+        self.PathEnergy = self.PathEnergy * np.asarray(factor, dtype=np.float64)
+
+
+class ImageSourceData(Simulation_Type):
+    """Image-source method (Classes_ImageSource.cs:30-1836) for planar objects
+    without edge diffraction, validated on the GPU (pach_image_sources).
+
+    Begin enumerates every sequence of 1..MaxOrder objects (repeats allowed,
+    TraverseOrders :266-348) for every receiver centre; Paths[rec] holds the
+    valid Specular_Paths in the reference's depth-first order.  Edge
+    diffraction and curved objects (Compound_Path) are not provided."""
+
+    def __init__(self, Source, Receiver, Direct, Rm, MaxOrder_in, SourceID_in=0, Edge_Diffraction=False,
+                 *, device=0, tracer=None):
+        if Edge_Diffraction:
+            raise NotImplementedError("edge diffraction (Compound_Path) is not provided")
+        self.Src, self.Rec, self.Direct, self.Room = Source, Receiver, Direct, Rm
+        self.MaxOrder = int(MaxOrder_in)
+        self.SrcNo = int(SourceID_in)
+        self.Diffraction = False
+        self._tr = tracer
+        self._device = device
+        self.ValidPaths = [[] for _ in range(len(Receiver.Origins))]
+        self._done = False
+        self.raw = None
+
+    def Sim_Type(self):
+        return "Image Source"
+
+    def Begin(self):
+        r, s, rb = self.Room, self.Src, self.Rec
+        tr = self._tr or A.Tracer(self._device)
+        try:
+            tr.upload_scene(r._tris, r._surf, r._alpha, r._scat, r._trans, r.grid)
+            tr.set_source(s.Origin, s.SoundPower, s.dir_mode, s.dirs)
+            tr.set_receivers(rb.Origins, rb.radius, r.atten, r.c, r.rho * r.c, rb.SampleRate, max(rb.SampleCT, 1))
+            P = tr.image_sources(self.MaxOrder, r._object, r.ObjectCount)
+        finally:
+            if self._tr is None:
+                tr.close()
+        self.raw = P
+        rho_c = r.rho * r.c
+        self.ValidPaths = [[] for _ in range(len(rb.Origins))]
+        for p in P:
+            dt = self.Direct.Min_Time(int(p["receiver"])) if self.Direct is not None else 0.0
+            self.ValidPaths[int(p["receiver"])].append(
+                Specular_Path(p["receiver"], p["order"], p["seq"], p["poly"], p["pts"], p["length"], p["time_s"],
+                              p["energy"], rho_c, self.SrcNo, dt))
+        self._done = True
+
+    def ThreadState(self):
+        return Stopped
+
+    def ProgressMsg(self):
+        return "Calculating Image Set: complete" if self._done else "Calculating Image Set..."
+
+    def Combine_ThreadLocal_Results(self):
+        pass
+
+    @property
+    def Paths(self):
+        return self.ValidPaths
+
+    def Count(self, rec_id):
+        return len(self.ValidPaths[rec_id])
+
+    def Set_Power(self, Factor):
+        for paths in self.ValidPaths:
+            for p in paths:
+                p.Set_Power(Factor)
+
+
 class _RoomView:
     """The fields of rooms.Room that pach_direct_sound reads."""
 
@@ -445,9 +572,10 @@ class IR_Construction:
                 Start_at_Zero=False):
         """Energy-time curve of one receiver and band (Octave 8: all bands) from
         the direct sound, image sources and the ray tracer's scaled histograms
-        (Classes_Math.cs:1417-1474).  Image-source paths are not provided here
-        (ISData entries must be None).  Where the reference would index past
-        the end of the histogram, the direct sound's bins are dropped."""
+        (Classes_Math.cs:1417-1474).  Each image-source path adds its energy at
+        ceil(fs (TravelTime + delay)) when that bin lies inside (0, n - 1).
+        Where the reference would index past the end of the histogram, the
+        direct sound's bins are dropped."""
         Direct = list(Direct) if Direct is not None else [None]
         ISData = list(ISData) if ISData is not None else [None]
         delay = Direct[Src_ID].Delay_ms if Direct[0] is not None else 0.0
@@ -464,7 +592,13 @@ class IR_Construction:
                 if start + i < H.size:
                     H[start + i] += v
         if ISData[Src_ID] is not None:
-            raise NotImplementedError("image-source paths are not provided")
+            for value in ISData[Src_ID].Paths[Rec_ID]:
+                place = int(np.ceil(Sampling_Frequency * (value.TravelTime + delay * 0.001)))
+                if 0 < place < H.size - 1:
+                    e = value.Energy(Octave, Sampling_Frequency)
+                    for t in range(e.size):
+                        if place + t < H.size - 1:
+                            H[place + t] += e[t]
         return H
 
 

@@ -82,3 +82,52 @@ def test_long_strings_use_two_length_bytes(tmp_path):
     b = f.read_bytes()
     assert b[0] == 0xC8 and b[1] == 0x01 and b[2:202] == date.encode()
     assert pac1.read_pac1(f)["date"] == date
+
+
+class _IS:
+    """ImageSourceData stand-in holding Specular_Paths (Paths[rec])."""
+
+    def __init__(self, paths):
+        self.Paths = paths
+
+
+def image_source_data(rb):
+    from oracle import pyoracle as O
+    room = rooms.shoebox()
+    med = O.standard_medium()
+    raw = O.Scene(room, grid=(4, 4, 4)).image_sources(2, room.source, np.ones(8), room.receivers, med)
+    paths = [[] for _ in range(len(rb.Origins))]
+    for p in raw:
+        paths[p["rec"]].append(P.Specular_Path(p["rec"], p["order"], p["seq"], p["poly"], p["pts"], p["length"],
+                                               p["time"], p["energy"], med.rho_c))
+    return _IS(paths), raw
+
+
+def test_image_source_section_round_trip(tmp_path):
+    rb, ds, src = bank_and_direct(cut_ms=5.0)
+    isd, raw = image_source_data(rb)
+    for special in (True, False):
+        f = tmp_path / f"is{int(special)}.pac1"
+        pac1.write_pac1(f, ds, rb, date="d", image_sources=isd, special_filter=special)
+        d = pac1.read_pac1(f)
+        got = d["image_sources"][0]
+        assert [len(x) for x in got] == [len(x) for x in isd.Paths]
+        for q, paths in enumerate(isd.Paths):
+            for g, p in zip(got[q], paths):
+                np.testing.assert_array_equal(g["pts"], p.Path[0])
+                np.testing.assert_array_equal(g["energy"], p.PathEnergy)
+                np.testing.assert_array_equal(g["prms"], p.prms)
+                assert g["time"] == p.TravelTime and g["seq"] == p.Reflection_Sequence
+                if special:
+                    assert g["filter"].shape == (16384,) and np.all(g["filter"] == 1)
+                else:
+                    assert g["filter"] is None
+        # the section's size from Write_Data's field list
+        n_paths = sum(len(x) for x in isd.Paths)
+        pts = sum(len(p.Path[0]) for x in isd.Paths for p in x)
+        seq = sum(len(p.Reflection_Sequence) for x in isd.Paths for p in x)
+        size = (1 + len("Image-Source_Data")) + 8 * len(isd.Paths) + n_paths * (2 + 4 + 64 + 1 + 64 + 8) \
+            + 24 * pts + 4 * seq + (n_paths * (4 + 16 * 16384) if special else 0)
+        f0 = tmp_path / "none.pac1"
+        pac1.write_pac1(f0, ds, rb, date="d")
+        assert f.stat().st_size - f0.stat().st_size == size
