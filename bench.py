@@ -27,6 +27,7 @@ BYTES_OCT_BOUNCE = 128      # read ray 40 B + write ray 40 B + hit triangle 48 B
 BYTES_BROAD_BOUNCE = 184    # broadband ray 68 B x 2 + hit triangle 48 B
 BYTES_SPAWN = 40            # each spawned octave ray written once
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
 
 
 def cpu_model():
@@ -82,7 +83,10 @@ def main():
                     help="pach_tune setting, e.g. dist_lds=0")
     ap.add_argument("--autotune", type=int, default=0, metavar="RAYS",
                     help="run pach_autotune with RAYS sample source rays before the warm-up")
-    ap.add_argument("--workload", choices=["hall", "batch"], default="hall",
+    ap.add_argument("--is-order", type=int, default=3, help="image_sources workload: reflection order")
+    ap.add_argument("--is-cpu-objects", type=int, default=16,
+                    help="image_sources workload: first objects of the CPU sample's sequences")
+    ap.add_argument("--workload", choices=["hall", "batch", "image_sources"], default="hall",
                     help="hall: the 2k-triangle hall; batch: 256 packed rooms in one launch sequence")
     ap.add_argument("--batch-ray-scale", type=float, default=0.01,
                     help="batch workload: scale of each room's log-uniform [1e4, 1e6] ray count")
@@ -108,6 +112,8 @@ def main():
     from pachydermacoustic_universal_amd import _abi as A
     from pachydermacoustic_universal_amd import rooms
 
+    if args.workload == "image_sources":
+        return bench_image_sources(args, A, rooms)
     at, c, rho = A.uniform_medium()
     prec = A.PREC_FP64 if args.precision == "fp64" else A.PREC_FP32
     tr = A.Tracer(0)
@@ -221,6 +227,73 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_note": traffic_note,
                      "kernel": "k_octave", "avg_launch_ms": tot["ms_oct"] / max(tot["n_oct"], 1),
                      "launches": tot["n_oct"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    tr.close()
+
+
+def bench_image_sources(args, A, rooms):
+    """Image-source path checks per second: every sequence of 1..is_order
+    polygons of the hall (1,016 polygons, each its own object) for each of
+    its 16 receivers, validated by k_image_sources (FP64)."""
+    at, c, rho = A.uniform_medium()
+    room = rooms.fan_hall()
+    grid = (args.grid or 32,) * 3
+    tr = A.Tracer(0)
+    tr.upload_scene(room.verts, room.surf, room.alpha, room.scat, room.trans, grid)
+    tr.set_source(room.source, np.ones(8), A.DIR_GEODESIC)
+    tr.set_receivers(room.receivers, room.radius, at, c, c * rho, 1000, 1000)
+    S, R, n = room.n_surfaces, int(room.receivers.shape[0]), args.is_order
+    for w in range(args.warmup):
+        tr.image_sources(n)
+        print(f"[bench] warmup {w} done", file=sys.stderr, flush=True)
+    tot = dict(checks=0, ms=0.0, launches=0, paths=0)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        P = tr.image_sources(n)
+        st = tr.image_source_stats()
+        for key in ("checks", "launches", "paths"):
+            tot[key] += st[key]
+        tot["ms"] += st["ms_kernel"]
+        print(f"[bench] step {k}: {st['checks']:.3e} checks, {len(P)} paths, kernel {st['ms_kernel']:.1f} ms",
+              file=sys.stderr, flush=True)
+    dt = time.perf_counter() - t0
+    # The tables a check reads (planes, receivers, one object's triangles) stay
+    # in cache, so the kernel is bound by FP64 issue, not HBM.  Algorithmic
+    # FP64 flops per check of a k-reflection sequence: k mirror images (17
+    # each), the ray direction (3) and the division-free triangle test (52)
+    # on each of the last object's triangles.
+    tris_per_obj = room.n_tris / S
+    per_step = sum(R * S ** k * (17 * k + 3 + 52 * tris_per_obj) for k in range(1, n + 1))
+    achieved = per_step * args.steps / (tot["ms"] / 1e3) / 1e12
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as O
+
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        med = O.Medium(c=c, atten=at, rho=rho)
+        q1 = min(args.is_cpu_objects, S)
+        sc = O.Scene(room, grid=grid)
+        t1 = time.perf_counter()
+        found = sc.image_source_count(n, room.source, np.ones(8), room.receivers, med, 0, q1, threads)
+        cdt = time.perf_counter() - t1
+        checks = q1 * R * sum(S ** k for k in range(0, n))
+        cpu = {"value": checks / cdt, "unit": "path checks/s", "cores": threads, "kind": "port",
+               "sample": f"sequences starting with polygons 0..{q1 - 1} ({checks:.3e} checks, {found} paths, "
+                         f"{cdt:.1f} s), C FP64 restatement, {threads} threads on {cpu_model()} "
+                         f"(nproc={os.cpu_count()})"}
+    out = {
+        "metric": "image-source path checks/s", "value": tot["checks"] / dt, "unit": "path checks/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"image sources of the seeded polyhedral hall: every sequence of 1..{n} of its "
+                               f"{S} polygons x {R} receivers, {grid[0]}^3 grid",
+                   "checks_per_step": tot["checks"] / args.steps, "paths_per_step": tot["paths"] / args.steps},
+        "roofline": {"bound": "fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "k_is_screen + k_image_sources",
+                     "avg_launch_ms": tot["ms"] / max(tot["launches"], 1), "launches": tot["launches"]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -243,6 +243,17 @@ typedef struct pach_is_path {
 int pach_image_sources(pach_ctx *ctx, const int *object_of_surface, int n_objects, int max_order,
                        pach_is_path *out, int64_t cap, int64_t *n_paths);
 
+/* Work and timing of the last pach_image_sources call (no counterpart in the
+ * reference). */
+typedef struct pach_is_stats {
+    int64_t checks;     /* (sequence, receiver) pairs validated */
+    int64_t paths;      /* valid paths found */
+    int64_t launches;   /* kernel launches */
+    double ms_kernel;   /* summed HIP-event durations of the validation kernel */
+    double ms_total;    /* wall time of the call (host clock) */
+} pach_is_stats;
+int pach_image_sources_stats(pach_ctx *ctx, pach_is_stats *out);
+
 /* One revolution of a convergence check on host arrays, the function the
  * convergence mode applies to each checked receiver (TaskAll:870-956 for
  * PACH_CONV_SPECMIN, :1063-1112 for PACH_CONV_MINIMUM).  energy, spec, diff,

@@ -1261,9 +1261,67 @@ static void is_traverse(is_run *r, int *seq, int order, double (*img)[3])
     }
 }
 
+static int64_t is_range(const or_scene *s, const int *obj_of_surf, int n_obj, int max_order, const double src[3],
+                        const double power[8], const double *recs, int R, const double atten[8], double c,
+                        double rho_c, or_is_path *out, int64_t cap, int q0_begin, int q0_end);
+
 int64_t or_image_sources(const or_scene *s, const int *obj_of_surf, int n_obj, int max_order,
                          const double src[3], const double power[8], const double *recs, int R,
                          const double atten[8], double c, double rho_c, or_is_path *out, int64_t cap)
+{
+    return is_range(s, obj_of_surf, n_obj, max_order, src, power, recs, R, atten, c, rho_c, out, cap, 0, n_obj);
+}
+
+typedef struct {
+    const or_scene *s;
+    const int *obj;
+    int n_obj, max_order, R, q0_begin, q0_end;
+    const double *src, *power, *recs, *atten;
+    double c, rho_c;
+    int64_t n;
+} is_mt_arg;
+
+static void *is_mt_worker(void *p)
+{
+    is_mt_arg *a = (is_mt_arg *)p;
+    a->n = is_range(a->s, a->obj, a->n_obj, a->max_order, a->src, a->power, a->recs, a->R, a->atten, a->c,
+                    a->rho_c, NULL, 0, a->q0_begin, a->q0_end);
+    return NULL;
+}
+
+/* Counts the valid paths of the sequences whose first object lies in
+ * [q0_begin, q0_end), split over n_threads as Begin splits the objects over
+ * its threads (:117-150).  The CPU timing sample. */
+int64_t or_image_sources_count_mt(const or_scene *s, const int *obj_of_surf, int n_obj, int max_order,
+                                  const double src[3], const double power[8], const double *recs, int R,
+                                  const double atten[8], double c, double rho_c, int q0_begin, int q0_end,
+                                  int n_threads)
+{
+    if (n_threads < 1) n_threads = 1;
+    if (q0_begin < 0 || q0_end > n_obj || q0_begin >= q0_end) return -1;
+    int span = q0_end - q0_begin;
+    if (n_threads > span) n_threads = span;
+    pthread_t th[256];
+    is_mt_arg args[256];
+    if (n_threads > 256) n_threads = 256;
+    for (int i = 0; i < n_threads; ++i) {
+        is_mt_arg a = {s, obj_of_surf, n_obj, max_order, R, q0_begin + (int)((int64_t)span * i / n_threads),
+                       q0_begin + (int)((int64_t)span * (i + 1) / n_threads), src, power, recs, atten, c, rho_c, 0};
+        args[i] = a;
+        pthread_create(&th[i], NULL, is_mt_worker, &args[i]);
+    }
+    int64_t n = 0;
+    for (int i = 0; i < n_threads; ++i) {
+        pthread_join(th[i], NULL);
+        if (args[i].n < 0) return -1;
+        n += args[i].n;
+    }
+    return n;
+}
+
+static int64_t is_range(const or_scene *s, const int *obj_of_surf, int n_obj, int max_order, const double src[3],
+                        const double power[8], const double *recs, int R, const double atten[8], double c,
+                        double rho_c, or_is_path *out, int64_t cap, int q0_begin, int q0_end)
 {
     if (max_order < 1 || max_order > OR_IS_MAX_ORDER || n_obj < 1) return -1;
     int *obj = malloc(sizeof(int) * s->S);
@@ -1291,7 +1349,7 @@ int64_t or_image_sources(const or_scene *s, const int *obj_of_surf, int n_obj, i
     int seq[OR_IS_MAX_ORDER];
     double img[OR_IS_MAX_ORDER][3];
     /* Begin / Calculate (:117-170): one depth-first traversal per first object */
-    for (int q0 = 0; q0 < n_obj; ++q0) {
+    for (int q0 = q0_begin; q0 < q0_end; ++q0) {
         seq[0] = q0;
         is_traverse(&r, seq, 0, img);
     }

@@ -159,6 +159,12 @@ typedef struct {
 int64_t or_image_sources(const or_scene *s, const int *obj_of_surf, int n_obj, int max_order,
                          const double src[3], const double power[8], const double *recs, int R,
                          const double atten[8], double c, double rho_c, or_is_path *out, int64_t cap);
+/* Valid-path count of the sequences starting with an object in
+ * [q0_begin, q0_end), on n_threads threads (the CPU timing sample). */
+int64_t or_image_sources_count_mt(const or_scene *s, const int *obj_of_surf, int n_obj, int max_order,
+                                  const double src[3], const double power[8], const double *recs, int R,
+                                  const double atten[8], double c, double rho_c, int q0_begin, int q0_end,
+                                  int n_threads);
 
 #ifdef __cplusplus
 }

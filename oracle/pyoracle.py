@@ -108,6 +108,9 @@ def lib():
         L.or_image_sources.restype = C.c_int64
         L.or_image_sources.argtypes = [C.c_void_p, _ip, C.c_int, C.c_int, _dp, _dp, _dp, C.c_int, _dp,
                                        C.c_double, C.c_double, C.c_void_p, C.c_int64]
+        L.or_image_sources_count_mt.restype = C.c_int64
+        L.or_image_sources_count_mt.argtypes = [C.c_void_p, _ip, C.c_int, C.c_int, _dp, _dp, _dp, C.c_int, _dp,
+                                                C.c_double, C.c_double, C.c_int, C.c_int, C.c_int]
         _lib = L
     return _lib
 
@@ -218,6 +221,19 @@ class Scene:
         out = np.zeros(n, dtype=IS_PATH)
         lib().or_image_sources(*args, out.ctypes.data, n)
         return out
+
+    def image_source_count(self, max_order, src, power, receivers, medium, q0_begin, q0_end, threads=1):
+        """Valid paths of the sequences whose first object is in [q0_begin, q0_end) (polygons as objects)."""
+        src = np.ascontiguousarray(src, dtype=np.float64)
+        pw = np.ascontiguousarray(power, dtype=np.float64)
+        rec = np.ascontiguousarray(receivers, dtype=np.float64).reshape(-1, 3)
+        at = np.ascontiguousarray(medium.atten, dtype=np.float64)
+        n = lib().or_image_sources_count_mt(self.h, None, self._a.shape[0], int(max_order), _p(src), _p(pw),
+                                            _p(rec), rec.shape[0], _p(at), float(medium.c), float(medium.rho_c),
+                                            int(q0_begin), int(q0_end), int(threads))
+        if n < 0:
+            raise ValueError("image sources: bad arguments")
+        return int(n)
 
     def sabine(self, volume, atten):
         out = np.zeros(8)

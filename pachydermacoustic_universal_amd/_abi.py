@@ -70,6 +70,14 @@ class Tuning(C.Structure):
         return d
 
 
+class IsStats(C.Structure):
+    _fields_ = [("checks", C.c_int64), ("paths", C.c_int64), ("launches", C.c_int64), ("ms_kernel", C.c_double),
+                ("ms_total", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 # pach_is_path (one valid image-source path), as a numpy record
 IS_MAX_ORDER = 8
 IS_PATH = np.dtype([("receiver", "<i4"), ("order", "<i4"), ("seq", "<i4", (IS_MAX_ORDER,)),
@@ -107,6 +115,7 @@ SIGNATURES = [
                                         C.POINTER(C.c_int)]),
     ("pach_image_sources", C.c_int, [C.c_void_p, _ip, C.c_int, C.c_int, C.c_void_p, C.c_int64,
                                      C.POINTER(C.c_int64)]),
+    ("pach_image_sources_stats", C.c_int, [C.c_void_p, C.POINTER(IsStats)]),
     ("pach_autotune", C.c_int, [C.c_void_p, C.POINTER(Params), C.c_int64, C.POINTER(Tuning)]),
     ("pach_trace", C.c_int, [C.c_void_p, C.POINTER(Params)]),
     ("pach_poll", C.c_int, [C.c_void_p, C.POINTER(Progress)]),
@@ -396,6 +405,11 @@ class Tracer:
                 continue
             self._chk(rc)
             return out[:n.value].copy()
+
+    def image_source_stats(self):
+        st = IsStats()
+        self._chk(lib().pach_image_sources_stats(self.h, C.byref(st)))
+        return st.as_dict()
 
     def histogram(self, plane=0, scaled=True):
         out = np.zeros((self.R, 8, self.sample_ct))

@@ -48,6 +48,56 @@ __device__ __forceinline__ int hit(const DevScene<double> &s, const double o[3],
     return tri;
 }
 
+// Can a closest-hit query from o along d end on object px at t >= 1e-6?
+// Only if the same intersection test accepts one of px's triangles there with
+// no distance limit (the walk accepts a subset of those), so a "no" decides
+// the check without walking the grid.
+__device__ bool may_hit_object(const DevScene<double> &s, const IsJob &j, const double o[3], const double d[3],
+                               int px)
+{
+    for (int k = j.obj_tri0[px]; k < j.obj_tri0[px + 1]; ++k) {
+        double tt;
+        if (tri_hit_mt(s, j.obj_tris[k], o[0], o[1], o[2], d[0], d[1], d[2], __builtin_huge_val(), tt) &&
+            !(tt < 0.000001))
+            return true;
+    }
+    return false;
+}
+
+// Conservative form of may_hit_object (entries [k_lo, k_hi) of the object's
+// triangles) for the screening pass, without
+// divisions: d = E - O unnormalised (barycentric coordinates do not depend on
+// its length), the barycentric bounds widened by 1e-7 and near-parallel rays
+// (|det| below 1e-6 of its terms) always kept.  It never rejects a triangle
+// that may_hit_object accepts, whose rounding errors are ~1e-15 relative, so
+// screening changes which items are validated but never a result.
+__device__ bool may_hit_tris_wide(const DevScene<double> &s, const IsJob &j, const double o[3], const double d[3],
+                                  int k_lo, int k_hi)
+{
+    constexpr double kEps = 1e-7;
+    for (int k = k_lo; k < k_hi; ++k) {
+        const int t = j.obj_tris[k];
+        const double4 a = s.v0[t], e1 = s.e1[t], e2 = s.e2[t];
+        const double px_ = d[1] * e2.z - d[2] * e2.y;
+        const double py_ = d[2] * e2.x - d[0] * e2.z;
+        const double pz_ = d[0] * e2.y - d[1] * e2.x;
+        const double t0 = e1.x * px_, t1 = e1.y * py_, t2 = e1.z * pz_;
+        const double det = t0 + t1 + t2;
+        const double ad = fabs(det);
+        if (ad <= 1e-6 * (fabs(t0) + fabs(t1) + fabs(t2))) return true;
+        const double sg = det > 0 ? 1.0 : -1.0;
+        const double sx = o[0] - a.x, sy = o[1] - a.y, sz = o[2] - a.z;
+        const double uu = (sx * px_ + sy * py_ + sz * pz_) * sg;
+        const double qx = sy * e1.z - sz * e1.y;
+        const double qy = sz * e1.x - sx * e1.z;
+        const double qz = sx * e1.y - sy * e1.x;
+        const double vv = (d[0] * qx + d[1] * qy + d[2] * qz) * sg;
+        const double tt = (e2.x * qx + e2.y * qy + e2.z * qz) * sg;
+        if (uu >= -kEps * ad && vv >= -kEps * ad && uu + vv <= (1 + kEps) * ad && tt > 0) return true;
+    }
+    return false;
+}
+
 // OcclusionIntersect: from O towards the image E, through transmissive
 // polygons of other objects, the first surface must belong to object px.
 __device__ bool occl_intersect(const DevScene<double> &s, const IsJob &j, const double O[3], const double E[3],
@@ -56,6 +106,8 @@ __device__ bool occl_intersect(const DevScene<double> &s, const IsJob &j, const 
     double D[3] = {E[0] - O[0], E[1] - O[1], E[2] - O[2]};
     const double L = sqrt(D[0] * D[0] + D[1] * D[1] + D[2] * D[2]);
     D[0] /= L; D[1] /= L; D[2] /= L;
+    // without transmissive polygons the first hit decides: it must lie on px
+    if (!j.any_trans && !may_hit_object(s, j, O, D, px)) return false;
     double o[3] = {O[0], O[1], O[2]}, X[3], t = 0;
     int tri = -1;
     for (int guard = 0; guard < kIsPassLimit; ++guard) {
@@ -114,30 +166,121 @@ __device__ __forceinline__ double seg_len(const double *a, const double *b)
     return sqrt(x * x + y * y + z * z);
 }
 
-__global__ void __launch_bounds__(kIsBlock) k_image_sources(DevScene<double> s, IsJob j, int64_t lo, int64_t hi)
+// Item lo + l of the launch: receiver and sequence digits (l's digits added
+// to those of lo, 32-bit arithmetic).
+// x / d for x < 2^31 and 1 <= d < 2^21 through a double reciprocal
+// (rinv = 1 / d): the estimate is off by at most one, then corrected.
+__device__ __forceinline__ uint32_t div_small(uint32_t x, uint32_t d, double rinv)
 {
-    const int n = j.order;
-    for (int64_t idx = lo + (int64_t)blockIdx.x * kIsBlock + threadIdx.x; idx < hi;
-         idx += (int64_t)gridDim.x * kIsBlock) {
-        const int rec = (int)(idx % j.R);
-        int64_t sq = idx / j.R;
-        int seq[PACH_IS_MAX_ORDER];
-        for (int q = n - 1; q >= 0; --q) {
-            seq[q] = (int)(sq % j.n_obj);
-            sq /= j.n_obj;
+    int64_t q = (int64_t)((double)x * rinv);
+    int64_t r = (int64_t)x - q * (int64_t)d;
+    if (r < 0) { --q; r += d; }
+    if (r >= (int64_t)d) ++q;
+    return (uint32_t)q;
+}
+
+template <int NO>
+__device__ __forceinline__ int decode_item(const IsJob &j, uint32_t l, int seq[NO])
+{
+    const uint32_t R = (uint32_t)j.R, N = (uint32_t)j.n_obj;
+    uint32_t rest = div_small(l, R, j.inv_R);
+    int rec = j.base_rec + (int)(l - rest * R);
+    int carry = 0;
+    if (rec >= j.R) { rec -= j.R; carry = 1; }
+#pragma unroll
+    for (int q = NO - 1; q >= 0; --q) {
+        const uint32_t next = div_small(rest, N, j.inv_N);
+        int v = j.base_seq[q] + (int)(rest - next * N) + carry;
+        rest = next;
+        carry = 0;
+        if (v >= j.n_obj) { v -= j.n_obj; carry = 1; }
+        seq[q] = v;
+    }
+    return rec;
+}
+
+// Polygon_Scene.Image: each image mirrors the previous one in the next plane.
+template <int NO>
+__device__ __forceinline__ void make_images(const IsJob &j, const int seq[NO], double img[NO][3])
+{
+#pragma unroll
+    for (int q = 0; q < NO; ++q) {
+        const double *prev = q > 0 ? img[q - 1] : j.src;
+        const IsPlane &pl = j.plane[seq[q]];
+        const double dist = pl.n[0] * (prev[0] - pl.p[0]) + pl.n[1] * (prev[1] - pl.p[1]) +
+                            pl.n[2] * (prev[2] - pl.p[2]);
+        for (int k = 0; k < 3; ++k) img[q][k] = prev[k] - 2 * pl.n[k] * dist;
+    }
+}
+
+// Screening pass (scenes without transmissive polygons): an item survives
+// only if the ray from its receiver towards the last image can end on the
+// last object (may_hit_tris_wide).  Survivors gather in a workgroup buffer in
+// LDS and go to the global queue in bulk (one atomic per flush), so the
+// validation pass runs on full waves and the queue counter stays cold.
+constexpr int kScreenBuf = 4096;
+
+template <int NO>
+__global__ void __launch_bounds__(kIsBlock) k_is_screen(DevScene<double> s, IsJob j, uint32_t count,
+                                                       uint32_t *queue, unsigned *queued)
+{
+    constexpr int n = NO;
+    __shared__ uint32_t buf[kScreenBuf];
+    __shared__ unsigned nbuf, gbase;
+    const int lane = threadIdx.x & 63;
+    if (threadIdx.x == 0) nbuf = 0;
+    __syncthreads();
+    auto flush = [&]() {
+        if (threadIdx.x == 0) gbase = atomicAdd(queued, nbuf);
+        __syncthreads();
+        for (unsigned i = threadIdx.x; i < nbuf; i += kIsBlock) queue[gbase + i] = buf[i];
+        __syncthreads();
+        if (threadIdx.x == 0) nbuf = 0;
+        __syncthreads();
+    };
+    for (uint32_t l0 = blockIdx.x * kIsBlock; l0 < count; l0 += gridDim.x * kIsBlock) {
+        const uint32_t l = l0 + threadIdx.x;
+        bool pass = false;
+        if (l < count) {
+            int seq[NO];
+            const int rec = decode_item<NO>(j, l, seq);
+            double img[NO][3];
+            make_images<NO>(j, seq, img);
+            const double4 rc = j.rec[rec];
+            const double D[3] = {img[n - 1][0] - rc.x, img[n - 1][1] - rc.y, img[n - 1][2] - rc.z};
+            const double O[3] = {rc.x, rc.y, rc.z};
+            pass = may_hit_tris_wide(s, j, O, D, j.obj_tri0[seq[n - 1]], j.obj_tri0[seq[n - 1] + 1]);
         }
-        // Polygon_Scene.Image: each image mirrors the previous one
-        double img[PACH_IS_MAX_ORDER][3];
-        for (int q = 0; q < n; ++q) {
-            const double *prev = q > 0 ? img[q - 1] : j.src;
-            const IsPlane &pl = j.plane[seq[q]];
-            const double dist = pl.n[0] * (prev[0] - pl.p[0]) + pl.n[1] * (prev[1] - pl.p[1]) +
-                                pl.n[2] * (prev[2] - pl.p[2]);
-            for (int k = 0; k < 3; ++k) img[q][k] = prev[k] - 2 * pl.n[k] * dist;
+        const uint64_t m = __ballot(pass);
+        if (m != 0) {
+            unsigned base = 0;
+            if (lane == 0) base = atomicAdd(&nbuf, (unsigned)__popcll(m));
+            base = __shfl(base, 0);
+            if (pass) buf[base + __popcll(m & ((1ull << lane) - 1))] = l;
         }
+        __syncthreads();
+        if (nbuf > kScreenBuf - kIsBlock) flush();
+    }
+    flush();
+}
+
+// Validation pass: ProcessImages for each queued item (every item when queue
+// is null).
+template <int NO>
+__global__ void __launch_bounds__(kIsBlock) k_image_sources(DevScene<double> s, IsJob j, uint32_t count,
+                                                           const uint32_t *queue, const unsigned *queued)
+{
+    constexpr int n = NO;
+    const uint32_t total = queue ? *queued : count;
+    for (uint32_t qi = blockIdx.x * kIsBlock + threadIdx.x; qi < total; qi += gridDim.x * kIsBlock) {
+        const uint32_t l = queue ? queue[qi] : qi;
+        int seq[NO];
+        const int rec = decode_item<NO>(j, l, seq);
+        double img[NO][3];
+        make_images<NO>(j, seq, img);
         const double4 rc = j.rec[rec];
-        double P[PACH_IS_MAX_ORDER + 2][3];
-        int polys[PACH_IS_MAX_ORDER];
+        double P[NO + 2][3];
+        int polys[NO];
         double tm[8] = {1, 1, 1, 1, 1, 1, 1, 1};
         P[0][0] = j.src[0]; P[0][1] = j.src[1]; P[0][2] = j.src[2];
         P[n + 1][0] = rc.x; P[n + 1][1] = rc.y; P[n + 1][2] = rc.z;
@@ -217,11 +360,25 @@ __global__ void __launch_bounds__(kIsBlock) k_image_sources(DevScene<double> s, 
 
 }  // namespace
 
-hipError_t launch_image_sources(hipStream_t st, int grid, const DevScene<double> &s, const IsJob &j, int64_t lo,
-                                int64_t hi)
+hipError_t launch_image_sources(hipStream_t st, int grid, const DevScene<double> &s, const IsJob &j, uint32_t count,
+                                uint32_t *queue, unsigned *queued)
 {
-    if (hi <= lo) return hipSuccess;
-    hipLaunchKernelGGL(k_image_sources, dim3(grid), dim3(kIsBlock), 0, st, s, j, lo, hi);
+    if (count == 0) return hipSuccess;
+#define PACH_IS_CASE(NO_)                                                                                   \
+    case NO_:                                                                                               \
+        if (queue) {                                                                                        \
+            hipLaunchKernelGGL(k_is_screen<NO_>, dim3(grid), dim3(kIsBlock), 0, st, s, j, count, queue, queued); \
+            if (hipError_t e = hipGetLastError(); e != hipSuccess) return e;                                \
+        }                                                                                                   \
+        hipLaunchKernelGGL(k_image_sources<NO_>, dim3(grid), dim3(kIsBlock), 0, st, s, j, count,            \
+                           (const uint32_t *)queue, (const unsigned *)queued);                              \
+        break;
+    switch (j.order) {
+        PACH_IS_CASE(1) PACH_IS_CASE(2) PACH_IS_CASE(3) PACH_IS_CASE(4)
+        PACH_IS_CASE(5) PACH_IS_CASE(6) PACH_IS_CASE(7) PACH_IS_CASE(8)
+        default: return hipErrorInvalidValue;
+    }
+#undef PACH_IS_CASE
     return hipGetLastError();
 }
 

@@ -133,6 +133,7 @@ struct pach_ctx {
     std::atomic<int64_t> oct_rays{0};
     int status = 0;
     pach_stats stats{};
+    pach_is_stats is_stats{};
     std::mutex mu;
 };
 
@@ -1267,6 +1268,8 @@ extern "C" int pach_image_sources(pach_ctx *c, const int *object_of_surface, int
             obj[p] = object_of_surface[p];
         }
     }
+    if (n_objects >= (1 << 21) || c->R >= (1 << 21))
+        return fail(c, PACH_EINVAL, "image sources: at most 2^21 - 1 objects and receivers");
     // threads per order: n_objects^order x R, kept below 2^62
     {
         double tot = c->R;
@@ -1281,6 +1284,14 @@ extern "C" int pach_image_sources(pach_ctx *c, const int *object_of_surface, int
     for (int t = 0; t < T; ++t) {
         const int o = obj[c->surf[t]];
         if (first_poly[o] == c->surf[t] && first_tri[o] < 0) first_tri[o] = t;
+    }
+    // triangles of each object (CSR), for the walk-free first test
+    std::vector<int> obj_tri0(n_objects + 1, 0), obj_tris(T);
+    for (int t = 0; t < T; ++t) obj_tri0[obj[c->surf[t]] + 1]++;
+    for (int o = 0; o < n_objects; ++o) obj_tri0[o + 1] += obj_tri0[o];
+    {
+        std::vector<int> fill(obj_tri0.begin(), obj_tri0.end() - 1);
+        for (int t = 0; t < T; ++t) obj_tris[fill[obj[c->surf[t]]]++] = t;
     }
     std::vector<IsPlane> planes(n_objects);
     for (int o = 0; o < n_objects; ++o) {
@@ -1299,24 +1310,43 @@ extern "C" int pach_image_sources(pach_ctx *c, const int *object_of_surface, int
         }
     }
     (void)hipSetDevice(c->device);
+    const auto t_start = std::chrono::steady_clock::now();
+    c->is_stats = pach_is_stats{};
     constexpr unsigned long long kChunk = 1ull << 16;  // device result records per launch
-    int *d_obj = nullptr;
+    int *d_obj = nullptr, *d_obj_tri0 = nullptr, *d_obj_tris = nullptr;
     IsPlane *d_plane = nullptr;
     pach_is_path *d_out = nullptr;
     unsigned long long *d_count = nullptr;
+    uint32_t *d_queue = nullptr;
+    unsigned *d_queued = nullptr;
+    constexpr int64_t kSpan = (int64_t)1 << 26;  // items per launch at most
     std::vector<pach_is_path> found;
     auto body = [&]() -> int {
         HIPCHK(c, dalloc(d_obj, S));
+        HIPCHK(c, dalloc(d_obj_tri0, n_objects + 1));
+        HIPCHK(c, dalloc(d_obj_tris, T));
+        HIPCHK(c, hipMemcpy(d_obj_tri0, obj_tri0.data(), (n_objects + 1) * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(d_obj_tris, obj_tris.data(), T * sizeof(int), hipMemcpyHostToDevice));
         HIPCHK(c, dalloc(d_plane, n_objects));
         HIPCHK(c, dalloc(d_out, kChunk));
         HIPCHK(c, dalloc(d_count, 1));
+        if (!c->any_trans) {
+            // screening queue: without transmissive polygons the first hit decides
+            HIPCHK(c, dalloc(d_queue, (size_t)kSpan));
+            HIPCHK(c, dalloc(d_queued, 1));
+        }
         HIPCHK(c, hipMemcpy(d_obj, obj.data(), S * sizeof(int), hipMemcpyHostToDevice));
         HIPCHK(c, hipMemcpy(d_plane, planes.data(), n_objects * sizeof(IsPlane), hipMemcpyHostToDevice));
         IsJob j{};
         j.obj = d_obj;
         j.plane = d_plane;
         j.rec = c->d_ctr64;
+        j.obj_tri0 = d_obj_tri0;
+        j.obj_tris = d_obj_tris;
+        j.any_trans = c->any_trans ? 1 : 0;
         j.n_obj = n_objects;
+        j.inv_R = 1.0 / c->R;
+        j.inv_N = 1.0 / n_objects;
         j.R = c->R;
         for (int k = 0; k < 3; ++k) j.src[k] = c->origin[k];
         for (int b = 0; b < 8; ++b) {
@@ -1335,36 +1365,64 @@ extern "C" int pach_image_sources(pach_ctx *c, const int *object_of_surface, int
             j.order = order;
             int64_t total = c->R;
             for (int o = 0; o < order; ++o) total *= n_objects;
-            int64_t span = std::min<int64_t>(total, (int64_t)1 << 26);
+            // launches hold whole sequences: spans are multiples of R
+            const int64_t span_max = kSpan / c->R * c->R;
+            int64_t span = std::min<int64_t>(total, span_max);
             for (int64_t lo = 0; lo < total;) {
                 const int64_t hi = std::min(total, lo + span);
                 unsigned long long n = 0;
                 HIPCHK(c, hipMemsetAsync(d_count, 0, sizeof(unsigned long long), st));
-                HIPCHK(c, launch_image_sources(st, grid, s, j, lo, hi));
+                if (d_queued) HIPCHK(c, hipMemsetAsync(d_queued, 0, sizeof(unsigned), st));
+                HIPCHK(c, hipEventRecord(c->ev[0], st));
+                // receiver and sequence digits of item lo
+                int64_t rest = lo / c->R;
+                j.base_rec = (int)(lo % c->R);
+                for (int q = order - 1; q >= 0; --q) {
+                    j.base_seq[q] = (int)(rest % n_objects);
+                    rest /= n_objects;
+                }
+                HIPCHK(c, launch_image_sources(st, grid, s, j, (uint32_t)(hi - lo), d_queue, d_queued));
+                HIPCHK(c, hipEventRecord(c->ev[1], st));
                 HIPCHK(c, hipMemcpyAsync(&n, d_count, sizeof(n), hipMemcpyDeviceToHost, st));
                 HIPCHK(c, hipStreamSynchronize(st));
+                float ms = 0;
+                HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+                c->is_stats.ms_kernel += ms;
+                c->is_stats.launches++;
                 if (n > kChunk) {
                     // more valid paths than the buffer holds: redo the range in smaller pieces
-                    span = std::max<int64_t>(1, (hi - lo) * (int64_t)kChunk / (int64_t)(2 * n));
+                    span = std::max<int64_t>(c->R, (hi - lo) * (int64_t)kChunk / (int64_t)(2 * n) / c->R * c->R);
                     continue;
                 }
+                c->is_stats.checks += hi - lo;
                 const size_t old = found.size();
                 found.resize(old + n);
                 if (n) HIPCHK(c, hipMemcpy(found.data() + old, d_out, n * sizeof(pach_is_path), hipMemcpyDeviceToHost));
                 lo = hi;
-                if (n < kChunk / 4) span = std::min<int64_t>(span * 2, (int64_t)1 << 26);
+                if (n < kChunk / 4) span = std::min<int64_t>(span * 2, span_max);
             }
         }
         return PACH_OK;
     };
     const int rc = body();
-    dfree(d_obj); dfree(d_plane); dfree(d_out); dfree(d_count);
+    dfree(d_obj); dfree(d_obj_tri0); dfree(d_obj_tris); dfree(d_plane); dfree(d_out); dfree(d_count);
+    dfree(d_queue); dfree(d_queued);
     if (rc) return rc;
     std::stable_sort(found.begin(), found.end(), is_path_before);
     *n_paths = (int64_t)found.size();
+    c->is_stats.paths = (int64_t)found.size();
+    c->is_stats.ms_total =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     const size_t w = std::min<size_t>(found.size(), (size_t)cap);
     if (w) std::memcpy(out, found.data(), w * sizeof(pach_is_path));
     return (int64_t)found.size() > cap ? fail(c, PACH_ERANGE, "image sources: output buffer too small") : PACH_OK;
+}
+
+extern "C" int pach_image_sources_stats(pach_ctx *c, pach_is_stats *out)
+{
+    if (!c || !out) return PACH_EINVAL;
+    *out = c->is_stats;
+    return PACH_OK;
 }
 
 extern "C" int pach_autotune(pach_ctx *c, const pach_params *p, int64_t sample_rays, pach_tuning *out)
