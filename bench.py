@@ -86,7 +86,7 @@ def main():
     ap.add_argument("--is-order", type=int, default=3, help="image_sources workload: reflection order")
     ap.add_argument("--is-cpu-objects", type=int, default=16,
                     help="image_sources workload: first objects of the CPU sample's sequences")
-    ap.add_argument("--workload", choices=["hall", "batch", "image_sources"], default="hall",
+    ap.add_argument("--workload", choices=["hall", "batch", "image_sources", "filters"], default="hall",
                     help="hall: the 2k-triangle hall; batch: 256 packed rooms in one launch sequence")
     ap.add_argument("--batch-ray-scale", type=float, default=0.01,
                     help="batch workload: scale of each room's log-uniform [1e4, 1e6] ray count")
@@ -114,6 +114,8 @@ def main():
 
     if args.workload == "image_sources":
         return bench_image_sources(args, A, rooms)
+    if args.workload == "filters":
+        return bench_filters(args, A, rooms)
     at, c, rho = A.uniform_medium()
     prec = A.PREC_FP64 if args.precision == "fp64" else A.PREC_FP32
     tr = A.Tracer(0)
@@ -294,6 +296,62 @@ def bench_image_sources(args, A, rooms):
         "roofline": {"bound": "fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "k_is_screen + k_image_sources",
                      "avg_launch_ms": tot["ms"] / max(tot["launches"], 1), "launches": tot["launches"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    tr.close()
+
+
+def bench_filters(args, A, rooms):
+    """Auralisation filters per second: the 7 filters (omni and 6 directional,
+    Directional_Histogram.Create_Filter) of each of the hall's 16 receivers
+    from a traced 3 s, 1 kHz histogram (SampleCT 3,000), one call of
+    pach_receiver_filters per step (k_etc_conv, FP64)."""
+    at, c, rho = A.uniform_medium()
+    room = rooms.fan_hall()
+    tr = A.Tracer(0)
+    A.trace_room(tr, room, rays=20000, cutoff_s=args.cutoff, seed=args.seed, grid=(32, 32, 32))
+    R, T = int(room.receivers.shape[0]), tr.sample_ct
+    swl = np.full(8, 120.0)
+    for w in range(args.warmup):
+        tr.receiver_filters(swl)
+        print(f"[bench] warmup {w} done", file=sys.stderr, flush=True)
+    ms_k, t0 = 0.0, time.perf_counter()
+    for k in range(args.steps):
+        tr.receiver_filters(swl)
+        st = tr.filter_stats()
+        ms_k += st["ms_kernel"]
+        print(f"[bench] step {k}: {st['channels']} channels, kernel {st['ms_kernel']:.2f} ms, "
+              f"call {st['ms_total']:.1f} ms", file=sys.stderr, flush=True)
+    dt = time.perf_counter() - t0
+    # algorithmic FP64 flops: every envelope sample meets every carrier tap
+    # once per band (9 bands incl. the 16 kHz one), one multiply-add each
+    flops = 2.0 * 9 * T * A.CARRIER_LEN * 7 * R
+    achieved = flops * args.steps / (ms_k / 1e3) / 1e12
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import filters_ref as FR
+
+        pl = tr.planes(scaled=True)
+        n_rec = 2
+        t1 = time.perf_counter()
+        for r in range(n_rec):
+            FR.receiver_filters(pl[0, r], pl[1, r], pl[2:5, r], pl[5:8, r], swl, 1000)
+        cdt = time.perf_counter() - t1
+        cpu = {"value": n_rec / cdt, "unit": "receiver filter sets/s", "cores": 1, "kind": "port",
+               "sample": f"{n_rec} receivers' 7 filters, numpy FP64 FFT convolution as the reference computes "
+                         f"them ({cdt:.2f} s) on {cpu_model()}"}
+    out = {
+        "metric": "receiver filter sets/s", "value": R * args.steps / dt, "unit": "receiver filter sets/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"auralisation filters (omni + 6 directional) of the hall's {R} receivers from "
+                               f"{T}-sample 1 kHz histograms of a 20,000-ray trace",
+                   "channels_per_step": 7 * R, "kernel_ms_per_step": ms_k / args.steps},
+        "roofline": {"bound": "fp64", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "k_etc_conv",
+                     "avg_launch_ms": ms_k / args.steps, "launches": args.steps},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

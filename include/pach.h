@@ -254,6 +254,35 @@ typedef struct pach_is_stats {
 } pach_is_stats;
 int pach_image_sources_stats(pach_ctx *ctx, pach_is_stats *out);
 
+/* Auralisation filters.  Pach_SP.ETCToFilter (Pach_SP.cs:1584-1676) of n_ch
+ * channels on the GPU: prms [n_ch][8][T] octave-band pressure envelopes
+ * (62.5 Hz .. 8 kHz) at sample_frequency, swl [8] source power levels (dB).
+ * Each band's |envelope| x 10^((120 - SWL) / 20) is convolved with the band's
+ * 16,384-tap band-limited carrier (GenerateBandLimitedSincPulse at
+ * sample_frequency) and the bands are summed, plus a 16 kHz band made from the
+ * 8 kHz envelope with ISO 9613-1's extra air attenuation over 343 m/s x t.
+ * out [n_ch][T + 16383]. */
+int pach_etc_to_filter(pach_ctx *ctx, const double *prms, int n_ch, int T, const double swl[8],
+                       int sample_frequency, double *out);
+
+/* Directional_Histogram.Create_Filter(SWL, Rho_C) (Classes_Receivers.cs:
+ * 1911-1936) for every receiver of the last trace, from its scaled histograms:
+ * 7 filters per receiver -- the pressure envelope, then for x, y, z the
+ * envelopes P neg / E and P pos / E (0 where E = 0) -- each ETCToFilter at the
+ * receivers' sample rate shortened by 4096 samples.
+ * out [R][7][SampleCT + 16383 - 4096]. */
+int pach_receiver_filters(pach_ctx *ctx, const double swl[8], double *out);
+
+/* Work and timing of the last filter call (no counterpart in the reference). */
+typedef struct pach_filter_stats {
+    int64_t channels;   /* channels convolved */
+    int64_t samples;    /* envelope samples per band and channel */
+    int64_t launches;
+    double ms_kernel;   /* HIP-event duration of the convolution kernel */
+    double ms_total;    /* wall time of the call (host clock) */
+} pach_filter_stats;
+int pach_filter_stats_get(pach_ctx *ctx, pach_filter_stats *out);
+
 /* One revolution of a convergence check on host arrays, the function the
  * convergence mode applies to each checked receiver (TaskAll:870-956 for
  * PACH_CONV_SPECMIN, :1063-1112 for PACH_CONV_MINIMUM).  energy, spec, diff,

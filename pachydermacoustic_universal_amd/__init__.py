@@ -15,6 +15,7 @@ from .pachyderm import (  # noqa: F401
     IR_Construction,
     ImageSourceData,
     Lambert_Scattering,
+    Pach_SP,
     Polygon_Scene,
     RandomSource,
     Receiver_Bank,
@@ -27,6 +28,6 @@ from .pachyderm import (  # noqa: F401
 )
 
 __all__ = ["AcousticalMath", "Basic_Material", "Direct_Sound", "GeodesicSource", "IR_Construction",
-           "ImageSourceData", "Lambert_Scattering", "Specular_Path",
+           "ImageSourceData", "Lambert_Scattering", "Pach_SP", "Specular_Path",
            "Polygon_Scene", "RandomSource", "Receiver_Bank", "Running", "Simulation_Type",
            "SplitRayTracer", "Stopped", "SuppliedDirectionSource", "rooms"]

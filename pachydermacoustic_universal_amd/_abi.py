@@ -78,6 +78,17 @@ class IsStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class FilterStats(C.Structure):
+    _fields_ = [("channels", C.c_int64), ("samples", C.c_int64), ("launches", C.c_int64),
+                ("ms_kernel", C.c_double), ("ms_total", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+CARRIER_LEN = 16384  # ETCToFilter's carrier length
+
+
 # pach_is_path (one valid image-source path), as a numpy record
 IS_MAX_ORDER = 8
 IS_PATH = np.dtype([("receiver", "<i4"), ("order", "<i4"), ("seq", "<i4", (IS_MAX_ORDER,)),
@@ -116,6 +127,9 @@ SIGNATURES = [
     ("pach_image_sources", C.c_int, [C.c_void_p, _ip, C.c_int, C.c_int, C.c_void_p, C.c_int64,
                                      C.POINTER(C.c_int64)]),
     ("pach_image_sources_stats", C.c_int, [C.c_void_p, C.POINTER(IsStats)]),
+    ("pach_etc_to_filter", C.c_int, [C.c_void_p, _dp, C.c_int, C.c_int, _dp, C.c_int, _dp]),
+    ("pach_receiver_filters", C.c_int, [C.c_void_p, _dp, _dp]),
+    ("pach_filter_stats_get", C.c_int, [C.c_void_p, C.POINTER(FilterStats)]),
     ("pach_autotune", C.c_int, [C.c_void_p, C.POINTER(Params), C.c_int64, C.POINTER(Tuning)]),
     ("pach_trace", C.c_int, [C.c_void_p, C.POINTER(Params)]),
     ("pach_poll", C.c_int, [C.c_void_p, C.POINTER(Progress)]),
@@ -405,6 +419,29 @@ class Tracer:
                 continue
             self._chk(rc)
             return out[:n.value].copy()
+
+    def etc_to_filter(self, prms, swl, sample_frequency):
+        """Pach_SP.ETCToFilter of prms [n_ch][8][T] (or [8][T]): [n_ch][T + 16383]."""
+        pr = np.ascontiguousarray(prms, dtype=np.float64)
+        single = pr.ndim == 2
+        pr = pr.reshape(-1, 8, pr.shape[-1])
+        sw = np.ascontiguousarray(swl, dtype=np.float64)
+        out = np.zeros((pr.shape[0], pr.shape[2] + CARRIER_LEN - 1))
+        self._chk(lib().pach_etc_to_filter(self.h, _p(pr), pr.shape[0], pr.shape[2], _p(sw), int(sample_frequency),
+                                           _p(out)))
+        return out[0] if single else out
+
+    def receiver_filters(self, swl=None):
+        """Create_Filter(SWL, Rho_C) of every receiver of the last trace: [R][7][SampleCT + 12287]."""
+        sw = np.ascontiguousarray(np.full(8, 120.0) if swl is None else swl, dtype=np.float64)
+        out = np.zeros((self.R, 7, self.sample_ct + CARRIER_LEN - 1 - 4096))
+        self._chk(lib().pach_receiver_filters(self.h, _p(sw), _p(out)))
+        return out
+
+    def filter_stats(self):
+        st = FilterStats()
+        self._chk(lib().pach_filter_stats_get(self.h, C.byref(st)))
+        return st.as_dict()
 
     def image_source_stats(self):
         st = IsStats()

@@ -37,6 +37,7 @@ UNITS = [
     ("trace_f32.hip", [], True),
     ("trace_f64.hip", ["-ffp-contract=off"], True),
     ("image_source.hip", ["-ffp-contract=off"], True),
+    ("filters.hip", [], True),
     ("denoise.hip", [], True),
 ]
 

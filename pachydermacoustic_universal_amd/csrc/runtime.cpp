@@ -6,6 +6,7 @@
 // (k_octave) together with the rays the previous launch handed over, until
 // every ray of every batch has terminated.
 #include "../../include/pach.h"
+#include "filters.h"
 #include "grid.h"
 #include "image_source.h"
 #include "trace_types.h"
@@ -134,6 +135,11 @@ struct pach_ctx {
     int status = 0;
     pach_stats stats{};
     pach_is_stats is_stats{};
+    pach_filter_stats filter_stats{};
+    // auralisation-filter buffers, kept between calls (grown on demand)
+    double *f_env = nullptr, *f_car = nullptr, *f_part = nullptr, *f_out = nullptr, *f_tab = nullptr;
+    size_t f_env_n = 0, f_part_n = 0, f_out_n = 0, f_tab_n = 0;
+    int f_car_fs = 0;
     std::mutex mu;
 };
 
@@ -164,6 +170,16 @@ template <class T> static hipError_t dalloc(T *&p, size_t n)
     dfree(p);
     if (n == 0) n = 1;
     return hipMalloc((void **)&p, n * sizeof(T));
+}
+
+// Device buffer of at least n elements, kept (cap tracks its size).
+template <class T> static hipError_t dgrow(T *&p, size_t &cap, size_t n)
+{
+    if (p && cap >= n) return hipSuccess;
+    cap = 0;
+    const hipError_t e = dalloc(p, n);
+    if (e == hipSuccess) cap = n;
+    return e;
 }
 
 template <class R> static void free_scene(SceneBuf<R> &s)
@@ -314,6 +330,7 @@ extern "C" void pach_destroy(pach_ctx *c)
     dfree(c->d_cell_rmask);
     dfree(c->d_dirs); dfree(c->d_geo);
     dfree(c->d_ctr32); dfree(c->d_ctr64); dfree(c->d_hist);
+    dfree(c->f_env); dfree(c->f_car); dfree(c->f_part); dfree(c->f_out); dfree(c->f_tab);
     dfree(c->d_cnt); dfree(c->d_scal); dfree(c->d_band); dfree(c->d_rooms);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -1607,6 +1624,177 @@ extern "C" int pach_histogram(pach_ctx *c, int plane, double *out, int scaled)
                 out[i] *= f;
         }
     }
+    return PACH_OK;
+}
+
+// ------------------------------------------------------------------ auralisation filters
+namespace {
+
+// GenerateBandLimitedSincPulse (Pach_SP.cs:1678-1700): sinc(bw t) cos(2 pi fc t)
+// about the centre sample, scaled by 2 / (L1 norm x length).
+void band_carrier(int fs, double fl, double fu, double *c)
+{
+    const double fc = (fl + fu) / 2.0, bw = fu - fl, t0 = (kCarrierLen - 1) / 2.0;
+    double norm = 0;
+    for (int n = 0; n < kCarrierLen; ++n) {
+        const double t = (n - t0) / fs;
+        const double sinc = bw == 0 ? 1.0 : std::sin(M_PI * bw * t) / (M_PI * bw * t);
+        c[n] = sinc * std::cos(2 * M_PI * fc * t);
+        norm += std::fabs(c[n]);
+    }
+    norm = 2 / (norm * kCarrierLen);
+    if (norm > 0)
+        for (int n = 0; n < kCarrierLen; ++n) c[n] *= norm;
+}
+
+// Medium_Properties.ISO9613_1_attencoef, scalar form (Classes_Medium_Props.cs:
+// 193-202), with its C# integer exponents (-1/2, -1/3 and 1/2 are 0, -5/2 is -2).
+double iso9613_scalar(double f, double tk, double pa, double hr)
+{
+    const double h = hr * std::pow(10, -6.8346 * std::pow(273.16 / tk, 1.261) + 4.6151) / (pa / 101.325);
+    const double fro = (pa / 101.325) * (24 + 4.04 * std::pow(10, 4) * h * ((0.02 + h) / (0.391 + h)));
+    const double frn = (pa / 101.325) * std::pow(tk / 293.15, 0) *
+                       (9 + 280 * h * std::exp(-4.170 * (std::pow(tk / 293.15, 0) - 1)));
+    const double z = 0.1068 * std::exp(-3352 / tk) * std::pow(frn + f * f / frn, -1);
+    const double y = std::pow(tk / 293.15, -2) * (0.01275 * std::exp(-2239.1 / tk) * std::pow(fro + f * f / fro, -1) + z);
+    return 8.686 * f * f * ((1.84 * std::pow(10, -11) * std::pow(pa / 101.325, -1) * std::pow(tk / 293.15, 0)) + y);
+}
+
+// Tables of Pach_SP.ETCToFilter (Pach_SP.cs:1584-1676) for T samples at fs:
+// the 8 octave-band carriers and the 16 kHz one, the level factors
+// 10^((120 - SWL) / 20) and the 16 kHz band's per-sample factors (the 8 kHz
+// envelope with the extra air attenuation over 343 m/s x t, ISO 9613-1 at
+// 20 C, 50 %, pressure passed in Pa).
+struct FiltTables {
+    std::vector<double> car, f16;
+    double p_mod[8];
+};
+
+FiltTables filt_tables(const pach_ctx *c, int T, const double swl[8], int fs)
+{
+    FiltTables f;
+    if (!c->f_car || c->f_car_fs != fs) {
+        // carriers depend only on the sample rate: built when it changes
+        f.car.resize((size_t)kFiltBands * kCarrierLen);
+        for (int b = 0; b < 8; ++b) {
+            const double fc = 62.5 * std::pow(2, b);
+            band_carrier(fs, fc / std::sqrt(2.0), fc * std::sqrt(2.0), &f.car[(size_t)b * kCarrierLen]);
+        }
+        band_carrier(fs, 16000.0 / std::sqrt(2.0), 16000.0 * std::sqrt(2.0), &f.car[(size_t)8 * kCarrierLen]);
+    }
+    for (int b = 0; b < 8; ++b) f.p_mod[b] = std::pow(10, (120 - swl[b]) / 20.0);
+    const double tk = 20 + 273.15, pa = 101.325 * 1000, hr = 50.0;
+    const double extra = iso9613_scalar(16000.0, tk, pa, hr) - iso9613_scalar(8000.0, tk, pa, hr);
+    f.f16.resize(T);
+    for (int t = 0; t < T; ++t) f.f16[t] = std::pow(10, -extra * (t * (1.0 / fs) * 343.0) / 20.0);
+    return f;
+}
+
+// Convolution of the device envelopes d_env (launch_etc_conv's layout) with
+// the carriers; out [n_ch][T + kCarrierLen - 1 - trim] on the host.
+int run_filters(pach_ctx *c, const double *d_env, const FiltTables &f, int fs, int n_ch, int T, double *out, int trim)
+{
+    const int out_len = T + kCarrierLen - 1;
+    const int n_chb = (n_ch + kFiltCh - 1) / kFiltCh;
+    if (!f.car.empty()) {
+        size_t car_n = 0;
+        HIPCHK(c, dgrow(c->f_car, car_n, f.car.size()));
+        HIPCHK(c, hipMemcpy(c->f_car, f.car.data(), f.car.size() * sizeof(double), hipMemcpyHostToDevice));
+        c->f_car_fs = fs;
+    }
+    HIPCHK(c, dgrow(c->f_part, c->f_part_n, (size_t)kFiltBands * n_ch * out_len));
+    HIPCHK(c, dgrow(c->f_out, c->f_out_n, (size_t)n_ch * out_len));
+    HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    HIPCHK(c, launch_etc_conv(c->stream, d_env, c->f_car, T, n_chb, n_ch, c->f_part, c->f_out));
+    HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+    c->filter_stats.ms_kernel = ms;
+    c->filter_stats.launches = 1;
+    c->filter_stats.channels = n_ch;
+    c->filter_stats.samples = T;
+    const int keep = out_len - trim;
+    HIPCHK(c, hipMemcpy2D(out, (size_t)keep * sizeof(double), c->f_out, (size_t)out_len * sizeof(double),
+                          (size_t)keep * sizeof(double), n_ch, hipMemcpyDeviceToHost));
+    return PACH_OK;
+}
+
+}  // namespace
+
+extern "C" int pach_etc_to_filter(pach_ctx *c, const double *prms, int n_ch, int T, const double swl[8],
+                                  int sample_frequency, double *out)
+{
+    if (!c || !prms || !swl || !out || n_ch <= 0 || T <= 0 || sample_frequency <= 0) return PACH_EINVAL;
+    if (c->running) return fail(c, PACH_ESTATE, "trace running");
+    if ((int64_t)T > (int64_t)1 << 26) return fail(c, PACH_EINVAL, "envelope too long");
+    const auto t_start = std::chrono::steady_clock::now();
+    (void)hipSetDevice(c->device);
+    const FiltTables f = filt_tables(c, T, swl, sample_frequency);
+    // envelopes |prms| x p_mod in launch_etc_conv's layout
+    const int n_chb = (n_ch + kFiltCh - 1) / kFiltCh;
+    std::vector<double> env((size_t)kFiltBands * n_chb * T * kFiltCh, 0.0);
+    for (int ch = 0; ch < n_ch; ++ch) {
+        const double *pr = prms + (size_t)ch * 8 * T;
+        const int cb = ch / kFiltCh, cl = ch % kFiltCh;
+        for (int b = 0; b < 8; ++b)
+            for (int t = 0; t < T; ++t)
+                env[(((size_t)b * n_chb + cb) * T + t) * kFiltCh + cl] = std::fabs(pr[(size_t)b * T + t]) * f.p_mod[b];
+        for (int t = 0; t < T; ++t)
+            env[(((size_t)8 * n_chb + cb) * T + t) * kFiltCh + cl] =
+                std::fabs(pr[(size_t)7 * T + t]) * f.p_mod[7] * f.f16[t];
+    }
+    int rc = PACH_OK;
+    if (dgrow(c->f_env, c->f_env_n, env.size()) != hipSuccess ||
+        hipMemcpy(c->f_env, env.data(), env.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(c, PACH_EHIP, "filter envelopes: device copy failed");
+    if (!rc) rc = run_filters(c, c->f_env, f, sample_frequency, n_ch, T, out, 0);
+    c->filter_stats.ms_total =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    return rc;
+}
+
+extern "C" int pach_receiver_filters(pach_ctx *c, const double swl[8], double *out)
+{
+    if (!c || !swl || !out) return PACH_EINVAL;
+    if (!c->has_recv) return fail(c, PACH_ESTATE, "no receivers");
+    if (c->running) return fail(c, PACH_ESTATE, "trace running");
+    const int R = c->R, T = c->sct;
+    if (T <= 1) return fail(c, PACH_ESTATE, "histograms too short for filters");
+    const auto t_start = std::chrono::steady_clock::now();
+    (void)hipSetDevice(c->device);
+    const FiltTables f = filt_tables(c, T, swl, (int)c->fs);
+    // Receiver_Bank.Scale: 1 / (completed source rays of each receiver's room)
+    std::vector<double> scale(R, 0.0);
+    const int64_t done = c->rays_done.load();
+    for (const RoomTab &t : c->room_tab) {
+        const double n = (double)std::min(std::max<int64_t>(done - t.ray_base, 0), t.ray_count);
+        for (int r = t.rec_base; r < t.rec_base + t.rec_count && r < R; ++r) scale[r] = n > 0 ? 1.0 / n : 0.0;
+    }
+    const int n_ch = 7 * R, n_chb = (n_ch + kFiltCh - 1) / kFiltCh;
+    auto body = [&]() -> int {
+        const size_t env_n = (size_t)kFiltBands * n_chb * T * kFiltCh;
+        HIPCHK(c, dgrow(c->f_env, c->f_env_n, env_n));
+        HIPCHK(c, hipMemsetAsync(c->f_env, 0, env_n * sizeof(double), c->stream));
+        std::vector<double> tab(scale);
+        tab.insert(tab.end(), f.p_mod, f.p_mod + 8);
+        tab.insert(tab.end(), f.f16.begin(), f.f16.end());
+        HIPCHK(c, dgrow(c->f_tab, c->f_tab_n, tab.size()));
+        HIPCHK(c, hipMemcpyAsync(c->f_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, launch_receiver_env(c->stream, c->d_hist, c->hist_n, R, T, c->f_tab, c->f_tab + R,
+                                      c->f_tab + R + 8, n_chb, c->f_env));
+        return run_filters(c, c->f_env, f, (int)c->fs, n_ch, T, out, 4096);
+    };
+    const int rc = body();
+    c->filter_stats.ms_total =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    return rc;
+}
+
+extern "C" int pach_filter_stats_get(pach_ctx *c, pach_filter_stats *out)
+{
+    if (!c || !out) return PACH_EINVAL;
+    *out = c->filter_stats;
     return PACH_OK;
 }
 

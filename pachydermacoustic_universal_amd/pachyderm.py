@@ -240,6 +240,45 @@ class Receiver_Bank:
         self.planes /= n
         self.planes[A.PLANES["pressure"]] *= np.sqrt(n)
 
+    def _filter_channels(self, rec):
+        """Directional_Histogram.Create_Filter's envelopes (Classes_Receivers.cs:
+        1911-1936): pressure, then per axis P neg / E and P pos / E (0 where
+        E = 0) -- [7][8][SampleCT]."""
+        E, P = self.planes[0, rec], self.planes[1, rec]
+        zero = E == 0
+        safe = np.where(zero, 1.0, E)
+        ch = [P]
+        for d in range(3):
+            ch.append(np.where(zero, 0.0, P * self.planes[5 + d, rec] / safe))
+            ch.append(np.where(zero, 0.0, P * self.planes[2 + d, rec] / safe))
+        return np.array(ch)
+
+    def _filters(self, recs, swl, device=0):
+        chans = np.concatenate([self._filter_channels(r) for r in recs])
+        with A.Tracer(device) as tr:
+            F = tr.etc_to_filter(chans, swl, self.SampleRate)
+        return F[:, :-4096].reshape(len(recs), 7, -1)
+
+    def Create_Filter(self, SWL=None, Rec_ID=None, dim=None, VB=None):
+        """Auralisation filters (Receiver_Bank.Create_Filter, Classes_Receivers.cs:
+        626-663; Directional_Histogram.Create_Filter :1911-1995), computed on the
+        GPU (pach_etc_to_filter) from the scaled histograms.
+
+        Create_Filter(): every receiver with SWL 120 dB; sets F [R][n] (omni)
+        and Fdir [R][6][n] (per axis: the P neg / E, then the P pos / E filter).
+        Create_Filter(SWL, Rec_ID): that receiver's 7 filters (omni, then per
+        axis neg, pos).  Create_Filter(SWL, Rec_ID, dim): one of them (dim 0
+        omni; 2d + 1 and 2d + 2 the neg / pos filters of axis d, zero where
+        E = 0; the reference's own single-direction path fails there)."""
+        if SWL is None or not hasattr(SWL, "__len__"):
+            F = self._filters(range(self.Count), np.full(8, 120.0))
+            self.F, self.Fdir = F[:, 0], F[:, 1:]
+            return None
+        F = self._filters([int(Rec_ID)], np.asarray(SWL, dtype=np.float64))[0]
+        if dim is None:
+            return F
+        return F[int(dim) if 0 <= int(dim) <= 6 else 0]
+
     def GetEnergyHistogram(self, Octave, Delay_ms, Rec_Index):
         """Energy histogram padded by the delay (:432-438); Octave 8 sums the bands."""
         E = self.planes[0, Rec_Index]
@@ -562,6 +601,20 @@ class _RoomView:
 
     def __init__(self, verts, surf, trans, source, receivers):
         self.verts, self.surf, self.trans, self.source, self.receivers = verts, surf, trans, source, receivers
+
+
+class Pach_SP:
+    """Pach_SP (Pach_SP.cs) signal processing used by the auralisation path."""
+
+    @staticmethod
+    def ETCToFilter(Octave_PRMS, SWL, sample_frequency=44100, sample_frequency_out=44100, device=0):
+        """Pach_SP.ETCToFilter (Pach_SP.cs:1584-1676) on the GPU: the sum over the
+        octave bands (and a 16 kHz band from the 8 kHz one) of each band's
+        |envelope| 10^((120 - SWL) / 20) convolved with its band-limited
+        carrier; length T + 16383.  sample_frequency_out is unused, as in the
+        reference."""
+        with A.Tracer(device) as tr:
+            return tr.etc_to_filter(np.asarray(Octave_PRMS, dtype=np.float64), SWL, sample_frequency)
 
 
 class IR_Construction:
