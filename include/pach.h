@@ -182,6 +182,23 @@ int pach_receivers_set(pach_ctx *ctx, const double *centers, int R, double radiu
 int pach_rooms_set(pach_ctx *ctx, int n, const double *src, const double *power, const int64_t *rays,
                    const int *surf_count, const int *rec_count);
 
+/* Mapping receivers (PachMapReceiver, Pach_Map_Receiver.cs:34-1690): M
+ * spheres of diameter `increment` at centers [M][3] that replace the
+ * spherical receivers in later traces (one room; RecMain is the map), with
+ * their own voxel grid (Partition :99-303) and voxel walks per ray segment
+ * (CheckRay :1493-1631, CheckBroadbandRay :1359-1491, as written there).
+ * Each receiver records at 1000 Hz, sample_ct samples, times relative to its
+ * direct arrival, deposits I 10^(-0.1 a t) / (pi r^2) at the chord midpoint;
+ * directional: Directional_Histogram (pressure sqrt(E rho c), direction
+ * planes) else Histogram (pressure sqrt(E)).  pach_receivers_set still
+ * supplies the medium. */
+int pach_map_receivers_set(pach_ctx *ctx, const double *centers, int M, double increment, int directional,
+                           int sample_ct);
+int pach_map_receivers_clear(pach_ctx *ctx);
+/* plane 0-7 (energy, pressure, +x, +y, +z, -x, -y, -z): out [M][8][sample_ct],
+ * scaled by the source rays traced (pressure by their square root). */
+int pach_map_histogram(pach_ctx *ctx, int plane, double *out, int scaled);
+
 /* Launch tuning of later traces (no counterpart in the reference; results do
  * not depend on it).  Keys: "dist_lds" (0/1): stage the voxel distance field
  * in LDS when it fits; "receiver_masks" (0/1): test only receivers whose

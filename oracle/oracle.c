@@ -525,6 +525,280 @@ void or_receiver_check(const or_setup *su, double **planes, const double o[3], c
 }
 
 /* ------------------------------------------------------------------------ */
+/* Mapping receivers (PachMapReceiver, Pach_Map_Receiver.cs).                */
+/*   Partition :99-303 (bounding box, voxel counts, receiver lists)          */
+/*   CheckBroadbandRay :1359-1491, CheckRay :1493-1631 (voxel walks)         */
+/*   CheckReceivers_* :349-403 (each receiver once per segment)              */
+/*   Map_Receiver :1721-1826 (sphere test, deposit)                          */
+/*   Histogram.Add / Directional_Histogram.Add (Classes_Receivers.cs:1159-  */
+/*   1166, :1813-1825)                                                       */
+/* Hare's AABB.IsPointInBox / Intersect are taken as the inclusive box and   */
+/* the slab entry point (t >= 0); parity of those two is unpinned.           */
+/* ------------------------------------------------------------------------ */
+struct or_map {
+    int M;
+    double *ctr;      /* [M][3] */
+    double *direct;   /* [M] Direct_Time */
+    double r, r2, sizemod, c, rho_c, atten[8];
+    int directional, sample_ct;
+    double bmin[3], bmax[3], vd[3];
+    int ct[3];
+    int *start;       /* [ncell + 1], cell = (x * ctY + y) * ctZ + z */
+    int *list;
+};
+
+or_map *or_map_create(const double *centers, int M, double increment, const double src[3], double c,
+                      double rho_c, const double atten[8], int directional, int sample_ct)
+{
+    if (M <= 0 || !(increment > 0) || sample_ct <= 0) return NULL;
+    or_map *m = calloc(1, sizeof(or_map));
+    m->M = M;
+    m->ctr = malloc(sizeof(double) * 3 * M);
+    m->direct = malloc(sizeof(double) * M);
+    memcpy(m->ctr, centers, sizeof(double) * 3 * M);
+    m->r = increment * 0.5;
+    m->r2 = m->r * m->r;
+    m->sizemod = 1 / (M_PI * m->r2);
+    m->c = c;
+    m->rho_c = rho_c;
+    memcpy(m->atten, atten, sizeof(m->atten));
+    m->directional = directional;
+    m->sample_ct = sample_ct;
+    for (int i = 0; i < M; ++i) {
+        const double *P = centers + 3 * i;
+        double lx = src[0] - P[0], ly = src[1] - P[1], lz = src[2] - P[2];
+        m->direct[i] = sqrt(lx * lx + ly * ly + lz * lz) / c;
+    }
+    /* bounds of all spheres (:183-192) */
+    for (int k = 0; k < 3; ++k) { m->bmin[k] = INFINITY; m->bmax[k] = -INFINITY; }
+    for (int i = 0; i < M; ++i)
+        for (int k = 0; k < 3; ++k) {
+            double v = centers[3 * i + k];
+            if (v + increment * .5 > m->bmax[k]) m->bmax[k] = v + increment * .5;
+            if (v - increment * .5 < m->bmin[k]) m->bmin[k] = v - increment * .5;
+        }
+    m->ct[0] = (int)ceil((m->bmax[0] - m->bmin[0]) / (increment * 4));
+    m->ct[1] = (int)ceil((m->bmax[1] - m->bmin[1]) / (increment * 4));
+    m->ct[2] = (int)ceil((m->bmax[2] - m->bmin[2]) / (increment * 2.5));
+    double inv[3];
+    for (int k = 0; k < 3; ++k) {
+        m->vd[k] = (m->bmax[k] - m->bmin[k]) / m->ct[k];
+        inv[k] = 1 / m->vd[k];
+    }
+    int nc = m->ct[0] * m->ct[1] * m->ct[2];
+    int *cnt = calloc(nc + 1, sizeof(int));
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1) {
+            m->start = malloc(sizeof(int) * (nc + 1));
+            m->start[0] = 0;
+            for (int i = 0; i < nc; ++i) m->start[i + 1] = m->start[i] + cnt[i];
+            m->list = malloc(sizeof(int) * (m->start[nc] > 0 ? m->start[nc] : 1));
+            memset(cnt, 0, sizeof(int) * (nc + 1));
+        }
+        for (int i = 0; i < M; ++i) {
+            const double *o = centers + 3 * i;
+            int lo[3], hi[3];
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = (int)floor((o[k] - m->r - m->bmin[k]) * inv[k]);
+                hi[k] = (int)floor((o[k] + m->r - m->bmin[k]) * inv[k]);
+                if (lo[k] < 0) lo[k] = 0;
+                if (hi[k] > m->ct[k] - 1) hi[k] = m->ct[k] - 1;
+            }
+            for (int x = lo[0]; x <= hi[0]; ++x)
+                for (int y = lo[1]; y <= hi[1]; ++y)
+                    for (int z = lo[2]; z <= hi[2]; ++z) {
+                        /* voxel box as Partition builds it, then the sphere-AABB test */
+                        double mn[3] = {x * m->vd[0] + m->bmin[0], y * m->vd[1] + m->bmin[1], z * m->vd[2] + m->bmin[2]};
+                        double mx[3] = {(x + 1) * m->vd[0] + m->bmin[0], (y + 1) * m->vd[1] + m->bmin[1],
+                                        (z + 1) * m->vd[2] + m->bmin[2]};
+                        double dd[3];
+                        for (int k = 0; k < 3; ++k) {
+                            double a = mn[k] - o[k], b = o[k] - mx[k];
+                            dd[k] = a > (b > 0 ? b : 0) ? a : (b > 0 ? b : 0);
+                        }
+                        if (dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2] < m->r2) {
+                            int cell = (x * m->ct[1] + y) * m->ct[2] + z;
+                            if (pass == 0) cnt[cell]++;
+                            else m->list[m->start[cell] + cnt[cell]++] = i;
+                        }
+                    }
+        }
+    }
+    free(cnt);
+    return m;
+}
+
+void or_map_destroy(or_map *m)
+{
+    if (!m) return;
+    free(m->ctr); free(m->direct); free(m->start); free(m->list);
+    free(m);
+}
+
+int or_map_grid(const or_map *m, int ct[3], double box_min[3], double voxel[3])
+{
+    for (int k = 0; k < 3; ++k) { ct[k] = m->ct[k]; box_min[k] = m->bmin[k]; voxel[k] = m->vd[k]; }
+    return m->ct[0] * m->ct[1] * m->ct[2];
+}
+
+int or_map_voxel(const or_map *m, int x, int y, int z, int *out, int cap)
+{
+    int cell = (x * m->ct[1] + y) * m->ct[2] + z;
+    int n = m->start[cell + 1] - m->start[cell];
+    for (int i = 0; i < n && i < cap; ++i) out[i] = m->list[m->start[cell] + i];
+    return n;
+}
+
+/* Histogram.Add / Directional_Histogram.Add at 1000 Hz; samples before 0
+ * are skipped (the directional form has no such test and would fail). */
+static void map_add(const or_map *m, double **P, int i, int band, double time, double e, const double *d,
+                    int64_t *dep)
+{
+    int sample = (int)(time * 1000);
+    if (sample >= m->sample_ct || sample < 0) return;
+    size_t idx = ((size_t)i * 8 + band) * m->sample_ct + sample;
+    if (P[0]) P[0][idx] += e;
+    if (m->directional) {
+        if (P[1]) P[1][idx] += sqrt(e * m->rho_c);
+        for (int k = 0; k < 3; ++k) {
+            double dk = -d[k];
+            double v = (double)(float)(dk * e);
+            if (dk > 0) { if (P[2 + k]) P[2 + k][idx] += v; }
+            else { if (P[5 + k]) P[5 + k][idx] += v; }
+        }
+    } else if (P[1]) {
+        P[1][idx] += sqrt(e);
+    }
+    if (dep) (*dep)++;
+}
+
+/* Map_Receiver.CheckRay / CheckBroadbandRay (:1780-1826): chord midpoint
+ * before EndPt, deposit at t / c + t_sum - Direct_Time (the air attenuation
+ * is applied over that time, as written). */
+static void map_receiver(const or_map *m, double **P, int i, const double *o, const double *d,
+                         const double *end, const double *E, int band, double t_sum, int64_t *dep)
+{
+    const double *C = m->ctr + 3 * i;
+    double mx = o[0] - C[0], my = o[1] - C[1], mz = o[2] - C[2];
+    double b = mx * d[0] + my * d[1] + mz * d[2];
+    double c = (mx * mx + my * my + mz * mz) - m->r2;
+    if (c > 0 && b > 0) return;
+    double discr = b * b - c;
+    if (discr < 0) return;
+    double t1 = -b - sqrt(discr), t2 = -b + sqrt(discr);
+    double t = (t1 + t2) / 2;
+    double ex = end[0] - o[0], ey = end[1] - o[1], ez = end[2] - o[2];
+    if (!(t > 0 && t * t < ex * ex + ey * ey + ez * ez)) return;
+    double raydist = t / m->c + t_sum - m->direct[i];
+    if (band >= 0) {
+        map_add(m, P, i, band, raydist, E[0] * pow(10, -.1 * m->atten[band] * raydist) * m->sizemod, d, dep);
+    } else {
+        for (int oc = 0; oc < 8; ++oc)
+            map_add(m, P, i, oc, raydist, E[oc] * pow(10, -.1 * m->atten[oc] * raydist) * m->sizemod, d, dep);
+    }
+}
+
+/* receivers of voxel (x, y, z) not yet checked on this segment */
+static void map_voxel_check(const or_map *m, double **P, unsigned char *seen, int x, int y, int z, const double *o,
+                            const double *d, const double *end, const double *E, int band, double t_sum,
+                            int64_t *dep)
+{
+    int cell = (x * m->ct[1] + y) * m->ct[2] + z;
+    for (int j = m->start[cell]; j < m->start[cell + 1]; ++j) {
+        int i = m->list[j];
+        if (seen[i]) continue;
+        seen[i] = 1;
+        map_receiver(m, P, i, o, d, end, E, band, t_sum, dep);
+    }
+}
+
+static int map_in_box(const or_map *m, const double *p)
+{
+    for (int k = 0; k < 3; ++k)
+        if (p[k] < m->bmin[k] || p[k] > m->bmax[k]) return 0;
+    return 1;
+}
+
+/* slab entry of the ray into the box (t >= 0) */
+static int map_box_entry(const or_map *m, const double *o, const double *d, double *entry)
+{
+    double t0 = 0, t1 = INFINITY;
+    for (int k = 0; k < 3; ++k) {
+        if (d[k] == 0) {
+            if (o[k] < m->bmin[k] || o[k] > m->bmax[k]) return 0;
+            continue;
+        }
+        double a = (m->bmin[k] - o[k]) / d[k], b = (m->bmax[k] - o[k]) / d[k];
+        if (a > b) { double t = a; a = b; b = t; }
+        if (a > t0) t0 = a;
+        if (b < t1) t1 = b;
+    }
+    if (t0 > t1) return 0;
+    for (int k = 0; k < 3; ++k) entry[k] = o[k] + d[k] * t0;
+    return 1;
+}
+
+static void map_walk(const or_map *m, double **P, const double *o, const double *d, const double *end,
+                     const double *E, int band, double t_sum, int broadband, int64_t *dep)
+{
+    double start[3] = {o[0], o[1], o[2]};
+    if (!map_in_box(m, o))
+        if (!map_box_entry(m, o, d, start)) return;
+    /* the broadband walk starts from the ray's own origin and steers by the
+     * signs of its position; the octave walk uses the entry point and the
+     * direction (the reference's two versions differ so) */
+    const double *from = broadband ? o : start;
+    const double *sgn = broadband ? o : d;
+    int X[3], Xe[3], step[3];
+    double tmax[3], tdel[3];
+    for (int k = 0; k < 3; ++k) {
+        X[k] = (int)floor((from[k] - m->bmin[k]) / m->vd[k]);
+        Xe[k] = (int)floor((end[k] - m->bmin[k]) / m->vd[k]);
+        if (X[k] < 0) X[k] = 0;
+        if (X[k] >= m->ct[k]) X[k] = m->ct[k] - 1;
+    }
+    for (int k = 0; k < 3; ++k)
+        if ((sgn[k] < 0 && X[k] < Xe[k]) || (sgn[k] >= 0 && X[k] > Xe[k])) return;
+    for (int k = 0; k < 3; ++k) {
+        double lo = X[k] * m->vd[k] + m->bmin[k], hi = (X[k] + 1) * m->vd[k] + m->bmin[k];
+        if (sgn[k] < 0) {
+            step[k] = -1;
+            tmax[k] = (lo - from[k]) / sgn[k];
+        } else {
+            step[k] = 1;
+            tmax[k] = (hi - from[k]) / sgn[k];
+        }
+        tdel[k] = m->vd[k] / sgn[k] * step[k];
+    }
+    unsigned char *seen = calloc(m->M, 1);
+    for (;;) {
+        map_voxel_check(m, P, seen, X[0], X[1], X[2], o, d, end, E, band, t_sum, dep);
+        int ax;
+        if (tmax[0] < tmax[1]) ax = tmax[0] < tmax[2] ? 0 : 2;
+        else ax = tmax[1] < tmax[2] ? 1 : 2;
+        X[ax] += step[ax];
+        if (X[ax] < 0 || X[ax] >= m->ct[ax]) break;
+        tmax[ax] = tmax[ax] + tdel[ax];
+        int more = broadband ? (X[0] != Xe[0] && X[1] != Xe[1] && X[2] != Xe[2])
+                             : (X[0] != Xe[0] || X[1] != Xe[1] || X[2] != Xe[2]);
+        if (!more) break;
+    }
+    free(seen);
+}
+
+void or_map_check_octave(const or_map *m, double **planes, const double o[3], const double d[3],
+                         const double end[3], double I, int band, double t_sum, int64_t *deposits)
+{
+    map_walk(m, planes, o, d, end, &I, band, t_sum, 0, deposits);
+}
+
+void or_map_check_broadband(const or_map *m, double **planes, const double o[3], const double d[3],
+                            const double end[3], const double E[8], double t_sum, int64_t *deposits)
+{
+    map_walk(m, planes, o, d, end, E, -1, t_sum, 1, deposits);
+}
+
+/* ------------------------------------------------------------------------ */
 /* Tracer (TaskAll:335-536)                                                  */
 /* ------------------------------------------------------------------------ */
 typedef struct {
@@ -710,12 +984,14 @@ static void trace_octave(const or_scene *s, const or_setup *su, const or_params 
         else cnt->bounces_verylate++;
         int ana = (r.mask & 1) ? 1 : ((r.mask & 2) ? 2 : 0);
         if (ht < 0) {
-            check_rays(x, r.o, r.d, hit, &r.I, r.band, 0, 0, r.t, ana);
+            if (su->map) or_map_check_octave(su->map, su->map_planes, r.o, r.d, hit, r.I, r.band, r.t, &cnt->deposits);
+            else check_rays(x, r.o, r.d, hit, &r.I, r.band, 0, 0, r.t, ana);
             cnt->lost += r.I;
             cnt->misses++;
             return;
         }
-        check_rays(x, r.o, r.d, hit, &r.I, r.band, 0, 0, r.t, ana);
+        if (su->map) or_map_check_octave(su->map, su->map_planes, r.o, r.d, hit, r.I, r.band, r.t, &cnt->deposits);
+        else check_rays(x, r.o, r.d, hit, &r.I, r.band, 0, 0, r.t, ana);
         r.t += leg / su->c;
         memcpy(r.o, hit, sizeof(hit));
         r.tri = ht;
@@ -775,13 +1051,19 @@ static void trace_source_ray(const or_scene *s, const or_setup *su, const or_par
         int ht = shoot(s, p, o_, d, hit, &leg);
         cnt->bounces_broad++;
         if (ht < 0) {
-            if (order > p->is_order) check_rays(x, o_, d, hit, E, -1, p->oct_lo, p->oct_hi, tb, 0);
+            if (order > p->is_order) {
+                if (su->map) or_map_check_broadband(su->map, su->map_planes, o_, d, hit, E, tb, &cnt->deposits);
+                else check_rays(x, o_, d, hit, E, -1, p->oct_lo, p->oct_hi, tb, 0);
+            }
             for (int j = 0; j < 8; ++j) cnt->lost += E[j];
             cnt->misses++;
             split = 0;
             break;
         }
-        if (order > p->is_order) check_rays(x, o_, d, hit, E, -1, p->oct_lo, p->oct_hi, tb, 0);
+        if (order > p->is_order) {
+            if (su->map) or_map_check_broadband(su->map, su->map_planes, o_, d, hit, E, tb, &cnt->deposits);
+            else check_rays(x, o_, d, hit, E, -1, p->oct_lo, p->oct_hi, tb, 0);
+        }
         tb += leg / su->c;
         memcpy(o_, hit, sizeof(hit));
         order++;
@@ -884,6 +1166,8 @@ int or_trace(const or_scene *s, const or_setup *su, const or_params *p,
 typedef struct {
     const or_scene *s;
     const or_setup *su;
+    or_setup su_own;            /* the caller's setup with this thread's map planes */
+    double *map_planes[8];
     or_params p;
     double *planes[OR_NPLANES];
     or_counters cnt;
@@ -901,6 +1185,7 @@ int or_trace_mt(const or_scene *s, const or_setup *su, const or_params *p,
 {
     if (n_threads <= 1) return or_trace(s, su, p, planes, cnt);
     size_t plane_n = (size_t)su->R * 8 * su->sample_ct;
+    size_t map_n = su->map ? (size_t)su->map->M * 8 * su->map->sample_ct : 0;
     mt_job *jobs = calloc(n_threads, sizeof(mt_job));
     pthread_t *th = calloc(n_threads, sizeof(pthread_t));
     int64_t n = p->ray_end - p->ray_begin;
@@ -910,6 +1195,13 @@ int or_trace_mt(const or_scene *s, const or_setup *su, const or_params *p,
         jobs[i].p.ray_end = p->ray_begin + n * (i + 1) / n_threads;
         for (int k = 0; k < OR_NPLANES; ++k)
             jobs[i].planes[k] = planes[k] ? calloc(plane_n, sizeof(double)) : NULL;
+        if (su->map) {
+            jobs[i].su_own = *su;
+            for (int k = 0; k < 8; ++k)
+                jobs[i].map_planes[k] = su->map_planes[k] ? calloc(map_n, sizeof(double)) : NULL;
+            jobs[i].su_own.map_planes = jobs[i].map_planes;
+            jobs[i].su = &jobs[i].su_own;
+        }
         pthread_create(&th[i], NULL, mt_worker, &jobs[i]);
     }
     for (int i = 0; i < n_threads; ++i) pthread_join(th[i], NULL);
@@ -918,6 +1210,11 @@ int or_trace_mt(const or_scene *s, const or_setup *su, const or_params *p,
             if (!planes[k]) continue;
             for (size_t q = 0; q < plane_n; ++q) planes[k][q] += jobs[i].planes[k][q];
             free(jobs[i].planes[k]);
+        }
+        for (int k = 0; k < 8 && su->map; ++k) {
+            if (!su->map_planes[k]) continue;
+            for (size_t q = 0; q < map_n; ++q) su->map_planes[k][q] += jobs[i].map_planes[k][q];
+            free(jobs[i].map_planes[k]);
         }
         or_counters *c = &jobs[i].cnt;
         cnt->bounces_broad += c->bounces_broad;

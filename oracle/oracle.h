@@ -91,6 +91,26 @@ typedef struct {
     int use_grid;           /* 1: voxel DDA, 0: brute force */
 } or_params;
 
+/* --- mapping receivers (PachMapReceiver, Pach_Map_Receiver.cs) ----------- */
+typedef struct or_map or_map;
+/* Spheres of diameter `increment` at centers [M][3], their voxel grid
+ * (Partition :99-303), Direct_Time = |src - centre| / c, rho c = rho_c.
+ * directional: Directional_Histogram (P = sqrt(E rho c), direction planes)
+ * or Histogram (P = sqrt(E)).  sample_ct samples at 1000 Hz. */
+or_map *or_map_create(const double *centers, int M, double increment, const double src[3], double c,
+                      double rho_c, const double atten[8], int directional, int sample_ct);
+void or_map_destroy(or_map *m);
+int or_map_grid(const or_map *m, int ct[3], double box_min[3], double voxel[3]);
+/* receivers listed in voxel (x, y, z): count, written to out up to cap */
+int or_map_voxel(const or_map *m, int x, int y, int z, int *out, int cap);
+/* one octave-ray segment (CheckRay :1493-1631) / broadband segment
+ * (CheckBroadbandRay :1359-1491); planes: 8 pointers [M][8][sample_ct]
+ * (E, P, +x, +y, +z, -x, -y, -z; NULL to skip) */
+void or_map_check_octave(const or_map *m, double **planes, const double o[3], const double d[3],
+                         const double end[3], double I, int band, double t_sum, int64_t *deposits);
+void or_map_check_broadband(const or_map *m, double **planes, const double o[3], const double d[3],
+                            const double end[3], const double E[8], double t_sum, int64_t *deposits);
+
 typedef struct {
     double src[3];
     double power[8];        /* W per band (Source.SoundPower) */
@@ -102,6 +122,8 @@ typedef struct {
     double rho_c;
     int sample_rate;
     int sample_ct;
+    const or_map *map;      /* non-NULL: the receivers are this map (RecMain = PachMapReceiver) */
+    double **map_planes;    /* 8 planes [M][8][map sample_ct] */
 } or_setup;
 
 typedef struct {

@@ -35,7 +35,7 @@ class Setup(C.Structure):
     _fields_ = [("src", C.c_double * 3), ("power", C.c_double * 8), ("R", C.c_int),
                 ("centers", _dp), ("radius", C.c_double), ("c", C.c_double),
                 ("atten", C.c_double * 8), ("rho_c", C.c_double), ("sample_rate", C.c_int),
-                ("sample_ct", C.c_int)]
+                ("sample_ct", C.c_int), ("map", C.c_void_p), ("map_planes", C.POINTER(_dp))]
 
 
 class Counters(C.Structure):
@@ -108,6 +108,17 @@ def lib():
         L.or_image_sources.restype = C.c_int64
         L.or_image_sources.argtypes = [C.c_void_p, _ip, C.c_int, C.c_int, _dp, _dp, _dp, C.c_int, _dp,
                                        C.c_double, C.c_double, C.c_void_p, C.c_int64]
+        L.or_map_create.restype = C.c_void_p
+        L.or_map_create.argtypes = [_dp, C.c_int, C.c_double, _dp, C.c_double, C.c_double, _dp, C.c_int, C.c_int]
+        L.or_map_destroy.argtypes = [C.c_void_p]
+        L.or_map_grid.restype = C.c_int
+        L.or_map_grid.argtypes = [C.c_void_p, _ip, _dp, _dp]
+        L.or_map_voxel.restype = C.c_int
+        L.or_map_voxel.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _ip, C.c_int]
+        L.or_map_check_octave.argtypes = [C.c_void_p, C.POINTER(_dp), _dp, _dp, _dp, C.c_double, C.c_int,
+                                          C.c_double, C.POINTER(C.c_int64)]
+        L.or_map_check_broadband.argtypes = [C.c_void_p, C.POINTER(_dp), _dp, _dp, _dp, _dp, C.c_double,
+                                             C.POINTER(C.c_int64)]
         L.or_image_sources_count_mt.restype = C.c_int64
         L.or_image_sources_count_mt.argtypes = [C.c_void_p, _ip, C.c_int, C.c_int, _dp, _dp, _dp, C.c_int, _dp,
                                                 C.c_double, C.c_double, C.c_int, C.c_int, C.c_int]
@@ -264,11 +275,67 @@ def geodesic_faces():
     return out
 
 
+class Map:
+    """Mapping receivers (or_map): spheres of diameter `increment` at `centers`."""
+
+    def __init__(self, centers, increment, src, medium, directional=True, sample_ct=1000):
+        self.centers = np.ascontiguousarray(centers, dtype=np.float64).reshape(-1, 3)
+        s = np.ascontiguousarray(src, dtype=np.float64)
+        at = np.ascontiguousarray(medium.atten, dtype=np.float64)
+        self.M, self.sample_ct = self.centers.shape[0], int(sample_ct)
+        self.h = lib().or_map_create(_p(self.centers), self.M, float(increment), _p(s), float(medium.c),
+                                     float(medium.rho_c), _p(at), int(directional), self.sample_ct)
+        if not self.h:
+            raise ValueError("oracle map creation failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_map_destroy(self.h)
+            self.h = None
+
+    def grid(self):
+        ct = (C.c_int * 3)()
+        mn, vd = np.zeros(3), np.zeros(3)
+        lib().or_map_grid(self.h, ct, _p(mn), _p(vd))
+        return tuple(ct), mn, vd
+
+    def voxel(self, x, y, z):
+        buf = (C.c_int * self.M)()
+        n = lib().or_map_voxel(self.h, x, y, z, buf, self.M)
+        return list(buf[:n])
+
+    def planes(self):
+        return np.zeros((8, self.M, 8, self.sample_ct))
+
+    @staticmethod
+    def _ptrs(pl):
+        ptrs = (_dp * 8)()
+        for k in range(8):
+            ptrs[k] = _p(pl[k])
+        return ptrs
+
+    def check_octave(self, pl, o, d, end, I, band, t_sum):
+        dep = C.c_int64()
+        a = [np.ascontiguousarray(v, dtype=np.float64) for v in (o, d, end)]
+        lib().or_map_check_octave(self.h, self._ptrs(pl), _p(a[0]), _p(a[1]), _p(a[2]), float(I), int(band),
+                                  float(t_sum), C.byref(dep))
+        return dep.value
+
+    def check_broadband(self, pl, o, d, end, E, t_sum):
+        dep = C.c_int64()
+        a = [np.ascontiguousarray(v, dtype=np.float64) for v in (o, d, end, E)]
+        lib().or_map_check_broadband(self.h, self._ptrs(pl), _p(a[0]), _p(a[1]), _p(a[2]), _p(a[3]),
+                                     float(t_sum), C.byref(dep))
+        return dep.value
+
+
 def trace(room, *, rays, cutoff_s, seed=1, mode=MODE_STOCHASTIC, dir_mode=DIR_GEODESIC,
           dirs=None, oct_lo=0, oct_hi=7, is_order=-1, sample_rate=1000, sample_ct=None,
           power=None, medium=None, grid=(16, 16, 16), use_grid=True, ray_begin=0,
-          p1_max_bounces=128, max_bounces=100000, threads=1, planes=NPLANES, scene=None):
+          p1_max_bounces=128, max_bounces=100000, threads=1, planes=NPLANES, scene=None, map_rx=None):
     """Trace ``rays`` source rays; returns (planes[11, R, 8, sample_ct], counters dict).
+    With ``map_rx`` (a Map) the receivers are the map's and the planes returned
+    are the map's [8, M, 8, map sample_ct].
 
     Histograms are raw sums (not divided by the ray count), i.e. what the
     receivers hold before Combine_ThreadLocal_Results' Scale."""
@@ -302,6 +369,12 @@ def trace(room, *, rays, cutoff_s, seed=1, mode=MODE_STOCHASTIC, dir_mode=DIR_GE
         p.dirs = _p(dd)
         p.n_dirs = dd.shape[0]
     p.p1_max_bounces, p.max_bounces, p.use_grid = p1_max_bounces, max_bounces, int(use_grid)
+    map_out = None
+    if map_rx is not None:
+        map_out = map_rx.planes()
+        map_ptrs = Map._ptrs(map_out)
+        su.map = map_rx.h
+        su.map_planes = map_ptrs
     out = np.zeros((NPLANES, R, 8, sample_ct))
     ptrs = (_dp * NPLANES)()
     for k in range(NPLANES):
@@ -313,6 +386,8 @@ def trace(room, *, rays, cutoff_s, seed=1, mode=MODE_STOCHASTIC, dir_mode=DIR_GE
         rc = lib().or_trace(sc.h, C.byref(su), C.byref(p), ptrs, C.byref(cnt))
     if rc != 0:
         raise RuntimeError(f"or_trace failed: {rc}")
+    if map_out is not None:
+        return map_out, cnt.as_dict()
     return out, cnt.as_dict()
 
 

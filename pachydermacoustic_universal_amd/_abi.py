@@ -118,6 +118,9 @@ SIGNATURES = [
     ("pach_receivers_set", C.c_int, [C.c_void_p, _dp, C.c_int, C.c_double, _dp, C.c_double,
                                      C.c_double, C.c_int, C.c_int]),
     ("pach_rooms_set", C.c_int, [C.c_void_p, C.c_int, _dp, _dp, C.POINTER(C.c_int64), _ip, _ip]),
+    ("pach_map_receivers_set", C.c_int, [C.c_void_p, _dp, C.c_int, C.c_double, C.c_int, C.c_int]),
+    ("pach_map_receivers_clear", C.c_int, [C.c_void_p]),
+    ("pach_map_histogram", C.c_int, [C.c_void_p, C.c_int, _dp, C.c_int]),
     ("pach_tune", C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
     ("pach_direct_sound", C.c_int, [_dp, C.c_int, _ip, _dp, C.c_int, _dp, _dp, _dp, C.c_int, _dp, C.c_double,
                                     _dp, _dp, _ip]),
@@ -447,6 +450,25 @@ class Tracer:
         st = IsStats()
         self._chk(lib().pach_image_sources_stats(self.h, C.byref(st)))
         return st.as_dict()
+
+    def set_map(self, centers, increment, directional=True, sample_ct=1000):
+        """Mapping receivers (pach_map_receivers_set) for the next traces."""
+        ctr = np.ascontiguousarray(centers, dtype=np.float64).reshape(-1, 3)
+        self._chk(lib().pach_map_receivers_set(self.h, _p(ctr), ctr.shape[0], float(increment), int(directional),
+                                               int(sample_ct)))
+        self.map_n, self.map_sct = ctr.shape[0], int(sample_ct)
+
+    def clear_map(self):
+        self._chk(lib().pach_map_receivers_clear(self.h))
+        self.map_n = 0
+
+    def map_histogram(self, plane=0, scaled=True):
+        out = np.zeros((self.map_n, 8, self.map_sct))
+        self._chk(lib().pach_map_histogram(self.h, plane, _p(out), int(scaled)))
+        return out
+
+    def map_planes(self, scaled=False):
+        return np.stack([self.map_histogram(k, scaled) for k in range(8)])
 
     def histogram(self, plane=0, scaled=True):
         out = np.zeros((self.R, 8, self.sample_ct))

@@ -96,6 +96,18 @@ struct pach_ctx {
     double *d_hist = nullptr;
     size_t hist_n = 0;  // per plane
 
+    // mapping receivers (pach_map_receivers_set)
+    bool map_on = false;
+    int map_n = 0, map_sct = 0, map_directional = 1;
+    double map_r = 0;
+    std::vector<double> map_ctr;  // [M][3]
+    double map_bmin[3] = {0, 0, 0}, map_bmax[3] = {0, 0, 0}, map_vd[3] = {1, 1, 1};
+    int map_ct[3] = {1, 1, 1};
+    double4 *d_map_ctr = nullptr;
+    uint32_t *d_map_start = nullptr, *d_map_list = nullptr;
+    double *d_map_hist = nullptr;
+    size_t map_hist_n = 0;  // per plane
+
     // rooms (pach_rooms_set); empty: one room made of the whole scene
     struct RoomIn {
         double src[3], power[8];
@@ -331,6 +343,7 @@ extern "C" void pach_destroy(pach_ctx *c)
     dfree(c->d_dirs); dfree(c->d_geo);
     dfree(c->d_ctr32); dfree(c->d_ctr64); dfree(c->d_hist);
     dfree(c->f_env); dfree(c->f_car); dfree(c->f_part); dfree(c->f_out); dfree(c->f_tab);
+    dfree(c->d_map_ctr); dfree(c->d_map_start); dfree(c->d_map_list); dfree(c->d_map_hist);
     dfree(c->d_cnt); dfree(c->d_scal); dfree(c->d_band); dfree(c->d_rooms);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -960,6 +973,38 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
     rv.sct = c->sct;
     rv.hist = c->d_hist;
     rv.plane_stride = c->hist_n;
+    rv.has_map = 0;
+    if (c->map_on) {
+        if (rooms.size() != 1) return fail(c, PACH_EINVAL, "mapping receivers trace one room");
+        // Direct_Time of each map receiver from the current source (Map_Receiver ctor)
+        std::vector<double4> ctr(c->map_n);
+        for (int i = 0; i < c->map_n; ++i) {
+            const double *P = &c->map_ctr[3 * i];
+            const double lx = c->origin[0] - P[0], ly = c->origin[1] - P[1], lz = c->origin[2] - P[2];
+            ctr[i] = make_double4(P[0], P[1], P[2], std::sqrt(lx * lx + ly * ly + lz * lz) / c->c);
+        }
+        HIPCHK(c, hipMemcpy(c->d_map_ctr, ctr.data(), c->map_n * sizeof(double4), hipMemcpyHostToDevice));
+        DevMap &m = rv.map;
+        m.ctr = c->d_map_ctr;
+        m.start = c->d_map_start;
+        m.list = c->d_map_list;
+        for (int k = 0; k < 3; ++k) {
+            m.bmin[k] = c->map_bmin[k];
+            m.bmax[k] = c->map_bmax[k];
+            m.vd[k] = c->map_vd[k];
+            m.ct[k] = c->map_ct[k];
+        }
+        m.n = c->map_n;
+        m.r2 = c->map_r * c->map_r;
+        m.sizemod = 1 / (M_PI * m.r2);
+        m.c = c->c;
+        m.rho_c = c->rho_c;
+        m.directional = c->map_directional;
+        m.sct = c->map_sct;
+        m.hist = c->d_map_hist;
+        m.plane_stride = c->map_hist_n;
+        rv.has_map = 1;
+    }
 
     const int64_t N = rooms.back().ray_base + rooms.back().ray_count;
     int64_t batch = p.batch_rays > 0 ? p.batch_rays : std::min<int64_t>(N, 32768);
@@ -975,6 +1020,7 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
     hipStream_t st = c->stream;
     HIPCHK(c, hipMemcpyAsync(c->d_band, &bt, sizeof(bt), hipMemcpyHostToDevice, st));
     HIPCHK(c, hipMemsetAsync(c->d_hist, 0, c->hist_n * kPlanes * sizeof(double), st));
+    if (c->map_on) HIPCHK(c, hipMemsetAsync(c->d_map_hist, 0, c->map_hist_n * 8 * sizeof(double), st));
     HIPCHK(c, hipMemsetAsync(c->d_cnt, 0, sizeof(DevCounters), st));
     HIPCHK(c, hipMemsetAsync(c->d_scal, 0, 8 * sizeof(unsigned), st));
 
@@ -1155,6 +1201,112 @@ extern "C" int pach_rooms_set(pach_ctx *c, int n, const double *src, const doubl
     }
     c->rooms = std::move(v);
     c->rmask_dirty = true;
+    return PACH_OK;
+}
+
+// PachMapReceiver.Partition (Pach_Map_Receiver.cs:99-303): spheres of
+// diameter `increment`, their bounding box, voxels of 4 x 4 x 2.5 increments,
+// each listing (ascending) the receivers whose sphere reaches into it.
+extern "C" int pach_map_receivers_set(pach_ctx *c, const double *centers, int M, double increment, int directional,
+                                      int sample_ct)
+{
+    if (!c || !centers || M <= 0 || !(increment > 0) || sample_ct <= 0) return PACH_EINVAL;
+    if (c->running) return fail(c, PACH_ESTATE, "trace running");
+    for (int i = 0; i < 3 * M; ++i)
+        if (!std::isfinite(centers[i])) return fail(c, PACH_EINVAL, "map: non-finite centre");
+    (void)hipSetDevice(c->device);
+    const double r = increment * 0.5, r2 = r * r;
+    double bmin[3], bmax[3], vd[3], inv[3];
+    int ct[3];
+    for (int k = 0; k < 3; ++k) { bmin[k] = INFINITY; bmax[k] = -INFINITY; }
+    for (int i = 0; i < M; ++i)
+        for (int k = 0; k < 3; ++k) {
+            const double v = centers[3 * i + k];
+            if (v + increment * .5 > bmax[k]) bmax[k] = v + increment * .5;
+            if (v - increment * .5 < bmin[k]) bmin[k] = v - increment * .5;
+        }
+    ct[0] = (int)std::ceil((bmax[0] - bmin[0]) / (increment * 4));
+    ct[1] = (int)std::ceil((bmax[1] - bmin[1]) / (increment * 4));
+    ct[2] = (int)std::ceil((bmax[2] - bmin[2]) / (increment * 2.5));
+    const int64_t ncell = (int64_t)ct[0] * ct[1] * ct[2];
+    if (ncell <= 0 || ncell > ((int64_t)1 << 26)) return fail(c, PACH_EINVAL, "map: receiver grid too large");
+    for (int k = 0; k < 3; ++k) {
+        vd[k] = (bmax[k] - bmin[k]) / ct[k];
+        inv[k] = 1 / vd[k];
+    }
+    std::vector<std::vector<uint32_t>> cells(ncell);
+    for (int i = 0; i < M; ++i) {
+        const double *o = centers + 3 * i;
+        int lo[3], hi[3];
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::max(0, (int)std::floor((o[k] - r - bmin[k]) * inv[k]));
+            hi[k] = std::min(ct[k] - 1, (int)std::floor((o[k] + r - bmin[k]) * inv[k]));
+        }
+        for (int x = lo[0]; x <= hi[0]; ++x)
+            for (int y = lo[1]; y <= hi[1]; ++y)
+                for (int z = lo[2]; z <= hi[2]; ++z) {
+                    const double mn[3] = {x * vd[0] + bmin[0], y * vd[1] + bmin[1], z * vd[2] + bmin[2]};
+                    const double mx[3] = {(x + 1) * vd[0] + bmin[0], (y + 1) * vd[1] + bmin[1], (z + 1) * vd[2] + bmin[2]};
+                    double dd[3];
+                    for (int k = 0; k < 3; ++k) dd[k] = std::max(mn[k] - o[k], std::max(0.0, o[k] - mx[k]));
+                    if (dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2] < r2)
+                        cells[((int64_t)x * ct[1] + y) * ct[2] + z].push_back((uint32_t)i);
+                }
+    }
+    std::vector<uint32_t> start(ncell + 1, 0), list;
+    for (int64_t q = 0; q < ncell; ++q) {
+        start[q + 1] = start[q] + (uint32_t)cells[q].size();
+        list.insert(list.end(), cells[q].begin(), cells[q].end());
+    }
+    if (list.empty()) list.push_back(0);
+    HIPCHK(c, dalloc(c->d_map_ctr, M));
+    HIPCHK(c, dalloc(c->d_map_start, (size_t)ncell + 1));
+    HIPCHK(c, dalloc(c->d_map_list, list.size()));
+    HIPCHK(c, hipMemcpy(c->d_map_start, start.data(), start.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_map_list, list.data(), list.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    const size_t per_plane = (size_t)M * kBands * sample_ct;
+    HIPCHK(c, dalloc(c->d_map_hist, per_plane * 8));
+    HIPCHK(c, hipMemset(c->d_map_hist, 0, per_plane * 8 * sizeof(double)));
+    c->map_ctr.assign(centers, centers + 3 * M);
+    c->map_n = M;
+    c->map_r = r;
+    c->map_sct = sample_ct;
+    c->map_directional = directional ? 1 : 0;
+    c->map_hist_n = per_plane;
+    for (int k = 0; k < 3; ++k) {
+        c->map_bmin[k] = bmin[k];
+        c->map_bmax[k] = bmax[k];
+        c->map_vd[k] = vd[k];
+        c->map_ct[k] = ct[k];
+    }
+    c->map_on = true;
+    return PACH_OK;
+}
+
+extern "C" int pach_map_receivers_clear(pach_ctx *c)
+{
+    if (!c) return PACH_EINVAL;
+    if (c->running) return fail(c, PACH_ESTATE, "trace running");
+    c->map_on = false;
+    return PACH_OK;
+}
+
+extern "C" int pach_map_histogram(pach_ctx *c, int plane, double *out, int scaled)
+{
+    if (!c || !out) return PACH_EINVAL;
+    if (plane < 0 || plane >= 8) return fail(c, PACH_EINVAL, "bad map plane");
+    if (!c->map_on) return fail(c, PACH_ESTATE, "no mapping receivers");
+    if (c->running) return fail(c, PACH_ESTATE, "trace running");
+    (void)hipSetDevice(c->device);
+    HIPCHK(c, hipMemcpy(out, c->d_map_hist + (size_t)plane * c->map_hist_n, c->map_hist_n * sizeof(double),
+                        hipMemcpyDeviceToHost));
+    if (scaled) {
+        const double n = (double)c->rays_done.load();
+        if (n > 0) {
+            const double f = plane == PACH_PLANE_PRESSURE ? 1.0 / std::sqrt(n) : 1.0 / n;
+            for (size_t i = 0; i < c->map_hist_n; ++i) out[i] *= f;
+        }
+    }
     return PACH_OK;
 }
 

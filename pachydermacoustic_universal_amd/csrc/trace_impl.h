@@ -485,6 +485,167 @@ __device__ void check_receivers(const DevRecv<R> &rv, const RoomTab &rm, uint64_
     }
 }
 
+// ---------------------------------------------------------------- mapping receivers
+// PachMapReceiver.CheckRay (Pach_Map_Receiver.cs:1493-1631) and
+// CheckBroadbandRay (:1359-1491): a voxel walk over the receiver grid from the
+// segment's origin (or its entry into the grid) towards EndPt, each listed
+// receiver tested once per segment (CheckReceivers_*, :349-403).  As written
+// there, the octave walk stops on entering the end voxel, and the broadband
+// walk starts from the ray's own origin, steers by the signs of the origin's
+// coordinates and stops once any index reaches its end value.  In FP64 so that
+// both precisions deposit what the reference would.
+struct MapWalk {
+    int X[3], step[3];
+    double tmax[3], tdel[3];
+};
+
+__device__ __forceinline__ bool map_walk_init(const DevMap &m, const double o[3], const double d[3],
+                                              const double end[3], bool broadband, MapWalk &w, int Xe[3])
+{
+    double start[3] = {o[0], o[1], o[2]};
+    bool inside = true;
+    for (int k = 0; k < 3; ++k) inside &= !(o[k] < m.bmin[k] || o[k] > m.bmax[k]);
+    if (!inside) {
+        // slab entry (t >= 0)
+        double t0 = 0, t1 = __builtin_huge_val();
+        for (int k = 0; k < 3; ++k) {
+            if (d[k] == 0) {
+                if (o[k] < m.bmin[k] || o[k] > m.bmax[k]) return false;
+                continue;
+            }
+            double a = (m.bmin[k] - o[k]) / d[k], b = (m.bmax[k] - o[k]) / d[k];
+            if (a > b) { const double t = a; a = b; b = t; }
+            if (a > t0) t0 = a;
+            if (b < t1) t1 = b;
+        }
+        if (t0 > t1) return false;
+        for (int k = 0; k < 3; ++k) start[k] = o[k] + d[k] * t0;
+    }
+    const double *from = broadband ? o : start;
+    const double *sgn = broadband ? o : d;
+    for (int k = 0; k < 3; ++k) {
+        w.X[k] = (int)floor((from[k] - m.bmin[k]) / m.vd[k]);
+        Xe[k] = (int)floor((end[k] - m.bmin[k]) / m.vd[k]);
+        if (w.X[k] < 0) w.X[k] = 0;
+        if (w.X[k] >= m.ct[k]) w.X[k] = m.ct[k] - 1;
+    }
+    for (int k = 0; k < 3; ++k)
+        if ((sgn[k] < 0 && w.X[k] < Xe[k]) || (sgn[k] >= 0 && w.X[k] > Xe[k])) return false;
+    for (int k = 0; k < 3; ++k) {
+        const double lo = w.X[k] * m.vd[k] + m.bmin[k], hi = (w.X[k] + 1) * m.vd[k] + m.bmin[k];
+        if (sgn[k] < 0) {
+            w.step[k] = -1;
+            w.tmax[k] = (lo - from[k]) / sgn[k];
+        } else {
+            w.step[k] = 1;
+            w.tmax[k] = (hi - from[k]) / sgn[k];
+        }
+        w.tdel[k] = m.vd[k] / sgn[k] * w.step[k];
+    }
+    return true;
+}
+
+// one step; false once the walk has ended
+__device__ __forceinline__ bool map_walk_step(const DevMap &m, MapWalk &w, const int Xe[3], bool broadband)
+{
+    int ax;
+    if (w.tmax[0] < w.tmax[1]) ax = w.tmax[0] < w.tmax[2] ? 0 : 2;
+    else ax = w.tmax[1] < w.tmax[2] ? 1 : 2;
+    w.X[ax] += w.step[ax];
+    if (w.X[ax] < 0 || w.X[ax] >= m.ct[ax]) return false;
+    w.tmax[ax] = w.tmax[ax] + w.tdel[ax];
+    return broadband ? (w.X[0] != Xe[0] && w.X[1] != Xe[1] && w.X[2] != Xe[2])
+                     : (w.X[0] != Xe[0] || w.X[1] != Xe[1] || w.X[2] != Xe[2]);
+}
+
+__device__ __forceinline__ int map_cell(const DevMap &m, const MapWalk &w)
+{
+    return (w.X[0] * m.ct[1] + w.X[1]) * m.ct[2] + w.X[2];
+}
+
+// is receiver j listed in cell? (lists are sorted)
+__device__ __forceinline__ bool map_listed(const DevMap &m, int cell, uint32_t j)
+{
+    uint32_t lo = m.start[cell], hi = m.start[cell + 1];
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        const uint32_t v = m.list[mid];
+        if (v == j) return true;
+        if (v < j) lo = mid + 1;
+        else hi = mid;
+    }
+    return false;
+}
+
+// Map_Receiver.CheckRay / CheckBroadbandRay (:1780-1826) and
+// Histogram.Add / Directional_Histogram.Add at 1000 Hz.  band >= 0: octave ray
+// of intensity E[0]; band < 0: broadband energies E[0..7].
+template <class R>
+__device__ void map_check(const DevRecv<R> &rv, const R *oR, const R *dR, R ex, R ey, R ez, const R *E, int band,
+                          double t_sum, LaneCounters &lc)
+{
+    const DevMap &m = rv.map;
+    const bool broadband = band < 0;
+    const double o[3] = {(double)oR[0], (double)oR[1], (double)oR[2]};
+    const double d[3] = {(double)dR[0], (double)dR[1], (double)dR[2]};
+    const double end[3] = {(double)ex, (double)ey, (double)ez};
+    MapWalk w;
+    int Xe[3];
+    if (!map_walk_init(m, o, d, end, broadband, w, Xe)) return;
+    const double qx = end[0] - o[0], qy = end[1] - o[1], qz = end[2] - o[2];
+    const double sqd = qx * qx + qy * qy + qz * qz;
+    for (int visit = 0;; ++visit) {
+        const int cell = map_cell(m, w);
+        for (uint32_t k = m.start[cell]; k < m.start[cell + 1]; ++k) {
+            const uint32_t j = m.list[k];
+            const double4 C = m.ctr[j];
+            const double mx = o[0] - C.x, my = o[1] - C.y, mz = o[2] - C.z;
+            const double b = mx * d[0] + my * d[1] + mz * d[2];
+            const double c = (mx * mx + my * my + mz * mz) - m.r2;
+            if (c > 0 && b > 0) continue;
+            const double discr = b * b - c;
+            if (discr < 0) continue;
+            const double t1 = -b - sqrt(discr), t2 = -b + sqrt(discr);
+            const double t = (t1 + t2) / 2;
+            if (!(t > 0 && t * t < sqd)) continue;
+            // tested once per segment: skip a receiver an earlier voxel of the
+            // walk lists (the walk is replayed from its start; hits are rare)
+            bool earlier = false;
+            if (visit > 0) {
+                MapWalk v;
+                int Xv[3];
+                map_walk_init(m, o, d, end, broadband, v, Xv);
+                for (int q = 0; q < visit && !earlier; ++q) {
+                    earlier = map_listed(m, map_cell(m, v), j);
+                    map_walk_step(m, v, Xv, broadband);
+                }
+            }
+            if (earlier) continue;
+            const double raydist = t / m.c + t_sum - C.w;
+            const int sample = (int)(raydist * 1000);
+            if (sample >= m.sct || sample < 0) continue;
+            const int b_lo = broadband ? 0 : band, b_hi = broadband ? kBands - 1 : band;
+            for (int oc = b_lo; oc <= b_hi; ++oc) {
+                const double e = (double)E[broadband ? oc : 0] * pow(10.0, -.1 * rv.band->atten[oc] * raydist) * m.sizemod;
+                const uint64_t idx = ((uint64_t)j * kBands + oc) * m.sct + sample;
+                unsafeAtomicAdd(m.hist + idx, e);
+                if (m.directional) {
+                    unsafeAtomicAdd(m.hist + m.plane_stride + idx, sqrt(e * m.rho_c));
+                    for (int a = 0; a < 3; ++a) {
+                        const double dk = -d[a];
+                        const double vv = (double)(float)(dk * e);
+                        unsafeAtomicAdd(m.hist + (uint64_t)(dk > 0 ? 2 + a : 5 + a) * m.plane_stride + idx, vv);
+                    }
+                } else {
+                    unsafeAtomicAdd(m.hist + m.plane_stride + idx, sqrt(e));
+                }
+                lc.v[C_DEP]++;
+            }
+        }
+        if (!map_walk_step(m, w, Xe, broadband)) break;
+    }
+}
+
 // ---------------------------------------------------------------- reflection
 // sin(2 pi u), cos(2 pi u) for u in [0, 1): quarter-turn reduction and the
 // Taylor series of sin / cos on [-pi/4, pi/4] to degree 17/18 in a fixed Horner
@@ -592,7 +753,7 @@ __device__ __forceinline__ void store_dead(ORay<R> *out, unsigned from, unsigned
 
 // cast_ray_distributed's broadband phase and split (TaskAll:345-424) for one
 // source ray; may suspend into resume_out when the output buffer is full.
-template <class R>
+template <class R, bool MAP>
 __global__ void __launch_bounds__(kBlock)
 k_phase1(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv, int64_t src_begin,
          int n_fresh, const P1State<R> *__restrict__ resume_in, int n_resume, ORay<R> *out,
@@ -640,9 +801,11 @@ k_phase1(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv, int64_t 
                         const R ex = st.o[0] + st.d[0] * R(1000000), ey = st.o[1] + st.d[1] * R(1000000),
                                 ez = st.o[2] + st.d[2] * R(1000000);
                         const R qx = ex - st.o[0], qy = ey - st.o[1], qz = ez - st.o[2];
-                        check_receivers<R>(rv, room, cand, st.o[0], st.o[1], st.o[2], st.d[0], st.d[1], st.d[2],
-                                           qx * qx + qy * qy + qz * qz, st.E, -1, tc.oct_lo,
-                                           tc.oct_hi, st.tb, 0, lc);
+                        if constexpr (MAP) map_check<R>(rv, st.o, st.d, ex, ey, ez, st.E, -1, st.tb, lc);
+                        else
+                            check_receivers<R>(rv, room, cand, st.o[0], st.o[1], st.o[2], st.d[0], st.d[1], st.d[2],
+                                               qx * qx + qy * qy + qz * qz, st.E, -1, tc.oct_lo,
+                                               tc.oct_hi, st.tb, 0, lc);
                     }
                     double l = 0;
                     for (int j = 0; j < kBands; ++j) l += (double)st.E[j];
@@ -674,9 +837,11 @@ k_phase1(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv, int64_t 
                 const R hx = st.o[0] + st.d[0] * leg, hy = st.o[1] + st.d[1] * leg, hz = st.o[2] + st.d[2] * leg;
                 if (st.order > tc.is_order) {
                     const R qx = hx - st.o[0], qy = hy - st.o[1], qz = hz - st.o[2];
-                    check_receivers<R>(rv, room, cand, st.o[0], st.o[1], st.o[2], st.d[0], st.d[1], st.d[2],
-                                       qx * qx + qy * qy + qz * qz, st.E, -1, tc.oct_lo, tc.oct_hi,
-                                       st.tb, 0, lc);
+                    if constexpr (MAP) map_check<R>(rv, st.o, st.d, hx, hy, hz, st.E, -1, st.tb, lc);
+                    else
+                        check_receivers<R>(rv, room, cand, st.o[0], st.o[1], st.o[2], st.d[0], st.d[1], st.d[2],
+                                           qx * qx + qy * qy + qz * qz, st.E, -1, tc.oct_lo, tc.oct_hi,
+                                           st.tb, 0, lc);
                 }
                 st.tb += (double)leg / rv.c;
                 st.o[0] = hx; st.o[1] = hy; st.o[2] = hz;
@@ -765,7 +930,7 @@ k_phase1(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv, int64_t 
 // checks, Absorb, Scatter_Late / Scatter_VeryLate, the phase hand-off.
 // Returns false when the ray terminates; sets has_child when Scatter_Late
 // spawned a specular child.
-template <class R>
+template <class R, bool MAP>
 __device__ __forceinline__ bool octave_shade(const TraceConst &tc, const DevScene<R> &s, const DevRecv<R> &rv,
                                              ORay<R> &r, int ht, R leg, uint64_t cand, bool &has_child,
                                              ORay<R> &child, LaneCounters &lc)
@@ -780,8 +945,10 @@ __device__ __forceinline__ bool octave_shade(const TraceConst &tc, const DevScen
         const R ex = r.o[0] + r.d[0] * R(1000000), ey = r.o[1] + r.d[1] * R(1000000),
                 ez = r.o[2] + r.d[2] * R(1000000);
         const R qx = ex - r.o[0], qy = ey - r.o[1], qz = ez - r.o[2];
-        check_receivers<R>(rv, room, cand, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2],
-                           qx * qx + qy * qy + qz * qz, &r.I, band, 0, 0, r.t, ana, lc);
+        if constexpr (MAP) map_check<R>(rv, r.o, r.d, ex, ey, ez, &r.I, band, r.t, lc);
+        else
+            check_receivers<R>(rv, room, cand, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2],
+                               qx * qx + qy * qy + qz * qz, &r.I, band, 0, 0, r.t, ana, lc);
         lc.lost += (double)r.I;
         lc.v[C_MISS]++;
         return false;
@@ -789,8 +956,10 @@ __device__ __forceinline__ bool octave_shade(const TraceConst &tc, const DevScen
     const R hx = r.o[0] + r.d[0] * leg, hy = r.o[1] + r.d[1] * leg, hz = r.o[2] + r.d[2] * leg;
     {
         const R qx = hx - r.o[0], qy = hy - r.o[1], qz = hz - r.o[2];
-        check_receivers<R>(rv, room, cand, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2],
-                           qx * qx + qy * qy + qz * qz, &r.I, band, 0, 0, r.t, ana, lc);
+        if constexpr (MAP) map_check<R>(rv, r.o, r.d, hx, hy, hz, &r.I, band, r.t, lc);
+        else
+            check_receivers<R>(rv, room, cand, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2],
+                               qx * qx + qy * qy + qz * qz, &r.I, band, 0, 0, r.t, ana, lc);
     }
     r.t += (double)leg / rv.c;
     r.o[0] = hx; r.o[1] = hy; r.o[2] = hz;
@@ -885,7 +1054,7 @@ __device__ __forceinline__ bool octave_shade(const TraceConst &tc, const DevScen
 // queue (kShadeQ items in HBM/L2) and refills at once; whenever 64 items wait
 // (or no lane is walking) every lane shades one, so the shading code runs on
 // full waves, and continuations and spawned rays go onto the wave's stack.
-template <class R, bool LDS, int BS, int OCC, bool SQ>
+template <class R, bool LDS, int BS, int OCC, bool SQ, bool MAP>
 __global__ void __launch_bounds__(BS, OCC > 0 ? OCC : 1)
 k_octave(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv,
          const ORay<R> *__restrict__ in0, const unsigned *n0_ptr, unsigned cap0,
@@ -1046,7 +1215,7 @@ k_octave(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv,
                     const ShadeItem<R> it = wq[qn - k + lane];
                     sr = it.r;
                     lc.v[C_SHADED]++;
-                    alive = octave_shade<R>(tc, s, rv, sr, it.best, it.tbest, it.rmask, has_child, child, lc);
+                    alive = octave_shade<R, MAP>(tc, s, rv, sr, it.best, it.tbest, it.rmask, has_child, child, lc);
                 }
                 qn -= k;
                 push(has_child, child);
@@ -1062,7 +1231,7 @@ k_octave(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv,
                 if (lane == 0) lc.v[C_SHADE]++;
                 if (ready) {
                     lc.v[C_SHADED]++;
-                    act = octave_shade<R>(tc, s, rv, r, w.best, w.tbest, w.rmask, has_child, child, lc);
+                    act = octave_shade<R, MAP>(tc, s, rv, r, w.best, w.tbest, w.rmask, has_child, child, lc);
                     ready = false;
                 }
                 if (__ballot(has_child)) {
@@ -1088,12 +1257,16 @@ hipError_t launch_phase1(hipStream_t st, const TraceConst &tc, const DevScene<R>
     const int n = resume_in ? n_resume : n_fresh;
     if (n <= 0) return hipSuccess;
     const int grid = (n + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(k_phase1<R>, dim3(grid), dim3(kBlock), 0, st, tc, s, rv, src_begin, n_fresh,
-                       resume_in, n_resume, out, out_count, out_cap, resume_out, resume_count, cnt);
+    if (rv.has_map)
+        hipLaunchKernelGGL((k_phase1<R, true>), dim3(grid), dim3(kBlock), 0, st, tc, s, rv, src_begin, n_fresh,
+                           resume_in, n_resume, out, out_count, out_cap, resume_out, resume_count, cnt);
+    else
+        hipLaunchKernelGGL((k_phase1<R, false>), dim3(grid), dim3(kBlock), 0, st, tc, s, rv, src_begin, n_fresh,
+                           resume_in, n_resume, out, out_count, out_cap, resume_out, resume_count, cnt);
     return hipGetLastError();
 }
 
-template <class R, int BS, int OCC, bool SQ>
+template <class R, int BS, int OCC, bool SQ, bool MAP = false>
 static void launch_octave_bs(hipStream_t st, int grid, size_t lds, const TraceConst &tc, const DevScene<R> &s,
                              const DevRecv<R> &rv, const ORay<R> *in0, const unsigned *n0, unsigned cap0,
                              const ORay<R> *in1, const unsigned *n1, unsigned cap1, unsigned *in_head,
@@ -1101,11 +1274,11 @@ static void launch_octave_bs(hipStream_t st, int grid, size_t lds, const TraceCo
                              unsigned *dump_count, unsigned dump_cap, int min_iters, DevCounters *cnt)
 {
     if (lds)
-        hipLaunchKernelGGL((k_octave<R, true, BS, OCC, SQ>), dim3(grid), dim3(BS), lds, st, tc, s, rv, in0, n0,
+        hipLaunchKernelGGL((k_octave<R, true, BS, OCC, SQ, MAP>), dim3(grid), dim3(BS), lds, st, tc, s, rv, in0, n0,
                            cap0, in1, n1, cap1, in_head, stacks, stack_cap, shq, dump, dump_count, dump_cap,
                            min_iters, cnt);
     else
-        hipLaunchKernelGGL((k_octave<R, false, BS, OCC, SQ>), dim3(grid), dim3(BS), 0, st, tc, s, rv, in0, n0,
+        hipLaunchKernelGGL((k_octave<R, false, BS, OCC, SQ, MAP>), dim3(grid), dim3(BS), 0, st, tc, s, rv, in0, n0,
                            cap0, in1, n1, cap1, in_head, stacks, stack_cap, shq, dump, dump_count, dump_cap,
                            min_iters, cnt);
 }
@@ -1130,6 +1303,19 @@ hipError_t launch_octave(hipStream_t st, int grid, const OctShape &sh, const Tra
         else PACH_OCT_Q(BS_, 0)                          \
     }
     if (sh.queue && !shq) return hipErrorInvalidValue;
+    if (rv.has_map) {
+        // mapping receivers: the default launch shape only (workgroups of
+        // 256, no shade queue), with the precision's register cap
+        if (sh.occ == 4)
+            launch_octave_bs<R, 256, 4, false, true>(st, grid, lds, tc, s, rv, in0, n0, cap0, in1, n1, cap1, in_head,
+                                                     stacks, stack_cap, shq, dump, dump_count, dump_cap, min_iters,
+                                                     cnt);
+        else
+            launch_octave_bs<R, 256, 0, false, true>(st, grid, lds, tc, s, rv, in0, n0, cap0, in1, n1, cap1, in_head,
+                                                     stacks, stack_cap, shq, dump, dump_count, dump_cap, min_iters,
+                                                     cnt);
+        return hipGetLastError();
+    }
     if (sh.block == 512) PACH_OCT_O(512)
     else PACH_OCT_O(256)
 #undef PACH_OCT_O
@@ -1142,8 +1328,8 @@ template <class R, int BS, int OCC, bool SQ> static int blocks_per_cu_bs(int lds
 {
     int nb = 0;
     hipError_t e =
-        lds_bytes > 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_octave<R, true, BS, OCC, SQ>, BS, lds_bytes)
-                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_octave<R, false, BS, OCC, SQ>, BS, 0);
+        lds_bytes > 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_octave<R, true, BS, OCC, SQ, false>, BS, lds_bytes)
+                      : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_octave<R, false, BS, OCC, SQ, false>, BS, 0);
     if (e != hipSuccess || nb < 1) nb = 1;
     return nb;
 }

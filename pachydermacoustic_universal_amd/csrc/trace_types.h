@@ -61,6 +61,22 @@ struct RoomTab {
     int pad;
 };
 
+// Mapping receivers (PachMapReceiver, Pach_Map_Receiver.cs): spheres of
+// radius r2^(1/2) culled through their own voxel grid (cells x-major:
+// (x * ct[1] + y) * ct[2] + z, receiver lists sorted ascending).
+struct DevMap {
+    const double4 *ctr;      // [n] centre, w = Direct_Time (|source - centre| / c)
+    const uint32_t *start;   // [ncell + 1] first list entry of each cell
+    const uint32_t *list;    // receiver indices
+    double bmin[3], bmax[3], vd[3];
+    int ct[3];
+    int n;
+    double r2, sizemod, c, rho_c;
+    int directional, sct;
+    double *hist;            // 8 planes (E, P, +x, +y, +z, -x, -y, -z) of [n][8][sct], 1000 Hz
+    uint64_t plane_stride;
+};
+
 template <class R> struct DevRecv {
     const typename V4<R>::T *ctr;  // [R] centre (w unused)
     int n;
@@ -73,6 +89,8 @@ template <class R> struct DevRecv {
     int fs, sct;
     double *hist;         // [kPlanes][n][8][sct]
     uint64_t plane_stride;
+    int has_map;          // 1: the receivers are the map (RecMain = PachMapReceiver)
+    DevMap map;
 };
 
 struct TraceConst {

@@ -1,0 +1,166 @@
+"""Mapping receivers in the oracle (oracle.c or_map_*: PachMapReceiver,
+Pach_Map_Receiver.cs) pinned on the CPU:
+
+* the receiver grid: voxel counts ceil(extent / 4 increment) in x, y and
+  ceil(extent / 2.5 increment) in z over the spheres' bounding box, and each
+  voxel lists exactly the receivers whose sphere reaches into it (brute force
+  over every voxel box);
+* one octave segment through a receiver centre deposits I 10^(-0.1 a t_r) /
+  (pi r^2) at sample floor(1000 t_r), t_r = t_mid / c + t_sum - Direct_Time;
+  a receiver straddling several voxels deposits once; receivers past the end
+  point or behind the origin get nothing;
+* an octave segment that leaves the map deposits in exactly the receivers
+  whose sphere it crosses before its end (brute force over all receivers);
+  one that ends inside the map skips receivers listed only in its end voxel
+  (the walk stops on entering that voxel);
+* the broadband walk steers by the signs of the ray's position (as written):
+  in a room of positive coordinates a ray moving towards -x deposits
+  nothing, and every broadband deposit is a receiver the segment crosses.
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as O
+
+INC = 1.0
+SRC = np.array([2.0, 2.0, 1.5])
+
+
+def medium():
+    return O.Medium(c=343.0, atten=np.linspace(0.001, 0.02, 8), rho=1.2)
+
+
+def plane_map(nx=8, ny=6, z=1.2, x0=3.0, y0=2.5, pitch=1.0, directional=True):
+    xs, ys = np.meshgrid(x0 + pitch * np.arange(nx), y0 + pitch * np.arange(ny), indexing="ij")
+    centers = np.stack([xs.ravel(), ys.ravel(), np.full(xs.size, z)], 1)
+    return O.Map(centers, INC, SRC, medium(), directional=directional, sample_ct=400), centers
+
+
+def test_grid_lists_exactly_the_overlapping_spheres():
+    rng = np.random.default_rng(1)
+    centers = rng.uniform([0, 0, 0], [13, 9, 4], (60, 3))
+    m = O.Map(centers, 0.8, SRC, medium())
+    ct, mn, vd = m.grid()
+    lo, hi = centers.min(0) - 0.4, centers.max(0) + 0.4
+    assert ct == (int(np.ceil((hi[0] - lo[0]) / 3.2)), int(np.ceil((hi[1] - lo[1]) / 3.2)),
+                  int(np.ceil((hi[2] - lo[2]) / 2.0)))
+    np.testing.assert_allclose(mn, lo)
+    for x in range(ct[0]):
+        for y in range(ct[1]):
+            for z in range(ct[2]):
+                bmin = mn + vd * np.array([x, y, z])
+                bmax = bmin + vd
+                d = np.maximum(np.maximum(bmin - centers, 0), centers - bmax)
+                want = np.nonzero((d ** 2).sum(1) < 0.16)[0].tolist()
+                assert m.voxel(x, y, z) == want
+
+
+def test_single_segment_deposit_closed_form():
+    m, centers = plane_map()
+    med = medium()
+    k = 10
+    C = centers[k]
+    o = C + np.array([-5.0, 0.0, 0.0])
+    d = np.array([1.0, 0.0, 0.0])
+    end = C + np.array([30.0, 0.0, 0.0])
+    pl = m.planes()
+    t_sum, I, band = 0.013, 2.5, 3
+    m.check_octave(pl, o, d, end, I, band, t_sum)
+    # receivers on the same row are crossed too; look at k only
+    tr = 5.0 / med.c + t_sum - np.linalg.norm(SRC - C) / med.c
+    want = I * 10 ** (-0.1 * med.atten[band] * tr) / (np.pi * 0.25)
+    s = int(tr * 1000)
+    assert pl[0, k, band, s] == pytest.approx(want, rel=1e-12)
+    assert pl[0, k].sum() == pytest.approx(want, rel=1e-12)  # once, in one bin
+    assert pl[1, k, band, s] == pytest.approx(np.sqrt(want * med.rho_c), rel=1e-12)
+    assert pl[5, k, band, s] == pytest.approx(np.float32(-want), rel=1e-6)  # -x: arrival from -x side
+    # the segment stops 1 m before the receiver: nothing
+    pl2 = m.planes()
+    m.check_octave(pl2, o, d, C + np.array([-1.0, 0, 0]), I, band, t_sum)
+    assert pl2[0, k].sum() == 0
+    # starts past it: nothing
+    pl3 = m.planes()
+    m.check_octave(pl3, C + np.array([0.6, 0, 0]), d, end, I, band, t_sum)
+    assert pl3[0, k].sum() == 0
+
+
+def crossed(centers, r, o, d, end):
+    m = o - centers
+    b = m @ d
+    c = (m * m).sum(1) - r * r
+    disc = b * b - c
+    ok = ~((c > 0) & (b > 0)) & (disc >= 0)
+    t = -b
+    L2 = ((end - o) ** 2).sum()
+    return set(np.nonzero(ok & (t > 0) & (t * t < L2))[0].tolist())
+
+
+def test_octave_walk_that_leaves_the_map_matches_brute_force():
+    m, centers = plane_map()
+    rng = np.random.default_rng(3)
+    n_hit = 0
+    for _ in range(400):
+        o = rng.uniform([0, 0, 0], [14, 10, 4])
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        end = o + 60 * d  # far outside the map box
+        pl = m.planes()
+        m.check_octave(pl, o, d, end, 1.0, 0, 0.1)
+        got = set(np.nonzero(pl[0, :, 0].sum(1) > 0)[0].tolist())
+        # crossed receivers whose arrival falls inside the 400 recorded samples
+        want = {i for i in crossed(centers, 0.5, o, d, end)
+                if 0 <= int((0.1 + (-(o - centers[i]) @ d) / 343.0 - np.linalg.norm(SRC - centers[i]) / 343.0)
+                            * 1000) < 400}
+        assert got == want
+        n_hit += len(got)
+    assert n_hit > 50
+
+
+def test_octave_walk_skips_its_end_voxel():
+    m, centers = plane_map(nx=12, ny=1, x0=0.5, y0=0.5, pitch=1.0)
+    ct, mn, vd = m.grid()
+    assert ct[0] == 3  # voxels of 4 m along x
+    o = np.array([0.5, 0.5, 1.2]) - np.array([0.49, 0, 0])
+    d = np.array([1.0, 0, 0])
+    end = np.array([10.3, 0.5, 1.2])  # inside the third voxel (x in [8, 12))
+    pl = m.planes()
+    m.check_octave(pl, o, d, end, 1.0, 0, 0.1)
+    got = set(np.nonzero(pl[0].sum((1, 2)) > 0)[0].tolist())
+    # receivers 0..9 have midpoints before the end; those listed only in the
+    # last voxel (centres 8.5, 9.5) are skipped; 7.5 reaches into voxel 2 (r 0.5)
+    assert got == {0, 1, 2, 3, 4, 5, 6, 7}
+
+
+def test_broadband_walk_steers_by_position_signs():
+    m, centers = plane_map()
+    E = np.ones(8)
+    C = centers[12]
+    pl = m.planes()
+    m.check_broadband(pl, C + np.array([5.0, 0, 0]), np.array([-1.0, 0, 0]), C - np.array([5.0, 0, 0]), E, 0.1)
+    assert pl.sum() == 0  # crosses receiver 12, deposits nothing
+    rng = np.random.default_rng(4)
+    n = 0
+    for _ in range(300):
+        o = rng.uniform([0, 0, 0], [14, 10, 4])
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        end = o + 40 * d
+        pl = m.planes()
+        m.check_broadband(pl, o, d, end, E, 0.1)
+        got = set(np.nonzero(pl[0].sum((1, 2)) > 0)[0].tolist())
+        assert got <= crossed(centers, 0.5, o, d, end)
+        n += len(got)
+    assert n > 0
+
+
+def test_trace_with_a_map_deposits_and_conserves_counts():
+    from pachydermacoustic_universal_amd import rooms
+    room = rooms.shoebox(14, 10, 4, alpha=0.2, scat=0.3, source=tuple(SRC), receivers=[(7, 5, 1.2)])
+    m, centers = plane_map()
+    med = O.standard_medium()
+    m = O.Map(centers, INC, SRC, med, sample_ct=500)
+    pl, cnt = O.trace(room, rays=300, cutoff_s=0.5, seed=2, medium=med, map_rx=m, grid=(8, 8, 8))
+    assert pl.shape == (8, len(centers), 8, 500)
+    assert cnt["deposits"] > 0 and (pl[0].sum((1, 2)) > 0).all()
+    pl2, _ = O.trace(room, rays=300, cutoff_s=0.5, seed=2, medium=med, map_rx=m, grid=(8, 8, 8), threads=3)
+    np.testing.assert_allclose(pl2, pl, rtol=1e-12, atol=0)
