@@ -86,7 +86,9 @@ def main():
     ap.add_argument("--is-order", type=int, default=3, help="image_sources workload: reflection order")
     ap.add_argument("--is-cpu-objects", type=int, default=16,
                     help="image_sources workload: first objects of the CPU sample's sequences")
-    ap.add_argument("--workload", choices=["hall", "batch", "image_sources", "filters"], default="hall",
+    ap.add_argument("--map-increment", type=float, default=0.5,
+                    help="map workload: receiver spacing and diameter (m)")
+    ap.add_argument("--workload", choices=["hall", "batch", "image_sources", "filters", "map"], default="hall",
                     help="hall: the 2k-triangle hall; batch: 256 packed rooms in one launch sequence")
     ap.add_argument("--batch-ray-scale", type=float, default=0.01,
                     help="batch workload: scale of each room's log-uniform [1e4, 1e6] ray count")
@@ -138,6 +140,10 @@ def main():
     tr.upload_scene(room.verts, room.surf, room.alpha, room.scat, room.trans, grid)
     tr.set_source(room.source, np.ones(8), A.DIR_GEODESIC)
     tr.set_receivers(room.receivers, room.radius, at, c, c * rho, sr, sct)
+    map_ctr = None
+    if args.workload == "map":
+        map_ctr = hall_map(room, args.map_increment)
+        tr.set_map(map_ctr, args.map_increment, True, sct)
     if args.workload == "batch":
         tr.set_rooms(packed.sources, np.ones((len(members), 8)), room_rays, packed.surf_count, packed.rec_count)
     trace_rays = 0 if args.workload == "batch" else n_rays
@@ -197,10 +203,11 @@ def main():
                 nr += k
             what = f"{nr} source rays spread over the 256 rooms in proportion"
         else:
+            omap = O.Map(map_ctr, args.map_increment, room.source, med, True, sct) if map_ctr is not None else None
             _, cnt = O.trace(room, rays=args.cpu_rays, cutoff_s=args.cutoff, seed=args.seed,
-                             medium=med, grid=grid, threads=threads)
+                             medium=med, grid=grid, threads=threads, map_rx=omap)
             cb = cnt["bounces_broad"] + cnt["bounces_late"] + cnt["bounces_verylate"]
-            what = f"{args.cpu_rays} source rays of the same hall workload"
+            what = f"{args.cpu_rays} source rays of the same {'map' if omap else 'hall'} workload"
         cdt = time.perf_counter() - t1
         cpu = {"value": cb / cdt, "unit": "ray-bounces/s", "cores": threads, "kind": "port",
                "sample": f"{what} ({cb:.3e} bounces, {cdt:.1f} s), C FP64 restatement, "
@@ -210,6 +217,10 @@ def main():
         workload_name = (f"256 packed rooms (shoeboxes 3-30 m and convex hulls of 12-500 tris), "
                          f"4 receivers each, {n_rays} rays (log-uniform [1e4, 1e6] x {args.batch_ray_scale}), "
                          f"{args.cutoff} s cut-off")
+    elif args.workload == "map":
+        workload_name = (f"seeded polyhedral hall, ~2000 tris, {len(map_ctr)} mapping receivers "
+                         f"({args.map_increment} m grid over the audience area, 1.2 m up), {n_rays} rays, "
+                         f"{args.cutoff} s cut-off, {tr_grid[0]}x{tr_grid[1]}x{tr_grid[2]} grid")
     else:
         workload_name = (f"seeded polyhedral hall, ~2000 tris, 16 receivers, {n_rays} rays, "
                          f"{args.cutoff} s cut-off, {tr_grid[0]}x{tr_grid[1]}x{tr_grid[2]} grid")
@@ -233,6 +244,17 @@ def main():
     }
     print(json.dumps(out))
     tr.close()
+
+
+def hall_map(room, inc):
+    """Mapping receivers of the hall: a square grid of spacing inc at the
+    height of its receivers, over their plan extent."""
+    R = np.asarray(room.receivers)
+    lo, hi = R.min(0), R.max(0)
+    xs = np.arange(lo[0], hi[0] + 1e-9, inc)
+    ys = np.arange(lo[1], hi[1] + 1e-9, inc)
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    return np.stack([X.ravel(), Y.ravel(), np.full(X.size, R[:, 2].mean())], 1)
 
 
 def bench_image_sources(args, A, rooms):

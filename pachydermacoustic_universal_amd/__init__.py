@@ -16,6 +16,7 @@ from .pachyderm import (  # noqa: F401
     ImageSourceData,
     Lambert_Scattering,
     Pach_SP,
+    PachMapReceiver,
     Polygon_Scene,
     RandomSource,
     Receiver_Bank,
@@ -28,6 +29,6 @@ from .pachyderm import (  # noqa: F401
 )
 
 __all__ = ["AcousticalMath", "Basic_Material", "Direct_Sound", "GeodesicSource", "IR_Construction",
-           "ImageSourceData", "Lambert_Scattering", "Pach_SP", "Specular_Path",
+           "ImageSourceData", "Lambert_Scattering", "Pach_SP", "PachMapReceiver", "Specular_Path",
            "Polygon_Scene", "RandomSource", "Receiver_Bank", "Running", "Simulation_Type",
            "SplitRayTracer", "Stopped", "SuppliedDirectionSource", "rooms"]

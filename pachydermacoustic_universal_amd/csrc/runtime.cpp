@@ -1011,6 +1011,7 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
     batch = std::max<int64_t>(1, std::min<int64_t>(batch, 1 << 22));
     OctShape shape = c->tune_shape;
     if (shape.occ < 0) shape.occ = sizeof(R) == 4 ? 4 : 0;
+    if (c->map_on) shape = OctShape{256, 0, 0};  // the map variant's launch shape
     const int grid = c->n_cu * octave_blocks_per_cu<R>(shape, s.dist_lds ? lds_bytes : 0);
     int rc = ensure_work<R>(c, batch, (size_t)grid * (shape.block / 64), shape.queue != 0);
     if (rc) return rc;

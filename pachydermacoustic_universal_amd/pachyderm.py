@@ -296,6 +296,110 @@ class Receiver_Bank:
         return self.planes[5:8, Rec_Index, Octave].copy()
 
 
+# ---------------------------------------------------------------- mapping receivers
+class PachMapReceiver(Receiver_Bank):
+    """Mapping receivers (Pach_Map_Receiver.cs:34-1690) over a mesh.
+
+    Map_Mesh: (vertices [V][3], polygons: lists of 3 or 4 vertex indices).
+    Receivers sit at the polygon centres offset by the unit polygon normal,
+    or (RecOnVertex) at the vertices offset by increment along the vertex
+    normal, or with Z_displacement at centre / vertex + increment / 2 in z
+    (Partition :99-181); spheres of diameter Increment, recording at 1000 Hz
+    (the rate the reference fixes), times relative to each receiver's direct
+    arrival.  A SplitRayTracer with this bank as RecMain traces into the map
+    (pach_map_receivers_set).  Time1Pt (single-sample histograms) is not
+    provided."""
+
+    def __init__(self, Map_Mesh, Src_Pt, SampleRate_in, Increment_in, Sc, Cutoff_time, Time1Pt=False,
+                 Z_displacementIn=False, DirectionalIn=True, RecOnVertex=False, Offset_Mesh=False):
+        if Time1Pt:
+            raise NotImplementedError("single-sample map histograms (Time1Pt) are not provided")
+        V = np.asarray(Map_Mesh[0], dtype=np.float64).reshape(-1, 3)
+        polys = [list(p) for p in Map_Mesh[1]]
+        self.increment = float(Increment_in)
+        self.Directional = bool(DirectionalIn)
+        self.Z_Displacement = bool(Z_displacementIn)
+        self.Rec_Vertex = bool(RecOnVertex)
+        self.Mesh_Offset = bool(Offset_Mesh)
+        self.Src = np.asarray(Src_Pt.Origin, dtype=np.float64)
+        self.CutOffTime = float(Cutoff_time)
+        self.SampleRate = 1000
+        self.SampleCT = int(np.floor(self.CutOffTime * self.SampleRate / 1000))
+        self.SWL = Src_Pt.SWL()
+        self.scene = Sc
+        self.Radius = self.increment * 0.5
+        normals = []
+        for p in polys:
+            n = np.cross(V[p[1]] - V[p[0]], V[p[2]] - V[p[0]])
+            normals.append(n / np.linalg.norm(n))
+        normals = np.array(normals)
+        if self.Rec_Vertex:
+            vn = np.zeros_like(V)
+            for p, n in zip(polys, normals):
+                for v in p:
+                    vn[v] += n
+            vn /= np.maximum(np.linalg.norm(vn, axis=1, keepdims=True), 1e-300)
+            pts = V + (np.array([0, 0, self.increment * .5]) if self.Z_Displacement else vn * self.increment)
+        else:
+            centers = np.array([V[p].mean(0) for p in polys])
+            pts = centers + (np.array([0, 0, self.increment * .5]) if self.Z_Displacement else normals)
+        self.Origins = pts
+        self.Map_Mesh = (V, polys)
+        self.Direct_Time = np.linalg.norm(self.Src - pts, axis=1) / Sc.Sound_speed(0)
+        self.planes = np.zeros((8, len(pts), 8, self.SampleCT))
+        self.delay_ms = 0.0
+
+    @property
+    def Count(self):
+        return len(self.Origins)
+
+    @staticmethod
+    def Get_SPL_Map(Rec_List, T_Bounds, Octave, SrcID, Coherent_Superposition=False, ZeroAtDirect=False):
+        """Incoherent SPL map (Pach_Map_Receiver.cs:410-540, the
+        Coherent_Superposition = false branch): the energy of samples
+        [floor(T_Bounds[1]), floor(T_Bounds[0])) of each receiver's histogram,
+        summed over the sources with their arrival offsets, each source's
+        histogram weighted by Pressure_Intensity(1) = sqrt(405) as written."""
+        if Coherent_Superposition:
+            raise NotImplementedError("the coherent map needs the pressure filters of each receiver")
+        SrcID = list(SrcID)
+        first = Rec_List[0]
+        out = np.zeros(first.Count)
+        t_max, t_min = int(np.floor(T_Bounds[0])), int(np.floor(T_Bounds[1]))
+        w = np.sqrt(1.0 * 405.0)
+        for i in range(first.Count):
+            hist = np.zeros(first.SampleCT)
+            zero = 0
+            if ZeroAtDirect:
+                zero = int(min(Rec_List[s].Direct_Time[i] for s in SrcID) * first.SampleRate)
+            for s in SrcID:
+                temp = Rec_List[s].GetEnergyHistogram(Octave, Rec_List[s].delay_ms, i)
+                arrival = int(Rec_List[s].Direct_Time[i] * first.SampleRate) - zero
+                for j in range(temp.size):
+                    t = j + arrival
+                    if hist.size == 1:
+                        hist[0] += temp[j] * w
+                    if t >= hist.size:
+                        break
+                    hist[t] += temp[j] * w
+            E = hist[t_min:t_max].sum()
+            out[i] = AcousticalMath.SPL_Intensity(E)
+        return out
+
+    @staticmethod
+    def Get_Clarity_Map(Rec_List, C_Cutoff, Octave, SrcID, Coherent_Superposition=False):
+        """Incoherent clarity map (:925-991): Clarity of each receiver's
+        energy-time curve from the start of its histogram."""
+        if Coherent_Superposition:
+            raise NotImplementedError("the coherent map needs the pressure filters of each receiver")
+        first = Rec_List[0]
+        out = np.zeros(first.Count)
+        for i in range(first.Count):
+            etc = sum(Rec_List[s].GetEnergyHistogram(Octave, 0.0, i) for s in SrcID)
+            out[i] = AcousticalMath.Clarity(etc, first.SampleRate, C_Cutoff / 1000.0, 0.0, False)
+        return out
+
+
 # ---------------------------------------------------------------- tracer
 class Simulation_Type:
     """Abstract simulation contract (Classes_Simulation_Type.cs:27-35)."""
@@ -356,8 +460,15 @@ class SplitRayTracer(Simulation_Type):
         r, s, rb = self.Room, self.Source, self.RecMain
         self._tr.upload_scene(r._tris, r._surf, r._alpha, r._scat, r._trans, r.grid)
         self._tr.set_source(s.Origin, s.SoundPower, s.dir_mode, s.dirs)
-        self._tr.set_receivers(rb.Origins, rb.radius, r.atten, r.c, r.rho * r.c, rb.SampleRate,
-                               rb.SampleCT)
+        if isinstance(rb, PachMapReceiver):
+            # the medium and cut-off still come through the receiver set-up
+            self._tr.set_receivers(rb.Origins[:1], rb.Radius, r.atten, r.c, r.rho * r.c, 1000, rb.SampleCT)
+            self._tr.set_map(rb.Origins, rb.increment, rb.Directional, rb.SampleCT)
+        else:
+            if getattr(self._tr, "map_n", 0):
+                self._tr.clear_map()
+            self._tr.set_receivers(rb.Origins, rb.radius, r.atten, r.c, r.rho * r.c, rb.SampleRate,
+                                   rb.SampleCT)
         self._st = time.time()
         self._started = True
         self._tr.trace(rays=self.Raycount, cutoff_m=self._cutoff, seed=self.seed, mode=self.mode,
@@ -389,7 +500,10 @@ class SplitRayTracer(Simulation_Type):
             raise A.PachError(rc, A.lib().pach_last_error(self._tr.h).decode())
         self._stats = self._tr.stats()
         n = max(self._stats["rays_done"], 1)
-        self.RecMain.planes = self._tr.planes(scaled=False)
+        if isinstance(self.RecMain, PachMapReceiver):
+            self.RecMain.planes = self._tr.map_planes(scaled=False)
+        else:
+            self.RecMain.planes = self._tr.planes(scaled=False)
         self.RecMain.Scale(n)
         self._lost_percent = self._tr.percent_lost()
 
@@ -674,6 +788,13 @@ class AcousticalMath:
     @staticmethod
     def EarlyDecayTime(Schroeder_Integral, sample_f):
         return A.early_decay_time(Schroeder_Integral, sample_f)
+
+    @staticmethod
+    def SPL_Intensity(Intensity):
+        """Classes_Math.cs:97-102: 10 log10(I / 1e-12), 0 below 0 dB."""
+        with np.errstate(divide="ignore"):
+            spl = 10 * np.log10(Intensity / 1e-12)
+        return 0.0 if not spl >= 0 else float(spl)
 
     @staticmethod
     def Clarity(etc, sample_f, seconds, startTime, pressure=False):

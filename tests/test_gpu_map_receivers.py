@@ -107,3 +107,41 @@ def test_fp32_map_close_to_fp64(tracer):
     # spherical receivers are back
     tracer.trace(rays=2000, cutoff_m=0.6 * c, seed=9, precision=A.PREC_FP64)
     assert tracer.histogram(0).sum() > 0
+
+
+def test_split_ray_tracer_with_a_map_mesh():
+    """PachMapReceiver over a 6 x 4 quad mesh at 1.2 m as RecMain of a
+    SplitRayTracer: the map histograms equal a direct pach_map_receivers_set
+    trace with the same receivers and seed, and the SPL map is the energy sum
+    of each receiver's window in dB."""
+    print("[case] map receivers through SplitRayTracer", flush=True)
+    from pachydermacoustic_universal_amd import pachyderm as P
+    room = box()
+    sc = P.Polygon_Scene.from_room(room)
+    src = P.GeodesicSource([120] * 8, room.source)
+    nx, ny = 6, 4
+    V = np.array([(3 + 1.5 * i, 2 + 1.5 * j, 0.2) for i in range(nx + 1) for j in range(ny + 1)])
+    quads = [[i * (ny + 1) + j, (i + 1) * (ny + 1) + j, (i + 1) * (ny + 1) + j + 1, i * (ny + 1) + j + 1]
+             for i in range(nx) for j in range(ny)]
+    mr = P.PachMapReceiver((V, quads), src, 1000, 1.0, sc, 500.0)
+    np.testing.assert_allclose(mr.Origins[:, 2], 1.2)  # polygon centres + unit normal (+z)
+    rt = P.SplitRayTracer(src, mr, sc, 0.5 * sc.Sound_speed(0), [0, 7], -1, 5000, precision="fp64", seed=4)
+    rt.Begin()
+    rt.Combine_ThreadLocal_Results()
+    got = mr.planes[0]
+    with A.Tracer(0) as tr:
+        at, c, rho = A.uniform_medium()
+        tr.upload_scene(room.verts, room.surf, room.alpha, room.scat, room.trans, (0, 0, 0))
+        tr.set_source(room.source, src.SoundPower, A.DIR_GEODESIC)
+        tr.set_receivers(mr.Origins[:1], 0.5, at, c, c * rho, 1000, mr.SampleCT)
+        tr.set_map(mr.Origins, 1.0, True, mr.SampleCT)
+        tr.trace(rays=5000, cutoff_m=0.5 * c, seed=4, precision=A.PREC_FP64)
+        want = tr.map_histogram(0, scaled=True)
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=0)
+    spl = P.PachMapReceiver.Get_SPL_Map([mr], [500, 0], 8, [0])
+    E = want.sum(1)  # all bands
+    for i in range(mr.Count):
+        arrival = int(mr.Direct_Time[i] * 1000)
+        e = E[i][: max(0, 500 - arrival)].sum() * np.sqrt(405.0)
+        assert spl[i] == pytest.approx(10 * np.log10(e / 1e-12), rel=1e-12)
+    rt.close()
