@@ -131,19 +131,18 @@ __device__ __forceinline__ bool tri_hit_dual(const DevScene<float> &s, uint32_t 
 {
     const float4 *rows = s.cell_rows + 3 * (size_t)j;
     const float4 r2 = rows[0];
+    const float4 r0 = rows[1];
+    const float4 r1 = rows[2];
     const float den = r2.x * dx + r2.y * dy + r2.z * dz;
     const float num = -(r2.x * ox + r2.y * oy + r2.z * oz + r2.w);
     const float tt = num * __builtin_amdgcn_rcpf(den);
-    if (!(tt > 0.0f && tt <= tlim)) return false;
     const float px = ox + dx * tt, py = oy + dy * tt, pz = oz + dz * tt;
-    const float4 r0 = rows[1];
     const float u = r0.x * px + r0.y * py + r0.z * pz + r0.w;
-    if (u < -4e-6f || u > 1.000004f) return false;
-    const float4 r1 = rows[2];
     const float v = r1.x * px + r1.y * py + r1.z * pz + r1.w;
-    if (v < -4e-6f || u + v > 1.000004f) return false;
-    tout = tt;
-    return true;
+    const bool ok = (tt > 0.0f && tt <= tlim) && !(u < -4e-6f || u > 1.000004f) &&
+                    !(v < -4e-6f || u + v > 1.000004f);
+    tout = ok ? tt : tout;
+    return ok;
 }
 
 // Triangle t, entry j of the voxel lists.
@@ -183,11 +182,15 @@ template <class R> struct Walk {
     int k;            // flattened walk: distance value of the current voxel
 };
 
+// t where the ray leaves voxel c along one axis: the upper face when d > 0,
+// the lower one when d < 0, never when d == 0.  Written as selects so a wave
+// whose lanes move in different directions runs one straight-line sequence.
 template <class R>
 __device__ __forceinline__ R axis_exit(R o, R d, R inv, int c, const R gmin, const R cs)
 {
     const R INF = R(__builtin_huge_val());
-    return d > R(0) ? (gmin + (c + 1) * cs - o) * inv : (d < R(0) ? (gmin + c * cs - o) * inv : INF);
+    const R v = (gmin + R(c + (d > R(0) ? 1 : 0)) * cs - o) * inv;
+    return d != R(0) ? v : INF;
 }
 
 template <class R>
@@ -286,6 +289,23 @@ __device__ bool walk_steps(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R
     return false;
 }
 
+// Enter the walk's current voxel: its distance value, and for an occupied
+// voxel its triangle-list range and receiver mask.
+template <class R, bool LDS>
+__device__ __forceinline__ void walk_enter(const DevScene<R> &s, Walk<R> &w, uint32_t &n_vox)
+{
+    ++n_vox;
+    const int cell = (w.cz * s.gy + w.cy) * s.gx + w.cx;
+    w.k = cell_dist<LDS>(s.cell_dist, cell);
+    w.j = w.je = 0;
+    if (w.k == 0) {
+        if (s.cell_rmask) w.rmask |= s.cell_rmask[cell];
+        const uint2 rg = s.cell_range[cell];
+        w.j = rg.x;
+        w.je = rg.x + rg.y;
+    }
+}
+
 // Flattened form of walk_steps: each of the `budget` units either tests one
 // triangle of the current voxel or leaves the voxel (one step, or one jump
 // across an empty block), so lanes in triangle-dense voxels do not hold the
@@ -298,20 +318,8 @@ template <class R, bool LDS, bool SCHED = false>
 __device__ bool walk_units(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R dx, R dy, R dz, int excl,
                            int budget, uint32_t &n_vox, uint32_t &n_test)
 {
+    if (w.k < 0) walk_enter<R, LDS>(s, w, n_vox);  // the first voxel of a new walk
     for (int it = 0; it < budget; ++it) {
-        if (w.k < 0) {
-            // enter the current voxel
-            ++n_vox;
-            const int cell = (w.cz * s.gy + w.cy) * s.gx + w.cx;
-            w.k = cell_dist<LDS>(s.cell_dist, cell);
-            w.j = w.je = 0;
-            if (w.k == 0) {
-                if (s.cell_rmask) w.rmask |= s.cell_rmask[cell];
-                const uint2 rg = s.cell_range[cell];
-                w.j = rg.x;
-                w.je = rg.x + rg.y;
-            }
-        }
         const R texit = fmin(w.tmx, fmin(w.tmy, w.tmz));
         const bool pending = w.j < w.je;
         if constexpr (SCHED) {
@@ -321,21 +329,47 @@ __device__ bool walk_units(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R
             if (pending != tests_now) continue;  // this lane's kind of unit waits
         }
         if (pending) {
-            ++n_test;
-            const uint32_t jj = w.j++;
-            const int t = (int)s.cell_tris[jj];
             const R tlim = fmin(w.tbest, texit * R(1.0000001) + R(1e-6));
-            R tt;
-            if (t != w.best && !(Tmin<R>::excl && t == excl) && tri_hit<R>(s, t, jj, ox, oy, oz, dx, dy, dz, tlim, tt) &&
-                (tt < w.tbest || (tt == w.tbest && t < w.best))) {
-                w.tbest = tt;
-                w.best = t;
+            // the tests first: their row loads then need not wait for the index loads
+            auto take = [&](bool hit, int t, R tt) {
+                if (hit && t != w.best && !(Tmin<R>::excl && t == excl) &&
+                    (tt < w.tbest || (tt == w.tbest && t < w.best))) {
+                    w.tbest = tt;
+                    w.best = t;
+                }
+            };
+            if constexpr (sizeof(R) == 4) {
+                // two list entries per unit (the last one twice when one is
+                // left: a repeated triangle never replaces itself), merged in
+                // list order, as two units would
+                const uint32_t j0 = w.j;
+                const bool two = j0 + 1 < w.je;
+                const uint32_t j1 = two ? j0 + 1 : j0;
+                w.j = j0 + (two ? 2u : 1u);
+                n_test += two ? 2u : 1u;
+                const int t0 = (int)s.cell_tris[j0];
+                const int t1 = (int)s.cell_tris[j1];
+                R tt0, tt1;
+                const bool h0 = tri_hit<R>(s, t0, j0, ox, oy, oz, dx, dy, dz, tlim, tt0);
+                const bool h1 = tri_hit<R>(s, t1, j1, ox, oy, oz, dx, dy, dz, tlim, tt1);
+                take(h0, t0, tt0);
+                take(h1, t1, tt1);
+            } else {
+                ++n_test;
+                const uint32_t jj = w.j++;
+                const int t = (int)s.cell_tris[jj];
+                R tt;
+                const bool h = tri_hit<R>(s, t, jj, ox, oy, oz, dx, dy, dz, tlim, tt);
+                take(h, t, tt);
             }
             continue;
         }
         if (w.best >= 0 && w.tbest <= texit) return true;
-        if (w.k >= 2) {
-            const int h = w.k - 1;
+        {
+            // one move: across the empty block around the voxel (k >= 2), else
+            // one voxel (the block is the voxel itself), the same straight-line
+            // code for both so stepping and jumping lanes do not diverge
+            const int h = max(w.k - 1, 0);
             const int lx = max(w.cx - h, 0), hx = min(w.cx + h, s.gx - 1);
             const int ly = max(w.cy - h, 0), hy = min(w.cy + h, s.gy - 1);
             const int lz = max(w.cz - h, 0), hz = min(w.cz + h, s.gz - 1);
@@ -348,31 +382,22 @@ __device__ bool walk_units(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R
             const int fx = (int)floor((px - s.gmin[0]) * s.inv_cs[0]);
             const int fy = (int)floor((py - s.gmin[1]) * s.inv_cs[1]);
             const int fz = (int)floor((pz - s.gmin[2]) * s.inv_cs[2]);
-            int nx = dx > R(0) ? min(max(fx, w.cx), hx) : (dx < R(0) ? max(min(fx, w.cx), lx) : w.cx);
-            int ny = dy > R(0) ? min(max(fy, w.cy), hy) : (dy < R(0) ? max(min(fy, w.cy), ly) : w.cy);
-            int nz = dz > R(0) ? min(max(fz, w.cz), hz) : (dz < R(0) ? max(min(fz, w.cz), lz) : w.cz);
-            if (ax == 0) nx = dx > R(0) ? hx + 1 : lx - 1;
-            else if (ax == 1) ny = dy > R(0) ? hy + 1 : ly - 1;
-            else nz = dz > R(0) ? hz + 1 : lz - 1;
-            if (nx < 0 || nx >= s.gx || ny < 0 || ny >= s.gy || nz < 0 || nz >= s.gz) return true;
+            // never move backwards; stay inside the block on the other axes
+            // (clamp to [c, h] moving up, [l, c] moving down, c when still)
+            int nx = min(max(fx, dx < R(0) ? lx : w.cx), dx > R(0) ? hx : w.cx);
+            int ny = min(max(fy, dy < R(0) ? ly : w.cy), dy > R(0) ? hy : w.cy);
+            int nz = min(max(fz, dz < R(0) ? lz : w.cz), dz > R(0) ? hz : w.cz);
+            nx = ax == 0 ? (dx > R(0) ? hx + 1 : lx - 1) : nx;
+            ny = ax == 1 ? (dy > R(0) ? hy + 1 : ly - 1) : ny;
+            nz = ax == 2 ? (dz > R(0) ? hz + 1 : lz - 1) : nz;
+            if ((unsigned)nx >= (unsigned)s.gx || (unsigned)ny >= (unsigned)s.gy || (unsigned)nz >= (unsigned)s.gz)
+                return true;
             w.cx = nx; w.cy = ny; w.cz = nz;
             w.tmx = axis_exit<R>(ox, dx, w.ix, nx, s.gmin[0], s.cs[0]);
             w.tmy = axis_exit<R>(oy, dy, w.iy, ny, s.gmin[1], s.cs[1]);
             w.tmz = axis_exit<R>(oz, dz, w.iz, nz, s.gmin[2], s.cs[2]);
-        } else if (w.tmx < w.tmy && w.tmx < w.tmz) {
-            w.cx += dx > R(0) ? 1 : -1;
-            if (w.cx < 0 || w.cx >= s.gx) return true;
-            w.tmx += s.cs[0] * fabs(w.ix);
-        } else if (w.tmy < w.tmz) {
-            w.cy += dy > R(0) ? 1 : -1;
-            if (w.cy < 0 || w.cy >= s.gy) return true;
-            w.tmy += s.cs[1] * fabs(w.iy);
-        } else {
-            w.cz += dz > R(0) ? 1 : -1;
-            if (w.cz < 0 || w.cz >= s.gz) return true;
-            w.tmz += s.cs[2] * fabs(w.iz);
         }
-        w.k = -1;
+        walk_enter<R, LDS>(s, w, n_vox);
     }
     return false;
 }
