@@ -6,11 +6,15 @@
 // GEMM tiling: a 256-thread workgroup computes a 128 x 128 tile of C; each of
 // its four wave64s owns a 64 x 64 quarter as 2 x 2 MFMA 32x32 blocks.  K is
 // consumed 64 at a time through LDS, where both operands are kept k-contiguous
-// (rows padded to 72 elements) so each lane fetches its 8-element MFMA operand
-// with one 16-byte read; the global loads of the next K tile are issued before
-// the MFMAs of the current one, so they are in flight while it computes.
-// Transposed operands are transposed on the way into LDS.  Weight gradients (K = every patch of the batch) split K over the grid
-// and sum the partial tiles in a second kernel, so results are deterministic.
+// (8 chunks of 8 per row, rows padded to 72 elements, chunk positions
+// XOR-swizzled by row so neither the writes nor the MFMA operand reads
+// conflict on banks); each lane fetches its 8-element MFMA operand with one
+// 16-byte read.  An operand stored with k as its row index is read in 8 x 8
+// blocks and transposed in registers, so it too is written 16 bytes at a time.
+// The global loads of the next K tile are issued before the MFMAs of the
+// current one.  Tiles are dealt to workgroups in an XCD-aware order.  Weight
+// gradients (K = every patch of the batch) split K over the grid and sum the
+// partial tiles in a second kernel, so results are deterministic.
 #include <hip/hip_runtime.h>
 
 #include "../../include/pach.h"
@@ -40,16 +44,20 @@ __device__ __forceinline__ uint16_t f2bf(float f)
     return (uint16_t)(u >> 16);
 }
 
+// tanh through one exponential: tanh y = 1 - 2 / (e^(2y) + 1), which saturates
+// to +-1 for large |y| (e^(2y) overflows to inf or underflows to 0)
+__device__ __forceinline__ float tanh_fast(float y) { return 1.0f - 2.0f / (__expf(2.0f * y) + 1.0f); }
+
 __device__ __forceinline__ float gelu(float x)
 {
     const float k = 0.7978845608028654f;  // sqrt(2 / pi)
-    return 0.5f * x * (1.0f + tanhf(k * (x + 0.044715f * x * x * x)));
+    return 0.5f * x * (1.0f + tanh_fast(k * (x + 0.044715f * x * x * x)));
 }
 
 __device__ __forceinline__ float gelu_grad(float x)
 {
     const float k = 0.7978845608028654f;
-    const float t = tanhf(k * (x + 0.044715f * x * x * x));
+    const float t = tanh_fast(k * (x + 0.044715f * x * x * x));
     return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * k * (1.0f + 3.0f * 0.044715f * x * x);
 }
 
@@ -72,7 +80,7 @@ struct Epi {
 // [z * kchunk, (z + 1) * kchunk), kchunk a multiple of kBK.  With `partial`
 // the raw tile goes to partial[z] and the epilogue is skipped.
 template <bool TA, bool TB>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kThreads, 3)
 k_gemm(const uint16_t *__restrict__ A, int lda, const uint16_t *__restrict__ B, int ldb, int M, int N, int K,
        int kchunk, Epi epi, float *__restrict__ partial)
 {
@@ -80,8 +88,20 @@ k_gemm(const uint16_t *__restrict__ A, int lda, const uint16_t *__restrict__ B, 
     __shared__ __attribute__((aligned(16))) uint16_t Bs[kBN * kLdsK];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wm = wv >> 1, wn = wv & 1;
-    const int m0 = blockIdx.y * kBM, n0 = blockIdx.x * kBN;
-    const int kb = blockIdx.z * kchunk, ke = min(K, kb + kchunk);
+    // XCD-aware tile order.  Workgroups are dealt to the 8 XCDs round-robin
+    // in launch order, and each XCD has its own L2; tiles that share operand
+    // rows (the N tiles of one M tile, the tiles of one K chunk) are given to
+    // workgroups of one XCD, so those rows cross HBM once rather than once
+    // per XCD.  XCD x holds launch ids x, x + 8, ... and takes the x-th
+    // contiguous range of logical tiles (a bijection for any tile count).
+    const int nx = (int)gridDim.x, ny = (int)gridDim.y;
+    const int ntile = nx * ny * (int)gridDim.z;
+    const int lid = (int)blockIdx.x + nx * ((int)blockIdx.y + ny * (int)blockIdx.z);
+    const int q = ntile >> 3, r8 = ntile & 7, xcd = lid & 7;
+    const int tile = xcd * q + min(xcd, r8) + (lid >> 3);
+    const int tx = tile % nx, ty = (tile / nx) % ny, tz = tile / (nx * ny);
+    const int m0 = ty * kBM, n0 = tx * kBN;
+    const int kb = tz * kchunk, ke = min(K, kb + kchunk);
     f32x16 acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -90,50 +110,79 @@ k_gemm(const uint16_t *__restrict__ A, int lda, const uint16_t *__restrict__ B, 
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.0f;
 
-    // chunk c of a tile: non-transposed operands read rows of kBK (8 chunks a
-    // row), transposed ones read k-rows of 128 (16 chunks a row)
-    uint4 ra[kChunks], rb[kChunks];
+    // LDS: row r of an operand tile holds its 64 k values as 8 chunks of 8;
+    // chunk c sits at position c ^ ((r / 8) % 8) of the row (rows padded to
+    // 72 elements).  With that swizzle the 16-byte writes of a transposed block
+    // (8 consecutive rows, one chunk) and the MFMA operand reads (consecutive
+    // rows, one chunk) both fall on distinct banks.
+    auto lds_off = [](int r, int c) { return r * kLdsK + ((c ^ ((r >> 3) & 7)) << 3); };
+    // Operands whose rows run along k (A unless TA, B when TB) are copied in
+    // 16-byte chunks: chunk c = tid + pass * 256 is row c / 8, k chunk c % 8.
+    // An operand stored with k as its row index (A when TA, B unless TB) is
+    // read in 8 x 8 blocks: thread t < 128 (of the 128 threads that own that
+    // operand) loads k rows 8 (t / 16) .. + 7 at columns 8 (t % 16) .. + 7 and
+    // transposes the block in registers into 8 k-contiguous chunks.
+    constexpr bool XA = TA, XB = !TB;
+    constexpr int NDA = XA ? 1 : kChunks, NDB = XB ? 1 : kChunks;
+    const bool blk_a = XA && tid < 128;
+    const bool blk_b = XB && (XA ? tid >= 128 : tid < 128);
+    const int bt = tid & 127, bkg = bt >> 4, bmg = bt & 15;
+    uint4 rda[NDA], rdb[NDB], rt[8];
     auto load = [&](int k0) {
+        if constexpr (!XA) {
 #pragma unroll
-        for (int pass = 0; pass < kChunks; ++pass) {
-            const int c = tid + pass * kThreads;
-            ra[pass] = make_uint4(0, 0, 0, 0);
-            rb[pass] = make_uint4(0, 0, 0, 0);
-            if (!TA) {
-                const int r = c >> 3, kc = (c & 7) * 8;
-                if (m0 + r < M && k0 + kc < ke) ra[pass] = *(const uint4 *)(A + (size_t)(m0 + r) * lda + k0 + kc);
-            } else {
-                const int kr = c >> 4, mc = (c & 15) * 8;
-                if (k0 + kr < ke && m0 + mc < M) ra[pass] = *(const uint4 *)(A + (size_t)(k0 + kr) * lda + m0 + mc);
+            for (int pass = 0; pass < kChunks; ++pass) {
+                const int c = tid + pass * kThreads, r = c >> 3, kc = (c & 7) * 8;
+                rda[pass] = (m0 + r < M && k0 + kc < ke) ? *(const uint4 *)(A + (size_t)(m0 + r) * lda + k0 + kc)
+                                                        : make_uint4(0, 0, 0, 0);
             }
-            if (TB) {
-                const int r = c >> 3, kc = (c & 7) * 8;
-                if (n0 + r < N && k0 + kc < ke) rb[pass] = *(const uint4 *)(B + (size_t)(n0 + r) * ldb + k0 + kc);
-            } else {
-                const int kr = c >> 4, nc = (c & 15) * 8;
-                if (k0 + kr < ke && n0 + nc < N) rb[pass] = *(const uint4 *)(B + (size_t)(k0 + kr) * ldb + n0 + nc);
+        }
+        if constexpr (!XB) {
+#pragma unroll
+            for (int pass = 0; pass < kChunks; ++pass) {
+                const int c = tid + pass * kThreads, r = c >> 3, kc = (c & 7) * 8;
+                rdb[pass] = (n0 + r < N && k0 + kc < ke) ? *(const uint4 *)(B + (size_t)(n0 + r) * ldb + k0 + kc)
+                                                        : make_uint4(0, 0, 0, 0);
+            }
+        }
+        if (blk_a || blk_b) {
+            const uint16_t *X = blk_a ? A : B;
+            const int ld = blk_a ? lda : ldb, col = (blk_a ? m0 : n0) + bmg * 8, ncol = blk_a ? M : N;
+#pragma unroll
+            for (int rr = 0; rr < 8; ++rr) {
+                const int k = k0 + bkg * 8 + rr;
+                rt[rr] = (k < ke && col < ncol) ? *(const uint4 *)(X + (size_t)k * ld + col) : make_uint4(0, 0, 0, 0);
             }
         }
     };
+    auto word = [](const uint4 &v, int d) { return d == 0 ? v.x : (d == 1 ? v.y : (d == 2 ? v.z : v.w)); };
     auto store = [&]() {
+        if constexpr (!XA) {
 #pragma unroll
-        for (int pass = 0; pass < kChunks; ++pass) {
-            const int c = tid + pass * kThreads;
-            if (!TA) {
-                *(uint4 *)(As + (c >> 3) * kLdsK + (c & 7) * 8) = ra[pass];
-            } else {
-                const int kr = c >> 4, mc = (c & 15) * 8;
-                const uint16_t *e = (const uint16_t *)&ra[pass];
-#pragma unroll
-                for (int i = 0; i < 8; ++i) As[(mc + i) * kLdsK + kr] = e[i];
+            for (int pass = 0; pass < kChunks; ++pass) {
+                const int c = tid + pass * kThreads;
+                *(uint4 *)(As + lds_off(c >> 3, c & 7)) = rda[pass];
             }
-            if (TB) {
-                *(uint4 *)(Bs + (c >> 3) * kLdsK + (c & 7) * 8) = rb[pass];
-            } else {
-                const int kr = c >> 4, nc = (c & 15) * 8;
-                const uint16_t *e = (const uint16_t *)&rb[pass];
+        }
+        if constexpr (!XB) {
 #pragma unroll
-                for (int i = 0; i < 8; ++i) Bs[(nc + i) * kLdsK + kr] = e[i];
+            for (int pass = 0; pass < kChunks; ++pass) {
+                const int c = tid + pass * kThreads;
+                *(uint4 *)(Bs + lds_off(c >> 3, c & 7)) = rdb[pass];
+            }
+        }
+        if (blk_a || blk_b) {
+            uint16_t *Xs = blk_a ? As : Bs;
+            // column mm of the block: word e packs k rows 2e (low half) and 2e + 1
+#pragma unroll
+            for (int mm = 0; mm < 8; ++mm) {
+                uint32_t w[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const uint32_t lo = word(rt[2 * e], mm >> 1), hi = word(rt[2 * e + 1], mm >> 1);
+                    w[e] = (mm & 1) ? ((lo >> 16) | (hi & 0xffff0000u)) : ((lo & 0xffffu) | (hi << 16));
+                }
+                *(uint4 *)(Xs + lds_off(bmg * 8 + mm, bkg)) = make_uint4(w[0], w[1], w[2], w[3]);
             }
         }
     };
@@ -146,14 +195,14 @@ k_gemm(const uint16_t *__restrict__ A, int lda, const uint16_t *__restrict__ B, 
         for (int kk = 0; kk < kBK; kk += 16) {
             // operand layout of the 32x32x16 MFMA: lane l supplies row (l % 32)
             // of A and column (l % 32) of B at k = 8 (l / 32) .. 8 (l / 32) + 7
-            const int ko = kk + (lane >> 5) * 8;
+            const int kc = (kk >> 3) + (lane >> 5);
             bf16x8 a[2], b[2];
 #pragma unroll
             for (int i = 0; i < 2; ++i)
-                a[i] = *(const bf16x8 *)(As + (wm * 64 + i * 32 + (lane & 31)) * kLdsK + ko);
+                a[i] = *(const bf16x8 *)(As + lds_off(wm * 64 + i * 32 + (lane & 31), kc));
 #pragma unroll
             for (int j = 0; j < 2; ++j)
-                b[j] = *(const bf16x8 *)(Bs + (wn * 64 + j * 32 + (lane & 31)) * kLdsK + ko);
+                b[j] = *(const bf16x8 *)(Bs + lds_off(wn * 64 + j * 32 + (lane & 31), kc));
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -162,40 +211,78 @@ k_gemm(const uint16_t *__restrict__ A, int lda, const uint16_t *__restrict__ B, 
         }
         __syncthreads();
     }
-    // result layout: lane l, register v holds C[8 (v / 4) + 4 (l / 32) + v % 4][l % 32]
+    // result layout: lane l, register v holds C[8 (v / 4) + 4 (l / 32) + v % 4][l % 32].
+    // One 32 x 32 block at a time, every input the epilogue reads (position
+    // rows, pre-activations, the output it accumulates into) is loaded for the
+    // whole block before its first store, so those loads are in flight
+    // together rather than one round trip per element.
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < 2; ++j) {
+            const int col = n0 + wn * 64 + j * 32 + (lane & 31);
+            const int rb = m0 + wm * 64 + i * 32 + (lane >> 5) * 4;
+            const bool cok = col < N;
+            if (partial) {
+                float *pz = partial + (size_t)tz * M * N;
+#pragma unroll
+                for (int v = 0; v < 16; ++v) {
+                    const int row = rb + (v >> 2) * 8 + (v & 3);
+                    if (cok && row < M) pz[(size_t)row * N + col] = acc[i][j][v];
+                }
+                continue;
+            }
+            const float bias = (epi.bias && cok) ? epi.bias[col] : 0.0f;
+            float posv[16], pre[16], old[16];
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
-                const int row = m0 + wm * 64 + i * 32 + (v >> 2) * 8 + (lane >> 5) * 4 + (v & 3);
-                const int col = n0 + wn * 64 + j * 32 + (lane & 31);
-                if (row >= M || col >= N) continue;
-                float r = acc[i][j][v];
-                if (partial) {
-                    partial[(size_t)blockIdx.z * M * N + (size_t)row * N + col] = r;
-                    continue;
-                }
+                const int row = rb + (v >> 2) * 8 + (v & 3);
+                const bool ok = cok && row < M;
                 const size_t o = (size_t)row * epi.ldo + col;
-                if (epi.bias) r += epi.bias[col];
-                if (epi.pos) r += epi.pos[(size_t)(row % epi.npos) * N + col];
-                if (epi.dgelu) r *= gelu_grad(epi.dgelu[o]);
+                posv[v] = (epi.pos && ok) ? epi.pos[(size_t)(row % epi.npos) * N + col] : 0.0f;
+                pre[v] = (epi.dgelu && ok) ? epi.dgelu[o] : 0.0f;
+                old[v] = (epi.out && epi.accumulate && ok) ? epi.out[o] : 0.0f;
+            }
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int row = rb + (v >> 2) * 8 + (v & 3);
+                if (!cok || row >= M) continue;
+                const size_t o = (size_t)row * epi.ldo + col;
+                float r = acc[i][j][v];
+                if (epi.bias) r += bias;
+                if (epi.pos) r += posv[v];
+                if (epi.dgelu) r *= gelu_grad(pre[v]);
                 if (epi.out) {
-                    if (epi.accumulate) r += epi.out[o];
+                    if (epi.accumulate) r += old[v];
                     epi.out[o] = r;
                 }
                 if (epi.out_bf) epi.out_bf[o] = f2bf(epi.gelu ? gelu(r) : r);
             }
+        }
+}
+
+// Sums of `parts` FP32 slices of n values (split-K partial tiles, column-sum
+// parts), in a fixed order: four interleaved running sums combined at the end,
+// so four loads are in flight per step instead of one.
+__device__ __forceinline__ float sum_slices(const float *__restrict__ p, int parts, size_t n, size_t i)
+{
+    float s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    int z = 0;
+    for (; z + 4 <= parts; z += 4) {
+        s0 += p[(size_t)z * n + i];
+        s1 += p[(size_t)(z + 1) * n + i];
+        s2 += p[(size_t)(z + 2) * n + i];
+        s3 += p[(size_t)(z + 3) * n + i];
+    }
+    for (; z < parts; ++z) s0 += p[(size_t)z * n + i];
+    return (s0 + s1) + (s2 + s3);
 }
 
 __global__ void k_splitk_sum(const float *__restrict__ partial, int splits, size_t n, float *__restrict__ out)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float s = 0;
-    for (int z = 0; z < splits; ++z) s += partial[(size_t)z * n + i];
-    out[i] = s;
+    out[i] = sum_slices(partial, splits, n, i);
 }
 
 // Column sums of in[M, N] in two deterministic stages: partial[gridDim.y][N].
@@ -204,10 +291,20 @@ __global__ void k_colsum_part(const float *__restrict__ in, int M, int N, float 
     __shared__ float red[4][64];
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     const int col = blockIdx.x * 64 + tx;
-    float s = 0;
-    if (col < N)
-        for (int r = blockIdx.y * 4 + ty; r < M; r += gridDim.y * 4) s += in[(size_t)r * N + col];
-    red[ty][tx] = s;
+    // rows blockIdx.y * 4 + ty + j * S (S = 4 gridDim.y), four running sums
+    float s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    if (col < N) {
+        const int S = gridDim.y * 4;
+        int r = blockIdx.y * 4 + ty;
+        for (; r + 3 * S < M; r += 4 * S) {
+            s0 += in[(size_t)r * N + col];
+            s1 += in[(size_t)(r + S) * N + col];
+            s2 += in[(size_t)(r + 2 * S) * N + col];
+            s3 += in[(size_t)(r + 3 * S) * N + col];
+        }
+        for (; r < M; r += S) s0 += in[(size_t)r * N + col];
+    }
+    red[ty][tx] = (s0 + s1) + (s2 + s3);
     __syncthreads();
     if (ty == 0 && col < N) partial[(size_t)blockIdx.y * N + col] = red[0][tx] + red[1][tx] + red[2][tx] + red[3][tx];
 }
@@ -216,9 +313,7 @@ __global__ void k_colsum_fin(const float *__restrict__ partial, int parts, int N
 {
     const int col = blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= N) return;
-    float s = 0;
-    for (int p = 0; p < parts; ++p) s += partial[(size_t)p * N + col];
-    out[col] = s;
+    out[col] = sum_slices(partial, parts, (size_t)N, (size_t)col);
 }
 
 // dPos[p][n] = sum over samples b of d[b * P + p][n]
@@ -392,7 +487,7 @@ struct pach_dn {
 
 namespace {
 
-constexpr int kColParts = 64;
+constexpr int kColParts = 128;
 
 int dn_fail(pach_dn *d, int code, const std::string &msg)
 {
