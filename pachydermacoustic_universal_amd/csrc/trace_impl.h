@@ -84,6 +84,12 @@ template <class R> __device__ __forceinline__ void normalize3(R &x, R &y, R &z)
     }
 }
 
+// Square root: FP64 IEEE (as the oracle); FP32 the hardware v_sqrt_f32
+// (1 ulp), where the FP32 tracer has no bit-level reference to match.
+template <class R> __device__ __forceinline__ R fsqrt(R x);
+template <> __device__ __forceinline__ double fsqrt<double>(double x) { return sqrt(x); }
+template <> __device__ __forceinline__ float fsqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // Smallest accepted hit distance.  FP64 keeps Hare's t > 1e-10
 // (Classes_VoxelGrid_PolyRefractive.cs:166).  FP32 instead excludes the
 // triangle the ray leaves (planar polygons cannot be re-hit) and accepts t > 0.
@@ -423,12 +429,13 @@ struct LaneCounters {
     double lost;
 };
 
+// p: the pressure sqrt(e rho_c), computed by the caller in the ray's precision
 __device__ __forceinline__ void hist_add(const double *base_unused, double *hist, uint64_t stride,
-                                         uint64_t idx, double e, double rho_c, double ndx,
+                                         uint64_t idx, double e, double p, double ndx,
                                          double ndy, double ndz, int ana)
 {
     unsafeAtomicAdd(hist + idx, e);
-    unsafeAtomicAdd(hist + stride + idx, sqrt(e * rho_c));
+    unsafeAtomicAdd(hist + stride + idx, p);
     // signed direction sums through a float cast, as Directional_Histogram.Add does
     const double vx = (double)(float)(ndx * e), vy = (double)(float)(ndy * e), vz = (double)(float)(ndz * e);
     unsafeAtomicAdd(hist + (ndx > 0 ? 2 : 5) * stride + idx, vx);
@@ -444,7 +451,15 @@ template <> __device__ __forceinline__ double band_gain<double>(const DevRecv<do
 }
 template <> __device__ __forceinline__ double band_gain<float>(const DevRecv<float> &rv, int band, double dist)
 {
-    return (double)exp2f(rv.band->kexp2[band] * (float)dist);
+    return (double)__builtin_amdgcn_exp2f(rv.band->kexp2[band] * (float)dist);
+}
+
+// Pressure of a deposit, sqrt(e rho_c): FP64 IEEE on the FP64 path, FP32
+// hardware square root on the FP32 path.
+template <class R> __device__ __forceinline__ double pressure(double e, double rho_c)
+{
+    if constexpr (sizeof(R) == 8) return sqrt(e * rho_c);
+    else return (double)__builtin_amdgcn_sqrtf((float)(e * rho_c));
 }
 
 // Receiver_Bank.CheckRay / CheckBroadbandRay over every receiver.
@@ -461,7 +476,7 @@ __device__ __forceinline__ void receiver_test(const DevRecv<R> &rv, int r, R ox,
     if (c > R(0) && b > R(0)) return;
     const R disc = b * b - c;
     if (disc < R(0)) return;
-    const R sq = sqrt(disc);
+    const R sq = fsqrt<R>(disc);
     const R t1 = -b - sq, t2 = -b + sq;
     const R t = (t1 + t2) * R(.5);
     if (!(t > R(0) && t * t < sqd)) return;
@@ -474,13 +489,13 @@ __device__ __forceinline__ void receiver_test(const DevRecv<R> &rv, int r, R ox,
     if (band >= 0) {
         const uint64_t idx = ((uint64_t)r * kBands + band) * rv.sct + sample;
         const double e = (double)E[0] * band_gain<R>(rv, band, dist) * rv.sizemod * ts;
-        hist_add(nullptr, rv.hist, rv.plane_stride, idx, e, rv.rho_c, ndx, ndy, ndz, ana);
+        hist_add(nullptr, rv.hist, rv.plane_stride, idx, e, pressure<R>(e, rv.rho_c), ndx, ndy, ndz, ana);
         lc.v[C_DEP]++;
     } else {
         for (int o = lo; o <= hi; ++o) {
             const uint64_t idx = ((uint64_t)r * kBands + o) * rv.sct + sample;
             const double e = (double)E[o] * band_gain<R>(rv, o, dist) * rv.sizemod * ts;
-            hist_add(nullptr, rv.hist, rv.plane_stride, idx, e, rv.rho_c, ndx, ndy, ndz, 0);
+            hist_add(nullptr, rv.hist, rv.plane_stride, idx, e, pressure<R>(e, rv.rho_c), ndx, ndy, ndz, 0);
             lc.v[C_DEP]++;
         }
     }
@@ -752,8 +767,11 @@ __device__ __forceinline__ void diffuse_dir(R nx, R ny, R nz, R uaz, R uzen, R &
         sincos_turn(uaz, sa, ca);
         st = sqrt((1.0 - uzen) * (1.0 + uzen));
     } else {
-        sincospif(2.0f * uaz, &sa, &ca);
-        st = sqrtf((1.0f - uzen) * (1.0f + uzen));
+        // hardware sine / cosine (HIP's __sinf / __cosf: v_sin / v_cos on the
+        // angle in turns), accurate to ~1e-6 for angles in [0, 2 pi)
+        sa = __sinf(6.2831853071795864f * uaz);
+        ca = __cosf(6.2831853071795864f * uaz);
+        st = fsqrt<float>((1.0f - uzen) * (1.0f + uzen));
     }
     const R x = ca * st, y = sa * st, z = uzen;
     ox = xx * x + yx * y + zx * z;
