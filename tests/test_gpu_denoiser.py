@@ -1,7 +1,9 @@
 """GPU checks of the denoiser kernels against the FP64 CPU oracle.
 
   * GEMM: every operand orientation, edge tiles and split K, against an FP64
-    product of the same BF16 operands (FP32 accumulation: 1e-5 of the scale).
+    product of the same BF16 operands (FP32 accumulation: 1e-5 of the scale);
+    tile counts of 30 and 36 (not multiples of the 8 XCDs the tile order
+    deals over) cover the XCD-aware tile order.
   * Forward pass: the oracle with BF16 rounding at the same points; 2e-3
     absolute on outputs of order one.
   * Loss (with per-bin weights) and gradient: relative 2-norm error per
@@ -38,7 +40,7 @@ def net():
 
 
 @pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1), (1, 1)])
-@pytest.mark.parametrize("M,N,K", [(136, 200, 96), (64, 128, 1088), (256, 72, 40)])
+@pytest.mark.parametrize("M,N,K", [(136, 200, 96), (64, 128, 1088), (256, 72, 40), (1160, 264, 200), (264, 264, 2056)])
 def test_gemm_matches_fp64(net, ta, tb, M, N, K):
     print(f"[case] gemm M={M} N={N} K={K} ta={ta} tb={tb}", flush=True)
     rng = np.random.default_rng(M + N + K)
