@@ -65,6 +65,30 @@ def measured_traffic(args, tot, n_rays):
             f"profiles/k_octave_fp32_hall_1e6_rays.md) x this run's octave bounces per launch")
 
 
+def valu_issue(args, tot, n_rays):
+    """The k_octave bound the HBM roofline does not show: VALU issue.  The
+    profiled VALU instructions per octave bounce (rocprofv3 SQ_INSTS_VALU of
+    the same workload), at 4 cycles per wave64 instruction, times this run's
+    octave bounces per second of kernel time, over the cycles of 1,024 SIMDs at
+    2.4 GHz.  None unless this run is the profiled workload."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            prof = json.load(f)
+    except (OSError, ValueError):
+        return None
+    same = (args.workload == "hall" and prof.get("rays") == n_rays and prof.get("cutoff") == args.cutoff
+            and prof.get("grid") == (args.grid or 32) and prof.get("precision") == args.precision
+            and not args.tune and not args.autotune)
+    ipb = prof.get("valu_instr_per_oct_bounce")
+    if not same or not ipb or tot["ms_oct"] <= 0:
+        return None
+    rate = tot["oct"] / (tot["ms_oct"] * 1e-3)
+    return {"instr_per_bounce": ipb, "lane_use": prof.get("valu_lane_utilisation"),
+            "busy_frac": ipb * 4 * rate / (1024 * 2.4e9),
+            "note": "estimate: profiled VALU instructions per octave bounce x 4 cycles x this run's octave "
+                    "bounces per second of k_octave time / (1,024 SIMDs x 2.4 GHz)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -239,7 +263,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_note": traffic_note,
                      "kernel": "k_octave", "avg_launch_ms": tot["ms_oct"] / max(tot["n_oct"], 1),
-                     "launches": tot["n_oct"]},
+                     "launches": tot["n_oct"], "valu_issue": valu_issue(args, tot, n_rays)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
