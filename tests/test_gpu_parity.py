@@ -78,6 +78,47 @@ def test_specular_fp64_per_bin(tracer, medium, room_fn, rays, cut):
         assert_bins(g[k], ora[k], 1e-5, f"plane {k}")
 
 
+# Directions with zero components: the walk's axis exits are infinite on those
+# axes (selects, not branches), and its moves and empty-block jumps must not
+# step along them.  In the box, no path of these directions meets an edge or
+# corner within the cut-off (a path that did would slip through the crack in
+# FP64 and not in FP32, whose edges are widened).
+AXIS_DIRS = np.array([(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1),
+                      (1, 1, 0), (1, 0, 1), (0, -1, 1), (-1, 1, 0)], dtype=np.float64)
+AXIS_DIRS /= np.linalg.norm(AXIS_DIRS, axis=1, keepdims=True)
+
+
+def axis_box():
+    # source off the faces' diagonal planes (a ray along an axis would hit a
+    # triangle edge exactly); receivers on the +x, +y and (1, 1, 0) lines
+    src = (2.3, 3.1, 1.7)
+    return rooms.shoebox(source=src, receivers=[(7.0, 3.1, 1.7), (2.3, 6.5, 1.7), (5.0, 5.8, 1.7)])
+
+
+@pytest.mark.parametrize("room_fn", [axis_box, rooms.fan_hall])
+def test_axis_aligned_rays(tracer, medium, room_fn):
+    room = room_fn()
+    st, g, cnt, ora = run_both(tracer, medium, room, rays=len(AXIS_DIRS), cut=1.0, seed=3,
+                               mode=A.MODE_SPECULAR, precision=A.PREC_FP64,
+                               dir_mode=A.DIR_SUPPLIED, dirs=AXIS_DIRS)
+    assert st["bounces"] == bounces(cnt)
+    assert st["deposits"] == cnt["deposits"]
+    for k in range(A.NPLANES):
+        assert_bins(g[k], ora[k], 1e-5, f"plane {k}")
+    if room_fn is not axis_box:
+        return
+    assert st["deposits"] > 0
+    # FP32 geometry in the box: reflections off axis-aligned walls keep these
+    # directions exact and both triangles of a face share its normal, so the
+    # paths, bounces and deposits are those of FP64
+    st32 = A.trace_room(tracer, room, rays=len(AXIS_DIRS), cutoff_s=1.0, seed=3, mode=A.MODE_SPECULAR,
+                        precision=A.PREC_FP32, dir_mode=A.DIR_SUPPLIED, dirs=AXIS_DIRS, medium=medium)
+    g32 = tracer.planes(scaled=False)
+    assert st32["bounces"] == st["bounces"]
+    assert st32["deposits"] == st["deposits"]
+    np.testing.assert_allclose(g32[0].sum(), g[0].sum(), rtol=1e-4)
+
+
 @pytest.mark.parametrize("room_fn,rays,cut", [(rooms.shoebox, 6, 0.6), (rooms.fan_hall, 12, 1.0)])
 def test_stochastic_fp64_per_bin(tracer, medium, room_fn, rays, cut):
     room = room_fn()
