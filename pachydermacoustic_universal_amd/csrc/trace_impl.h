@@ -128,10 +128,11 @@ __device__ __forceinline__ bool tri_hit_mt(const DevScene<double> &s, int t, dou
 
 // FP32 test against the triangle's dual basis (host-precomputed rows r0, r1,
 // r2): r2 . x + r2.w is the signed plane distance, r0 / r1 give the barycentric
-// coordinates u, v of a point x of the plane.  The plane row decides t first,
-// so triangles crossed behind the ray or past tlim cost 8 multiply-adds.
-// Edges are widened by 4e-6 (barycentric) so no ray slips between neighbours.
-// The rows are read from the voxel list's copy at list index j.
+// coordinates u, v of a point x of the plane.  The three rows are read
+// together and the test has no early return, so lanes whose candidates fail at
+// different stages run one instruction stream.  Edges are widened by 4e-6
+// (barycentric) so no ray slips between neighbours.  The rows are read from
+// the voxel list's copy at list index j.
 __device__ __forceinline__ bool tri_hit_dual(const DevScene<float> &s, uint32_t j, float ox, float oy, float oz,
                                              float dx, float dy, float dz, float tlim, float &tout)
 {
@@ -312,10 +313,11 @@ __device__ __forceinline__ void walk_enter(const DevScene<R> &s, Walk<R> &w, uin
     }
 }
 
-// Flattened form of walk_steps: each of the `budget` units either tests one
-// triangle of the current voxel or leaves the voxel (one step, or one jump
-// across an empty block), so lanes in triangle-dense voxels do not hold the
-// wave on a long inner loop.  Same walk, same result as walk_steps.
+// Flattened form of walk_steps: each of the `budget` units either tests the
+// next list entries of the current voxel (two in FP32, one in FP64) or leaves
+// the voxel (one step, or one jump across an empty block, by the same
+// straight-line code), so lanes in triangle-dense voxels do not hold the wave
+// on a long inner loop.  Same walk, same result as walk_steps.
 // SCHED: the wave takes one kind of unit per iteration -- triangle tests when
 // at least half of its walking lanes have one pending, else voxel moves -- so
 // an iteration runs one of the two code paths instead of both; lanes of the
