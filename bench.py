@@ -106,7 +106,8 @@ def main():
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="pach_tune setting, e.g. dist_lds=0")
     ap.add_argument("--autotune", type=int, default=0, metavar="RAYS",
-                    help="run pach_autotune with RAYS sample source rays before the warm-up")
+                    help="run pach_autotune with RAYS sample source rays before the warm-up "
+                         "(-1: the tuner sizes its sample to ~2e9 bounces per trial)")
     ap.add_argument("--is-order", type=int, default=3, help="image_sources workload: reflection order")
     ap.add_argument("--is-cpu-objects", type=int, default=16,
                     help="image_sources workload: first objects of the CPU sample's sequences")

@@ -329,8 +329,10 @@ typedef struct pach_tuning {
  * timing sample traces (coordinate descent over each knob's candidates,
  * starting from the current settings), then applies the best: the grid is
  * rebuilt at the chosen resolution and the knobs are set as by pach_tune.
- * sample_rays: source rays per trial (<= 0: min(ray count, 8192)); with rooms
- * set, every room's ray count is scaled down to the sample in proportion.
+ * sample_rays: source rays per trial (<= 0: enough for ~2e9 bounces per
+ * trial, from the bounces per source ray of a first 8192-ray trial, at most
+ * the ray count); with rooms set, every room's ray count is scaled down to the
+ * sample in proportion.
  * Overwrites the histograms and statistics (tune before the real trace).
  * No counterpart in the reference (its Voxel_Grid resolution is a user input,
  * Classes_Scenes.cs:709-755). */

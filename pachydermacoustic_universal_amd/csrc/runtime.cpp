@@ -1610,20 +1610,24 @@ extern "C" int pach_autotune(pach_ctx *c, const pach_params *p, int64_t sample_r
         total = 0;
         for (const auto &rm : saved) total += rm.rays;
     }
-    const int64_t sample = sample_rays > 0 ? std::min(sample_rays, total) : std::min<int64_t>(total, 8192);
+    int64_t sample = sample_rays > 0 ? std::min(sample_rays, total) : std::min<int64_t>(total, 8192);
     if (sample <= 0) return fail(c, PACH_EINVAL, "autotune: no rays to sample");
-    if (!saved.empty()) {
-        // every room keeps its share of the sample (at least one ray if it has any)
-        int64_t n = 0;
-        for (auto &rm : c->rooms) {
-            if (rm.rays > 0)
-                rm.rays = std::max<int64_t>(1, (int64_t)std::llround((double)rm.rays * (double)sample / (double)total));
-            n += rm.rays;
+    auto set_sample = [&](int64_t smp) {
+        if (!saved.empty()) {
+            // every room keeps its share of the sample (at least one ray if it has any)
+            int64_t n = 0;
+            c->rooms = saved;
+            for (auto &rm : c->rooms) {
+                if (rm.rays > 0)
+                    rm.rays = std::max<int64_t>(1, (int64_t)std::llround((double)rm.rays * (double)smp / (double)total));
+                n += rm.rays;
+            }
+            pp.ray_count = n;
+        } else {
+            pp.ray_count = smp;
         }
-        pp.ray_count = n;
-    } else {
-        pp.ray_count = sample;
-    }
+    };
+    set_sample(sample);
     struct Knobs {
         int grid[3], block, occ, queue, walk_flat, walk_steps, shade_min, refill_min, dump_min;
     };
@@ -1660,6 +1664,22 @@ extern "C" int pach_autotune(pach_ctx *c, const pach_params *p, int64_t sample_r
     do {
         double warm;
         if ((rc = rate(warm))) break;  // first trial allocates the work buffers
+        if (sample_rays <= 0 && sample < total) {
+            // A trial must be long enough for the launch's drain (few rays left,
+            // waves emptying) to be a small part of it, or settings that drain
+            // well win on the sample and lose on the full trace: size the
+            // default sample to ~2e9 bounces from the bounces per source ray
+            // of the first trial.
+            const pach_stats &S = c->stats;
+            const double bpr = (double)(S.bounces_broad + S.bounces_late + S.bounces_verylate) /
+                               (double)std::max<int64_t>(pp.ray_count, 1);
+            const int64_t want = (int64_t)std::ceil(2e9 / std::max(bpr, 1.0));
+            if (want > sample) {
+                sample = std::min(total, want);
+                set_sample(sample);
+                if ((rc = rate(warm))) break;
+            }
+        }
         if ((rc = rate(r0))) break;
         best_r = r0;
         auto try_knob = [&](auto set, const std::vector<int> &vals) -> int {

@@ -67,3 +67,25 @@ def test_autotune_batch_keeps_room_rays():
             r0 += int(k)
     finally:
         tr.close()
+
+
+def test_autotune_default_sample_is_sized_from_bounces():
+    """With no sample size given, the tuner sizes its trials from the bounces
+    per source ray of a first trial (here the whole, small ray count) and
+    leaves the rooms' own ray counts in force afterwards."""
+    rl, _ = rooms.batch_rooms(seed=17, n=6)
+    pk = rooms.pack_rooms(rl)
+    rays = np.random.default_rng(2).integers(100, 400, len(rl))
+    tr = A.Tracer(0)
+    try:
+        at, c, rho = A.uniform_medium()
+        A.trace_batch(tr, pk, rays, cutoff_s=0.2, seed=2)
+        tr.set_rooms(pk.sources, np.ones((len(rl), 8)), rays, pk.surf_count, pk.rec_count)
+        print("[case] autotune batch rooms=6 default sample", flush=True)
+        t = tr.autotune(rays=0, cutoff_m=0.2 * c, sample_rays=0, seed=2)
+        print("[tuning]", t, flush=True)
+        assert t["trials"] >= 10 and t["rate_best"] >= t["rate_start"] > 0
+        tr.trace(rays=0, cutoff_m=0.2 * c, seed=2)
+        assert tr.stats()["rays_done"] == rays.sum()
+    finally:
+        tr.close()
