@@ -41,7 +41,8 @@ def main(argv=None):
                     help="IS_Order of every trace; with 0 the direct sound comes from Direct_Sound (ETCurve)")
     ap.add_argument("--weights", default=DN.DEFAULT_WEIGHTS)
     ap.add_argument("--ref-cache", default="", help="npz with a reference histogram to reuse (and to write)")
-    ap.add_argument("--autotune", type=int, default=4096)
+    ap.add_argument("--autotune", type=int, default=-1,
+                    help="sample source rays of pach_autotune (-1: sized by the tuner, 0: no tuning)")
     ap.add_argument("--report", default="")
     args = ap.parse_args(argv)
 
