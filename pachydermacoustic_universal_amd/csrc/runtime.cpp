@@ -1012,7 +1012,8 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
     OctShape shape = c->tune_shape;
     if (shape.occ < 0) shape.occ = sizeof(R) == 4 ? 4 : 0;
     if (c->map_on) shape = OctShape{256, 0, 0};  // the map variant's launch shape
-    const int grid = c->n_cu * octave_blocks_per_cu<R>(shape, s.dist_lds ? lds_bytes : 0);
+    const int grid = c->n_cu * (c->map_on ? octave_map_blocks_per_cu<R>(s.dist_lds ? lds_bytes : 0)
+                                          : octave_blocks_per_cu<R>(shape, s.dist_lds ? lds_bytes : 0));
     int rc = ensure_work<R>(c, batch, (size_t)grid * (shape.block / 64), shape.queue != 0);
     if (rc) return rc;
     WorkBuf<R> &w = work_of<R>(c);

@@ -1324,6 +1324,10 @@ k_octave(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv,
 }
 
 // ---------------------------------------------------------------- launchers
+// Register cap of the mapping-receiver variant: the compiler's allocation
+// (231 VGPRs, two waves per SIMD); a cap at three waves spills 88 registers
+// and runs 15 % slower on the hall's 1,184-receiver map.
+constexpr int kMapOcc = 0;
 template <class R>
 hipError_t launch_phase1(hipStream_t st, const TraceConst &tc, const DevScene<R> &s,
                          const DevRecv<R> &rv, int64_t src_begin, int n_fresh,
@@ -1381,11 +1385,13 @@ hipError_t launch_octave(hipStream_t st, int grid, const OctShape &sh, const Tra
     }
     if (sh.queue && !shq) return hipErrorInvalidValue;
     if (rv.has_map) {
-        // mapping receivers: workgroups of 256, no shade queue, the
-        // compiler's register allocation (a 128-register cap spills the
-        // map walk: 26.0 s against 18.3 s on the hall's 1,184-receiver map)
-        launch_octave_bs<R, 256, 0, false, true>(st, grid, lds, tc, s, rv, in0, n0, cap0, in1, n1, cap1, in_head,
-                                                 stacks, stack_cap, shq, dump, dump_count, dump_cap, min_iters, cnt);
+        // mapping receivers: workgroups of 256, no shade queue, kMapOcc (a
+        // 128-register cap spills the map walk: 26.0 s against 18.3 s on the
+        // hall's 1,184-receiver map); the grid comes from
+        // octave_map_blocks_per_cu, this kernel's own residency
+        launch_octave_bs<R, 256, kMapOcc, false, true>(st, grid, lds, tc, s, rv, in0, n0, cap0, in1, n1, cap1,
+                                                       in_head, stacks, stack_cap, shq, dump, dump_count, dump_cap,
+                                                       min_iters, cnt);
         return hipGetLastError();
     }
     if (sh.block == 512) PACH_OCT_O(512)
@@ -1423,6 +1429,19 @@ template <class R> int octave_blocks_per_cu(const OctShape &sh, int lds_bytes)
     return sh.queue ? blocks_per_cu_q<R, true>(sh, lds_bytes) : blocks_per_cu_q<R, false>(sh, lds_bytes);
 }
 
+// Resident workgroups per CU of the mapping-receiver variant (its own register
+// count, not the plain kernel's: a persistent kernel's grid must be resident).
+template <class R> int octave_map_blocks_per_cu(int lds_bytes)
+{
+    int nb = 0;
+    hipError_t e =
+        lds_bytes > 0
+            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_octave<R, true, 256, kMapOcc, false, true>, 256, lds_bytes)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_octave<R, false, 256, kMapOcc, false, true>, 256, 0);
+    if (e != hipSuccess || nb < 1) nb = 1;
+    return nb;
+}
+
 #define PACH_INSTANTIATE(R)                                                                        \
     template hipError_t launch_phase1<R>(hipStream_t, const TraceConst &, const DevScene<R> &,     \
                                          const DevRecv<R> &, int64_t, int, const P1State<R> *, int, \
@@ -1434,6 +1453,7 @@ template <class R> int octave_blocks_per_cu(const OctShape &sh, int lds_bytes)
                                          const unsigned *, unsigned, unsigned *, ORay<R> *, int,    \
                                          ShadeItem<R> *, ORay<R> *, unsigned *, unsigned, int,      \
                                          DevCounters *);                                            \
-    template int octave_blocks_per_cu<R>(const OctShape &, int);
+    template int octave_blocks_per_cu<R>(const OctShape &, int);                                   \
+    template int octave_map_blocks_per_cu<R>(int);
 
 }  // namespace pach

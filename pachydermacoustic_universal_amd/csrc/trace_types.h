@@ -212,5 +212,6 @@ hipError_t launch_octave(hipStream_t st, int grid, const OctShape &sh, const Tra
                          unsigned dump_cap, int min_iters, DevCounters *cnt);
 
 template <class R> int octave_blocks_per_cu(const OctShape &sh, int lds_bytes);
+template <class R> int octave_map_blocks_per_cu(int lds_bytes);
 
 }  // namespace pach
