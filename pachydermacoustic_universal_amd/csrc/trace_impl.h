@@ -103,6 +103,10 @@ template <> struct Tmin<float> { static constexpr float v = 0.0f; static constex
 __device__ __forceinline__ bool tri_hit_mt(const DevScene<double> &s, int t, double ox, double oy, double oz,
                                            double dx, double dy, double dz, double tlim, double &tout)
 {
+    // Moller-Trumbore without early returns (the rejections are combined at the
+    // end), so a wave whose lanes fail at different stages runs one stream;
+    // every value that decides is computed by the same operations as in the
+    // oracle, so the decisions are bit-identical.
     const double4 a = s.v0[t];
     const double4 e1 = s.e1[t];
     const double4 e2 = s.e2[t];
@@ -110,20 +114,18 @@ __device__ __forceinline__ bool tri_hit_mt(const DevScene<double> &s, int t, dou
     const double py = dz * e2.x - dx * e2.z;
     const double pz = dx * e2.y - dy * e2.x;
     const double det = e1.x * px + e1.y * py + e1.z * pz;
-    if (det == 0.0) return false;
-    const double inv = 1.0 / det;
+    const double inv = 1.0 / det;  // infinite for det == 0, which is rejected below
     const double sx = ox - a.x, sy = oy - a.y, sz = oz - a.z;
     const double u = (sx * px + sy * py + sz * pz) * inv;
-    if (u < 0.0 || u > 1.0) return false;
     const double qx = sy * e1.z - sz * e1.y;
     const double qy = sz * e1.x - sx * e1.z;
     const double qz = sx * e1.y - sy * e1.x;
     const double v = (dx * qx + dy * qy + dz * qz) * inv;
-    if (v < 0.0 || u + v > 1.0) return false;
     const double tt = (e2.x * qx + e2.y * qy + e2.z * qz) * inv;
-    if (!(tt > 1e-10) || !(tt <= tlim)) return false;
-    tout = tt;
-    return true;
+    const bool ok = det != 0.0 && !(u < 0.0 || u > 1.0) && !(v < 0.0 || u + v > 1.0) && (tt > 1e-10) &&
+                    (tt <= tlim);
+    tout = ok ? tt : tout;
+    return ok;
 }
 
 // FP32 test against the triangle's dual basis (host-precomputed rows r0, r1,
