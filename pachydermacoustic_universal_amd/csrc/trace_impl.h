@@ -316,7 +316,7 @@ __device__ __forceinline__ void walk_enter(const DevScene<R> &s, Walk<R> &w, uin
 }
 
 // Flattened form of walk_steps: each of the `budget` units either tests the
-// next list entries of the current voxel (two in FP32, one in FP64) or leaves
+// next two list entries of the current voxel or leaves
 // the voxel (one step, or one jump across an empty block, by the same
 // straight-line code), so lanes in triangle-dense voxels do not hold the wave
 // on a long inner loop.  Same walk, same result as walk_steps.
@@ -348,7 +348,7 @@ __device__ bool walk_units(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R
                     w.best = t;
                 }
             };
-            if constexpr (sizeof(R) == 4) {
+            {
                 // two list entries per unit (the last one twice when one is
                 // left: a repeated triangle never replaces itself), merged in
                 // list order, as two units would
@@ -364,13 +364,6 @@ __device__ bool walk_units(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R
                 const bool h1 = tri_hit<R>(s, t1, j1, ox, oy, oz, dx, dy, dz, tlim, tt1);
                 take(h0, t0, tt0);
                 take(h1, t1, tt1);
-            } else {
-                ++n_test;
-                const uint32_t jj = w.j++;
-                const int t = (int)s.cell_tris[jj];
-                R tt;
-                const bool h = tri_hit<R>(s, t, jj, ox, oy, oz, dx, dy, dz, tlim, tt);
-                take(h, t, tt);
             }
             continue;
         }
