@@ -7,7 +7,12 @@ bins, 32^3 voxel grid, FP32 traversal with FP64 histograms.  One step is one
 complete trace (SplitRayTracer.Begin -> Combine) of all rays; the inputs are
 resident in HBM before the timed region starts.
 
-Prints one JSON line (see DESIGN.md for the field definitions).
+Prints one JSON line: `value` is ray-bounces per second over the timed
+steps; `roofline` gives the octave kernel's algorithmic bytes per second of its
+own launch time against the 8 TB/s HBM peak, the HBM bytes per launch measured
+by rocprofv3 (`traffic`) and the kernel's VALU-issue load (`valu_issue`);
+`cpu_baseline` times the C FP64 restatement on the host's cores over a bounded
+sample.  Other workloads (--workload) report their own metric in the same form.
 """
 from __future__ import annotations
 
