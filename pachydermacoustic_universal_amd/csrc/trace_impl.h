@@ -316,7 +316,7 @@ __device__ __forceinline__ void walk_enter(const DevScene<R> &s, Walk<R> &w, uin
 }
 
 // Flattened form of walk_steps: each of the `budget` units either tests the
-// next two list entries of the current voxel or leaves
+// next three list entries of the current voxel or leaves
 // the voxel (one step, or one jump across an empty block, by the same
 // straight-line code), so lanes in triangle-dense voxels do not hold the wave
 // on a long inner loop.  Same walk, same result as walk_steps.
@@ -349,21 +349,24 @@ __device__ bool walk_units(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R
                 }
             };
             {
-                // two list entries per unit (the last one twice when one is
-                // left: a repeated triangle never replaces itself), merged in
-                // list order, as two units would
-                const uint32_t j0 = w.j;
-                const bool two = j0 + 1 < w.je;
-                const uint32_t j1 = two ? j0 + 1 : j0;
-                w.j = j0 + (two ? 2u : 1u);
-                n_test += two ? 2u : 1u;
+                // three list entries per unit (the last one repeated when
+                // fewer are left: a repeated triangle never replaces itself),
+                // merged in list order, as single units would (four spill
+                // scalar registers and run 7 % slower)
+                const uint32_t j0 = w.j, jl = w.je - 1;
+                const uint32_t j1 = min(j0 + 1, jl), j2 = min(j0 + 2, jl);
+                w.j = j2 + 1;
+                n_test += j2 - j0 + 1;
                 const int t0 = (int)s.cell_tris[j0];
                 const int t1 = (int)s.cell_tris[j1];
-                R tt0, tt1;
+                const int t2 = (int)s.cell_tris[j2];
+                R tt0, tt1, tt2;
                 const bool h0 = tri_hit<R>(s, t0, j0, ox, oy, oz, dx, dy, dz, tlim, tt0);
                 const bool h1 = tri_hit<R>(s, t1, j1, ox, oy, oz, dx, dy, dz, tlim, tt1);
+                const bool h2 = tri_hit<R>(s, t2, j2, ox, oy, oz, dx, dy, dz, tlim, tt2);
                 take(h0, t0, tt0);
                 take(h1, t1, tt1);
+                take(h2, t2, tt2);
             }
             continue;
         }
