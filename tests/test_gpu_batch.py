@@ -74,7 +74,9 @@ def test_batch_fp64_matches_oracle_per_room(tracer):
 
 
 def test_batch_rooms_are_isolated(tracer):
-    """Dropping every ray of one room leaves the other rooms' bins bit-identical."""
+    """Dropping every ray of one room leaves the other rooms' bins unchanged: the
+    same deposits, up to the order in which FP64 atomics add two deposits into
+    one bin (a last-bit difference, so 1e-14 relative)."""
     pk, rays = small_batch(seed=9, n=5, rays_lo=20, rays_hi=40)
     print(f"[case] batch isolation rooms={len(rays)} rays={list(rays)}", flush=True)
     A.trace_batch(tracer, pk, rays, cutoff_s=0.3, seed=2, precision=A.PREC_FP32)
@@ -86,7 +88,8 @@ def test_batch_rooms_are_isolated(tracer):
     A.trace_batch(tracer, pk, r2, cutoff_s=0.3, seed=2, precision=A.PREC_FP32)
     part = tracer.planes(scaled=False)
     k0 = int(pk.rec_count[:2].sum())
-    np.testing.assert_array_equal(full[:, :k0], part[:, :k0])
+    np.testing.assert_allclose(part[:, :k0], full[:, :k0], rtol=1e-14, atol=0)
+    assert np.array_equal(part[:, :k0] != 0, full[:, :k0] != 0)
     k1 = k0 + int(pk.rec_count[2])
     assert np.all(part[:, k0:k1] == 0)
     assert full[0, k0:k1].sum() > 0
