@@ -316,10 +316,11 @@ __device__ __forceinline__ void walk_enter(const DevScene<R> &s, Walk<R> &w, uin
 }
 
 // Flattened form of walk_steps: each of the `budget` units either tests the
-// next three list entries of the current voxel or leaves
-// the voxel (one step, or one jump across an empty block, by the same
-// straight-line code), so lanes in triangle-dense voxels do not hold the wave
-// on a long inner loop.  Same walk, same result as walk_steps.
+// next three list entries of the current voxel or leaves the voxel (one step,
+// or one jump across an empty block, by the same straight-line code), so
+// lanes in triangle-dense voxels do not hold the wave on a long inner loop.
+// Same voxels, same closest hit as walk_steps (the voxel exits are recomputed
+// after each move rather than accumulated, which changes only their rounding).
 // SCHED: the wave takes one kind of unit per iteration -- triangle tests when
 // at least half of its walking lanes have one pending, else voxel moves -- so
 // an iteration runs one of the two code paths instead of both; lanes of the
