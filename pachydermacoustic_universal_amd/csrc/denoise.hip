@@ -119,8 +119,9 @@ k_gemm(const uint16_t *__restrict__ A, int lda, const uint16_t *__restrict__ B, 
     // Operands whose rows run along k (A unless TA, B when TB) are copied in
     // 16-byte chunks: chunk c = tid + pass * 256 is row c / 8, k chunk c % 8.
     // An operand stored with k as its row index (A when TA, B unless TB) is
-    // read in 8 x 8 blocks: thread t < 128 (of the 128 threads that own that
-    // operand) loads k rows 8 (t / 16) .. + 7 at columns 8 (t % 16) .. + 7 and
+    // read in 8 x 8 blocks by 128 threads (threads 0-127; when both operands
+    // are stored so, threads 128-255 take B): thread t loads k rows
+    // 8 (b / 16) .. + 7 at columns 8 (b % 16) .. + 7, b = t % 128, and
     // transposes the block in registers into 8 k-contiguous chunks.
     constexpr bool XA = TA, XB = !TB;
     constexpr int NDA = XA ? 1 : kChunks, NDB = XB ? 1 : kChunks;
