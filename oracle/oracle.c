@@ -970,6 +970,121 @@ static void scatter_verylate(const or_scene *s, oray *r, const double *N, double
     if (transmit) { r->d[0] = -r->d[0]; r->d[1] = -r->d[1]; r->d[2] = -r->d[2]; }
 }
 
+/* Lambert_Scattering.Scatter_Early (Classes_Materials.cs:4498-4570) of a
+ * broadband ray at hit point o on triangle ht of polygon sf, energies E after
+ * Absorb: per band with E > 0 a diffuse child BroadRay.SplitRay(oct, s)
+ * (E s; the ray keeps E (1 - s), Classes_Rays.cs:345-351); with transmission
+ * tau > 0 the ray splits again, tr = SplitRay(oct, tau) along the incoming
+ * direction, and the child splits td = R.SplitRay(tau), reversed
+ * (:4554-4563).  Children are pushed tr, td, R (the reference's enqueue
+ * order); d becomes the specular reflection about the flipped normal. */
+static void scatter_early(const or_scene *s, int sf, int ht, const double o_[3], double d[3], const double *N,
+                          double cos_t, double tb, double E[8], uint64_t root, uint32_t k, const uint32_t w[16],
+                          ostack *st, or_counters *cnt)
+{
+    double Nf[3] = {N[0], N[1], N[2]}, cf = cos_t;
+    if (cf > 0) { Nf[0] = -N[0]; Nf[1] = -N[1]; Nf[2] = -N[2]; cf = -cf; }
+    for (int oc = 0; oc < 8; ++oc) {
+        if (E[oc] == 0) continue;
+        double sc = s->scat[8 * sf + oc];
+        oray R;
+        memcpy(R.o, o_, sizeof(R.o));
+        R.I = E[oc] * sc;
+        E[oc] *= (1 - sc);
+        R.t = tb; R.band = oc; R.tri = ht; R.phase = 0; R.k = 0;
+        R.uid = or_child_uid(root, TAG_EARLY(k, oc));
+        diffuse_dir(Nf, u01(w[2 * oc]), u01(w[2 * oc + 1]), 0, R.d);
+        R.mask = 3;
+        double tau = s->trans[8 * sf + oc];
+        if (tau > 0) {
+            oray tr = R;
+            memcpy(tr.d, d, sizeof(tr.d)); /* incoming direction */
+            tr.I = E[oc] * tau;
+            E[oc] *= (1 - tau);
+            tr.mask = 0;
+            tr.uid = or_child_uid(root, TAG_EARLY_TR(k, oc));
+            push(st, &tr);
+            oray td = R;
+            td.I = R.I * tau;
+            R.I *= (1 - tau);
+            td.d[0] = -R.d[0]; td.d[1] = -R.d[1]; td.d[2] = -R.d[2];
+            td.mask = 3;
+            td.uid = or_child_uid(root, TAG_EARLY_TD(k, oc));
+            push(st, &td);
+            cnt->spawned += 2;
+            cnt->octave_rays += 2;
+        }
+        push(st, &R);
+        cnt->spawned++;
+        cnt->octave_rays++;
+    }
+    specular(d, Nf, cf);
+}
+
+/* ---- single scattering events, for the pins of the Lambert scheme ---- */
+int or_lambert_bounds(const or_scene *s, int surf, int band, double *lo, double *hi)
+{
+    if (!s || surf < 0 || surf >= s->S || band < 0 || band > 7) return -1;
+    *lo = s->lo[8 * surf + band];
+    *hi = s->hi[8 * surf + band];
+    return 0;
+}
+
+static void ray_out(const oray *r, or_ray_out *o)
+{
+    memcpy(o->d, r->d, sizeof(o->d));
+    o->I = r->I;
+    o->mask = r->mask;
+    o->band = r->band;
+}
+
+int or_scatter_late_once(const or_scene *s, int sf, int band, const double d_in[3], const double N[3], double I,
+                         int mask_in, int transmit, const uint32_t w[4], or_ray_out *parent, or_ray_out *child)
+{
+    oray r;
+    memset(&r, 0, sizeof(r));
+    memcpy(r.d, d_in, sizeof(r.d));
+    r.I = I; r.band = band; r.mask = mask_in;
+    ostack st = {NULL, 0, 0};
+    or_counters cnt;
+    memset(&cnt, 0, sizeof(cnt));
+    const double cos_t = dot3(N, r.d);
+    scatter_late(s, &r, N, cos_t, sf, w, transmit, 0, &st, &cnt);
+    ray_out(&r, parent);
+    const int n = (int)st.n;
+    if (n > 0 && child) ray_out(&st.a[0], child);
+    free(st.a);
+    return n;
+}
+
+void or_scatter_verylate_once(const or_scene *s, int sf, int band, const double d_in[3], const double N[3], double I,
+                              int mask_in, int transmit, const uint32_t w[4], or_ray_out *out)
+{
+    oray r;
+    memset(&r, 0, sizeof(r));
+    memcpy(r.d, d_in, sizeof(r.d));
+    r.I = I; r.band = band; r.mask = mask_in;
+    scatter_verylate(s, &r, N, dot3(N, r.d), sf, w, transmit);
+    ray_out(&r, out);
+}
+
+int or_scatter_early_once(const or_scene *s, int sf, const double d_in[3], const double N[3], const double E_in[8],
+                          const uint32_t w[16], double d_out[3], double E_out[8], or_ray_out *children, int cap)
+{
+    double o_[3] = {0, 0, 0}, d[3] = {d_in[0], d_in[1], d_in[2]}, E[8];
+    memcpy(E, E_in, sizeof(E));
+    ostack st = {NULL, 0, 0};
+    or_counters cnt;
+    memset(&cnt, 0, sizeof(cnt));
+    scatter_early(s, sf, 0, o_, d, N, dot3(d, N), 0.0, E, 0, 0, w, &st, &cnt);
+    memcpy(d_out, d, sizeof(d));
+    memcpy(E_out, E, sizeof(E));
+    const int n = (int)st.n;
+    for (int i = 0; i < n && i < cap; ++i) ray_out(&st.a[i], &children[i]);
+    free(st.a);
+    return n;
+}
+
 /* Phase 2 (TaskAll:427-480) then End_Trace (TaskAll:489-536) for one octave ray. */
 static void trace_octave(const or_scene *s, const or_setup *su, const or_params *p,
                          const rx_ctx *x, const rng_ctx *rng, const thresholds *th,
@@ -1078,46 +1193,9 @@ static void trace_source_ray(const or_scene *s, const or_setup *su, const or_par
             cnt->absorbed += e0 - E[o];
         }
         if (stoch) {
-            /* Lambert_Scattering.Scatter_Early (Classes_Materials.cs:4498-4570) */
-            double Nf[3] = {N[0], N[1], N[2]}, cf = cos_t;
-            if (cf > 0) { Nf[0] = -N[0]; Nf[1] = -N[1]; Nf[2] = -N[2]; cf = -cf; }
             uint32_t w[16];
             for (int j = 0; j < 4; ++j) rng_block(&rng, root, 4 * k + j, w + 4 * j);
-            for (int oc = 0; oc < 8; ++oc) {
-                if (E[oc] == 0) continue;
-                double sc = s->scat[8 * sf + oc];
-                oray R;
-                memcpy(R.o, o_, sizeof(o_));
-                R.I = E[oc] * sc;
-                E[oc] *= (1 - sc);
-                R.t = tb; R.band = oc; R.tri = ht; R.phase = 0; R.k = 0;
-                R.uid = or_child_uid(root, TAG_EARLY(k, oc));
-                diffuse_dir(Nf, u01(w[2 * oc]), u01(w[2 * oc + 1]), 0, R.d);
-                R.mask = 3;
-                double tau = s->trans[8 * sf + oc];
-                if (tau > 0) {
-                    oray tr = R;
-                    memcpy(tr.d, d, sizeof(d)); /* incoming direction */
-                    tr.I = E[oc] * tau;
-                    E[oc] *= (1 - tau);
-                    tr.mask = 0;
-                    tr.uid = or_child_uid(root, TAG_EARLY_TR(k, oc));
-                    push(st, &tr);
-                    oray td = R;
-                    td.I = R.I * tau;
-                    R.I *= (1 - tau);
-                    td.d[0] = -R.d[0]; td.d[1] = -R.d[1]; td.d[2] = -R.d[2];
-                    td.mask = 3;
-                    td.uid = or_child_uid(root, TAG_EARLY_TD(k, oc));
-                    push(st, &td);
-                    cnt->spawned += 2;
-                    cnt->octave_rays += 2;
-                }
-                push(st, &R);
-                cnt->spawned++;
-                cnt->octave_rays++;
-            }
-            specular(d, Nf, cf);
+            scatter_early(s, sf, ht, o_, d, N, cos_t, tb, E, root, k, w, st, cnt);
         } else {
             specular(d, N, cos_t);
         }

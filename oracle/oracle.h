@@ -151,6 +151,26 @@ void or_receiver_check(const or_setup *su, double **planes, const double o[3], c
 void or_sample_diffuse(const double N[3], double u_az, double u_zen, double out[3]);
 void or_sincos_turn(double u, double *s, double *c);
 
+/* Single scattering events of the Lambert scheme, run by the same code as
+ * or_trace (for the pins of Classes_Materials.cs:4463-4702).  w: the
+ * event's uniforms as 32-bit words (Scatter_Late / VeryLate: w[0] is the
+ * transmission draw already taken by the caller, w[1] the selector, w[2..3]
+ * the diffuse direction; Scatter_Early: two words per band). */
+typedef struct {
+    double d[3];
+    double I;
+    int mask, band;
+} or_ray_out;
+int or_lambert_bounds(const or_scene *s, int surf, int band, double *lo, double *hi);
+/* returns the number of children (0 or 1; *child filled when 1) */
+int or_scatter_late_once(const or_scene *s, int sf, int band, const double d_in[3], const double N[3], double I,
+                         int mask_in, int transmit, const uint32_t w[4], or_ray_out *parent, or_ray_out *child);
+void or_scatter_verylate_once(const or_scene *s, int sf, int band, const double d_in[3], const double N[3], double I,
+                              int mask_in, int transmit, const uint32_t w[4], or_ray_out *out);
+/* returns the number of children (written up to cap, in the reference's enqueue order) */
+int or_scatter_early_once(const or_scene *s, int sf, const double d_in[3], const double N[3], const double E_in[8],
+                          const uint32_t w[16], double d_out[3], double E_out[8], or_ray_out *children, int cap);
+
 /* --- decay-curve integration (Classes_Math.cs) ---------------------------- */
 void or_schroeder(const double *etc, int n, double *out);
 void or_log10data(const double *x, int n, double limit_db, double *out);

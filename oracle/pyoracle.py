@@ -50,6 +50,13 @@ class Counters(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class RayOut(C.Structure):
+    _fields_ = [("d", C.c_double * 3), ("I", C.c_double), ("mask", C.c_int), ("band", C.c_int)]
+
+    def as_tuple(self):
+        return np.array(self.d[:]), self.I, self.mask, self.band
+
+
 _lib = None
 
 
@@ -119,6 +126,17 @@ def lib():
                                           C.c_double, C.POINTER(C.c_int64)]
         L.or_map_check_broadband.argtypes = [C.c_void_p, C.POINTER(_dp), _dp, _dp, _dp, _dp, C.c_double,
                                              C.POINTER(C.c_int64)]
+        L.or_lambert_bounds.restype = C.c_int
+        L.or_lambert_bounds.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp]
+        _u32p = C.POINTER(C.c_uint32)
+        L.or_scatter_late_once.restype = C.c_int
+        L.or_scatter_late_once.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_int, C.c_int,
+                                           _u32p, C.POINTER(RayOut), C.POINTER(RayOut)]
+        L.or_scatter_verylate_once.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_int, C.c_int,
+                                               _u32p, C.POINTER(RayOut)]
+        L.or_scatter_early_once.restype = C.c_int
+        L.or_scatter_early_once.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _u32p, _dp, _dp,
+                                            C.POINTER(RayOut), C.c_int]
         L.or_image_sources_count_mt.restype = C.c_int64
         L.or_image_sources_count_mt.argtypes = [C.c_void_p, _ip, C.c_int, C.c_int, _dp, _dp, _dp, C.c_int, _dp,
                                                 C.c_double, C.c_double, C.c_int, C.c_int, C.c_int]
@@ -245,6 +263,40 @@ class Scene:
         if n < 0:
             raise ValueError("image sources: bad arguments")
         return int(n)
+
+    def lambert_bounds(self, surf, band):
+        """(lo, hi) of Lambert_Scattering for polygon surf, band (Classes_Materials.cs:4463-4485)."""
+        lo, hi = C.c_double(), C.c_double()
+        if lib().or_lambert_bounds(self.h, surf, band, C.byref(lo), C.byref(hi)) != 0:
+            raise ValueError("bad polygon or band")
+        return lo.value, hi.value
+
+    def scatter_late(self, sf, band, d, N, I, w, mask=0, transmit=False):
+        """One Scatter_Late event; returns (parent, child or None) as (d, I, mask, band)."""
+        d, N = (np.ascontiguousarray(v, dtype=np.float64) for v in (d, N))
+        ww = (C.c_uint32 * 4)(*[int(x) for x in w])
+        par, ch = RayOut(), RayOut()
+        n = lib().or_scatter_late_once(self.h, sf, band, _p(d), _p(N), float(I), int(mask), int(transmit), ww,
+                                       C.byref(par), C.byref(ch))
+        return par.as_tuple(), (ch.as_tuple() if n else None)
+
+    def scatter_verylate(self, sf, band, d, N, I, w, mask=0, transmit=False):
+        d, N = (np.ascontiguousarray(v, dtype=np.float64) for v in (d, N))
+        ww = (C.c_uint32 * 4)(*[int(x) for x in w])
+        out = RayOut()
+        lib().or_scatter_verylate_once(self.h, sf, band, _p(d), _p(N), float(I), int(mask), int(transmit), ww,
+                                       C.byref(out))
+        return out.as_tuple()
+
+    def scatter_early(self, sf, d, N, E, w):
+        """One Scatter_Early event of a broadband ray with energies E (after Absorb):
+        returns (outgoing direction, remaining energies, children in enqueue order)."""
+        d, N, E = (np.ascontiguousarray(v, dtype=np.float64) for v in (d, N, E))
+        ww = (C.c_uint32 * 16)(*[int(x) for x in w])
+        d_out, E_out = np.zeros(3), np.zeros(8)
+        ch = (RayOut * 24)()
+        n = lib().or_scatter_early_once(self.h, sf, _p(d), _p(N), _p(E), ww, _p(d_out), _p(E_out), ch, 24)
+        return d_out, E_out, [ch[i].as_tuple() for i in range(n)]
 
     def sabine(self, volume, atten):
         out = np.zeros(8)

@@ -9,6 +9,7 @@
 #include "filters.h"
 #include "grid.h"
 #include "image_source.h"
+#include "is_plan.h"
 #include "trace_types.h"
 
 #include <algorithm>
@@ -202,9 +203,11 @@ template <class R> static void free_scene(SceneBuf<R> &s)
 template <class R> static void free_work(WorkBuf<R> &w)
 {
     dfree(w.p1_out); dfree(w.res[0]); dfree(w.res[1]); dfree(w.stacks); dfree(w.dump[0]); dfree(w.dump[1]);
+    dfree(w.shq);
     w.p1_cap = w.res_cap = w.dump_cap = 0;
     w.stack_cap = 0;
     w.n_waves = 0;
+    w.shq_waves = 0;
 }
 
 // ------------------------------------------------------------------ medium
@@ -1560,17 +1563,21 @@ extern "C" int pach_image_sources(pach_ctx *c, const int *object_of_surface, int
                 HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
                 c->is_stats.ms_kernel += ms;
                 c->is_stats.launches++;
-                if (n > kChunk) {
-                    // more valid paths than the buffer holds: redo the range in smaller pieces
-                    span = std::max<int64_t>(c->R, (hi - lo) * (int64_t)kChunk / (int64_t)(2 * n) / c->R * c->R);
-                    continue;
+                // more valid paths than the buffer holds: redo the range smaller,
+                // or with a larger buffer when it is a single sequence
+                const IsSpanStep step = is_after_launch(span, lo, hi, n, j.cap, c->R, span_max);
+                span = step.span;
+                if (step.grow_to) {
+                    HIPCHK(c, dalloc(d_out, (size_t)step.grow_to));
+                    j.out = d_out;
+                    j.cap = step.grow_to;
                 }
+                if (step.redo) continue;
                 c->is_stats.checks += hi - lo;
                 const size_t old = found.size();
                 found.resize(old + n);
                 if (n) HIPCHK(c, hipMemcpy(found.data() + old, d_out, n * sizeof(pach_is_path), hipMemcpyDeviceToHost));
                 lo = hi;
-                if (n < kChunk / 4) span = std::min<int64_t>(span * 2, span_max);
             }
         }
         return PACH_OK;
