@@ -101,7 +101,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--rays", type=int, default=1_000_000)
     ap.add_argument("--cutoff", type=float, default=3.0)
-    ap.add_argument("--precision", choices=["fp32", "fp64"], default="fp32")
+    ap.add_argument("--precision", choices=["fp32", "fp64", "mixed"], default="fp32")
     ap.add_argument("--grid", type=int, default=None,
                     help="voxels per axis (default: 32 for the hall, automatic for the batch)")
     ap.add_argument("--cpu-rays", type=int, default=5000, help="source rays of the CPU sample")
@@ -149,7 +149,7 @@ def main():
     if args.workload == "filters":
         return bench_filters(args, A, rooms)
     at, c, rho = A.uniform_medium()
-    prec = A.PREC_FP64 if args.precision == "fp64" else A.PREC_FP32
+    prec = {"fp64": A.PREC_FP64, "mixed": A.PREC_MIXED}.get(args.precision, A.PREC_FP32)
     tr = A.Tracer(0)
     for kv in args.tune:
         k, v = kv.split("=")
@@ -264,7 +264,8 @@ def main():
                    "grid": tr_grid, "tuning": tuning,
                    "bounces_per_step": tot["bounces"] / args.steps,
                    "walk": {k: last[k] for k in ("loop_steps", "voxels", "tri_tests", "shade_passes",
-                                                 "lanes_shaded", "deposits", "spills")},
+                                                 "lanes_shaded", "deposits", "spills", "exact_rewalks",
+                                                 "exact_walks")},
                    "rays_per_s": n_rays * args.steps / dt},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_note": traffic_note,

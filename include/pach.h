@@ -45,6 +45,10 @@ extern "C" {
 /* geometry precision of the traversal and reflection arithmetic */
 #define PACH_PREC_FP32 0
 #define PACH_PREC_FP64 1
+/* FP64 rays, hits and shading (the FP64 results) with an FP32 screen for the
+ * voxel walk: every hit is decided by the FP64 test; candidates the FP32 screen
+ * cannot order with certainty are walked again in FP64 */
+#define PACH_PREC_MIXED 2
 
 /* source direction modes */
 #define PACH_DIR_GEODESIC 0   /* GeodesicSource (Classes_Sources.cs:272-404) */
@@ -121,6 +125,10 @@ typedef struct pach_stats {
     int conv_receiver_visible;    /* farthest receiver the source sees (-1: none), checked */
     int conv_receiver_occluded;   /* farthest receiver it does not see (-1: none), checked */
     int converged;                /* 1: every check passed */
+    /* mixed precision */
+    int64_t exact_rewalks;        /* bounces the FP32 screen left undecided, redone by the FP64 walk */
+    int64_t exact_walks;          /* grazing departures, walked in FP64 from the start */
+    double ms_exact;              /* summed HIP-event durations of the FP64-walk kernel */
 } pach_stats;
 
 /* Convergence checks of a trace with ray_count < 1 (TaskAll:111-176). */
@@ -211,11 +219,27 @@ int pach_map_histogram(pach_ctx *ctx, int plane, double *out, int scaled);
  * (1-64): idle lanes a wave gathers before refilling them (the refill
  * threshold); "dump_min" (>= 1): loop steps a drained wave runs before handing
  * its leftover rays to the next launch; "block" (256/512): workgroup size of
- * the octave kernel; "occupancy" (-1/0/4/5): 4 and 5 cap registers at 128 and
+ * the octave kernel; "occupancy" (-1/0/3/4/5): 4 and 5 cap registers at 128 and
  * 102 VGPRs so four or five waves share each SIMD, 0 leaves the compiler's
- * allocation, -1 (default) takes 4 for FP32 and 0 for FP64; "shade_queue" (0/1): a lane whose walk ended queues
- * its hit and walks on, and the wave shades 64 queued hits at a time. */
+ * allocation, -1 (default) takes 4 for FP32 and 3 for mixed and FP64, 3 caps
+ * at 168 VGPRs; "shade_queue" (0/1): a lane whose walk ended queues
+ * its hit and walks on, and the wave shades 64 queued hits at a time;
+ * "exact_queue" (>= 1, mixed precision): rays the queue of bounces left to the
+ * FP64 walk holds per launch beyond a reserve of 128 per wave (a wave drains
+ * once its push lands past that; default 2^25). */
 int pach_tune(pach_ctx *ctx, const char *key, int64_t value);
+
+/* Polygon_Scene.shoot (Classes_Scenes.cs:757-819; Hare Voxel_Grid.Shoot, whose
+ * contract is restated at Classes_VoxelGrid_PolyRefractive.cs:70-316) for n
+ * rays on the uploaded scene: origins, dirs [n][3] (unit directions); out
+ * tri [n] (triangle index, -1: the ray leaves the model; surf_of_tri gives
+ * the polygon) and t [n] (hit distance, 0 on a miss).  PACH_PREC_FP64 and
+ * PACH_PREC_MIXED return the FP64 closest hit (Moller-Trumbore, t > 1e-10,
+ * the lower triangle index on equal t); PACH_PREC_FP32 the FP32 walk of the
+ * FP32 tracer (no triangle excluded).  n_exact (may be NULL): mixed walks
+ * repeated with every candidate decided in FP64. */
+int pach_shoot(pach_ctx *ctx, const double *origins, const double *dirs, int64_t n, int precision, int *tri,
+               double *t, int64_t *n_exact);
 
 /* Deterministic direct sound of a point source (Direct_Sound,
  * Classes_Direct.cs:654-840), for the energy-time curve that

@@ -325,6 +325,15 @@ int or_shoot_brute(const or_scene *s, const double o[3], const double d[3], doub
 
 /* Amanatides-Woo DDA (Classes_VoxelGrid_PolyRefractive.cs:84-130, :199-313):
  * a hit ends the walk once its t lies within the current voxel's exit. */
+void or_shoot_brute_n(const or_scene *s, const double *o, const double *d, int64_t n, int *tri, double *t)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        double tt = 0;
+        tri[i] = or_shoot_brute(s, o + 3 * i, d + 3 * i, &tt);
+        t[i] = tri[i] >= 0 ? tt : 0.0;
+    }
+}
+
 int or_shoot_grid(const or_scene *s, const double o[3], const double d[3], double *t_out)
 {
     int c[3], step[3];

@@ -65,6 +65,8 @@ void or_scene_destroy(or_scene *s);
 /* closest hit with t > 1e-10; returns triangle index or -1 */
 int or_shoot_brute(const or_scene *s, const double o[3], const double d[3], double *t_out);
 int or_shoot_grid(const or_scene *s, const double o[3], const double d[3], double *t_out);
+/* or_shoot_brute for n rays (o, d [n][3]); tri -1 and t 0 on a miss */
+void or_shoot_brute_n(const or_scene *s, const double *o, const double *d, int64_t n, int *tri, double *t);
 double or_surface_area(const or_scene *s, int surf);
 void or_tri_normal(const or_scene *s, int tri, double n[3]);
 

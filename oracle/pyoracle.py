@@ -82,6 +82,7 @@ def lib():
         L.or_scene_destroy.argtypes = [C.c_void_p]
         L.or_shoot_brute.restype = C.c_int
         L.or_shoot_brute.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.or_shoot_brute_n.argtypes = [C.c_void_p, _dp, _dp, C.c_int64, _ip, _dp]
         L.or_shoot_grid.restype = C.c_int
         L.or_shoot_grid.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.or_surface_area.restype = C.c_double
@@ -220,6 +221,16 @@ class Scene:
         f = lib().or_shoot_grid if grid else lib().or_shoot_brute
         tri = f(self.h, _p(o), _p(d), _p(t))
         return tri, (float(t[0]) if tri >= 0 else None)
+
+    def shoot_brute_many(self, origins, dirs):
+        """or_shoot_brute for every ray: (triangle [n] (-1: miss), distance [n])."""
+        o = np.ascontiguousarray(origins, dtype=np.float64).reshape(-1, 3)
+        d = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+        n = o.shape[0]
+        tri = np.zeros(n, dtype=np.int32)
+        t = np.zeros(n)
+        lib().or_shoot_brute_n(self.h, _p(o), _p(d), n, tri.ctypes.data_as(_ip), _p(t))
+        return tri, t
 
     def surface_area(self, s):
         return lib().or_surface_area(self.h, s)

@@ -17,7 +17,7 @@ PACH_OK = 0
 ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "ESTATE", -4: "ENOMEM", -5: "ECANCEL", -6: "ERANGE"}
 PACH_ERANGE = -6
 MODE_STOCHASTIC, MODE_SPECULAR = 0, 1
-PREC_FP32, PREC_FP64 = 0, 1
+PREC_FP32, PREC_FP64, PREC_MIXED = 0, 1, 2
 DIR_GEODESIC, DIR_RANDOM, DIR_SUPPLIED = 0, 1, 2
 NPLANES = 11
 PLANES = {"energy": 0, "pressure": 1, "pos_x": 2, "pos_y": 3, "pos_z": 4, "neg_x": 5,
@@ -50,7 +50,8 @@ class Stats(C.Structure):
                 ("launches_octave", C.c_int64), ("ms_trace", C.c_double),
                 ("loop_steps", C.c_int64), ("voxels", C.c_int64), ("tri_tests", C.c_int64),
                 ("shade_passes", C.c_int64), ("lanes_shaded", C.c_int64), ("revolutions", C.c_int64),
-                ("conv_receiver_visible", C.c_int), ("conv_receiver_occluded", C.c_int), ("converged", C.c_int)]
+                ("conv_receiver_visible", C.c_int), ("conv_receiver_occluded", C.c_int), ("converged", C.c_int),
+                ("exact_rewalks", C.c_int64), ("exact_walks", C.c_int64), ("ms_exact", C.c_double)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}
@@ -140,6 +141,7 @@ SIGNATURES = [
     ("pach_cancel", C.c_int, [C.c_void_p]),
     ("pach_histogram", C.c_int, [C.c_void_p, C.c_int, _dp, C.c_int]),
     ("pach_stats_get", C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    ("pach_shoot", C.c_int, [C.c_void_p, _dp, _dp, C.c_int64, C.c_int, _ip, _dp, C.POINTER(C.c_int64)]),
     ("pach_percent_lost", C.c_int, [C.c_void_p, _dp]),
     ("pach_schroeder_integral", C.c_int, [_dp, C.c_int, _dp]),
     ("pach_t_x", C.c_int, [_dp, C.c_int, C.c_int, C.c_int, _dp]),
@@ -482,6 +484,18 @@ class Tracer:
         st = Stats()
         self._chk(lib().pach_stats_get(self.h, C.byref(st)))
         return st.as_dict()
+
+    def shoot(self, origins, dirs, precision=PREC_FP64):
+        """Polygon_Scene.shoot for each ray: (triangle [n] (-1: miss), distance [n], exact re-walks)."""
+        o = np.ascontiguousarray(origins, dtype=np.float64).reshape(-1, 3)
+        d = np.ascontiguousarray(dirs, dtype=np.float64).reshape(-1, 3)
+        n = o.shape[0]
+        tri = np.zeros(n, dtype=np.int32)
+        t = np.zeros(n)
+        ne = C.c_int64()
+        self._chk(lib().pach_shoot(self.h, _p(o), _p(d), n, int(precision), tri.ctypes.data_as(_ip), _p(t),
+                                   C.byref(ne)))
+        return tri, t, ne.value
 
     def percent_lost(self):
         out = C.c_double()

@@ -36,6 +36,7 @@ UNITS = [
     ("direct.cpp", [], False),
     ("trace_f32.hip", [], True),
     ("trace_f64.hip", ["-ffp-contract=off"], True),
+    ("trace_mix.hip", ["-ffp-contract=off"], True),
     ("image_source.hip", ["-ffp-contract=off"], True),
     ("filters.hip", [], True),
     ("denoise.hip", [], True),

@@ -44,6 +44,8 @@ template <class R> struct WorkBuf {
     size_t shq_waves = 0;
     ORay<R> *dump[2] = {nullptr, nullptr};
     unsigned dump_cap = 0;
+    ORay<R> *xq = nullptr;  // mixed precision: bounces for the FP64 walk (k_exact)
+    unsigned xq_cap = 0;
 };
 
 }  // namespace
@@ -51,7 +53,7 @@ template <class R> struct WorkBuf {
 struct pach_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     int n_cu = 0;
     std::string err;
 
@@ -67,6 +69,7 @@ struct pach_ctx {
     uint32_t *d_cell_range = nullptr, *d_cell_tris = nullptr;
     float4 *d_cell_rows = nullptr;  // FP32 test rows (r2, r0, r1) of every voxel-list entry
     std::vector<float> tri_rows32;  // [T][12] FP32 rows r2, r0, r1 per triangle
+    double row_norm = 0;            // largest |r0|, |r1| (barycentric gradient, 1 / shortest altitude)
     uint8_t *d_cell_dist = nullptr;
     uint64_t *d_cell_rmask = nullptr;
     int ncell = 0;
@@ -136,6 +139,7 @@ struct pach_ctx {
     int tune_walk_flat = 2;    // flattened walk, one kind of unit per wave iteration
     int tune_refill_min = 1;   // idle lanes gathered before a wave refills
     int tune_dump_min = 256;   // loop steps before a drained wave hands over its work
+    int64_t tune_xq = (int64_t)32 << 20;  // mixed precision: exact-queue rays beyond the per-wave reserve
     OctShape tune_shape;       // k_octave workgroup size and register cap
 
     // trace state
@@ -153,6 +157,12 @@ struct pach_ctx {
     double *f_env = nullptr, *f_car = nullptr, *f_part = nullptr, *f_out = nullptr, *f_tab = nullptr;
     size_t f_env_n = 0, f_part_n = 0, f_out_n = 0, f_tab_n = 0;
     int f_car_fs = 0;
+    // closest-hit query buffers (pach_shoot), grown on demand
+    double *q_o = nullptr, *q_d = nullptr, *q_t = nullptr;
+    int *q_tri = nullptr;
+    unsigned *q_exact = nullptr;
+    size_t q_o_n = 0, q_d_n = 0, q_t_n = 0, q_tri_n = 0;
+    int64_t q_exact_last = 0;
     std::mutex mu;
 };
 
@@ -204,6 +214,8 @@ template <class R> static void free_work(WorkBuf<R> &w)
 {
     dfree(w.p1_out); dfree(w.res[0]); dfree(w.res[1]); dfree(w.stacks); dfree(w.dump[0]); dfree(w.dump[1]);
     dfree(w.shq);
+    dfree(w.xq);
+    w.xq_cap = 0;
     w.p1_cap = w.res_cap = w.dump_cap = 0;
     w.stack_cap = 0;
     w.n_waves = 0;
@@ -341,6 +353,7 @@ extern "C" void pach_destroy(pach_ctx *c)
     free_scene(c->s64);
     free_work(c->w32);
     free_work(c->w64);
+    dfree(c->q_o); dfree(c->q_d); dfree(c->q_t); dfree(c->q_tri); dfree(c->q_exact);
     dfree(c->d_surf); dfree(c->d_cell_range); dfree(c->d_cell_tris); dfree(c->d_cell_dist); dfree(c->d_cell_rows);
     dfree(c->d_cell_rmask);
     dfree(c->d_dirs); dfree(c->d_geo);
@@ -485,6 +498,12 @@ extern "C" int pach_scene_upload(pach_ctx *c, const double *verts, int T, const 
         d1[4 * t + 3] = w1;
         d2[4 * t + 3] = w2;
     }
+    c->row_norm = 0;
+    for (int t = 0; t < T; ++t)
+        for (const std::vector<double> *dv : {&d0, &d1}) {
+            const double *q = dv->data() + 4 * t;
+            c->row_norm = std::max(c->row_norm, std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]));
+        }
     c->verts.assign(verts, verts + 9 * (size_t)T);
     c->surf.assign(surf, surf + T);
     c->tri_rows32.resize(12 * (size_t)T);
@@ -655,13 +674,17 @@ template <class R> const typename V4<R>::T *ctr_of(pach_ctx *c);
 template <> const float4 *ctr_of<float>(pach_ctx *c) { return c->d_ctr32; }
 template <> const double4 *ctr_of<double>(pach_ctx *c) { return c->d_ctr64; }
 
-template <class R> int ensure_work(pach_ctx *c, int64_t batch, size_t waves, bool queue)
+template <class R> int ensure_work(pach_ctx *c, int64_t batch, size_t waves, bool queue, bool exact_q = false)
 {
     WorkBuf<R> &w = work_of<R>(c);
     const unsigned p1_cap = (unsigned)std::min<int64_t>(batch * 200 + 4096, (int64_t)1 << 31);
     const unsigned res_cap = (unsigned)batch;
     const int stack_cap = 4096;
-    const unsigned dump_cap = (unsigned)std::min<size_t>(waves * (stack_cap + 64) + (1u << 20), (size_t)1 << 31);
+    // hand-over queue: every wave's stack and lanes, a batch's broadband output
+    // beside a backlog, and the continuations of the bounces the mixed kernel
+    // leaves to the FP64 walk
+    const unsigned dump_cap = (unsigned)std::min<size_t>(
+        waves * (stack_cap + 64) + (size_t)p1_cap + (exact_q ? ((size_t)16 << 20) : (1u << 20)), (size_t)1 << 31);
     hipError_t e = hipSuccess;
     if (w.p1_cap < p1_cap) {
         if ((e = dalloc(w.p1_out, p1_cap)) != hipSuccess) return fail(c, PACH_ENOMEM, "phase-1 buffer");
@@ -685,6 +708,12 @@ template <class R> int ensure_work(pach_ctx *c, int64_t batch, size_t waves, boo
         if ((e = dalloc(w.dump[0], dump_cap)) != hipSuccess || (e = dalloc(w.dump[1], dump_cap)) != hipSuccess)
             return fail(c, PACH_ENOMEM, "dump buffers");
         w.dump_cap = dump_cap;
+    }
+    // exact queue: a wave drains once its push lands past cap - 128 x waves
+    const unsigned xq_cap = (unsigned)std::min<size_t>(waves * 128 + (size_t)c->tune_xq, (size_t)1 << 31);
+    if (exact_q && w.xq_cap != xq_cap) {
+        if ((e = dalloc(w.xq, xq_cap)) != hipSuccess) return fail(c, PACH_ENOMEM, "exact queue");
+        w.xq_cap = xq_cap;
     }
     return PACH_OK;
 }
@@ -912,6 +941,40 @@ template <class R> DevScene<R> dev_scene(pach_ctx *c)
     return s;
 }
 
+// Error bounds of the mixed walk's FP32 screen (trace_impl.h, tri_filter and
+// mix_limits) for the current scene and grid.  eps = 2^-24; L: the largest
+// distance of a grid point from the coordinate origin, D: the grid diagonal,
+// G: the largest barycentric row norm.  The screen's plane distance is off by
+// at most 12 eps L + 8 eps |t| over |n . d| plus 3 eps |t| (rounded inputs,
+// three fused products, the reciprocal), its hit point by that plus
+// eps (2 L + D), its barycentric coordinates by G times that plus 11 eps L G;
+// each bound is doubled.  A voxel exit is off by at most 8 eps L / |d_axis|
+// + 8 eps t.  FP64 can hit the triangle a ray leaves again only when the
+// departure point's FP64 distance from its plane, ~2.2e-16 (L + D), exceeds
+// 1e-10 |n . d|.
+void mix_bounds(const pach_ctx *c, TraceConst &tc)
+{
+    const double eps = std::ldexp(1.0, -24);
+    double L = 0, D2 = 0;
+    for (int k = 0; k < 3; ++k) {
+        const double lo = c->gmin[k], hi = c->gmin[k] + c->g[k] * c->cs[k];
+        const double m = std::max(std::fabs(lo), std::fabs(hi));
+        L += m * m;
+        D2 += (hi - lo) * (hi - lo);
+    }
+    L = std::sqrt(L);
+    const double D = std::sqrt(D2), G = c->row_norm;
+    const double kt = 2 * eps * (12 * L + 8 * D), kt0 = 2 * 3 * eps * D;
+    tc.mx_kt = (float)kt;
+    tc.mx_kt0 = (float)kt0;
+    tc.mx_ku = (float)(2 * G * kt);
+    tc.mx_ku0 = (float)(2 * G * (kt0 + eps * (2 * L + D) + 11 * eps * L));
+    tc.mx_kx = (float)(8 * eps * L);
+    // grid.cpp lists a triangle in every voxel its box, grown by 1e-3 of a voxel per side, touches
+    tc.mx_infl = (float)(0.9e-3 * std::min({c->cs[0], c->cs[1], c->cs[2]}));
+    tc.mx_self = (float)std::max(1e-4, 16 * 1.1102230246251565e-16 * (L + D) / 1e-10);
+}
+
 template <class R> int run_trace(pach_ctx *c, const pach_params p)
 {
     (void)hipSetDevice(c->device);
@@ -934,6 +997,7 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
     for (int o = 0; o < 8; ++o) {
         bt.atten[o] = c->atten[o];
         bt.kexp2[o] = (float)(-0.1 * c->atten[o] * std::log2(10.0));
+        bt.kexp2d[o] = -0.1 * c->atten[o] * std::log2(10.0);
     }
     tc.band = c->d_band;
     // convergence mode (rayCount < 1): revolutions of 80 source rays until the checks pass
@@ -961,6 +1025,12 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
     s.cell_rmask = c->rmask_on ? c->d_cell_rmask : nullptr;
     const int lds_bytes = (c->ncell + 15) & ~15;
     s.dist_lds = (lds_bytes <= kMaxLdsDist && c->tune_dist_lds) ? 1 : 0;
+    // mixed precision: FP64 rays and decisions, the FP32 grid and rows screen the walk
+    const bool mix = sizeof(R) == 8 && p.precision == PACH_PREC_MIXED;
+    DevScene<float> sg = dev_scene<float>(c);
+    sg.cell_rmask = s.cell_rmask;
+    sg.dist_lds = s.dist_lds;
+    mix_bounds(c, tc);
 
     DevRecv<R> rv{};
     rv.ctr = ctr_of<R>(c);
@@ -1013,15 +1083,19 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
     int64_t batch = p.batch_rays > 0 ? p.batch_rays : std::min<int64_t>(N, 32768);
     batch = std::max<int64_t>(1, std::min<int64_t>(batch, 1 << 22));
     OctShape shape = c->tune_shape;
-    if (shape.occ < 0) shape.occ = sizeof(R) == 4 ? 4 : 0;
+    if (shape.occ < 0) shape.occ = sizeof(R) == 4 ? 4 : 3;
+    if (mix) shape.queue = 0;  // the mixed walk shades in place
     if (c->map_on) shape = OctShape{256, 0, 0};  // the map variant's launch shape
-    const int grid = c->n_cu * (c->map_on ? octave_map_blocks_per_cu<R>(s.dist_lds ? lds_bytes : 0)
-                                          : octave_blocks_per_cu<R>(shape, s.dist_lds ? lds_bytes : 0));
-    int rc = ensure_work<R>(c, batch, (size_t)grid * (shape.block / 64), shape.queue != 0);
+    const int lds_use = s.dist_lds ? lds_bytes : 0;
+    const int grid = c->n_cu * (c->map_on ? octave_map_blocks_per_cu<R>(lds_use)
+                                : mix    ? octave_mix_blocks_per_cu(shape, lds_use)
+                                         : octave_blocks_per_cu<R>(shape, lds_use));
+    int rc = ensure_work<R>(c, batch, (size_t)grid * (shape.block / 64), shape.queue != 0, mix);
     if (rc) return rc;
     WorkBuf<R> &w = work_of<R>(c);
     unsigned *p1_count = c->d_scal + 0, *res_count = c->d_scal + 1, *in_head = c->d_scal + 2;
     unsigned *dump_count = c->d_scal + 3;  // [2]
+    unsigned *xq_count = c->d_scal + 5;
     hipStream_t st = c->stream;
     HIPCHK(c, hipMemcpyAsync(c->d_band, &bt, sizeof(bt), hipMemcpyHostToDevice, st));
     HIPCHK(c, hipMemsetAsync(c->d_hist, 0, c->hist_n * kPlanes * sizeof(double), st));
@@ -1029,7 +1103,7 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
     HIPCHK(c, hipMemsetAsync(c->d_cnt, 0, sizeof(DevCounters), st));
     HIPCHK(c, hipMemsetAsync(c->d_scal, 0, 8 * sizeof(unsigned), st));
 
-    double ms_p1 = 0, ms_oct = 0;
+    double ms_p1 = 0, ms_oct = 0, ms_exact = 0;
     int64_t n_p1 = 0, n_oct = 0;
     int64_t started = 0;
     const int min_iters = c->tune_dump_min;
@@ -1055,7 +1129,9 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
                 HIPCHK(c, hipEventRecord(c->ev[1], st));
                 res_cur = 1 - res_cur;
                 p1_launched = true;
-            } else if (next < end && !stop_new) {
+            } else if (next < end && !stop_new && (size_t)pending + w.p1_cap <= w.dump_cap) {
+                // (a new batch only while the hand-over queue can take its output
+                // beside the backlog: a launch that drained early leaves work behind)
                 const int nb = (int)std::min<int64_t>(batch, end - next);
                 HIPCHK(c, hipEventRecord(c->ev[0], st));
                 HIPCHK(c, launch_phase1<R>(st, tc, s, rv, next, nb, nullptr, 0, w.p1_out, p1_count, w.p1_cap,
@@ -1069,18 +1145,39 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
             if (!p1_launched && pending == 0) break;
             HIPCHK(c, hipMemsetAsync(in_head, 0, sizeof(unsigned), st));
             HIPCHK(c, hipMemsetAsync(dump_count + (1 - cur), 0, sizeof(unsigned), st));
+            if (mix) HIPCHK(c, hipMemsetAsync(xq_count, 0, sizeof(unsigned), st));
             HIPCHK(c, hipEventRecord(c->ev[2], st));
-            HIPCHK(c, launch_octave<R>(st, grid, shape, tc, s, rv, pending ? w.dump[cur] : nullptr, dump_count + cur,
-                                       w.dump_cap, p1_launched ? w.p1_out : nullptr, p1_count, w.p1_cap, in_head,
-                                       w.stacks, w.stack_cap, shape.queue ? w.shq : nullptr, w.dump[1 - cur],
-                                       dump_count + (1 - cur), w.dump_cap, min_iters, c->d_cnt));
+            if constexpr (sizeof(R) == 8) {
+                if (mix) {
+                    HIPCHK(c, launch_octave_mix(st, grid, shape, tc, s, sg, rv, pending ? w.dump[cur] : nullptr,
+                                                dump_count + cur, w.dump_cap, p1_launched ? w.p1_out : nullptr,
+                                                p1_count, w.p1_cap, in_head, w.stacks, w.stack_cap, w.dump[1 - cur],
+                                                dump_count + (1 - cur), w.dump_cap, min_iters, c->d_cnt, w.xq,
+                                                xq_count, w.xq_cap));
+                }
+            }
+            if (!mix)
+                HIPCHK(c, launch_octave<R>(st, grid, shape, tc, s, sg, rv, pending ? w.dump[cur] : nullptr,
+                                           dump_count + cur, w.dump_cap, p1_launched ? w.p1_out : nullptr, p1_count,
+                                           w.p1_cap, in_head, w.stacks, w.stack_cap,
+                                           shape.queue ? w.shq : nullptr, w.dump[1 - cur], dump_count + (1 - cur),
+                                           w.dump_cap, min_iters, c->d_cnt));
             HIPCHK(c, hipEventRecord(c->ev[3], st));
+            if constexpr (sizeof(R) == 8) {
+                if (mix) {
+                    // the bounces the FP32 screen left undecided, by the FP64 walk, into the next input
+                    HIPCHK(c, launch_exact(st, c->n_cu * 4, tc, s, rv, w.xq, xq_count, w.xq_cap, w.dump[1 - cur],
+                                           dump_count + (1 - cur), w.dump_cap, c->d_cnt));
+                    HIPCHK(c, hipEventRecord(c->ev[4], st));
+                }
+            }
             unsigned h[8];
             HIPCHK(c, hipMemcpyAsync(h, c->d_scal, sizeof(h), hipMemcpyDeviceToHost, st));
             HIPCHK(c, hipStreamSynchronize(st));
             float ms = 0;
             if (p1_launched && hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) { ms_p1 += ms; ++n_p1; }
             if (hipEventElapsedTime(&ms, c->ev[2], c->ev[3]) == hipSuccess) { ms_oct += ms; ++n_oct; }
+            if (mix && hipEventElapsedTime(&ms, c->ev[3], c->ev[4]) == hipSuccess) ms_exact += ms;
             n_resume = h[1];
             cur = 1 - cur;
             pending = std::min(h[3 + cur], w.dump_cap);
@@ -1154,6 +1251,9 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
     S.tri_tests = (int64_t)hc.v[C_TEST];
     S.shade_passes = (int64_t)hc.v[C_SHADE];
     S.lanes_shaded = (int64_t)hc.v[C_SHADED];
+    S.exact_rewalks = (int64_t)hc.v[C_EXACT];
+    S.exact_walks = (int64_t)hc.v[C_EXACT0];
+    S.ms_exact = ms_exact;
     S.lost = hc.lost;
     S.e_init = 0;
     for (const RoomTab &t : rooms) {
@@ -1352,7 +1452,11 @@ extern "C" int pach_tune(pach_ctx *c, const char *key, int64_t value)
         c->tune_shape.block = (int)value;
         return PACH_OK;
     }
-    if (std::strcmp(key, "occupancy") == 0 && (value == -1 || value == 0 || value == 4 || value == 5)) {
+    if (std::strcmp(key, "exact_queue") == 0 && value >= 1 && value <= ((int64_t)1 << 30)) {
+        c->tune_xq = value;
+        return PACH_OK;
+    }
+    if (std::strcmp(key, "occupancy") == 0 && (value == -1 || value == 0 || value == 3 || value == 4 || value == 5)) {
         c->tune_shape.occ = (int)value;
         return PACH_OK;
     }
@@ -1384,7 +1488,10 @@ extern "C" int pach_trace(pach_ctx *c, const pach_params *p)
     if (p->oct_lo < 0 || p->oct_hi > 7 || p->oct_lo > p->oct_hi) return fail(c, PACH_EINVAL, "bad octave range");
     if (!(p->cutoff_m > 0)) return fail(c, PACH_EINVAL, "cutoff must be positive");
     if (p->mode != PACH_MODE_STOCHASTIC && p->mode != PACH_MODE_SPECULAR) return fail(c, PACH_EINVAL, "bad mode");
-    if (p->precision != PACH_PREC_FP32 && p->precision != PACH_PREC_FP64) return fail(c, PACH_EINVAL, "bad precision");
+    if (p->precision != PACH_PREC_FP32 && p->precision != PACH_PREC_FP64 && p->precision != PACH_PREC_MIXED)
+        return fail(c, PACH_EINVAL, "bad precision");
+    if (p->precision == PACH_PREC_MIXED && c->map_on)
+        return fail(c, PACH_EINVAL, "mixed precision does not trace mapping receivers (use FP32 or FP64)");
     c->cancel = false;
     c->rays_done = 0;
     c->rays_total = total;
@@ -1392,7 +1499,7 @@ extern "C" int pach_trace(pach_ctx *c, const pach_params *p)
     c->running = true;
     const pach_params pp = *p;
     auto body = [c, pp]() {
-        const int rc = pp.precision == PACH_PREC_FP64 ? run_trace<double>(c, pp) : run_trace<float>(c, pp);
+        const int rc = pp.precision == PACH_PREC_FP32 ? run_trace<float>(c, pp) : run_trace<double>(c, pp);
         c->status = rc;
         c->running = false;
         return rc;
@@ -1976,6 +2083,48 @@ extern "C" int pach_filter_stats_get(pach_ctx *c, pach_filter_stats *out)
 {
     if (!c || !out) return PACH_EINVAL;
     *out = c->filter_stats;
+    return PACH_OK;
+}
+
+extern "C" int pach_shoot(pach_ctx *c, const double *origins, const double *dirs, int64_t n, int precision,
+                          int *tri, double *t, int64_t *n_exact)
+{
+    if (!c || n < 0 || (n > 0 && (!origins || !dirs || !tri || !t))) return PACH_EINVAL;
+    if (c->running) return fail(c, PACH_ESTATE, "trace running");
+    if (!c->has_scene) return fail(c, PACH_ESTATE, "no scene");
+    if (precision != PACH_PREC_FP32 && precision != PACH_PREC_FP64 && precision != PACH_PREC_MIXED)
+        return fail(c, PACH_EINVAL, "bad precision");
+    if (n > ((int64_t)1 << 31)) return fail(c, PACH_EINVAL, "at most 2^31 rays per call");
+    if (n_exact) *n_exact = 0;
+    if (n == 0) return PACH_OK;
+    (void)hipSetDevice(c->device);
+    hipStream_t st = c->stream;
+    HIPCHK(c, dgrow(c->q_o, c->q_o_n, 3 * (size_t)n));
+    HIPCHK(c, dgrow(c->q_d, c->q_d_n, 3 * (size_t)n));
+    HIPCHK(c, dgrow(c->q_t, c->q_t_n, (size_t)n));
+    HIPCHK(c, dgrow(c->q_tri, c->q_tri_n, (size_t)n));
+    if (!c->q_exact) HIPCHK(c, dalloc(c->q_exact, 1));
+    HIPCHK(c, hipMemcpyAsync(c->q_o, origins, 3 * n * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipMemcpyAsync(c->q_d, dirs, 3 * n * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipMemsetAsync(c->q_exact, 0, sizeof(unsigned), st));
+    TraceConst tc{};
+    mix_bounds(c, tc);
+    DevScene<float> sg = dev_scene<float>(c);
+    if (precision == PACH_PREC_FP32) {
+        HIPCHK(c, launch_shoot<float>(st, tc, sg, sg, c->q_o, c->q_d, n, c->q_tri, c->q_t, c->q_exact));
+    } else {
+        const DevScene<double> s = dev_scene<double>(c);
+        if (precision == PACH_PREC_MIXED)
+            HIPCHK(c, launch_shoot_mix(st, tc, s, sg, c->q_o, c->q_d, n, c->q_tri, c->q_t, c->q_exact));
+        else
+            HIPCHK(c, launch_shoot<double>(st, tc, s, sg, c->q_o, c->q_d, n, c->q_tri, c->q_t, c->q_exact));
+    }
+    unsigned ne = 0;
+    HIPCHK(c, hipMemcpyAsync(tri, c->q_tri, n * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipMemcpyAsync(t, c->q_t, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipMemcpyAsync(&ne, c->q_exact, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    if (n_exact) *n_exact = ne;
     return PACH_OK;
 }
 

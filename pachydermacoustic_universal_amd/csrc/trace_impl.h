@@ -24,6 +24,8 @@
 
 #include "trace_types.h"
 
+#include <type_traits>
+
 namespace pach {
 
 // ---------------------------------------------------------------- RNG
@@ -68,6 +70,12 @@ template <> __device__ __forceinline__ float unif<float>(uint32_t w) { return (f
 template <class R> __device__ __forceinline__ R rcp(R x);
 template <> __device__ __forceinline__ float rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 template <> __device__ __forceinline__ double rcp<double>(double x) { return 1.0 / x; }
+
+// a * b + c: one fused operation in FP32; in FP64 two separately rounded
+// operations (the FP64 units are built without contraction, as the oracle is).
+template <class R> __device__ __forceinline__ R mad(R a, R b, R c);
+template <> __device__ __forceinline__ float mad<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+template <> __device__ __forceinline__ double mad<double>(double a, double b, double c) { return a * b + c; }
 
 // FP64: sqrt and IEEE division, as the oracle; FP32: one reciprocal square root.
 template <class R> __device__ __forceinline__ void normalize3(R &x, R &y, R &z)
@@ -130,28 +138,42 @@ __device__ __forceinline__ bool tri_hit_mt(const DevScene<double> &s, int t, dou
 
 // FP32 test against the triangle's dual basis (host-precomputed rows r0, r1,
 // r2): r2 . x + r2.w is the signed plane distance, r0 / r1 give the barycentric
-// coordinates u, v of a point x of the plane.  The three rows are read
-// together and the test has no early return, so lanes whose candidates fail at
-// different stages run one instruction stream.  Edges are widened by 4e-6
-// (barycentric) so no ray slips between neighbours.  The rows are read from
-// the voxel list's copy at list index j.
-__device__ __forceinline__ bool tri_hit_dual(const DevScene<float> &s, uint32_t j, float ox, float oy, float oz,
-                                             float dx, float dy, float dz, float tlim, float &tout)
+// coordinates u, v of a point x of the plane.  The rows are read from the
+// voxel list's copy at list index j (load_rows); the test has no early return
+// and combines its conditions without short-circuiting, so lanes whose
+// candidates fail at different stages run one instruction stream and no row
+// load is deferred behind a condition.  Edges are widened by 4e-6
+// (barycentric) so no ray slips between neighbours.
+struct Rows {
+    float4 r2, r0, r1;
+};
+
+__device__ __forceinline__ Rows load_rows(const DevScene<float> &s, uint32_t j)
 {
     const float4 *rows = s.cell_rows + 3 * (size_t)j;
-    const float4 r2 = rows[0];
-    const float4 r0 = rows[1];
-    const float4 r1 = rows[2];
+    return Rows{rows[0], rows[1], rows[2]};
+}
+
+__device__ __forceinline__ bool tri_hit_rows(const Rows &q, float ox, float oy, float oz, float dx, float dy,
+                                             float dz, float tlim, float &tout)
+{
+    const float4 r2 = q.r2, r0 = q.r0, r1 = q.r1;
     const float den = r2.x * dx + r2.y * dy + r2.z * dz;
     const float num = -(r2.x * ox + r2.y * oy + r2.z * oz + r2.w);
     const float tt = num * __builtin_amdgcn_rcpf(den);
     const float px = ox + dx * tt, py = oy + dy * tt, pz = oz + dz * tt;
     const float u = r0.x * px + r0.y * py + r0.z * pz + r0.w;
     const float v = r1.x * px + r1.y * py + r1.z * pz + r1.w;
-    const bool ok = (tt > 0.0f && tt <= tlim) && !(u < -4e-6f || u > 1.000004f) &&
-                    !(v < -4e-6f || u + v > 1.000004f);
+    const bool ok = (tt > 0.0f) & (tt <= tlim) & !(u < -4e-6f) & !(u > 1.000004f) & !(v < -4e-6f) &
+                    !(u + v > 1.000004f);
     tout = ok ? tt : tout;
     return ok;
+}
+
+__device__ __forceinline__ bool tri_hit_dual(const DevScene<float> &s, uint32_t j, float ox, float oy, float oz,
+                                             float dx, float dy, float dz, float tlim, float &tout)
+{
+    return tri_hit_rows(load_rows(s, j), ox, oy, oz, dx, dy, dz, tlim, tout);
 }
 
 // Triangle t, entry j of the voxel lists.
@@ -161,6 +183,32 @@ __device__ __forceinline__ bool tri_hit(const DevScene<R> &s, int t, uint32_t j,
 {
     if constexpr (sizeof(R) == 8) return tri_hit_mt(s, t, ox, oy, oz, dx, dy, dz, tlim, tout);
     else return tri_hit_dual(s, j, ox, oy, oz, dx, dy, dz, tlim, tout);
+}
+
+// Conservative FP32 screen of list entry j for the mixed walk (the FP64
+// decisions come from tri_hit_mt).  The screen passes every triangle the FP64
+// test could accept with a distance t <= tlim: tt is the FP32 distance, e a
+// bound on its error, and the barycentric test is widened by its own error
+// bound; both grow with 1 / |n . d| (grazing rays).  Every product is rounded
+// once (fused), which the bounds assume.  A ray parallel to the plane in FP32
+// (n . d == 0) is flagged in `para`: FP32 cannot place its hit.
+__device__ __forceinline__ bool tri_filter(const Rows &q, const TraceConst &tc, float ox, float oy, float oz,
+                                           float dx, float dy, float dz, float tlim, float &tt, float &e, bool &para)
+{
+    const float4 r2 = q.r2, r0 = q.r0, r1 = q.r1;
+    const float den = __builtin_fmaf(r2.x, dx, __builtin_fmaf(r2.y, dy, r2.z * dz));
+    const float num = -__builtin_fmaf(r2.x, ox, __builtin_fmaf(r2.y, oy, __builtin_fmaf(r2.z, oz, r2.w)));
+    const float inv = __builtin_amdgcn_rcpf(den);
+    tt = num * inv;
+    const float ai = fabsf(inv);
+    e = __builtin_fmaf(tc.mx_kt, ai, tc.mx_kt0);
+    const float eu = __builtin_fmaf(tc.mx_ku, ai, tc.mx_ku0);
+    const float px = __builtin_fmaf(dx, tt, ox), py = __builtin_fmaf(dy, tt, oy), pz = __builtin_fmaf(dz, tt, oz);
+    const float u = __builtin_fmaf(r0.x, px, __builtin_fmaf(r0.y, py, __builtin_fmaf(r0.z, pz, r0.w)));
+    const float v = __builtin_fmaf(r1.x, px, __builtin_fmaf(r1.y, py, __builtin_fmaf(r1.z, pz, r1.w)));
+    para = den == 0.0f;
+    // non-short-circuit &, so no row load is sunk behind the distance test
+    return (tt + e > 0.0f) & (tt - e <= tlim) & !(u < -eu) & !(v < -eu) & !(u + v > 1.0f + 2.0f * eu);
 }
 
 // Voxel distances staged in LDS by k_octave (dynamic shared memory).
@@ -189,6 +237,12 @@ template <class R> struct Walk {
     uint64_t rmask;   // receivers whose spheres touch a voxel the walk crossed
     uint32_t j, je;   // flattened walk: untested list entries of the current voxel
     int k;            // flattened walk: distance value of the current voxel
+    // mixed walk only (unused, and removed by the compiler, elsewhere)
+    R eb;             // error bound of tbest
+    R sl, sl2;        // least and second least lower bound t - e of the other candidates (-inf: undecidable)
+    int sid;          // the candidate of sl
+    int zid;          // a candidate whose interval reaches t <= 0 (a neighbour at the departure point)
+    R tlim, tend;     // current voxel: candidates accepted up to tlim; the walk ends once tbest + eb <= tend
 };
 
 // t where the ray leaves voxel c along one axis: the upper face when d > 0,
@@ -198,7 +252,7 @@ template <class R>
 __device__ __forceinline__ R axis_exit(R o, R d, R inv, int c, const R gmin, const R cs)
 {
     const R INF = R(__builtin_huge_val());
-    const R v = (gmin + R(c + (d > R(0) ? 1 : 0)) * cs - o) * inv;
+    const R v = (mad<R>(R(c + (d > R(0) ? 1 : 0)), cs, gmin) - o) * inv;
     return d != R(0) ? v : INF;
 }
 
@@ -315,6 +369,43 @@ __device__ __forceinline__ void walk_enter(const DevScene<R> &s, Walk<R> &w, uin
     }
 }
 
+// One move of the flattened walk: across the empty block around the voxel
+// (k >= 2), else one voxel (the block is the voxel itself), by the same
+// straight-line code for both so stepping and jumping lanes do not diverge.
+// Returns true when the walk leaves the grid.
+template <class R>
+__device__ __forceinline__ bool walk_move(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R dx, R dy, R dz)
+{
+    const int h = max(w.k - 1, 0);
+    const int lx = max(w.cx - h, 0), hx = min(w.cx + h, s.gx - 1);
+    const int ly = max(w.cy - h, 0), hy = min(w.cy + h, s.gy - 1);
+    const int lz = max(w.cz - h, 0), hz = min(w.cz + h, s.gz - 1);
+    const R bx = axis_exit<R>(ox, dx, w.ix, dx > R(0) ? hx : lx, s.gmin[0], s.cs[0]);
+    const R by = axis_exit<R>(oy, dy, w.iy, dy > R(0) ? hy : ly, s.gmin[1], s.cs[1]);
+    const R bz = axis_exit<R>(oz, dz, w.iz, dz > R(0) ? hz : lz, s.gmin[2], s.cs[2]);
+    const int ax = (bx < by) ? (bx < bz ? 0 : 2) : (by < bz ? 1 : 2);
+    const R tb = ax == 0 ? bx : (ax == 1 ? by : bz);
+    const R px = mad<R>(dx, tb, ox), py = mad<R>(dy, tb, oy), pz = mad<R>(dz, tb, oz);
+    const int fx = (int)floor((px - s.gmin[0]) * s.inv_cs[0]);
+    const int fy = (int)floor((py - s.gmin[1]) * s.inv_cs[1]);
+    const int fz = (int)floor((pz - s.gmin[2]) * s.inv_cs[2]);
+    // never move backwards; stay inside the block on the other axes
+    // (clamp to [c, h] moving up, [l, c] moving down, c when still)
+    int nx = min(max(fx, dx < R(0) ? lx : w.cx), dx > R(0) ? hx : w.cx);
+    int ny = min(max(fy, dy < R(0) ? ly : w.cy), dy > R(0) ? hy : w.cy);
+    int nz = min(max(fz, dz < R(0) ? lz : w.cz), dz > R(0) ? hz : w.cz);
+    nx = ax == 0 ? (dx > R(0) ? hx + 1 : lx - 1) : nx;
+    ny = ax == 1 ? (dy > R(0) ? hy + 1 : ly - 1) : ny;
+    nz = ax == 2 ? (dz > R(0) ? hz + 1 : lz - 1) : nz;
+    if ((unsigned)nx >= (unsigned)s.gx || (unsigned)ny >= (unsigned)s.gy || (unsigned)nz >= (unsigned)s.gz)
+        return true;
+    w.cx = nx; w.cy = ny; w.cz = nz;
+    w.tmx = axis_exit<R>(ox, dx, w.ix, nx, s.gmin[0], s.cs[0]);
+    w.tmy = axis_exit<R>(oy, dy, w.iy, ny, s.gmin[1], s.cs[1]);
+    w.tmz = axis_exit<R>(oz, dz, w.iz, nz, s.gmin[2], s.cs[2]);
+    return false;
+}
+
 // Flattened form of walk_steps: each of the `budget` units either tests the
 // next three list entries of the current voxel or leaves the voxel (one step,
 // or one jump across an empty block, by the same straight-line code), so
@@ -342,7 +433,7 @@ __device__ bool walk_units(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R
         if (pending) {
             const R tlim = fmin(w.tbest, texit * R(1.0000001) + R(1e-6));
             // the tests first: their row loads then need not wait for the index loads
-            auto take = [&](bool hit, int t, R tt) {
+            auto take = [&](bool hit, int t, R tt) __attribute__((always_inline)) {
                 if (hit && t != w.best && !(Tmin<R>::excl && t == excl) &&
                     (tt < w.tbest || (tt == w.tbest && t < w.best))) {
                     w.tbest = tt;
@@ -362,9 +453,18 @@ __device__ bool walk_units(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R
                 const int t1 = (int)s.cell_tris[j1];
                 const int t2 = (int)s.cell_tris[j2];
                 R tt0, tt1, tt2;
-                const bool h0 = tri_hit<R>(s, t0, j0, ox, oy, oz, dx, dy, dz, tlim, tt0);
-                const bool h1 = tri_hit<R>(s, t1, j1, ox, oy, oz, dx, dy, dz, tlim, tt1);
-                const bool h2 = tri_hit<R>(s, t2, j2, ox, oy, oz, dx, dy, dz, tlim, tt2);
+                bool h0, h1, h2;
+                if constexpr (sizeof(R) == 4) {
+                    // every row load of the unit first, then the three tests
+                    const Rows q0 = load_rows(s, j0), q1 = load_rows(s, j1), q2 = load_rows(s, j2);
+                    h0 = tri_hit_rows(q0, ox, oy, oz, dx, dy, dz, tlim, tt0);
+                    h1 = tri_hit_rows(q1, ox, oy, oz, dx, dy, dz, tlim, tt1);
+                    h2 = tri_hit_rows(q2, ox, oy, oz, dx, dy, dz, tlim, tt2);
+                } else {
+                    h0 = tri_hit<R>(s, t0, j0, ox, oy, oz, dx, dy, dz, tlim, tt0);
+                    h1 = tri_hit<R>(s, t1, j1, ox, oy, oz, dx, dy, dz, tlim, tt1);
+                    h2 = tri_hit<R>(s, t2, j2, ox, oy, oz, dx, dy, dz, tlim, tt2);
+                }
                 take(h0, t0, tt0);
                 take(h1, t1, tt1);
                 take(h2, t2, tt2);
@@ -372,41 +472,172 @@ __device__ bool walk_units(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R
             continue;
         }
         if (w.best >= 0 && w.tbest <= texit) return true;
-        {
-            // one move: across the empty block around the voxel (k >= 2), else
-            // one voxel (the block is the voxel itself), the same straight-line
-            // code for both so stepping and jumping lanes do not diverge
-            const int h = max(w.k - 1, 0);
-            const int lx = max(w.cx - h, 0), hx = min(w.cx + h, s.gx - 1);
-            const int ly = max(w.cy - h, 0), hy = min(w.cy + h, s.gy - 1);
-            const int lz = max(w.cz - h, 0), hz = min(w.cz + h, s.gz - 1);
-            const R bx = axis_exit<R>(ox, dx, w.ix, dx > R(0) ? hx : lx, s.gmin[0], s.cs[0]);
-            const R by = axis_exit<R>(oy, dy, w.iy, dy > R(0) ? hy : ly, s.gmin[1], s.cs[1]);
-            const R bz = axis_exit<R>(oz, dz, w.iz, dz > R(0) ? hz : lz, s.gmin[2], s.cs[2]);
-            const int ax = (bx < by) ? (bx < bz ? 0 : 2) : (by < bz ? 1 : 2);
-            const R tb = ax == 0 ? bx : (ax == 1 ? by : bz);
-            const R px = ox + dx * tb, py = oy + dy * tb, pz = oz + dz * tb;
-            const int fx = (int)floor((px - s.gmin[0]) * s.inv_cs[0]);
-            const int fy = (int)floor((py - s.gmin[1]) * s.inv_cs[1]);
-            const int fz = (int)floor((pz - s.gmin[2]) * s.inv_cs[2]);
-            // never move backwards; stay inside the block on the other axes
-            // (clamp to [c, h] moving up, [l, c] moving down, c when still)
-            int nx = min(max(fx, dx < R(0) ? lx : w.cx), dx > R(0) ? hx : w.cx);
-            int ny = min(max(fy, dy < R(0) ? ly : w.cy), dy > R(0) ? hy : w.cy);
-            int nz = min(max(fz, dz < R(0) ? lz : w.cz), dz > R(0) ? hz : w.cz);
-            nx = ax == 0 ? (dx > R(0) ? hx + 1 : lx - 1) : nx;
-            ny = ax == 1 ? (dy > R(0) ? hy + 1 : ly - 1) : ny;
-            nz = ax == 2 ? (dz > R(0) ? hz + 1 : lz - 1) : nz;
-            if ((unsigned)nx >= (unsigned)s.gx || (unsigned)ny >= (unsigned)s.gy || (unsigned)nz >= (unsigned)s.gz)
-                return true;
-            w.cx = nx; w.cy = ny; w.cz = nz;
-            w.tmx = axis_exit<R>(ox, dx, w.ix, nx, s.gmin[0], s.cs[0]);
-            w.tmy = axis_exit<R>(oy, dy, w.iy, ny, s.gmin[1], s.cs[1]);
-            w.tmz = axis_exit<R>(oz, dz, w.iz, nz, s.gmin[2], s.cs[2]);
-        }
+        if (walk_move<R>(s, w, ox, oy, oz, dx, dy, dz)) return true;
         walk_enter<R, LDS>(s, w, n_vox);
     }
     return false;
+}
+
+// ---------------------------------------------------------------- mixed walk
+// The mixed walk gives the FP64 closest hit (the Moller-Trumbore test of the
+// FP64 path and the oracle, t > 1e-10, lowest triangle on equal t) at the cost
+// of an FP32 walk.  The FP32 walk screens candidates conservatively
+// (tri_filter) and keeps the one nearest by FP32 distance with its error bound
+// e, plus the least lower bounds t - e of the other candidates.  After the
+// walk the FP64 test decides the few candidates that can be nearest
+// (mix_confirm); when that is not enough to be certain, the bounce is redone
+// by the FP64 walk (k_exact, or shoot<double>).  The triangle the ray leaves is
+// skipped in FP32: FP64 accepts it again only when the ray leaves at a grazing
+// angle, |n . d| < mx_self, and such bounces go to the FP64 walk directly.
+
+// Limits of the voxel just entered.  texit, the FP32 exit distance, is off by
+// at most kx / |d_a| + 8 eps texit on its exit axis a.  Every voxel list reaches
+// mx_infl past its voxel, so a hit up to mx_infl beyond the exit is tested in
+// this voxel: candidates are accepted up to texit + infl + err, and once the
+// best hit's upper bound is below texit + infl - err no untested triangle can
+// be nearer.
+__device__ __forceinline__ void mix_limits(const TraceConst &tc, Walk<float> &w)
+{
+    const float texit = fminf(w.tmx, fminf(w.tmy, w.tmz));
+    // select values, not fields: a conditional over two fields is an lvalue, and
+    // loading through the selected address keeps the whole walk state in memory
+    const float ix = w.ix, iy = w.iy, iz = w.iz;
+    const float ia = w.tmx <= w.tmy ? (w.tmx <= w.tmz ? float(ix) : float(iz)) : (w.tmy <= w.tmz ? float(iy) : float(iz));
+    const float err = __builtin_fmaf(tc.mx_kx, fabsf(ia), 4.8e-7f * texit);
+    w.tlim = texit + tc.mx_infl + err;
+    w.tend = texit + tc.mx_infl - err;
+}
+
+// Records lower bound lb of candidate id (not the best) among the two least.
+__device__ __forceinline__ void mix_lower(Walk<float> &w, float lb, int id)
+{
+    if (id == w.sid) return;  // listed again in a later voxel
+    if (lb < w.sl) {
+        w.sl2 = w.sl;
+        w.sl = lb;
+        w.sid = id;
+    } else {
+        w.sl2 = fminf(w.sl2, lb);
+    }
+}
+
+__device__ __forceinline__ void mix_walk_init(const DevScene<float> &s, Walk<float> &w, const float *o, const float *d)
+{
+    walk_init<float>(s, w, o[0], o[1], o[2], d[0], d[1], d[2]);
+    const float INF = __builtin_huge_valf();
+    w.eb = 0.0f;
+    w.sl = w.sl2 = INF;
+    w.sid = w.zid = -1;
+    w.tlim = w.tend = 0.0f;
+}
+
+// Flattened walk (as walk_units, SCHED: one kind of unit per wave iteration)
+// in mixed form; excl: the triangle left.
+template <bool LDS, bool SCHED>
+__device__ bool walk_units_mix(const DevScene<float> &s, const TraceConst &tc, Walk<float> &w, float ox, float oy,
+                               float oz, float dx, float dy, float dz, int excl, int budget, uint32_t &n_vox,
+                               uint32_t &n_test)
+{
+    if (w.k < 0) {
+        walk_enter<float, LDS>(s, w, n_vox);
+        mix_limits(tc, w);
+    }
+    for (int it = 0; it < budget; ++it) {
+        const bool pending = w.j < w.je;
+        if constexpr (SCHED) {
+            const int n_pending = __popcll(__ballot(pending));
+            const int n_walking = __popcll(__ballot(true));
+            const bool tests_now = 2 * n_pending >= n_walking;
+            if (pending != tests_now) continue;
+        }
+        if (pending) {
+            const uint32_t j0 = w.j, jl = w.je - 1;
+            const uint32_t j1 = min(j0 + 1, jl), j2 = min(j0 + 2, jl);
+            w.j = j2 + 1;
+            n_test += j2 - j0 + 1;
+            const int t0 = (int)s.cell_tris[j0];
+            const int t1 = (int)s.cell_tris[j1];
+            const int t2 = (int)s.cell_tris[j2];
+            // a candidate can matter only while its lower bound is below the best's upper bound
+            const float lim = fminf(w.tlim, w.tbest + w.eb);
+            float tt0, tt1, tt2, e0, e1, e2;
+            bool p0, p1, p2;
+            // every row load of the unit first, then the three screens
+            const Rows q0 = load_rows(s, j0), q1 = load_rows(s, j1), q2 = load_rows(s, j2);
+            const bool c0 = tri_filter(q0, tc, ox, oy, oz, dx, dy, dz, lim, tt0, e0, p0);
+            const bool c1 = tri_filter(q1, tc, ox, oy, oz, dx, dy, dz, lim, tt1, e1, p1);
+            const bool c2 = tri_filter(q2, tc, ox, oy, oz, dx, dy, dz, lim, tt2, e2, p2);
+            auto take = [&](bool c, bool par, int t, float tt, float e) __attribute__((always_inline)) {
+                if (t == w.best || !(c || par)) return;  // not a candidate, or listed again
+                if (t != excl) {
+                    if (par) {
+                        w.sl = -__builtin_huge_valf();  // FP32 cannot place this hit
+                    } else if (tt - e <= 0.0f) {
+                        // possibly a hit FP64 rejects (t <= 1e-10): decided after the walk
+                        if (w.zid < 0 || w.zid == t) w.zid = t;
+                        else w.sl = -__builtin_huge_valf();
+                    } else if (tt < w.tbest) {
+                        if (w.best >= 0) mix_lower(w, w.tbest - w.eb, w.best);
+                        w.tbest = tt;
+                        w.eb = e;
+                        w.best = t;
+                    } else {
+                        mix_lower(w, tt - e, t);
+                    }
+                }
+            };
+            take(c0, p0, t0, tt0, e0);
+            take(c1, p1, t1, tt1, e1);
+            take(c2, p2, t2, tt2, e2);
+            continue;
+        }
+        if (w.best >= 0 && w.tbest + w.eb <= w.tend) return true;
+        if (walk_move<float>(s, w, ox, oy, oz, dx, dy, dz)) return true;
+        walk_enter<float, LDS>(s, w, n_vox);
+        mix_limits(tc, w);
+    }
+    return false;
+}
+
+// After a mixed walk: the FP64 closest hit (hit, t64; hit -1: a miss) when it
+// is certain, else false (the bounce needs the FP64 walk).  The FP64 test
+// decides the best candidate, the candidate with the least other lower bound
+// when the two intervals overlap (two triangles at a shared edge) or the best
+// fails, and the candidate near the departure point.  The nearest accepted one
+// is certain when every candidate not tested lies beyond it (the recorded
+// lower bounds; the walk skipped only candidates beyond the best's upper
+// bound tbest + eb) and it lies before the end of the walk.
+__device__ __forceinline__ bool mix_confirm(const DevScene<double> &s64, const Walk<float> &w, const double *o,
+                                            const double *d, int &hit, double &t64)
+{
+    if (w.sl == -__builtin_huge_valf()) return false;
+    double tm = __builtin_huge_val();
+    int im = -1;
+    auto test = [&](int t) __attribute__((always_inline)) {
+        double tt;
+        if (tri_hit_mt(s64, t, o[0], o[1], o[2], d[0], d[1], d[2], __builtin_huge_val(), tt) &&
+            (tt < tm || (tt == tm && t < im))) {
+            tm = tt;
+            im = t;
+        }
+    };
+    if (w.best >= 0) test(w.best);
+    float lb = w.sl;  // least lower bound of the candidates not tested in FP64
+    if (w.sid >= 0 && (im < 0 || w.sl <= w.tbest + w.eb)) {
+        test(w.sid);
+        lb = w.sl2;
+    }
+    if (w.zid >= 0) test(w.zid);
+    if (im < 0) {
+        hit = -1;
+        t64 = 0.0;
+        return w.best < 0 && w.zid < 0;  // a miss is certain only without candidates
+    }
+    if (!(tm < (double)lb) || !(tm <= (double)w.tend)) return false;
+    if (w.best >= 0 && !(tm <= (double)(w.tbest + w.eb))) return false;
+    hit = im;
+    t64 = tm;
+    return true;
 }
 
 // Complete closest-hit query (the broadband phase uses it directly).
@@ -425,10 +656,35 @@ __device__ __forceinline__ int shoot(const DevScene<R> &s, R ox, R oy, R oz, R d
 }
 
 // ---------------------------------------------------------------- receivers
+// Event counters of a launch.  The voxel and test counts the walk bumps in its
+// inner loop stay in registers; every other event goes to its workgroup's
+// table in LDS by atomics, so no register holds those counts.
 struct LaneCounters {
-    uint32_t v[C_N];
-    double lost;
+    uint32_t vox, test;            // per lane
+    unsigned long long *sc;        // LDS [C_N]
+    double *sl;                    // LDS: lost energy
+    __device__ __forceinline__ void add(int c, unsigned n = 1u) const { atomicAdd(sc + c, (unsigned long long)n); }
+    __device__ __forceinline__ void lose(double e) const { atomicAdd(sl, e); }
 };
+
+// The workgroup's counter table: cleared by its threads, flushed to the
+// launch's device counters once every wave of the workgroup is done.
+#define PACH_COUNTERS_BEGIN(lc)                                                  \
+    __shared__ unsigned long long pach_sc[C_N];                                   \
+    __shared__ double pach_sl;                                                     \
+    for (int i_ = threadIdx.x; i_ < C_N; i_ += blockDim.x) pach_sc[i_] = 0;       \
+    if (threadIdx.x == 0) pach_sl = 0;                                             \
+    __syncthreads();                                                               \
+    LaneCounters lc{0u, 0u, pach_sc, &pach_sl}
+
+__device__ __forceinline__ void counters_end(const LaneCounters &lc, DevCounters *cnt)
+{
+    if (lc.vox) atomicAdd(lc.sc + C_VOXEL, (unsigned long long)lc.vox);
+    if (lc.test) atomicAdd(lc.sc + C_TEST, (unsigned long long)lc.test);
+    __syncthreads();
+    if (threadIdx.x < C_N && lc.sc[threadIdx.x]) atomicAdd(&cnt->v[threadIdx.x], lc.sc[threadIdx.x]);
+    if (threadIdx.x == 0 && *lc.sl != 0) unsafeAtomicAdd(&cnt->lost, *lc.sl);
+}
 
 // p: the pressure sqrt(e rho_c), computed by the caller in the ray's precision
 __device__ __forceinline__ void hist_add(const double *base_unused, double *hist, uint64_t stride,
@@ -446,9 +702,11 @@ __device__ __forceinline__ void hist_add(const double *base_unused, double *hist
 }
 
 template <class R> __device__ __forceinline__ double band_gain(const DevRecv<R> &rv, int band, double dist);
+// 10^(-0.1 a x) as exp2(k x), k = -0.1 a log2(10) (within ~1e-15 of the
+// reference's Math.Pow(10, ...); libm's pow needs far more registers)
 template <> __device__ __forceinline__ double band_gain<double>(const DevRecv<double> &rv, int band, double dist)
 {
-    return pow(10.0, -.1 * rv.band->atten[band] * dist);
+    return exp2(rv.band->kexp2d[band] * dist);
 }
 template <> __device__ __forceinline__ double band_gain<float>(const DevRecv<float> &rv, int band, double dist)
 {
@@ -463,11 +721,14 @@ template <class R> __device__ __forceinline__ double pressure(double e, double r
     else return (double)__builtin_amdgcn_sqrtf((float)(e * rho_c));
 }
 
-// Receiver_Bank.CheckRay / CheckBroadbandRay over every receiver.
-// sqd: squared distance to EndPt; E: intensity (octave) or energies (broadband).
-template <class R>
+// Receiver_Bank.CheckRay (BROAD false: an octave ray of intensity I in
+// `band`) / CheckBroadbandRay (BROAD true: energies E[lo..hi]) for receiver r.
+// sqd: squared distance to EndPt.  The octave form takes its intensity by
+// value: a pointer into the ray that a runtime band could index would keep the
+// whole ray record in private memory.
+template <class R, bool BROAD>
 __device__ __forceinline__ void receiver_test(const DevRecv<R> &rv, int r, R ox, R oy, R oz, R dx, R dy, R dz,
-                                              R sqd, const R *E, int band, int lo, int hi, double tsum, int ana,
+                                              R sqd, const R *E, R I, int band, int lo, int hi, double tsum, int ana,
                                               LaneCounters &lc)
 {
     const auto C = rv.ctr[r];
@@ -487,17 +748,17 @@ __device__ __forceinline__ void receiver_test(const DevRecv<R> &rv, int r, R ox,
     if (sample >= rv.sct || sample < 0) return;
     const double dist = rt * rv.c;
     const double ndx = -(double)dx, ndy = -(double)dy, ndz = -(double)dz;
-    if (band >= 0) {
+    if constexpr (!BROAD) {
         const uint64_t idx = ((uint64_t)r * kBands + band) * rv.sct + sample;
-        const double e = (double)E[0] * band_gain<R>(rv, band, dist) * rv.sizemod * ts;
+        const double e = (double)I * band_gain<R>(rv, band, dist) * rv.sizemod * ts;
         hist_add(nullptr, rv.hist, rv.plane_stride, idx, e, pressure<R>(e, rv.rho_c), ndx, ndy, ndz, ana);
-        lc.v[C_DEP]++;
+        lc.add(C_DEP);
     } else {
         for (int o = lo; o <= hi; ++o) {
             const uint64_t idx = ((uint64_t)r * kBands + o) * rv.sct + sample;
             const double e = (double)E[o] * band_gain<R>(rv, o, dist) * rv.sizemod * ts;
             hist_add(nullptr, rv.hist, rv.plane_stride, idx, e, pressure<R>(e, rv.rho_c), ndx, ndy, ndz, 0);
-            lc.v[C_DEP]++;
+            lc.add(C_DEP);
         }
     }
 }
@@ -507,9 +768,9 @@ __device__ __forceinline__ void receiver_test(const DevRecv<R> &rv, int r, R ox,
 // masks, only receivers whose spheres touch a voxel the walk crossed are
 // tested: a sphere the segment passes through contains the chord midpoint,
 // which lies in a crossed voxel, so the deposits are those of the full loop.
-template <class R>
+template <class R, bool BROAD>
 __device__ void check_receivers(const DevRecv<R> &rv, const RoomTab &rm, uint64_t cand, R ox, R oy, R oz, R dx,
-                                R dy, R dz, R sqd, const R *E, int band, int lo, int hi, double tsum, int ana,
+                                R dy, R dz, R sqd, const R *E, R I, int band, int lo, int hi, double tsum, int ana,
                                 LaneCounters &lc)
 {
     // voxel mask bits are receiver indices relative to the room's first receiver
@@ -518,11 +779,12 @@ __device__ void check_receivers(const DevRecv<R> &rv, const RoomTab &rm, uint64_
         while (cand) {
             const int r = __builtin_ctzll(cand);
             cand &= cand - 1;
-            receiver_test<R>(rv, rm.rec_base + r, ox, oy, oz, dx, dy, dz, sqd, E, band, lo, hi, tsum, ana, lc);
+            receiver_test<R, BROAD>(rv, rm.rec_base + r, ox, oy, oz, dx, dy, dz, sqd, E, I, band, lo, hi, tsum, ana,
+                                    lc);
         }
     } else {
         for (int r = rm.rec_base; r < rm.rec_base + rm.rec_count; ++r)
-            receiver_test<R>(rv, r, ox, oy, oz, dx, dy, dz, sqd, E, band, lo, hi, tsum, ana, lc);
+            receiver_test<R, BROAD>(rv, r, ox, oy, oz, dx, dy, dz, sqd, E, I, band, lo, hi, tsum, ana, lc);
     }
 }
 
@@ -644,12 +906,12 @@ __device__ __forceinline__ bool map_listed(const DevMap &m, int cell, uint32_t j
 // Map_Receiver.CheckRay / CheckBroadbandRay (:1780-1826) and
 // Histogram.Add / Directional_Histogram.Add at 1000 Hz.  band >= 0: octave ray
 // of intensity E[0]; band < 0: broadband energies E[0..7].
-template <class R>
-__device__ void map_check(const DevRecv<R> &rv, const R *oR, const R *dR, R ex, R ey, R ez, const R *E, int band,
+template <class R, bool BROAD>
+__device__ void map_check(const DevRecv<R> &rv, const R *oR, const R *dR, R ex, R ey, R ez, const R *E, R I, int band,
                           double t_sum, LaneCounters &lc)
 {
     const DevMap &m = rv.map;
-    const bool broadband = band < 0;
+    constexpr bool broadband = BROAD;
     const double o[3] = {(double)oR[0], (double)oR[1], (double)oR[2]};
     const double d[3] = {(double)dR[0], (double)dR[1], (double)dR[2]};
     const double end[3] = {(double)ex, (double)ey, (double)ez};
@@ -699,7 +961,10 @@ __device__ void map_check(const DevRecv<R> &rv, const R *oR, const R *dR, R ex, 
             if (sample >= m.sct || sample < 0) continue;
             const int b_lo = broadband ? 0 : band, b_hi = broadband ? kBands - 1 : band;
             for (int oc = b_lo; oc <= b_hi; ++oc) {
-                const double e = (double)E[broadband ? oc : 0] * pow(10.0, -.1 * rv.band->atten[oc] * raydist) * m.sizemod;
+                double Ei;
+                if constexpr (BROAD) Ei = (double)E[oc];
+                else Ei = (double)I;
+                const double e = Ei * pow(10.0, -.1 * rv.band->atten[oc] * raydist) * m.sizemod;
                 const uint64_t idx = ((uint64_t)j * kBands + oc) * m.sct + sample;
                 unsafeAtomicAdd(m.hist + idx, e);
                 if (m.directional) {
@@ -712,7 +977,7 @@ __device__ void map_check(const DevRecv<R> &rv, const R *oR, const R *dR, R ex, 
                 } else {
                     unsafeAtomicAdd(m.hist + m.plane_stride + idx, sqrt(e));
                 }
-                lc.v[C_DEP]++;
+                lc.add(C_DEP);
             }
         }
         if (!map_walk_step(m, w, Xe, broadband)) break;
@@ -821,6 +1086,17 @@ __device__ __forceinline__ void make_oray(ORay<R> &r, R ox, R oy, R oz, R dx, R 
     r.meta = (uint32_t)band | mask | ((uint32_t)room << M_ROOM_SHIFT);
 }
 
+// Buffer-to-buffer copy of a ray record in 16-byte pieces (a struct
+// assignment between two buffers goes through a private temporary).
+template <class R> __device__ __forceinline__ void copy_ray(ORay<R> *dst, const ORay<R> *src)
+{
+    static_assert(sizeof(ORay<R>) % 16 == 0, "ray records are whole 16-byte pieces");
+    const uint4 *a = reinterpret_cast<const uint4 *>(src);
+    uint4 *b = reinterpret_cast<uint4 *>(dst);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(ORay<R>) / 16); ++i) b[i] = a[i];
+}
+
 template <class R>
 __device__ __forceinline__ void store_dead(ORay<R> *out, unsigned from, unsigned to)
 {
@@ -836,8 +1112,7 @@ k_phase1(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv, int64_t 
          unsigned *out_count, unsigned out_cap, P1State<R> *resume_out, unsigned *resume_count,
          DevCounters *cnt)
 {
-    LaneCounters lc = {};
-    lc.lost = 0;
+    PACH_COUNTERS_BEGIN(lc);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = resume_in ? n_resume : n_fresh;
     if (i < n) {
@@ -872,21 +1147,21 @@ k_phase1(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv, int64_t 
                 const int ht = shoot<R, false>(s, st.o[0], st.o[1], st.o[2], st.d[0], st.d[1], st.d[2],
                                                excl_on ? st.tri : -1, leg, cand);
                 if (ht < 0) {
-                    lc.v[C_BROAD]++;
+                    lc.add(C_BROAD);
                     if (st.order > tc.is_order) {
                         const R ex = st.o[0] + st.d[0] * R(1000000), ey = st.o[1] + st.d[1] * R(1000000),
                                 ez = st.o[2] + st.d[2] * R(1000000);
                         const R qx = ex - st.o[0], qy = ey - st.o[1], qz = ez - st.o[2];
-                        if constexpr (MAP) map_check<R>(rv, st.o, st.d, ex, ey, ez, st.E, -1, st.tb, lc);
+                        if constexpr (MAP) map_check<R, true>(rv, st.o, st.d, ex, ey, ez, st.E, R(0), -1, st.tb, lc);
                         else
-                            check_receivers<R>(rv, room, cand, st.o[0], st.o[1], st.o[2], st.d[0], st.d[1], st.d[2],
-                                               qx * qx + qy * qy + qz * qz, st.E, -1, tc.oct_lo,
+                            check_receivers<R, true>(rv, room, cand, st.o[0], st.o[1], st.o[2], st.d[0], st.d[1], st.d[2],
+                                               qx * qx + qy * qy + qz * qz, st.E, R(0), -1, tc.oct_lo,
                                                tc.oct_hi, st.tb, 0, lc);
                     }
                     double l = 0;
                     for (int j = 0; j < kBands; ++j) l += (double)st.E[j];
-                    lc.lost += l;
-                    lc.v[C_MISS]++;
+                    lc.lose(l);
+                    lc.add(C_MISS);
                     split = false;
                     break;
                 }
@@ -904,19 +1179,19 @@ k_phase1(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv, int64_t 
                     if (base + nch > out_cap) {
                         if (base < out_cap) store_dead<R>(out, base, out_cap);
                         resume_out[atomicAdd(resume_count, 1u)] = st;
-                        lc.v[C_RESUME]++;
+                        lc.add(C_RESUME);
                         suspended = true;
                         break;
                     }
                 }
-                lc.v[C_BROAD]++;
+                lc.add(C_BROAD);
                 const R hx = st.o[0] + st.d[0] * leg, hy = st.o[1] + st.d[1] * leg, hz = st.o[2] + st.d[2] * leg;
                 if (st.order > tc.is_order) {
                     const R qx = hx - st.o[0], qy = hy - st.o[1], qz = hz - st.o[2];
-                    if constexpr (MAP) map_check<R>(rv, st.o, st.d, hx, hy, hz, st.E, -1, st.tb, lc);
+                    if constexpr (MAP) map_check<R, true>(rv, st.o, st.d, hx, hy, hz, st.E, R(0), -1, st.tb, lc);
                     else
-                        check_receivers<R>(rv, room, cand, st.o[0], st.o[1], st.o[2], st.d[0], st.d[1], st.d[2],
-                                           qx * qx + qy * qy + qz * qz, st.E, -1, tc.oct_lo, tc.oct_hi,
+                        check_receivers<R, true>(rv, room, cand, st.o[0], st.o[1], st.o[2], st.d[0], st.d[1], st.d[2],
+                                           qx * qx + qy * qy + qz * qz, st.E, R(0), -1, tc.oct_lo, tc.oct_hi,
                                            st.tb, 0, lc);
                 }
                 st.tb += (double)leg / rv.c;
@@ -953,15 +1228,15 @@ k_phase1(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv, int64_t 
                                          st.src, child_uid(0, PACH_TAG_EARLY_TD(st.k, oc)), M_SCAT | M_EVER, st.room);
                             childI *= (R(1) - tau);
                             out[q++] = td;
-                            lc.v[C_SPAWN] += 2;
-                            lc.v[C_OCT] += 2;
+                            lc.add(C_SPAWN, 2);
+                            lc.add(C_OCT, 2);
                         }
                         ORay<R> ch;
                         make_oray<R>(ch, hx, hy, hz, ddx, ddy, ddz, childI, st.tb, oc, ht, st.src,
                                      child_uid(0, PACH_TAG_EARLY(st.k, oc)), M_SCAT | M_EVER, st.room);
                         out[q++] = ch;
-                        lc.v[C_SPAWN]++;
-                        lc.v[C_OCT]++;
+                        lc.add(C_SPAWN);
+                        lc.add(C_OCT);
                     }
                     st.d[0] -= fx * cf * R(2);
                     st.d[1] -= fy * cf * R(2);
@@ -973,7 +1248,7 @@ k_phase1(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv, int64_t 
                 }
                 st.k++;
                 if (!((double)st.E[room.h] > room.T1)) break;
-                if (st.order >= tc.p1_max) { lc.v[C_P1CAP]++; break; }
+                if (st.order >= tc.p1_max) { lc.add(C_P1CAP); break; }
             }
         }
         if (split && !suspended) {
@@ -983,7 +1258,7 @@ k_phase1(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv, int64_t 
                 if (base < out_cap) store_dead<R>(out, base, out_cap);
                 st.flags |= F_SPLIT_PENDING;
                 resume_out[atomicAdd(resume_count, 1u)] = st;
-                lc.v[C_RESUME]++;
+                lc.add(C_RESUME);
             } else {
                 // BroadRay.SplitRay(o) per octave (Classes_Rays.cs:331-337)
                 for (int oc = tc.oct_lo; oc <= tc.oct_hi; ++oc) {
@@ -991,51 +1266,50 @@ k_phase1(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv, int64_t 
                     make_oray<R>(r, st.o[0], st.o[1], st.o[2], st.d[0], st.d[1], st.d[2], st.E[oc], st.tb,
                                  oc, st.tri, st.src, child_uid(0, PACH_TAG_SPLIT(oc)), 0u, st.room);
                     out[base + (oc - tc.oct_lo)] = r;
-                    lc.v[C_OCT]++;
+                    lc.add(C_OCT);
                 }
             }
         }
     }
-    for (int c = 0; c < C_N; ++c)
-        if (lc.v[c]) atomicAdd(&cnt->v[c], (unsigned long long)lc.v[c]);
-    if (lc.lost != 0) unsafeAtomicAdd(&cnt->lost, lc.lost);
+    counters_end(lc, cnt);
 }
 
 // ---------------------------------------------------------------- phases 2, 3
 // Everything an octave-ray bounce does after its closest-hit walk: receiver
 // checks, Absorb, Scatter_Late / Scatter_VeryLate, the phase hand-off.
-// Returns false when the ray terminates; sets has_child when Scatter_Late
-// spawned a specular child.
-template <class R, bool MAP>
+// Returns false when the ray terminates.  A specular child Scatter_Late
+// spawns is handed to spawn(child) where it is made (so it need not be held
+// through the diffuse reflection that follows); in the octave kernel only the
+// lanes that spawn execute that call.
+template <class R, bool MAP, class Spawn>
 __device__ __forceinline__ bool octave_shade(const TraceConst &tc, const DevScene<R> &s, const DevRecv<R> &rv,
-                                             ORay<R> &r, int ht, R leg, uint64_t cand, bool &has_child,
-                                             ORay<R> &child, LaneCounters &lc)
+                                             ORay<R> &r, int ht, R leg, uint64_t cand, Spawn &&spawn,
+                                             LaneCounters &lc)
 {
-    has_child = false;
     const int band = (int)(r.meta & M_BAND);
     const bool late3 = (r.meta & M_LATE3) != 0;
     const RoomTab &room = tc.rooms[r.meta >> M_ROOM_SHIFT];
     const int ana = (r.meta & M_SCAT) ? 1 : ((r.meta & M_EVER) ? 2 : 0);
-    lc.v[late3 ? C_VLATE : C_LATE]++;
+    lc.add(late3 ? C_VLATE : C_LATE);
     if (ht < 0) {
         const R ex = r.o[0] + r.d[0] * R(1000000), ey = r.o[1] + r.d[1] * R(1000000),
                 ez = r.o[2] + r.d[2] * R(1000000);
         const R qx = ex - r.o[0], qy = ey - r.o[1], qz = ez - r.o[2];
-        if constexpr (MAP) map_check<R>(rv, r.o, r.d, ex, ey, ez, &r.I, band, r.t, lc);
+        if constexpr (MAP) map_check<R, false>(rv, r.o, r.d, ex, ey, ez, nullptr, r.I, band, r.t, lc);
         else
-            check_receivers<R>(rv, room, cand, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2],
-                               qx * qx + qy * qy + qz * qz, &r.I, band, 0, 0, r.t, ana, lc);
-        lc.lost += (double)r.I;
-        lc.v[C_MISS]++;
+            check_receivers<R, false>(rv, room, cand, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2],
+                               qx * qx + qy * qy + qz * qz, nullptr, r.I, band, 0, 0, r.t, ana, lc);
+        lc.lose((double)r.I);
+        lc.add(C_MISS);
         return false;
     }
     const R hx = r.o[0] + r.d[0] * leg, hy = r.o[1] + r.d[1] * leg, hz = r.o[2] + r.d[2] * leg;
     {
         const R qx = hx - r.o[0], qy = hy - r.o[1], qz = hz - r.o[2];
-        if constexpr (MAP) map_check<R>(rv, r.o, r.d, hx, hy, hz, &r.I, band, r.t, lc);
+        if constexpr (MAP) map_check<R, false>(rv, r.o, r.d, hx, hy, hz, nullptr, r.I, band, r.t, lc);
         else
-            check_receivers<R>(rv, room, cand, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2],
-                               qx * qx + qy * qy + qz * qz, &r.I, band, 0, 0, r.t, ana, lc);
+            check_receivers<R, false>(rv, room, cand, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2],
+                               qx * qx + qy * qy + qz * qz, nullptr, r.I, band, 0, 0, r.t, ana, lc);
     }
     r.t += (double)leg / rv.c;
     r.o[0] = hx; r.o[1] = hy; r.o[2] = hz;
@@ -1060,7 +1334,7 @@ __device__ __forceinline__ bool octave_shade(const TraceConst &tc, const DevScen
                 diffuse = true;
                 if (sel > M.z) {
                     const R emc = R(1) - M.y;
-                    child = r;
+                    ORay<R> child = r;
                     child.I = r.I * emc;
                     r.I *= (R(1) - emc);
                     child.d[0] -= N.x * cos_t * R(2);
@@ -1072,9 +1346,9 @@ __device__ __forceinline__ bool octave_shade(const TraceConst &tc, const DevScen
                     child.uid_lo = (uint32_t)cu;
                     child.uid_hi = (uint32_t)(cu >> 32);
                     child.k = 0;
-                    has_child = true;
-                    lc.v[C_SPAWN]++;
-                    lc.v[C_OCT]++;
+                    spawn(child);
+                    lc.add(C_SPAWN);
+                    lc.add(C_OCT);
                 }
             }
         } else {
@@ -1105,6 +1379,43 @@ __device__ __forceinline__ bool octave_shade(const TraceConst &tc, const DevScen
     return r.t < tc.cotime && (double)r.I > room.T3[band];
 }
 
+// Exact queue of the mixed kernel (unused by the others).
+template <class R> struct ExactQ {
+    ORay<R> *rays;
+    unsigned *count;
+    unsigned cap;
+    unsigned drain_at;  // a wave whose push ends past this drains (every wave can push <= 128 more)
+};
+
+#ifdef PACH_MIX_UNIT
+// One bounce by the FP64 walk for every ray in `in` (the bounces the mixed
+// kernel's FP32 screen could not decide): the FP64 closest hit (shoot, t >
+// 1e-10), then the bounce's shading; the continuation and any spawned ray go
+// to `out` (the next launch's input).  Grid-stride over *n_in rays.
+__global__ void __launch_bounds__(kBlock)
+k_exact(const TraceConst tc, const DevScene<double> s, const DevRecv<double> rv, const ORay<double> *__restrict__ in,
+        const unsigned *n_in, unsigned cap_in, ORay<double> *out, unsigned *out_count, unsigned out_cap,
+        DevCounters *cnt)
+{
+    PACH_COUNTERS_BEGIN(lc);
+    const unsigned n = min(*n_in, cap_in);
+    auto put = [&](const ORay<double> &ray) __attribute__((always_inline)) {
+        const unsigned q = atomicAdd(out_count, 1u);
+        if (q < out_cap) out[q] = ray;
+        else { lc.add(C_DROP); lc.lose((double)ray.I); }
+    };
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        ORay<double> r = in[i];
+        double tb = 0;
+        uint64_t cand = 0;
+        const int ht = shoot<double, false>(s, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2], -1, tb, cand);
+        lc.add(C_SHADED);
+        if (octave_shade<double, false>(tc, s, rv, r, ht, tb, cand, put, lc)) put(r);
+    }
+    counters_end(lc, cnt);
+}
+#endif
+
 // Persistent, wavefront-compacting octave-ray kernel.
 //
 // Each lane holds one octave ray.  A loop step (a) refills idle lanes (once
@@ -1130,13 +1441,15 @@ __device__ __forceinline__ bool octave_shade(const TraceConst &tc, const DevScen
 // queue (kShadeQ items in HBM/L2) and refills at once; whenever 64 items wait
 // (or no lane is walking) every lane shades one, so the shading code runs on
 // full waves, and continuations and spawned rays go onto the wave's stack.
-template <class R, bool LDS, int BS, int OCC, bool SQ, bool MAP>
+// MIX (R = double): mixed walk -- the FP32 walk over sg screens candidates,
+// FP64 decides every hit and shades (the results of the FP64 path).
+template <class R, bool LDS, int BS, int OCC, bool SQ, bool MAP, bool MIX = false>
 __global__ void __launch_bounds__(BS, OCC > 0 ? OCC : 1)
-k_octave(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv,
+k_octave(const TraceConst tc, const DevScene<R> s, const DevScene<float> sg, const DevRecv<R> rv,
          const ORay<R> *__restrict__ in0, const unsigned *n0_ptr, unsigned cap0,
          const ORay<R> *__restrict__ in1, const unsigned *n1_ptr, unsigned cap1, unsigned *in_head,
          ORay<R> *stacks, int stack_cap, ShadeItem<R> *shq, ORay<R> *dump, unsigned *dump_count,
-         unsigned dump_cap, int min_iters, DevCounters *cnt)
+         unsigned dump_cap, int min_iters, DevCounters *cnt, ExactQ<R> xq)
 {
     if constexpr (LDS) {
         // stage the voxel distances (ncell rounded up to 16 bytes) in LDS
@@ -1144,10 +1457,9 @@ k_octave(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv,
         uint4 *dst = (uint4 *)pach_lds_dist;
         const int n16 = (s.ncell + 15) >> 4;
         for (int i = threadIdx.x; i < n16; i += BS) dst[i] = src[i];
-        __syncthreads();
     }
-    LaneCounters lc = {};
-    lc.lost = 0;
+    PACH_COUNTERS_BEGIN(lc);
+    unsigned shades = 0;  // shading passes of this wave
     const int lane = (int)__lane_id();
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const uint64_t wave = (uint64_t)blockIdx.x * (BS / 64) + (threadIdx.x >> 6);
@@ -1160,9 +1472,29 @@ k_octave(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv,
     int stack_n = 0;
     int iters = 0;
     bool input_done = false, draining = false;
+    static_assert(!MIX || (sizeof(R) == 8 && !SQ && !MAP), "mixed walk: FP64 rays, no shade queue, no map");
+    using G = typename std::conditional<MIX, float, R>::type;
     bool act = false, walking = false, ready = false;
     ORay<R> r;
-    Walk<R> w;
+    Walk<G> w;
+    G go[3], gd[3];  // MIX: the ray in FP32 for the walk
+    // MIX: a bounce the FP32 screen cannot decide goes to the exact queue (the
+    // FP64 walk of k_exact, after this launch); a wave whose push lands past
+    // xq.drain_at drains, so the queue never overflows
+    auto to_exact = [&](bool give) __attribute__((always_inline)) {
+        const uint64_t gm = __ballot(give);
+        if (!gm) return;
+        unsigned base = 0;
+        if (lane == 0) base = atomicAdd(xq.count, (unsigned)__popcll(gm));
+        base = __shfl(base, 0);
+        if (give) {
+            const unsigned q = base + (unsigned)__popcll(gm & lt_mask);
+            if (q < xq.cap) xq.rays[q] = r;
+            else { lc.add(C_DROP); lc.lose((double)r.I); }
+            act = false;
+        }
+        if (base + (unsigned)__popcll(gm) > xq.drain_at) draining = true;
+    };
     for (;;) {
         // ---- (a) refill idle lanes once refill_min of them are idle (or all
         // are): own stack first, then the input queue
@@ -1186,7 +1518,8 @@ k_octave(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv,
                 const int j = rank - from_stack;
                 if (!act && j >= 0 && j < avail) {
                     const unsigned q = base + (unsigned)j;
-                    r = q < n0 ? in0[q] : in1[q - n0];
+                    const ORay<R> *src = q < n0 ? in0 + q : in1 + (q - n0);
+                    r = *src;
                     act = (r.meta & M_DEAD) == 0;
                 }
             }
@@ -1203,17 +1536,45 @@ k_octave(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv,
                 base = __shfl(base, 0);
                 for (int i = lane; i < stack_n; i += 64) {
                     const unsigned q = base + (unsigned)i;
-                    if (q < dump_cap) dump[q] = ws[i];
-                    else { lc.v[C_DROP]++; lc.lost += (double)ws[i].I; }
+                    if (q < dump_cap) copy_ray(dump + q, ws + i);
+                    else { lc.add(C_DROP); lc.lose((double)ws[i].I); }
                 }
                 if (give) {
                     const unsigned q = base + (unsigned)stack_n + (unsigned)__popcll(gm & lt_mask);
                     if (q < dump_cap) dump[q] = r;
-                    else { lc.v[C_DROP]++; lc.lost += (double)r.I; }
+                    else { lc.add(C_DROP); lc.lose((double)r.I); }
                     act = false;
                 }
-                lc.v[C_SPILL] += (lane == 0) ? n_dump : 0u;
+                if (lane == 0) lc.add(C_SPILL, n_dump);
                 stack_n = 0;
+            }
+            if (!input_done) {
+                // drained before the input ran out (the exact queue filled):
+                // hand the unclaimed input to the next launch as well
+                unsigned base = 0;
+                if (lane == 0) base = atomicAdd(in_head, 64u);
+                base = __shfl(base, 0);
+                const int avail = base >= n_in ? 0 : (int)min(64u, n_in - base);
+                if (avail < 64) input_done = true;
+                bool mv = false;
+                const ORay<R> *src = nullptr;
+                if (lane < avail) {
+                    const unsigned q = base + (unsigned)lane;
+                    src = q < n0 ? in0 + q : in1 + (q - n0);
+                    mv = (src->meta & M_DEAD) == 0;
+                }
+                const uint64_t mm = __ballot(mv);
+                if (mm) {
+                    unsigned db = 0;
+                    if (lane == 0) db = atomicAdd(dump_count, (unsigned)__popcll(mm));
+                    db = __shfl(db, 0);
+                    if (mv) {
+                        const unsigned q = db + (unsigned)__popcll(mm & lt_mask);
+                        if (q < dump_cap) copy_ray(dump + q, src);
+                        else { lc.add(C_DROP); lc.lose((double)src->I); }
+                    }
+                    if (lane == 0) lc.add(C_SPILL, (unsigned)__popcll(mm));
+                }
             }
         }
         const uint64_t am = __ballot(act);
@@ -1222,35 +1583,58 @@ k_octave(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv,
             continue;  // the claim returned only dead slots: claim again
         }
         // ---- (b) start and advance closest-hit walks
+        bool grazing = false;
         if (act && !walking && !ready) {
             if ((int)r.k >= tc.max_bounces) {
-                lc.v[C_GUARD]++;
+                lc.add(C_GUARD);
                 act = false;
             } else {
-                walk_init<R>(s, w, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2]);
-                walking = true;
+                if constexpr (MIX) {
+                    for (int k = 0; k < 3; ++k) {
+                        go[k] = (float)r.o[k];
+                        gd[k] = (float)r.d[k];
+                    }
+                    // a grazing departure: FP64 may hit the triangle left again
+                    if (r.tri >= 0) {
+                        const float4 n = sg.nrm[r.tri];
+                        grazing = fabsf(n.x * gd[0] + n.y * gd[1] + n.z * gd[2]) < tc.mx_self;
+                    }
+                    if (grazing) lc.add(C_EXACT0);
+                    else mix_walk_init(sg, w, go, gd);
+                } else {
+                    walk_init<R>(s, w, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2]);
+                }
+                walking = !grazing;
             }
         }
+        if constexpr (MIX) to_exact(grazing);
         if (walking) {
             const int ex = Tmin<R>::excl ? r.tri : -1;
-            const bool done =
+            bool done;
+            if constexpr (MIX)
+                done = tc.walk_flat == 2
+                           ? walk_units_mix<LDS, true>(sg, tc, w, go[0], go[1], go[2], gd[0], gd[1], gd[2], r.tri,
+                                                       tc.walk_steps, lc.vox, lc.test)
+                           : walk_units_mix<LDS, false>(sg, tc, w, go[0], go[1], go[2], gd[0], gd[1], gd[2], r.tri,
+                                                        tc.walk_steps, lc.vox, lc.test);
+            else
+            done =
                 tc.walk_flat == 2
                     ? walk_units<R, LDS, true>(s, w, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2], ex, tc.walk_steps,
-                                               lc.v[C_VOXEL], lc.v[C_TEST])
+                                               lc.vox, lc.test)
                 : tc.walk_flat
                     ? walk_units<R, LDS>(s, w, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2], ex, tc.walk_steps,
-                                         lc.v[C_VOXEL], lc.v[C_TEST])
+                                         lc.vox, lc.test)
                     : walk_steps<R, LDS>(s, w, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2], ex, tc.walk_steps,
-                                         lc.v[C_VOXEL], lc.v[C_TEST]);
+                                         lc.vox, lc.test);
             if (done) {
                 walking = false;
                 ready = true;
             }
         }
         ++iters;
-        if (lane == 0) lc.v[C_ITER]++;
         // push a ray onto the wave's stack (overflow goes to the dump queue)
-        auto push = [&](bool has, const ORay<R> &ray) {
+        auto push = [&](bool has, const ORay<R> &ray) __attribute__((always_inline)) {
             const uint64_t cm = __ballot(has);
             if (!cm) return;
             if (has) {
@@ -1259,11 +1643,32 @@ k_octave(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv,
                     ws[pos] = ray;
                 } else {
                     const unsigned q = atomicAdd(dump_count, 1u);
-                    if (q < dump_cap) { dump[q] = ray; lc.v[C_SPILL]++; }
-                    else { lc.v[C_DROP]++; lc.lost += (double)ray.I; }
+                    if (q < dump_cap) { dump[q] = ray; lc.add(C_SPILL); }
+                    else { lc.add(C_DROP); lc.lose((double)ray.I); }
                 }
             }
             stack_n = min(stack_n + __popcll(cm), stack_cap);
+        };
+        // a spawned child goes onto the stack where octave_shade makes it: only
+        // the spawning lanes run this, so the active mask orders them; the
+        // caller then advances stack_n by their count
+        bool spawned = false;
+        auto spawn = [&](const ORay<R> &ray) __attribute__((always_inline)) {
+            spawned = true;
+            const int pos = stack_n + __popcll(__ballot(true) & lt_mask);
+            if (pos < stack_cap) {
+                ws[pos] = ray;
+            } else {
+                const unsigned q = atomicAdd(dump_count, 1u);
+                if (q < dump_cap) { dump[q] = ray; lc.add(C_SPILL); }
+                else { lc.add(C_DROP); lc.lose((double)ray.I); }
+            }
+        };
+        auto spawned_done = [&]() __attribute__((always_inline)) {
+            const int n_sp = __popcll(__ballot(spawned));
+            stack_n = min(stack_n + n_sp, stack_cap);
+            spawned = false;
+            return n_sp;
         };
         if constexpr (SQ) {
             // ---- (c) queue the finished walks, shade 64 at a time
@@ -1284,17 +1689,17 @@ k_octave(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv,
             }
             if (qn >= 64 || (qn > 0 && __ballot(walking) == 0)) {
                 const int k = min(qn, 64);
-                bool alive = false, has_child = false;
-                ORay<R> sr, child;
-                if (lane == 0) lc.v[C_SHADE]++;
+                bool alive = false;
+                ORay<R> sr;
+                ++shades;
                 if (lane < k) {
                     const ShadeItem<R> it = wq[qn - k + lane];
                     sr = it.r;
-                    lc.v[C_SHADED]++;
-                    alive = octave_shade<R, MAP>(tc, s, rv, sr, it.best, it.tbest, it.rmask, has_child, child, lc);
+                    lc.add(C_SHADED);
+                    alive = octave_shade<R, MAP>(tc, s, rv, sr, it.best, it.tbest, it.rmask, spawn, lc);
                 }
                 qn -= k;
-                push(has_child, child);
+                spawned_done();
                 push(alive, sr);  // on top: the continuation is walked next
                 __threadfence_block();
             }
@@ -1302,24 +1707,87 @@ k_octave(const TraceConst tc, const DevScene<R> s, const DevRecv<R> rv,
             // ---- (c) shade the lanes whose walk ended
             const int n_ready = __popcll(__ballot(ready));
             if (n_ready > 0 && (n_ready >= tc.shade_min || __ballot(walking) == 0)) {
-                bool has_child = false;
-                ORay<R> child;
-                if (lane == 0) lc.v[C_SHADE]++;
+                ++shades;
+                bool unsure = false;
                 if (ready) {
-                    lc.v[C_SHADED]++;
-                    act = octave_shade<R, MAP>(tc, s, rv, r, w.best, w.tbest, w.rmask, has_child, child, lc);
+                    R tb = (R)w.tbest;
+                    int hb = w.best;
+                    if constexpr (MIX) {
+                        double t64 = 0;
+                        unsure = !mix_confirm(s, w, r.o, r.d, hb, t64);
+                        tb = t64;
+                    }
                     ready = false;
+                    if (unsure) {
+                        lc.add(C_EXACT);
+                    } else {
+                        lc.add(C_SHADED);
+                        act = octave_shade<R, MAP>(tc, s, rv, r, hb, tb, w.rmask, spawn, lc);
+                    }
                 }
-                if (__ballot(has_child)) {
-                    push(has_child, child);
-                    __threadfence_block();
-                }
+                if (spawned_done()) __threadfence_block();
+                if constexpr (MIX) to_exact(unsure);
             }
         }
     }
-    for (int c = 0; c < C_N; ++c)
-        if (lc.v[c]) atomicAdd(&cnt->v[c], (unsigned long long)lc.v[c]);
-    if (lc.lost != 0) unsafeAtomicAdd(&cnt->lost, lc.lost);
+    if (lane == 0) {
+        lc.add(C_ITER, (unsigned)iters);
+        lc.add(C_SHADE, shades);
+    }
+    counters_end(lc, cnt);
+}
+
+// ---------------------------------------------------------------- closest-hit queries
+// Polygon_Scene.shoot (Classes_Scenes.cs:757-819) for a batch of rays, one
+// thread per ray: the triangle hit (-1: none) and its distance.  FP64: the FP64
+// walk; FP32: the octave kernel's FP32 walk; MIX: the mixed walk, confirmed
+// in FP64 or walked again in exact mode (counted in *n_exact).
+template <class R, bool MIX>
+__global__ void __launch_bounds__(kBlock)
+k_shoot(const TraceConst tc, const DevScene<R> s, const DevScene<float> sg, const double *__restrict__ o,
+        const double *__restrict__ d, int64_t n, int *__restrict__ tri, double *__restrict__ tout, unsigned *n_exact)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double O[3] = {o[3 * i], o[3 * i + 1], o[3 * i + 2]};
+    const double D[3] = {d[3 * i], d[3 * i + 1], d[3 * i + 2]};
+    uint32_t nv = 0, nt = 0;
+    if constexpr (MIX) {
+        const float go[3] = {(float)O[0], (float)O[1], (float)O[2]};
+        const float gd[3] = {(float)D[0], (float)D[1], (float)D[2]};
+        Walk<float> w;
+        double t64 = 0;
+        int hit = -1;
+        mix_walk_init(sg, w, go, gd);
+        while (!walk_units_mix<false, false>(sg, tc, w, go[0], go[1], go[2], gd[0], gd[1], gd[2], -1, 64, nv, nt)) {
+        }
+        if (!mix_confirm(s, w, O, D, hit, t64)) {
+            uint64_t rm = 0;
+            hit = shoot<double, false>(s, O[0], O[1], O[2], D[0], D[1], D[2], -1, t64, rm);
+            atomicAdd(n_exact, 1u);
+        }
+        tri[i] = hit;
+        tout[i] = hit >= 0 ? t64 : 0.0;
+    } else {
+        const R ox = (R)O[0], oy = (R)O[1], oz = (R)O[2], dx = (R)D[0], dy = (R)D[1], dz = (R)D[2];
+        Walk<R> w;
+        walk_init<R>(s, w, ox, oy, oz, dx, dy, dz);
+        while (!walk_units<R, false>(s, w, ox, oy, oz, dx, dy, dz, -1, 64, nv, nt)) {
+        }
+        tri[i] = w.best;
+        tout[i] = w.best >= 0 ? (double)w.tbest : 0.0;
+    }
+}
+
+template <class R>
+hipError_t launch_shoot(hipStream_t st, const TraceConst &tc, const DevScene<R> &s, const DevScene<float> &sg,
+                        const double *o, const double *d, int64_t n, int *tri, double *t, unsigned *n_exact)
+{
+    if (n <= 0) return hipSuccess;
+    const int64_t grid = (n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL((k_shoot<R, false>), dim3((unsigned)grid), dim3(kBlock), 0, st, tc, s, sg, o, d, n, tri, t,
+                       n_exact);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------- launchers
@@ -1346,33 +1814,34 @@ hipError_t launch_phase1(hipStream_t st, const TraceConst &tc, const DevScene<R>
     return hipGetLastError();
 }
 
-template <class R, int BS, int OCC, bool SQ, bool MAP = false>
+template <class R, int BS, int OCC, bool SQ, bool MAP = false, bool MIX = false>
 static void launch_octave_bs(hipStream_t st, int grid, size_t lds, const TraceConst &tc, const DevScene<R> &s,
-                             const DevRecv<R> &rv, const ORay<R> *in0, const unsigned *n0, unsigned cap0,
-                             const ORay<R> *in1, const unsigned *n1, unsigned cap1, unsigned *in_head,
+                             const DevScene<float> &sg, const DevRecv<R> &rv, const ORay<R> *in0, const unsigned *n0,
+                             unsigned cap0, const ORay<R> *in1, const unsigned *n1, unsigned cap1, unsigned *in_head,
                              ORay<R> *stacks, int stack_cap, ShadeItem<R> *shq, ORay<R> *dump,
-                             unsigned *dump_count, unsigned dump_cap, int min_iters, DevCounters *cnt)
+                             unsigned *dump_count, unsigned dump_cap, int min_iters, DevCounters *cnt,
+                             ExactQ<R> xq = ExactQ<R>{})
 {
     if (lds)
-        hipLaunchKernelGGL((k_octave<R, true, BS, OCC, SQ, MAP>), dim3(grid), dim3(BS), lds, st, tc, s, rv, in0, n0,
-                           cap0, in1, n1, cap1, in_head, stacks, stack_cap, shq, dump, dump_count, dump_cap,
-                           min_iters, cnt);
+        hipLaunchKernelGGL((k_octave<R, true, BS, OCC, SQ, MAP, MIX>), dim3(grid), dim3(BS), lds, st, tc, s, sg, rv,
+                           in0, n0, cap0, in1, n1, cap1, in_head, stacks, stack_cap, shq, dump, dump_count, dump_cap,
+                           min_iters, cnt, xq);
     else
-        hipLaunchKernelGGL((k_octave<R, false, BS, OCC, SQ, MAP>), dim3(grid), dim3(BS), 0, st, tc, s, rv, in0, n0,
-                           cap0, in1, n1, cap1, in_head, stacks, stack_cap, shq, dump, dump_count, dump_cap,
-                           min_iters, cnt);
+        hipLaunchKernelGGL((k_octave<R, false, BS, OCC, SQ, MAP, MIX>), dim3(grid), dim3(BS), 0, st, tc, s, sg, rv,
+                           in0, n0, cap0, in1, n1, cap1, in_head, stacks, stack_cap, shq, dump, dump_count, dump_cap,
+                           min_iters, cnt, xq);
 }
 
 template <class R>
 hipError_t launch_octave(hipStream_t st, int grid, const OctShape &sh, const TraceConst &tc, const DevScene<R> &s,
-                         const DevRecv<R> &rv, const ORay<R> *in0, const unsigned *n0, unsigned cap0,
-                         const ORay<R> *in1, const unsigned *n1, unsigned cap1, unsigned *in_head,
+                         const DevScene<float> &sg, const DevRecv<R> &rv, const ORay<R> *in0, const unsigned *n0,
+                         unsigned cap0, const ORay<R> *in1, const unsigned *n1, unsigned cap1, unsigned *in_head,
                          ORay<R> *stacks, int stack_cap, ShadeItem<R> *shq, ORay<R> *dump, unsigned *dump_count,
                          unsigned dump_cap, int min_iters, DevCounters *cnt)
 {
     const size_t lds = s.dist_lds ? (size_t)((s.ncell + 15) & ~15) : 0;
-#define PACH_OCT(BS_, OCC_, SQ_)                                                                            \
-    launch_octave_bs<R, BS_, OCC_, SQ_>(st, grid, lds, tc, s, rv, in0, n0, cap0, in1, n1, cap1, in_head,     \
+#define PACH_OCT(BS_, OCC_, SQ_)                                                                               \
+    launch_octave_bs<R, BS_, OCC_, SQ_>(st, grid, lds, tc, s, sg, rv, in0, n0, cap0, in1, n1, cap1, in_head,    \
                                         stacks, stack_cap, shq, dump, dump_count, dump_cap, min_iters, cnt)
 #define PACH_OCT_Q(BS_, OCC_) \
     { if (sh.queue) PACH_OCT(BS_, OCC_, true); else PACH_OCT(BS_, OCC_, false); }
@@ -1380,6 +1849,7 @@ hipError_t launch_octave(hipStream_t st, int grid, const OctShape &sh, const Tra
     {                                                    \
         if (sh.occ == 4) PACH_OCT_Q(BS_, 4)              \
         else if (sh.occ == 5) PACH_OCT_Q(BS_, 5)         \
+        else if (sh.occ == 3) PACH_OCT_Q(BS_, 3)         \
         else PACH_OCT_Q(BS_, 0)                          \
     }
     if (sh.queue && !shq) return hipErrorInvalidValue;
@@ -1388,7 +1858,7 @@ hipError_t launch_octave(hipStream_t st, int grid, const OctShape &sh, const Tra
         // 128-register cap spills the map walk: 26.0 s against 18.3 s on the
         // hall's 1,184-receiver map); the grid comes from
         // octave_map_blocks_per_cu, this kernel's own residency
-        launch_octave_bs<R, 256, kMapOcc, false, true>(st, grid, lds, tc, s, rv, in0, n0, cap0, in1, n1, cap1,
+        launch_octave_bs<R, 256, kMapOcc, false, true>(st, grid, lds, tc, s, sg, rv, in0, n0, cap0, in1, n1, cap1,
                                                        in_head, stacks, stack_cap, shq, dump, dump_count, dump_cap,
                                                        min_iters, cnt);
         return hipGetLastError();
@@ -1400,6 +1870,84 @@ hipError_t launch_octave(hipStream_t st, int grid, const OctShape &sh, const Tra
 #undef PACH_OCT
     return hipGetLastError();
 }
+
+#ifdef PACH_MIX_UNIT
+// Mixed walk (FP64 rays, FP32 screen): workgroup size and register cap as
+// OctShape says (occupancy 3, 4 or the compiler's; no shade queue, no mapping
+// receivers).
+hipError_t launch_octave_mix(hipStream_t st, int grid, const OctShape &sh, const TraceConst &tc,
+                             const DevScene<double> &s, const DevScene<float> &sg, const DevRecv<double> &rv,
+                             const ORay<double> *in0, const unsigned *n0, unsigned cap0, const ORay<double> *in1,
+                             const unsigned *n1, unsigned cap1, unsigned *in_head, ORay<double> *stacks,
+                             int stack_cap, ORay<double> *dump, unsigned *dump_count, unsigned dump_cap,
+                             int min_iters, DevCounters *cnt, ORay<double> *xq, unsigned *xq_count, unsigned xq_cap)
+{
+    if (sh.queue || rv.has_map) return hipErrorInvalidValue;
+    const size_t lds = s.dist_lds ? (size_t)((s.ncell + 15) & ~15) : 0;
+    const unsigned waves = (unsigned)grid * (unsigned)(sh.block / 64);
+    if (xq_cap <= 128u * waves) return hipErrorInvalidValue;
+    const ExactQ<double> q{xq, xq_count, xq_cap, xq_cap - 128u * waves};
+#define PACH_MIX(BS_, OCC_)                                                                                        \
+    launch_octave_bs<double, BS_, OCC_, false, false, true>(st, grid, lds, tc, s, sg, rv, in0, n0, cap0, in1, n1,     \
+                                                            cap1, in_head, stacks, stack_cap, nullptr, dump,          \
+                                                            dump_count, dump_cap, min_iters, cnt, q)
+#define PACH_MIX_O(BS_)                                     \
+    {                                                       \
+        if (sh.occ == 4) PACH_MIX(BS_, 4);                  \
+        else if (sh.occ == 3) PACH_MIX(BS_, 3);             \
+        else PACH_MIX(BS_, 0);                              \
+    }
+    if (sh.block == 512) PACH_MIX_O(512)
+    else PACH_MIX_O(256)
+#undef PACH_MIX_O
+#undef PACH_MIX
+    return hipGetLastError();
+}
+
+template <int BS, int OCC> static int mix_blocks_per_cu_bs(int lds_bytes)
+{
+    int nb = 0;
+    hipError_t e =
+        lds_bytes > 0
+            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_octave<double, true, BS, OCC, false, false, true>, BS,
+                                                           lds_bytes)
+            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_octave<double, false, BS, OCC, false, false, true>, BS,
+                                                           0);
+    if (e != hipSuccess || nb < 1) nb = 1;
+    return nb;
+}
+
+hipError_t launch_exact(hipStream_t st, int grid, const TraceConst &tc, const DevScene<double> &s,
+                        const DevRecv<double> &rv, const ORay<double> *in, const unsigned *n_in, unsigned cap_in,
+                        ORay<double> *out, unsigned *out_count, unsigned out_cap, DevCounters *cnt)
+{
+    hipLaunchKernelGGL(k_exact, dim3(grid), dim3(kBlock), 0, st, tc, s, rv, in, n_in, cap_in, out, out_count, out_cap,
+                       cnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_shoot_mix(hipStream_t st, const TraceConst &tc, const DevScene<double> &s,
+                            const DevScene<float> &sg, const double *o, const double *d, int64_t n, int *tri,
+                            double *t, unsigned *n_exact)
+{
+    if (n <= 0) return hipSuccess;
+    const int64_t grid = (n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL((k_shoot<double, true>), dim3((unsigned)grid), dim3(kBlock), 0, st, tc, s, sg, o, d, n, tri,
+                       t, n_exact);
+    return hipGetLastError();
+}
+
+// Resident workgroups per CU of the mixed-walk kernel (its own register count).
+int octave_mix_blocks_per_cu(const OctShape &sh, int lds_bytes)
+{
+    if (sh.block == 512)
+        return sh.occ == 4 ? mix_blocks_per_cu_bs<512, 4>(lds_bytes)
+                           : (sh.occ == 3 ? mix_blocks_per_cu_bs<512, 3>(lds_bytes) : mix_blocks_per_cu_bs<512, 0>(lds_bytes));
+    return sh.occ == 4 ? mix_blocks_per_cu_bs<256, 4>(lds_bytes)
+                       : (sh.occ == 3 ? mix_blocks_per_cu_bs<256, 3>(lds_bytes) : mix_blocks_per_cu_bs<256, 0>(lds_bytes));
+}
+
+#endif
 
 template <class R, int BS, int OCC, bool SQ> static int blocks_per_cu_bs(int lds_bytes)
 {
@@ -1415,6 +1963,7 @@ template <class R, int BS, bool SQ> static int blocks_per_cu_o(const OctShape &s
 {
     if (sh.occ == 4) return blocks_per_cu_bs<R, BS, 4, SQ>(lds_bytes);
     if (sh.occ == 5) return blocks_per_cu_bs<R, BS, 5, SQ>(lds_bytes);
+    if (sh.occ == 3) return blocks_per_cu_bs<R, BS, 3, SQ>(lds_bytes);
     return blocks_per_cu_bs<R, BS, 0, SQ>(lds_bytes);
 }
 
@@ -1447,12 +1996,16 @@ template <class R> int octave_map_blocks_per_cu(int lds_bytes)
                                          ORay<R> *, unsigned *, unsigned, P1State<R> *, unsigned *, \
                                          DevCounters *);                                           \
     template hipError_t launch_octave<R>(hipStream_t, int, const OctShape &, const TraceConst &,   \
-                                         const DevScene<R> &, const DevRecv<R> &, const ORay<R> *,  \
+                                         const DevScene<R> &, const DevScene<float> &,              \
+                                         const DevRecv<R> &, const ORay<R> *,                       \
                                          const unsigned *, unsigned, const ORay<R> *,               \
                                          const unsigned *, unsigned, unsigned *, ORay<R> *, int,    \
                                          ShadeItem<R> *, ORay<R> *, unsigned *, unsigned, int,      \
                                          DevCounters *);                                            \
     template int octave_blocks_per_cu<R>(const OctShape &, int);                                   \
+    template hipError_t launch_shoot<R>(hipStream_t, const TraceConst &, const DevScene<R> &,      \
+                                        const DevScene<float> &, const double *, const double *,   \
+                                        int64_t, int *, double *, unsigned *);                     \
     template int octave_map_blocks_per_cu<R>(int);
 
 }  // namespace pach

@@ -44,6 +44,7 @@ template <class R> struct DevScene {
 struct BandTab {
     double atten[kBands];  // air attenuation, dB/m
     float kexp2[kBands];   // -0.1 * atten * log2(10): 10^(-0.1 a x) = exp2(k x)
+    double kexp2d[kBands]; // the same in FP64
 };
 
 // One room of a trace (a single-scene trace is one room).  Source rays
@@ -109,6 +110,13 @@ struct TraceConst {
     int64_t n_dirs;
     const double *geo;    // 80 geodesic faces [80][9]
     int has_trans;        // any transmission coefficient > 0
+    // Mixed walk (FP32 screen, FP64 decisions): bounds on the rounding error of
+    // the FP32 screen, from the scene's extent and triangle shapes.
+    float mx_kt, mx_kt0;  // |t32 - t| <= kt |1 / (n . d)| + kt0 (m)
+    float mx_ku, mx_ku0;  // barycentric error <= ku |1 / (n . d)| + ku0
+    float mx_kx;          // voxel-exit error <= kx |1 / d_axis| + 8 eps t (m)
+    float mx_infl;        // how far past its voxel every voxel list reaches (m)
+    float mx_self;        // |n . d| below which FP64 may hit the triangle left again
 };
 
 // Octave ray (OctaveRay, Classes_Rays.cs:30-181).  meta bits: 0-2 band,
@@ -166,6 +174,8 @@ enum : int {
     C_TEST,      // triangle tests started (per lane)
     C_SHADE,     // shading passes (per wave)
     C_SHADED,    // lanes shaded (per lane)
+    C_EXACT,     // mixed walk: bounces the FP32 screen could not decide (redone by the FP64 walk)
+    C_EXACT0,    // mixed walk: grazing departures (walked in FP64 from the start)
     C_N
 };
 
@@ -206,12 +216,34 @@ constexpr int kShadeQ = 128;  // per-wave queue capacity (at most 63 wait + 64 p
 
 template <class R>
 hipError_t launch_octave(hipStream_t st, int grid, const OctShape &sh, const TraceConst &tc, const DevScene<R> &s,
-                         const DevRecv<R> &rv, const ORay<R> *in0, const unsigned *n0, unsigned cap0,
+                         const DevScene<float> &sg, const DevRecv<R> &rv, const ORay<R> *in0, const unsigned *n0,
+                         unsigned cap0,
                          const ORay<R> *in1, const unsigned *n1, unsigned cap1, unsigned *in_head,
                          ORay<R> *stacks, int stack_cap, ShadeItem<R> *shq, ORay<R> *dump, unsigned *dump_count,
                          unsigned dump_cap, int min_iters, DevCounters *cnt);
 
 template <class R> int octave_blocks_per_cu(const OctShape &sh, int lds_bytes);
+
+// mixed walk (trace_mix.hip): FP64 rays and decisions, FP32 screen over sg
+hipError_t launch_octave_mix(hipStream_t st, int grid, const OctShape &sh, const TraceConst &tc,
+                             const DevScene<double> &s, const DevScene<float> &sg, const DevRecv<double> &rv,
+                             const ORay<double> *in0, const unsigned *n0, unsigned cap0, const ORay<double> *in1,
+                             const unsigned *n1, unsigned cap1, unsigned *in_head, ORay<double> *stacks,
+                             int stack_cap, ORay<double> *dump, unsigned *dump_count, unsigned dump_cap,
+                             int min_iters, DevCounters *cnt, ORay<double> *xq, unsigned *xq_count, unsigned xq_cap);
+int octave_mix_blocks_per_cu(const OctShape &sh, int lds_bytes);
+// the mixed kernel's undecided bounces, by the FP64 walk (k_exact)
+hipError_t launch_exact(hipStream_t st, int grid, const TraceConst &tc, const DevScene<double> &s,
+                        const DevRecv<double> &rv, const ORay<double> *in, const unsigned *n_in, unsigned cap_in,
+                        ORay<double> *out, unsigned *out_count, unsigned out_cap, DevCounters *cnt);
+
+// closest-hit queries (k_shoot): o, d [n][3] in device memory
+template <class R>
+hipError_t launch_shoot(hipStream_t st, const TraceConst &tc, const DevScene<R> &s, const DevScene<float> &sg,
+                        const double *o, const double *d, int64_t n, int *tri, double *t, unsigned *n_exact);
+hipError_t launch_shoot_mix(hipStream_t st, const TraceConst &tc, const DevScene<double> &s,
+                            const DevScene<float> &sg, const double *o, const double *d, int64_t n, int *tri,
+                            double *t, unsigned *n_exact);
 template <class R> int octave_map_blocks_per_cu(int lds_bytes);
 
 }  // namespace pach

@@ -157,7 +157,8 @@ def test_fp32_matches_fp64_statistics(tracer, medium):
 
 
 DEFAULT_KNOBS = {"walk_flat": 2, "walk_steps": 12, "receiver_masks": 1, "dist_lds": 1, "shade_min": 16,
-                 "refill_min": 1, "dump_min": 256, "block": 256, "occupancy": -1, "shade_queue": 0}
+                 "refill_min": 1, "dump_min": 256, "block": 256, "occupancy": -1, "shade_queue": 0,
+                 "exact_queue": 1 << 25}
 
 
 @pytest.mark.parametrize("knobs", [{"walk_flat": 0, "walk_steps": 4}, {"receiver_masks": 0, "dist_lds": 0},
