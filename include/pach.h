@@ -306,6 +306,21 @@ int pach_image_sources_stats(pach_ctx *ctx, pach_is_stats *out);
 int pach_etc_to_filter(pach_ctx *ctx, const double *prms, int n_ch, int T, const double swl[8],
                        int sample_frequency, double *out);
 
+/* Allocation-free pach_etc_to_filter.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  d_prms [n_ch][8][T]
+ * and d_out [n_ch][T + 16383] are DEVICE pointers the caller owns; the work
+ * buffers (carriers, envelopes, the nine band planes) live in one fixed
+ * device array of PACH_NOALLOC_SCRATCH_BYTES per GPU, held until the process
+ * ends, so the call allocates nothing.  Every call on a GPU shares that
+ * array: calls that overlap on one GPU race.  Returns PACH_ERANGE when
+ * 8 (9 x 16384 + 8 + T + 72 ceil(n_ch / 8) T + 9 n_ch (T + 16383)) bytes
+ * (each of the four parts rounded up to 256 bytes) exceed the array;
+ * otherwise the same result as pach_etc_to_filter. */
+#define PACH_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int pach_etc_to_filter_noalloc(pach_ctx *ctx, const double *d_prms, int n_ch, int T, const double swl[8],
+                               int sample_frequency, double *d_out);
+
 /* Directional_Histogram.Create_Filter(SWL, Rho_C) (Classes_Receivers.cs:
  * 1911-1936) for every receiver of the last trace, from its scaled histograms:
  * 7 filters per receiver -- the pressure envelope, then for x, y, z the

@@ -133,6 +133,7 @@ SIGNATURES = [
     ("pach_image_sources_stats", C.c_int, [C.c_void_p, C.POINTER(IsStats)]),
     ("pach_etc_to_filter", C.c_int, [C.c_void_p, _dp, C.c_int, C.c_int, _dp, C.c_int, _dp]),
     ("pach_receiver_filters", C.c_int, [C.c_void_p, _dp, _dp]),
+    ("pach_etc_to_filter_noalloc", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, _dp, C.c_int, C.c_void_p]),
     ("pach_filter_stats_get", C.c_int, [C.c_void_p, C.POINTER(FilterStats)]),
     ("pach_autotune", C.c_int, [C.c_void_p, C.POINTER(Params), C.c_int64, C.POINTER(Tuning)]),
     ("pach_trace", C.c_int, [C.c_void_p, C.POINTER(Params)]),
@@ -435,6 +436,13 @@ class Tracer:
         self._chk(lib().pach_etc_to_filter(self.h, _p(pr), pr.shape[0], pr.shape[2], _p(sw), int(sample_frequency),
                                            _p(out)))
         return out[0] if single else out
+
+    def etc_to_filter_noalloc(self, d_prms, n_ch, T, swl, sample_frequency, d_out):
+        """pach_etc_to_filter_noalloc on device buffers (addresses as integers):
+        d_prms [n_ch][8][T], d_out [n_ch][T + 16383] doubles."""
+        sw = np.ascontiguousarray(swl, dtype=np.float64)
+        self._chk(lib().pach_etc_to_filter_noalloc(self.h, C.c_void_p(int(d_prms)), int(n_ch), int(T), _p(sw),
+                                                   int(sample_frequency), C.c_void_p(int(d_out))))
 
     def receiver_filters(self, swl=None):
         """Create_Filter(SWL, Rho_C) of every receiver of the last trace: [R][7][SampleCT + 12287]."""

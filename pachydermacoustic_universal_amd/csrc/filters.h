@@ -23,4 +23,16 @@ hipError_t launch_etc_conv(hipStream_t st, const double *env, const double *car,
 hipError_t launch_receiver_env(hipStream_t st, const double *hist, size_t plane_stride, int R, int T,
                                const double *scale, const double *p_mod, const double *f16, int n_chb, double *env);
 
+// The envelopes of pach_etc_to_filter from a device array prms [n_ch][8][T]
+// (every element of env written, padding channels zero).
+hipError_t launch_env_from_prms(hipStream_t st, const double *prms, int n_ch, int T, const double *p_mod,
+                                const double *f16, int n_chb, double *env);
+
+// Allocation-free filters: a fixed device array of kNoAllocScratch bytes per
+// GPU, defined in filters.hip; noalloc_scratch returns its device address on
+// `device` (looked up once per device with that device current).
+constexpr size_t kNoAllocScratch = (size_t)64 << 20;
+constexpr int kMaxDevices = 64;
+hipError_t noalloc_scratch(int device, unsigned char **ptr);
+
 }  // namespace pach

@@ -20,6 +20,7 @@
 #include "filters.h"
 
 #include <algorithm>
+#include <mutex>
 
 namespace pach {
 namespace {
@@ -146,7 +147,70 @@ __global__ void __launch_bounds__(256) k_receiver_env(const double *__restrict__
     }
 }
 
+// Envelopes of pach_etc_to_filter in launch_etc_conv's layout from the
+// caller's device array prms [n_ch][8][T]: |prms| x p_mod[band], the 16 kHz
+// band from band 7 with f16[t], zero for the padding channels of the last
+// block of 8 (every element is written, so nothing needs clearing first).
+__global__ void __launch_bounds__(256) k_env_from_prms(const double *__restrict__ prms, int n_ch, int T,
+                                                       const double *__restrict__ p_mod,
+                                                       const double *__restrict__ f16, int n_chb,
+                                                       double *__restrict__ env)
+{
+    const size_t n = (size_t)kFiltBands * n_chb * kFiltCh * T;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const int cl = (int)(i % kFiltCh);
+        const int t = (int)((i / kFiltCh) % T);
+        const size_t rest = i / ((size_t)kFiltCh * T);
+        const int cb = (int)(rest % n_chb);
+        const int b = (int)(rest / n_chb);
+        const int ch = cb * kFiltCh + cl;
+        double v = 0.0;
+        if (ch < n_ch) {
+            const int bs = b < 8 ? b : 7;
+            v = fabs(prms[((size_t)ch * 8 + bs) * T + t]) * p_mod[bs];
+            if (b == 8) v = v * f16[t];
+        }
+        env[i] = v;
+    }
+}
+
+// Work buffers of the allocation-free entry point (pach_etc_to_filter_noalloc):
+// one fixed array per GPU for the life of the process.  Host code must not
+// pass `scratch` itself to a kernel (in host code it names a host-side
+// stand-in); noalloc_scratch() returns its device address.
+__device__ __align__(256) unsigned char scratch[kNoAllocScratch];
+
 }  // namespace
+
+hipError_t noalloc_scratch(int device, unsigned char **ptr)
+{
+    static std::mutex mu;
+    static unsigned char *addr[kMaxDevices] = {};
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        int cur = 0;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if ((e = hipSetDevice(device)) != hipSuccess) return e;
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        (void)hipSetDevice(cur);
+        if (e != hipSuccess) return e;
+        addr[device] = (unsigned char *)p;
+    }
+    *ptr = addr[device];
+    return hipSuccess;
+}
+
+hipError_t launch_env_from_prms(hipStream_t st, const double *prms, int n_ch, int T, const double *p_mod,
+                                const double *f16, int n_chb, double *env)
+{
+    const size_t n = (size_t)kFiltBands * n_chb * kFiltCh * T;
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_env_from_prms, dim3(blocks), dim3(256), 0, st, prms, n_ch, T, p_mod, f16, n_chb, env);
+    return hipGetLastError();
+}
 
 hipError_t launch_etc_conv(hipStream_t st, const double *env, const double *car, int T, int n_chb, int n_ch,
                            double *part, double *out)

@@ -157,6 +157,7 @@ struct pach_ctx {
     double *f_env = nullptr, *f_car = nullptr, *f_part = nullptr, *f_out = nullptr, *f_tab = nullptr;
     size_t f_env_n = 0, f_part_n = 0, f_out_n = 0, f_tab_n = 0;
     int f_car_fs = 0;
+    unsigned char *noalloc_scratch = nullptr;  // this device's allocation-free filter scratch
     // closest-hit query buffers (pach_shoot), grown on demand
     double *q_o = nullptr, *q_d = nullptr, *q_t = nullptr;
     int *q_tri = nullptr;
@@ -297,6 +298,11 @@ extern "C" int pach_create(pach_ctx **out, int device)
     for (auto &e : c->ev) (void)hipEventCreate(&e);
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->n_cu = prop.multiProcessorCount;
+    // device address of the allocation-free filters' scratch array (once per device)
+    if (hipError_t e = noalloc_scratch(device, &c->noalloc_scratch); e != hipSuccess) {
+        pach_destroy(c);
+        return PACH_EHIP;
+    }
     if (c->n_cu <= 0) c->n_cu = 256;
     if (dalloc(c->d_cnt, 1) != hipSuccess || dalloc(c->d_scal, 8) != hipSuccess || dalloc(c->d_band, 1) != hipSuccess || dalloc(c->d_geo, 80 * 9) != hipSuccess) {
         pach_destroy(c);
@@ -1958,18 +1964,31 @@ struct FiltTables {
     double p_mod[8];
 };
 
+// The nine carriers at sample rate fs ([9][kCarrierLen]); they depend only on
+// fs, so the last rate's set is kept.
+const std::vector<double> &carriers(int fs)
+{
+    static std::mutex mu;
+    static int cached_fs = 0;
+    static std::vector<double> car;
+    std::lock_guard<std::mutex> lock(mu);
+    if (cached_fs != fs) {
+        car.assign((size_t)kFiltBands * kCarrierLen, 0.0);
+        for (int b = 0; b < 8; ++b) {
+            const double fc = 62.5 * std::pow(2, b);
+            band_carrier(fs, fc / std::sqrt(2.0), fc * std::sqrt(2.0), &car[(size_t)b * kCarrierLen]);
+        }
+        band_carrier(fs, 16000.0 / std::sqrt(2.0), 16000.0 * std::sqrt(2.0), &car[(size_t)8 * kCarrierLen]);
+        cached_fs = fs;
+    }
+    return car;
+}
+
 FiltTables filt_tables(const pach_ctx *c, int T, const double swl[8], int fs)
 {
     FiltTables f;
-    if (!c->f_car || c->f_car_fs != fs) {
-        // carriers depend only on the sample rate: built when it changes
-        f.car.resize((size_t)kFiltBands * kCarrierLen);
-        for (int b = 0; b < 8; ++b) {
-            const double fc = 62.5 * std::pow(2, b);
-            band_carrier(fs, fc / std::sqrt(2.0), fc * std::sqrt(2.0), &f.car[(size_t)b * kCarrierLen]);
-        }
-        band_carrier(fs, 16000.0 / std::sqrt(2.0), 16000.0 * std::sqrt(2.0), &f.car[(size_t)8 * kCarrierLen]);
-    }
+    // the device copy of the carriers is replaced when the rate changes
+    if (!c->f_car || c->f_car_fs != fs) f.car = carriers(fs);
     for (int b = 0; b < 8; ++b) f.p_mod[b] = std::pow(10, (120 - swl[b]) / 20.0);
     const double tk = 20 + 273.15, pa = 101.325 * 1000, hr = 50.0;
     const double extra = iso9613_scalar(16000.0, tk, pa, hr) - iso9613_scalar(8000.0, tk, pa, hr);
@@ -2040,6 +2059,58 @@ extern "C" int pach_etc_to_filter(pach_ctx *c, const double *prms, int n_ch, int
     c->filter_stats.ms_total =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     return rc;
+}
+
+static_assert(kNoAllocScratch == PACH_NOALLOC_SCRATCH_BYTES, "the header states the scratch size");
+
+extern "C" int pach_etc_to_filter_noalloc(pach_ctx *c, const double *d_prms, int n_ch, int T, const double swl[8],
+                                          int sample_frequency, double *d_out)
+{
+    if (!c || !d_prms || !swl || !d_out || n_ch <= 0 || T <= 0 || sample_frequency <= 0) return PACH_EINVAL;
+    if (c->running) return fail(c, PACH_ESTATE, "trace running");
+    const auto t_start = std::chrono::steady_clock::now();
+    // scratch layout in doubles, every part 256-byte aligned: carriers, the
+    // p_mod / f16 tables, the envelopes, the nine band planes
+    auto al = [](size_t n) { return (n + 31) & ~(size_t)31; };
+    const size_t out_len = (size_t)T + kCarrierLen - 1;
+    const size_t n_chb = ((size_t)n_ch + kFiltCh - 1) / kFiltCh;
+    const size_t o_tab = al((size_t)kFiltBands * kCarrierLen);
+    const size_t o_env = al(o_tab + 8 + (size_t)T);
+    const size_t o_part = al(o_env + (size_t)kFiltBands * n_chb * kFiltCh * T);
+    const size_t need = (o_part + (size_t)kFiltBands * n_ch * out_len) * sizeof(double);
+    if (need > kNoAllocScratch)
+        return fail(c, PACH_ERANGE,
+                    "pach_etc_to_filter_noalloc: " + std::to_string(n_ch) + " channels of " + std::to_string(T) +
+                        " samples need " + std::to_string(need) + " scratch bytes, above the fixed " +
+                        std::to_string(kNoAllocScratch) + " (use pach_etc_to_filter)");
+    (void)hipSetDevice(c->device);
+    unsigned char *base = c->noalloc_scratch;
+    if (!base) HIPCHK(c, noalloc_scratch(c->device, &base));
+    double *sc = reinterpret_cast<double *>(base);
+    const std::vector<double> &car = carriers(sample_frequency);
+    std::vector<double> tab(8 + (size_t)T);
+    {
+        const FiltTables f = filt_tables(c, T, swl, sample_frequency);
+        std::copy(f.p_mod, f.p_mod + 8, tab.begin());
+        std::copy(f.f16.begin(), f.f16.end(), tab.begin() + 8);
+    }
+    hipStream_t st = c->stream;
+    HIPCHK(c, hipMemcpyAsync(sc, car.data(), car.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(c, hipMemcpyAsync(sc + o_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(c, launch_env_from_prms(st, d_prms, n_ch, T, sc + o_tab, sc + o_tab + 8, (int)n_chb, sc + o_env));
+    HIPCHK(c, hipEventRecord(c->ev[2], st));
+    HIPCHK(c, launch_etc_conv(st, sc + o_env, sc, T, (int)n_chb, n_ch, sc + o_part, d_out));
+    HIPCHK(c, hipEventRecord(c->ev[3], st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+    c->filter_stats.ms_kernel = ms;
+    c->filter_stats.launches = 1;
+    c->filter_stats.channels = n_ch;
+    c->filter_stats.samples = T;
+    c->filter_stats.ms_total =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    return PACH_OK;
 }
 
 extern "C" int pach_receiver_filters(pach_ctx *c, const double swl[8], double *out)
