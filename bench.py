@@ -3,16 +3,24 @@
 Workload: the seeded polyhedral hall (fan-shaped plan extruded to 12 m,
 ~2,000 jittered triangles, per-surface random absorption and scattering per
 band, 16 receivers r = 0.5 m), 1e6 geodesic source rays, 3 s cut-off, 1 ms
-bins, 32^3 voxel grid, FP32 traversal with FP64 histograms.  One step is one
-complete trace (SplitRayTracer.Begin -> Combine) of all rays; the inputs are
-resident in HBM before the timed region starts.
+bins, 32^3 voxel grid.  One step is one complete trace (SplitRayTracer.Begin
+-> Combine) of all rays; the inputs are resident in HBM before the timed
+region starts.
 
-Prints one JSON line: `value` is ray-bounces per second over the timed
-steps; `roofline` gives the octave kernel's algorithmic bytes per second of its
-own launch time against the 8 TB/s HBM peak, the HBM bytes per launch measured
-by rocprofv3 (`traffic`) and the kernel's VALU-issue load (`valu_issue`);
-`cpu_baseline` times the C FP64 restatement on the host's cores over a bounded
-sample.  Other workloads (--workload) report their own metric in the same form.
+Precision (--precision): "mixed" (default) walks the voxel grid with FP32
+triangle tests widened by their error bounds and decides every hit, and does
+all shading, in FP64: its histograms equal the FP64 tracer's bin for bin.
+"fp64" walks in FP64 as well; "fp32" is the all-FP32 walk (statistically
+equivalent, faster).
+
+Prints one JSON line: `value` is ray-bounces per second over the timed steps.
+`roofline` prices the octave kernel in algorithmic HBM bytes (128 B per octave
+bounce + 40 B per spawned ray) per second of its own launch time against the
+8 TB/s peak, gives the HBM bytes per launch rocprofv3 measured (`traffic`),
+and names the resource that actually binds (`binding`: VALU issue, from the
+profiled VALU instructions per bounce).  `cpu_baseline` times the C FP64
+restatement on the host's cores over a bounded sample.  Other workloads
+(--workload) report their own metric in the same form.
 """
 from __future__ import annotations
 
@@ -32,6 +40,9 @@ BYTES_OCT_BOUNCE = 128      # read ray 40 B + write ray 40 B + hit triangle 48 B
 BYTES_BROAD_BOUNCE = 184    # broadband ray 68 B x 2 + hit triangle 48 B
 BYTES_SPAWN = 40            # each spawned octave ray written once
 HBM_PEAK_GBS = 8000.0
+# arithmetic of the path: walk test / hit decision and shading / histograms
+DTYPES = {"fp32": "f32 (f64 histograms)", "mixed": "f32 walk screen + f64 hits, shading, histograms",
+          "fp64": "f64"}
 FP64_PEAK_TFLOPS = 78.6
 
 
@@ -46,7 +57,22 @@ def cpu_model():
     return platform.processor()
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_octave_hall_traffic.json")
+def traffic_file(precision):
+    return os.path.join(ROOT, "profiles", f"k_octave_{precision}_hall_traffic.json")
+
+
+def load_profile(args, n_rays):
+    """The committed rocprofv3 counter summary of this precision's k_octave on
+    the 1e6-ray hall, and whether this run is that same workload."""
+    try:
+        with open(traffic_file(args.precision)) as f:
+            prof = json.load(f)
+    except (OSError, ValueError):
+        return None, False
+    same = (args.workload == "hall" and prof.get("rays") == n_rays and prof.get("cutoff") == args.cutoff
+            and prof.get("grid") == (args.grid or 32) and prof.get("precision") == args.precision
+            and not args.tune and not args.autotune)
+    return prof, same
 
 
 def measured_traffic(args, tot, n_rays):
@@ -55,43 +81,34 @@ def measured_traffic(args, tot, n_rays):
     of known size): the profiled command's bytes per octave bounce times this
     run's octave bounces per launch (launch counts vary from run to run; the
     bounces of a workload do not).  None when no profile is committed."""
-    try:
-        with open(TRAFFIC_FILE) as f:
-            prof = json.load(f)
-    except (OSError, ValueError):
+    prof, same = load_profile(args, n_rays)
+    if prof is None or args.workload != "hall":
         return None, None
-    same = (args.workload == "hall" and prof.get("rays") == n_rays and prof.get("cutoff") == args.cutoff
-            and prof.get("grid") == (args.grid or 32) and prof.get("precision") == args.precision
-            and not args.tune and not args.autotune)
     per_bounce = prof["hbm_bytes_per_bounce"]
     where = "this workload" if same else "the 1e6-ray hall"
     return (per_bounce * tot["oct"] / max(tot["n_oct"], 1),
             f"{per_bounce:.2f} HBM B per octave bounce measured on {where} ('{prof['command']}', "
-            f"profiles/k_octave_fp32_hall_1e6_rays.md) x this run's octave bounces per launch")
+            f"{os.path.relpath(traffic_file(args.precision), ROOT)}) x this run's octave bounces per launch")
 
 
 def valu_issue(args, tot, n_rays):
-    """The k_octave bound the HBM roofline does not show: VALU issue.  The
-    profiled VALU instructions per octave bounce (rocprofv3 SQ_INSTS_VALU of
-    the same workload), at 4 cycles per wave64 instruction, times this run's
-    octave bounces per second of kernel time, over the cycles of 1,024 SIMDs at
+    """The resource that binds k_octave: VALU issue.  The profiled VALU
+    instructions per octave bounce (rocprofv3 SQ_INSTS_VALU of the same
+    workload), at 4 cycles per wave64 instruction, times this run's octave
+    bounces per second of kernel time, over the cycles of 1,024 SIMDs at
     2.4 GHz.  None unless this run is the profiled workload."""
-    try:
-        with open(TRAFFIC_FILE) as f:
-            prof = json.load(f)
-    except (OSError, ValueError):
-        return None
-    same = (args.workload == "hall" and prof.get("rays") == n_rays and prof.get("cutoff") == args.cutoff
-            and prof.get("grid") == (args.grid or 32) and prof.get("precision") == args.precision
-            and not args.tune and not args.autotune)
-    ipb = prof.get("valu_instr_per_oct_bounce")
+    prof, same = load_profile(args, n_rays)
+    ipb = prof.get("valu_instr_per_oct_bounce") if prof else None
     if not same or not ipb or tot["ms_oct"] <= 0:
         return None
     rate = tot["oct"] / (tot["ms_oct"] * 1e-3)
-    return {"instr_per_bounce": ipb, "lane_use": prof.get("valu_lane_utilisation"),
-            "busy_frac": ipb * 4 * rate / (1024 * 2.4e9),
-            "note": "estimate: profiled VALU instructions per octave bounce x 4 cycles x this run's octave "
-                    "bounces per second of k_octave time / (1,024 SIMDs x 2.4 GHz)"}
+    busy = ipb * 4 * rate / (1024 * 2.4e9)
+    lanes = prof.get("valu_lane_utilisation")
+    return {"resource": "valu_issue", "frac": busy, "instr_per_bounce": ipb, "lane_use": lanes,
+            "lane_frac": busy * lanes if lanes else None,
+            "note": "VALU issue busy fraction: profiled VALU instructions per octave bounce x 4 cycles x "
+                    "this run's octave bounces per second of k_octave time / (1,024 SIMDs x 2.4 GHz); "
+                    "lane_frac also weights it by the profiled active-lane share"}
 
 
 def main():
@@ -101,7 +118,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--rays", type=int, default=1_000_000)
     ap.add_argument("--cutoff", type=float, default=3.0)
-    ap.add_argument("--precision", choices=["fp32", "fp64", "mixed"], default="fp32")
+    ap.add_argument("--precision", choices=["fp32", "fp64", "mixed"], default="mixed")
     ap.add_argument("--grid", type=int, default=None,
                     help="voxels per axis (default: 32 for the hall, automatic for the batch)")
     ap.add_argument("--cpu-rays", type=int, default=5000, help="source rays of the CPU sample")
@@ -192,7 +209,7 @@ def main():
     for w in range(args.warmup):
         step(args.seed + 1000 + w)
         print(f"[bench] warmup {w} done", file=sys.stderr, flush=True)
-    tot = dict(bounces=0, oct=0, broad=0, spawn=0, ms_oct=0.0, n_oct=0, ms_p1=0.0)
+    tot = dict(bounces=0, oct=0, broad=0, spawn=0, ms_oct=0.0, n_oct=0, ms_p1=0.0, ms_exact=0.0)
     t0 = time.perf_counter()
     for k in range(args.steps):
         st = step(args.seed + k)
@@ -204,6 +221,7 @@ def main():
         tot["ms_oct"] += st["ms_octave"]
         tot["n_oct"] += st["launches_octave"]
         tot["ms_p1"] += st["ms_phase1"]
+        tot["ms_exact"] += st["ms_exact"]
         print(f"[bench] step {k}: {st['bounces']:.3e} bounces in {st['ms_trace']:.1f} ms",
               file=sys.stderr, flush=True)
     dt = time.perf_counter() - t0
@@ -212,6 +230,7 @@ def main():
     oct_bytes = tot["oct"] * BYTES_OCT_BOUNCE + tot["spawn"] * BYTES_SPAWN
     achieved = oct_bytes / (tot["ms_oct"] / 1e3) / 1e9 if tot["ms_oct"] > 0 else 0.0
     traffic, traffic_note = measured_traffic(args, tot, n_rays)
+    avg_ms = tot["ms_oct"] / max(tot["n_oct"], 1)
 
     cpu = None
     if not args.no_cpu:
@@ -258,7 +277,7 @@ def main():
         "metric": "ray-bounces/s", "value": value, "unit": "ray-bounces/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": args.precision, "data": "synthetic",
+        "dtype": DTYPES[args.precision], "data": "synthetic",
         "config": {"workload": workload_name,
                    "rays": n_rays, "triangles": room.n_tris, "receivers": int(room.receivers.shape[0]),
                    "grid": tr_grid, "tuning": tuning,
@@ -269,8 +288,12 @@ def main():
                    "rays_per_s": n_rays * args.steps / dt},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_note": traffic_note,
-                     "kernel": "k_octave", "avg_launch_ms": tot["ms_oct"] / max(tot["n_oct"], 1),
-                     "launches": tot["n_oct"], "valu_issue": valu_issue(args, tot, n_rays)},
+                     "achieved_note": f"algorithmic bytes: {BYTES_OCT_BOUNCE} B per octave bounce (ray read "
+                                      f"+ written, hit triangle) + {BYTES_SPAWN} B per spawned ray",
+                     "measured_gbs": (traffic / (avg_ms * 1e-3) / 1e9) if traffic and avg_ms > 0 else None,
+                     "kernel": "k_octave", "avg_launch_ms": avg_ms,
+                     "launches": tot["n_oct"], "binding": valu_issue(args, tot, n_rays),
+                     "ms_fp64_fallback": tot["ms_exact"]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -126,6 +126,9 @@ struct pach_ctx {
     // counters and small device scalars
     DevCounters *d_cnt = nullptr;
     unsigned *d_scal = nullptr;  // [0] p1_count [1] res_count [2] in_head [3..4] dump_count[2]
+                                 // [5] exact-queue count [6] source rays with live rays
+    uint32_t *d_live = nullptr;  // one bit per source ray of the range being traced
+    size_t live_words = 0;
     BandTab *d_band = nullptr;
 
     WorkBuf<float> w32;
@@ -366,7 +369,7 @@ extern "C" void pach_destroy(pach_ctx *c)
     dfree(c->d_ctr32); dfree(c->d_ctr64); dfree(c->d_hist);
     dfree(c->f_env); dfree(c->f_car); dfree(c->f_part); dfree(c->f_out); dfree(c->f_tab);
     dfree(c->d_map_ctr); dfree(c->d_map_start); dfree(c->d_map_list); dfree(c->d_map_hist);
-    dfree(c->d_cnt); dfree(c->d_scal); dfree(c->d_band); dfree(c->d_rooms);
+    dfree(c->d_cnt); dfree(c->d_scal); dfree(c->d_band); dfree(c->d_rooms); dfree(c->d_live);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1102,6 +1105,7 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
     unsigned *p1_count = c->d_scal + 0, *res_count = c->d_scal + 1, *in_head = c->d_scal + 2;
     unsigned *dump_count = c->d_scal + 3;  // [2]
     unsigned *xq_count = c->d_scal + 5;
+    unsigned *live_count = c->d_scal + 6;
     hipStream_t st = c->stream;
     HIPCHK(c, hipMemcpyAsync(c->d_band, &bt, sizeof(bt), hipMemcpyHostToDevice, st));
     HIPCHK(c, hipMemsetAsync(c->d_hist, 0, c->hist_n * kPlanes * sizeof(double), st));
@@ -1123,6 +1127,12 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
         int res_cur = 0;
         int64_t next = first;   // next fresh source ray
         const int64_t end = first + count;
+        const size_t words = (size_t)(count + 31) / 32;
+        if (words > c->live_words) {
+            HIPCHK(c, hipStreamSynchronize(st));
+            HIPCHK(c, dalloc(c->d_live, words));
+            c->live_words = words;
+        }
         HIPCHK(c, hipMemsetAsync(c->d_scal, 0, 8 * sizeof(unsigned), st));
         for (;;) {
             const bool stop_new = c->cancel.load();
@@ -1177,6 +1187,14 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
                     HIPCHK(c, hipEventRecord(c->ev[4], st));
                 }
             }
+            // progress: the source rays started in this range that still have
+            // rays in the next launch's input (octave or suspended broadband)
+            HIPCHK(c, hipMemsetAsync(c->d_live, 0, (size_t)(next - first + 31) / 32 * sizeof(uint32_t), st));
+            HIPCHK(c, hipMemsetAsync(live_count, 0, sizeof(unsigned), st));
+            HIPCHK(c, launch_live_sources<R>(st, w.dump[1 - cur], dump_count + (1 - cur), w.dump_cap,
+                                             w.res[res_cur], res_count, (unsigned)w.res_cap, (uint32_t)first,
+                                             (uint32_t)count,
+                                             c->d_live, live_count));
             unsigned h[8];
             HIPCHK(c, hipMemcpyAsync(h, c->d_scal, sizeof(h), hipMemcpyDeviceToHost, st));
             HIPCHK(c, hipStreamSynchronize(st));
@@ -1190,8 +1208,9 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
             if (n_resume > w.res_cap) return fail(c, PACH_EHIP, "resume buffer overflow");
             // every launch completes the bounces its waves start, so this only trips on a defect
             if (n_oct > ((int64_t)1 << 20)) return fail(c, PACH_EHIP, "octave launches exceed 2^20: no progress");
-            // a source ray is complete once its broadband phase and every ray it spawned have ended
-            if (n_resume == 0 && pending == 0) c->rays_done = started;
+            // a source ray is complete once its broadband phase and every ray it
+            // spawned have ended (earlier ranges are complete when this one starts)
+            c->rays_done = started - (int64_t)h[6];
         }
         return PACH_OK;
     };

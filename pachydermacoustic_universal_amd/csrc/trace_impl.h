@@ -736,7 +736,18 @@ __device__ __forceinline__ void receiver_test(const DevRecv<R> &rv, int r, R ox,
     const R b = mx * dx + my * dy + mz * dz;
     const R c = (mx * mx + my * my + mz * mz) - rv.r2;
     if (c > R(0) && b > R(0)) return;
-    const R disc = b * b - c;
+    R disc;
+    if constexpr (sizeof(R) == 4) {
+        // the squared half-chord as r^2 - |m - b d|^2: in FP32, b^2 - c loses
+        // ~|m|^2 x 2^-24 (1e-4 m^2 at 40 m) to cancellation against half-chords
+        // of at most r^2; the perpendicular offset keeps it to ~|m| x 2^-24 x r
+        // (Haines et al., "Precision Improvements for Ray/Sphere Intersection",
+        // Ray Tracing Gems, 2019).  FP64 keeps the reference's b^2 - c.
+        const R qx = mx - b * dx, qy = my - b * dy, qz = mz - b * dz;
+        disc = rv.r2 - (qx * qx + qy * qy + qz * qz);
+    } else {
+        disc = b * b - c;
+    }
     if (disc < R(0)) return;
     const R sq = fsqrt<R>(disc);
     const R t1 = -b - sq, t2 = -b + sq;
@@ -1814,6 +1825,45 @@ hipError_t launch_phase1(hipStream_t st, const TraceConst &tc, const DevScene<R>
     return hipGetLastError();
 }
 
+// A source ray is complete once its broadband phase and every octave ray it
+// spawned have ended (TaskAll:274-312 advances the completed count per source
+// ray): between launches the live ones are the sources of the rays handed to
+// the next launch.  One bit per source ray; each newly set bit counts once.
+template <class R>
+__global__ void __launch_bounds__(256) k_live_sources(const ORay<R> *__restrict__ rays, const unsigned *n_rays,
+                                                      unsigned cap, const P1State<R> *__restrict__ res,
+                                                      const unsigned *n_res, unsigned res_cap, uint32_t first,
+                                                      uint32_t span, uint32_t *bits, unsigned *live)
+{
+    const unsigned nr = min(*n_rays, cap), ns = min(*n_res, res_cap);
+    const size_t n = (size_t)nr + ns;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        uint32_t src;
+        bool alive = true;
+        if (i < nr) {
+            alive = !(rays[i].meta & M_DEAD);
+            src = rays[i].src;
+        } else {
+            src = res[i - nr].src;
+        }
+        const uint32_t k = src - first;
+        if (alive && k < span) {
+            const uint32_t m = 1u << (k & 31u);
+            if (!(atomicOr(&bits[k >> 5], m) & m)) atomicAdd(live, 1u);
+        }
+    }
+}
+
+template <class R>
+hipError_t launch_live_sources(hipStream_t st, const ORay<R> *rays, const unsigned *n_rays, unsigned cap,
+                               const P1State<R> *res, const unsigned *n_res, unsigned res_cap, uint32_t first,
+                               uint32_t span, uint32_t *bits, unsigned *live)
+{
+    hipLaunchKernelGGL(k_live_sources<R>, dim3(1024), dim3(256), 0, st, rays, n_rays, cap, res, n_res, res_cap,
+                       first, span, bits, live);
+    return hipGetLastError();
+}
+
 template <class R, int BS, int OCC, bool SQ, bool MAP = false, bool MIX = false>
 static void launch_octave_bs(hipStream_t st, int grid, size_t lds, const TraceConst &tc, const DevScene<R> &s,
                              const DevScene<float> &sg, const DevRecv<R> &rv, const ORay<R> *in0, const unsigned *n0,
@@ -2006,6 +2056,9 @@ template <class R> int octave_map_blocks_per_cu(int lds_bytes)
     template hipError_t launch_shoot<R>(hipStream_t, const TraceConst &, const DevScene<R> &,      \
                                         const DevScene<float> &, const double *, const double *,   \
                                         int64_t, int *, double *, unsigned *);                     \
-    template int octave_map_blocks_per_cu<R>(int);
+    template int octave_map_blocks_per_cu<R>(int);                                              \
+    template hipError_t launch_live_sources<R>(hipStream_t, const ORay<R> *, const unsigned *,     \
+                                               unsigned, const P1State<R> *, const unsigned *,    \
+                                               unsigned, uint32_t, uint32_t, uint32_t *, unsigned *);
 
 }  // namespace pach

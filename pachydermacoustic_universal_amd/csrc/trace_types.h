@@ -193,6 +193,17 @@ hipError_t launch_phase1(hipStream_t st, const TraceConst &tc, const DevScene<R>
                          unsigned *out_count, unsigned out_cap, P1State<R> *resume_out,
                          unsigned *resume_count, DevCounters *cnt);
 
+// Progress: marks in bits[] the source rays [first, first + span) that still
+// have live rays between launches (the n_rays octave rays of rays, dead slots
+// skipped, and the n_res suspended broadband rays of res; counts read on the
+// device, at most cap and res_cap) and adds each newly marked source ray to
+// *live.  The caller clears
+// bits and *live first.
+template <class R>
+hipError_t launch_live_sources(hipStream_t st, const ORay<R> *rays, const unsigned *n_rays, unsigned cap,
+                               const P1State<R> *res, const unsigned *n_res, unsigned res_cap, uint32_t first,
+                               uint32_t span, uint32_t *bits, unsigned *live);
+
 // Launch shape of k_octave: workgroup size (256 or 512) and register cap
 // (occ 0: the compiler's choice; 4 / 5: at most 128 / 102 VGPRs, so 4 / 5
 // waves share a SIMD; -1: 4 for FP32, 0 for FP64, whose 4-wave build spills

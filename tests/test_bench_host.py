@@ -1,7 +1,8 @@
 """bench.py's host-side contract, on the CPU: more than one GPU is refused
 with a message, the measured-traffic figure scales the profiled bytes per
-octave bounce by a run's bounces per launch, and the VALU-issue estimate is
-given only for the profiled workload."""
+octave bounce by a run's bounces per launch (per precision, from that
+precision's committed profile), and the VALU-issue figure that names the
+binding resource is given only for the profiled workload."""
 import importlib.util
 import json
 import os
@@ -34,26 +35,44 @@ def test_more_than_one_gpu_is_refused():
     assert "one GPU" in (r.stdout + r.stderr)
 
 
-def test_traffic_and_valu_issue_follow_the_profile():
+PROFILED = [p for p in ("fp32", "mixed") if os.path.exists(
+    os.path.join(ROOT, "profiles", f"k_octave_{p}_hall_traffic.json"))]
+
+
+@pytest.mark.parametrize("precision", PROFILED)
+def test_traffic_and_valu_issue_follow_the_profile(precision):
     b = bench_module()
-    prof = json.load(open(b.TRAFFIC_FILE))
+    prof = json.load(open(b.traffic_file(precision)))
+    assert prof["precision"] == precision
     tot = dict(oct=4.0e10, n_oct=20, ms_oct=9000.0)
-    traffic, note = b.measured_traffic(default_args(), tot, prof["rays"])
+    args = default_args(precision=precision)
+    traffic, note = b.measured_traffic(args, tot, prof["rays"])
     assert traffic == pytest.approx(prof["hbm_bytes_per_bounce"] * 4.0e10 / 20)
     assert "this workload" in note
-    v = b.valu_issue(default_args(), tot, prof["rays"])
+    v = b.valu_issue(args, tot, prof["rays"])
     rate = 4.0e10 / 9.0
-    assert v["busy_frac"] == pytest.approx(prof["valu_instr_per_oct_bounce"] * 4 * rate / (1024 * 2.4e9))
+    assert v["resource"] == "valu_issue"
+    assert v["frac"] == pytest.approx(prof["valu_instr_per_oct_bounce"] * 4 * rate / (1024 * 2.4e9))
+    assert v["lane_frac"] == pytest.approx(v["frac"] * prof["valu_lane_utilisation"])
     assert v["instr_per_bounce"] == prof["valu_instr_per_oct_bounce"]
 
 
-@pytest.mark.parametrize("change", [dict(workload="map"), dict(precision="fp64"), dict(grid=48),
-                                    dict(tune=["walk_steps=8"])])
-def test_valu_issue_only_for_the_profiled_workload(change):
+@pytest.mark.parametrize("change", [dict(grid=48), dict(tune=["walk_steps=8"]), dict(cutoff=2.0)])
+@pytest.mark.parametrize("precision", PROFILED)
+def test_valu_issue_only_for_the_profiled_workload(change, precision):
     b = bench_module()
-    prof = json.load(open(b.TRAFFIC_FILE))
+    prof = json.load(open(b.traffic_file(precision)))
     tot = dict(oct=4.0e10, n_oct=20, ms_oct=9000.0)
-    assert b.valu_issue(default_args(**change), tot, prof["rays"]) is None
+    args = default_args(precision=precision, **change)
+    assert b.valu_issue(args, tot, prof["rays"]) is None
     # the traffic figure is still given, labelled as measured on the profiled hall
-    traffic, note = b.measured_traffic(default_args(**change), tot, prof["rays"])
+    traffic, note = b.measured_traffic(args, tot, prof["rays"])
     assert traffic > 0 and "1e6-ray hall" in note
+
+
+def test_no_figures_without_a_profile_of_that_precision_or_workload():
+    b = bench_module()
+    tot = dict(oct=4.0e10, n_oct=20, ms_oct=9000.0)
+    assert b.measured_traffic(default_args(precision="fp64"), tot, 1_000_000) == (None, None)
+    assert b.valu_issue(default_args(precision="fp64"), tot, 1_000_000) is None
+    assert b.measured_traffic(default_args(workload="map"), tot, 1_000_000) == (None, None)

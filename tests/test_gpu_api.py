@@ -74,3 +74,36 @@ def test_cancel_keeps_completed_rays_only():
     ref = O.scale(ora, k)[0]
     np.testing.assert_allclose(rt.GetReceiver.planes[0], ref, rtol=1e-5, atol=1e-12 * ref.max())
     rt.close()
+
+
+def test_progress_counts_completed_source_rays_while_tracing():
+    """pach_poll's rays_done advances while the trace runs, as the reference
+    advances its completed-ray count per finished source ray (TaskAll:274-312):
+    it never falls, stays below the ray count until the trace ends, takes many
+    values on the way, and ends at the ray count."""
+    from pachydermacoustic_universal_amd import _abi as A
+
+    room = rooms.fan_hall()
+    at, c, rho = A.uniform_medium()
+    tr = A.Tracer(0)
+    tr.upload_scene(room.verts, room.surf, room.alpha, room.scat, room.trans, (32, 32, 32))
+    tr.set_source(room.source, np.ones(8), A.DIR_GEODESIC)
+    tr.set_receivers(room.receivers, room.radius, at, c, c * rho, 1000, 1500)
+    n = 300_000
+    print(f"[case] async hall trace n={n} batch 4096", flush=True)
+    tr.trace(rays=n, cutoff_m=1.5 * c, seed=3, precision=A.PREC_FP32, batch_rays=4096, async_=True)
+    seen = []
+    while True:
+        p = tr.poll()
+        seen.append((p["rays_done"], p["running"]))
+        if not p["running"]:
+            break
+        time.sleep(0.002)
+    assert tr.wait() == 0
+    done = [d for d, _ in seen]
+    assert all(b >= a for a, b in zip(done, done[1:])), done
+    mid = sorted({d for d, running in seen if running and 0 < d < n})
+    print(f"  {len(seen)} polls, {len(mid)} distinct intermediate counts", flush=True)
+    assert len(mid) >= 5, seen[:20]
+    assert tr.poll()["rays_done"] == n and tr.stats()["rays_done"] == n
+    tr.close()
