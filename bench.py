@@ -125,6 +125,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--batch-rays", type=int, default=0, help="source rays per phase-1 launch (0: automatic)")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="pach_tune setting, e.g. dist_lds=0")
     ap.add_argument("--autotune", type=int, default=0, metavar="RAYS",
@@ -203,7 +204,7 @@ def main():
         print(f"[bench] autotune: {tuning}", file=sys.stderr, flush=True)
 
     def step(seed):
-        tr.trace(rays=trace_rays, cutoff_m=args.cutoff * c, seed=seed, precision=prec)
+        tr.trace(rays=trace_rays, cutoff_m=args.cutoff * c, seed=seed, precision=prec, batch_rays=args.batch_rays)
         return tr.stats()
 
     for w in range(args.warmup):

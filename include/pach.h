@@ -83,7 +83,7 @@ typedef struct pach_params {
     uint64_t seed;        /* Philox key of every ray stream */
     int p1_max_bounces;   /* broadband-phase bounce cap (0 = 128) */
     int max_bounces;      /* per octave ray bounce guard, phases 2+3 (0 = 100000) */
-    int64_t batch_rays;   /* source rays per phase-1 launch (0 = auto) */
+    int64_t batch_rays;   /* source rays per phase-1 launch (0 = min(ray count, 131072)) */
 } pach_params;
 
 typedef struct pach_progress {
@@ -165,6 +165,13 @@ int pach_scene_upload(pach_ctx *ctx, const double *verts, int T, const int *surf
  * is filled only when large enough, so call once with tris = NULL to size it. */
 int pach_grid_build(const double *verts, int T, int dims[3], double origin[3], double cell[3],
                     uint32_t *ranges, uint32_t *tris, int64_t *n_list, uint8_t *dist);
+
+/* The walk's per-face skip masks of the same grid (inspection and tests; no
+ * counterpart in the reference, whose per-ray mailbox,
+ * Classes_VoxelGrid_PolyRefractive.cs:162-164, they replace): skip
+ * [ncell][6] (faces -x, +x, -y, +y, -z, +z), bit i set when entry i < 32 of
+ * the voxel's list is also listed in the neighbour across that face. */
+int pach_grid_face_skip(const double *verts, int T, int dims[3], uint32_t *skip);
 
 /* Source origin, per-band power in W (Source.SoundPower, Classes_Sources.cs:96-104)
  * and direction mode.  dirs [n_dirs][3] only for PACH_DIR_SUPPLIED. */

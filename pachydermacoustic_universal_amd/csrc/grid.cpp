@@ -121,6 +121,35 @@ void receiver_cells(const Grid &g, const double *centers, int R, const int *bit,
     linf_distance(g.dims, dist);
 }
 
+void face_skip(const Grid &g, std::vector<uint32_t> &skip)
+{
+    const int gx = g.dims[0], gy = g.dims[1], gz = g.dims[2];
+    const size_t ncell = (size_t)gx * gy * gz;
+    skip.assign(ncell * 6, 0u);
+    std::vector<uint32_t> nb;
+    for (int z = 0; z < gz; ++z)
+        for (int y = 0; y < gy; ++y)
+            for (int x = 0; x < gx; ++x) {
+                const size_t c = ((size_t)z * gy + y) * gx + x;
+                const uint32_t first = g.ranges[2 * c], n = std::min<uint32_t>(g.ranges[2 * c + 1], 32);
+                if (n == 0) continue;
+                for (int f = 0; f < 6; ++f) {
+                    int q[3] = {x, y, z};
+                    q[f / 2] += (f & 1) ? 1 : -1;
+                    if (q[f / 2] < 0 || q[f / 2] >= g.dims[f / 2]) continue;
+                    const size_t cn = ((size_t)q[2] * gy + q[1]) * gx + q[0];
+                    const uint32_t f0 = g.ranges[2 * cn], fn = g.ranges[2 * cn + 1];
+                    if (fn == 0) continue;
+                    nb.assign(g.tris.begin() + f0, g.tris.begin() + f0 + fn);
+                    std::sort(nb.begin(), nb.end());
+                    uint32_t m = 0;
+                    for (uint32_t i = 0; i < n; ++i)
+                        if (std::binary_search(nb.begin(), nb.end(), g.tris[first + i])) m |= 1u << i;
+                    skip[6 * c + f] = m;
+                }
+            }
+}
+
 int build_grid(const double *verts, int T, const int dims_in[3], Grid &g, std::string &err)
 {
     if (!verts || T <= 0) { err = "grid: empty scene"; return -1; }
@@ -208,5 +237,18 @@ extern "C" int pach_grid_build(const double *verts, int T, int dims[3], double o
     if (tris && *n_list >= (int64_t)g.tris.size()) std::copy(g.tris.begin(), g.tris.end(), tris);
     if (dist) std::copy(g.dist.begin(), g.dist.begin() + ncell, dist);
     *n_list = (int64_t)g.tris.size();
+    return 0;
+}
+
+extern "C" int pach_grid_face_skip(const double *verts, int T, int dims[3], uint32_t *skip)
+{
+    if (!verts || !dims || !skip) return -1;
+    pach::Grid g;
+    std::string err;
+    if (pach::build_grid(verts, T, dims, g, err) != 0) return -1;
+    for (int k = 0; k < 3; ++k) dims[k] = g.dims[k];
+    std::vector<uint32_t> m;
+    pach::face_skip(g, m);
+    std::copy(m.begin(), m.end(), skip);
     return 0;
 }

@@ -25,6 +25,11 @@ void linf_distance(const int dims[3], std::vector<uint8_t> &dist);
 // as occupied
 void receiver_cells(const Grid &g, const double *centers, int R, const int *bit, double radius,
                     std::vector<uint64_t> &mask, std::vector<uint8_t> &dist);
+// For each voxel and each of its faces f (0: -x, 1: +x, 2: -y, 3: +y, 4: -z,
+// 5: +z), skip[6 voxel + f] has bit i set when entry i (i < 32) of the voxel's
+// list is also listed in the neighbour across f: a walk that steps in through
+// face f has already tested that triangle in the neighbour.
+void face_skip(const Grid &g, std::vector<uint32_t> &skip);
 // dims_in: requested voxel counts (any <= 0: chosen from the triangle count)
 int build_grid(const double *verts, int T, const int dims_in[3], Grid &g, std::string &err);
 

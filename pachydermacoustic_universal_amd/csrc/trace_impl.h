@@ -223,11 +223,17 @@ __device__ __forceinline__ uint8_t cell_dist(const uint8_t *g, int cell)
 
 // Resumable closest-hit walk over the uniform grid (Amanatides-Woo), with the
 // reference's rule that a hit ends the walk once it lies inside the current
-// voxel (Classes_VoxelGrid_PolyRefractive.cs:172).  Triangles whose hit lies
-// past the current voxel's exit are left for the voxel that contains it (every
-// voxel a triangle touches lists it), so each voxel runs the full inside test
-// on few candidates.  Empty space: a voxel at L-inf distance k >= 2 from every
-// occupied voxel lies inside a (2k-1)^3 empty block, crossed in one jump.
+// voxel (Classes_VoxelGrid_PolyRefractive.cs:172).  Empty space: a voxel at
+// L-inf distance k >= 2 from every occupied voxel lies inside a (2k-1)^3 empty
+// block, crossed in one jump.
+//
+// Each triangle is tested once per walk where the reference keeps a per-ray
+// mailbox (Classes_VoxelGrid_PolyRefractive.cs:162-164): the flattened walk
+// keeps the nearest hit found so far wherever it lies (not only hits inside
+// the current voxel), and a voxel entered by one step through face f skips
+// the list entries its neighbour across f also lists (DevScene::cell_skip),
+// which the walk tested there.  After a jump, and in the first voxel, every
+// entry is tested.
 template <class R> struct Walk {
     int cx, cy, cz;
     R tmx, tmy, tmz;  // t where the ray leaves the current voxel along each axis
@@ -235,14 +241,16 @@ template <class R> struct Walk {
     int best;
     R tbest;
     uint64_t rmask;   // receivers whose spheres touch a voxel the walk crossed
-    uint32_t j, je;   // flattened walk: untested list entries of the current voxel
+    uint32_t j, je;   // flattened walk: list window [j, j + 32) of the current voxel, list end je
+    uint32_t todo;    // flattened walk: entries of the window still to test (bit i: entry j + i)
+    int from;         // face of the current voxel the walk stepped in through (-1: none)
     int k;            // flattened walk: distance value of the current voxel
     // mixed walk only (unused, and removed by the compiler, elsewhere)
     R eb;             // error bound of tbest
     R sl, sl2;        // least and second least lower bound t - e of the other candidates (-inf: undecidable)
     int sid;          // the candidate of sl
     int zid;          // a candidate whose interval reaches t <= 0 (a neighbour at the departure point)
-    R tlim, tend;     // current voxel: candidates accepted up to tlim; the walk ends once tbest + eb <= tend
+    R tend;           // current voxel: the walk ends once tbest + eb <= tend
 };
 
 // t where the ray leaves voxel c along one axis: the upper face when d > 0,
@@ -273,6 +281,8 @@ __device__ __forceinline__ void walk_init(const DevScene<R> &s, Walk<R> &w, R ox
     w.best = -1;
     w.tbest = INF;
     w.j = w.je = 0;
+    w.todo = 0;
+    w.from = -1;
     w.k = -1;  // current voxel not loaded yet
 }
 
@@ -361,11 +371,15 @@ __device__ __forceinline__ void walk_enter(const DevScene<R> &s, Walk<R> &w, uin
     const int cell = (w.cz * s.gy + w.cy) * s.gx + w.cx;
     w.k = cell_dist<LDS>(s.cell_dist, cell);
     w.j = w.je = 0;
+    w.todo = 0;
     if (w.k == 0) {
         if (s.cell_rmask) w.rmask |= s.cell_rmask[cell];
         const uint2 rg = s.cell_range[cell];
         w.j = rg.x;
         w.je = rg.x + rg.y;
+        const uint32_t all = rg.y >= 32 ? ~0u : (1u << rg.y) - 1u;
+        const uint32_t seen = w.from >= 0 ? s.cell_skip[6 * cell + w.from] : 0u;
+        w.todo = all & ~seen;
     }
 }
 
@@ -399,6 +413,10 @@ __device__ __forceinline__ bool walk_move(const DevScene<R> &s, Walk<R> &w, R ox
     nz = ax == 2 ? (dz > R(0) ? hz + 1 : lz - 1) : nz;
     if ((unsigned)nx >= (unsigned)s.gx || (unsigned)ny >= (unsigned)s.gy || (unsigned)nz >= (unsigned)s.gz)
         return true;
+    // a single step enters through the face towards the voxel left (-x when
+    // moving in +x); a jump leaves no neighbour whose list was tested
+    const R da = ax == 0 ? dx : (ax == 1 ? dy : dz);
+    w.from = h == 0 ? 2 * ax + (da > R(0) ? 0 : 1) : -1;
     w.cx = nx; w.cy = ny; w.cz = nz;
     w.tmx = axis_exit<R>(ox, dx, w.ix, nx, s.gmin[0], s.cs[0]);
     w.tmy = axis_exit<R>(oy, dy, w.iy, ny, s.gmin[1], s.cs[1]);
@@ -406,12 +424,39 @@ __device__ __forceinline__ bool walk_move(const DevScene<R> &s, Walk<R> &w, R ox
     return false;
 }
 
+// Picks the next three entries of the walk's current voxel to test (the last
+// one repeated when fewer are left: a repeated triangle never replaces
+// itself), or opens the next 32-entry window of a longer list (never skipped).
+struct Three {
+    uint32_t j0, j1, j2, n;
+};
+template <class R> __device__ __forceinline__ Three next_three(Walk<R> &w)
+{
+    if (w.todo == 0) {
+        w.j += 32;
+        const uint32_t n = w.je - w.j;
+        w.todo = n >= 32 ? ~0u : (1u << n) - 1u;
+    }
+    const uint32_t m0 = w.todo, m1 = m0 & (m0 - 1u), m2 = m1 & (m1 - 1u);
+    const uint32_t b0 = __builtin_ctz(m0);
+    const uint32_t b1 = m1 ? __builtin_ctz(m1 | (1u << 31)) : b0;
+    const uint32_t b2 = m2 ? __builtin_ctz(m2 | (1u << 31)) : b1;
+    w.todo = m2 & (m2 - 1u);
+    return Three{w.j + b0, w.j + b1, w.j + b2, 1u + (m1 != 0u) + (m2 != 0u)};
+}
+
+// Entries of the current voxel still to test.
+template <class R> __device__ __forceinline__ bool walk_pending(const Walk<R> &w)
+{
+    return (w.todo != 0u) | (w.j + 32u < w.je);
+}
+
 // Flattened form of walk_steps: each of the `budget` units either tests the
-// next three list entries of the current voxel or leaves the voxel (one step,
-// or one jump across an empty block, by the same straight-line code), so
-// lanes in triangle-dense voxels do not hold the wave on a long inner loop.
-// Same voxels, same closest hit as walk_steps (the voxel exits are recomputed
-// after each move rather than accumulated, which changes only their rounding).
+// next three untested list entries of the current voxel or leaves the voxel
+// (one step, or one jump across an empty block, by the same straight-line
+// code), so lanes in triangle-dense voxels do not hold the wave on a long
+// inner loop.  Every triangle is tested once (Walk), and the nearest hit is
+// kept wherever it lies, so the closest hit is that of walk_steps.
 // SCHED: the wave takes one kind of unit per iteration -- triangle tests when
 // at least half of its walking lanes have one pending, else voxel moves -- so
 // an iteration runs one of the two code paths instead of both; lanes of the
@@ -423,7 +468,7 @@ __device__ bool walk_units(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R
     if (w.k < 0) walk_enter<R, LDS>(s, w, n_vox);  // the first voxel of a new walk
     for (int it = 0; it < budget; ++it) {
         const R texit = fmin(w.tmx, fmin(w.tmy, w.tmz));
-        const bool pending = w.j < w.je;
+        const bool pending = walk_pending(w);
         if constexpr (SCHED) {
             const int n_pending = __popcll(__ballot(pending));
             const int n_walking = __popcll(__ballot(true));
@@ -431,7 +476,8 @@ __device__ bool walk_units(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R
             if (pending != tests_now) continue;  // this lane's kind of unit waits
         }
         if (pending) {
-            const R tlim = fmin(w.tbest, texit * R(1.0000001) + R(1e-6));
+            // any hit nearer than the best so far, inside this voxel or not
+            const R tlim = w.tbest;
             // the tests first: their row loads then need not wait for the index loads
             auto take = [&](bool hit, int t, R tt) __attribute__((always_inline)) {
                 if (hit && t != w.best && !(Tmin<R>::excl && t == excl) &&
@@ -441,14 +487,12 @@ __device__ bool walk_units(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R
                 }
             };
             {
-                // three list entries per unit (the last one repeated when
-                // fewer are left: a repeated triangle never replaces itself),
-                // merged in list order, as single units would (four spill
-                // scalar registers and run 7 % slower)
-                const uint32_t j0 = w.j, jl = w.je - 1;
-                const uint32_t j1 = min(j0 + 1, jl), j2 = min(j0 + 2, jl);
-                w.j = j2 + 1;
-                n_test += j2 - j0 + 1;
+                // three list entries per unit, merged in list order, as
+                // single units would (four spill scalar registers and run 7 %
+                // slower)
+                const Three e = next_three(w);
+                const uint32_t j0 = e.j0, j1 = e.j1, j2 = e.j2;
+                n_test += e.n;
                 const int t0 = (int)s.cell_tris[j0];
                 const int t1 = (int)s.cell_tris[j1];
                 const int t2 = (int)s.cell_tris[j2];
@@ -490,12 +534,11 @@ __device__ bool walk_units(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R
 // skipped in FP32: FP64 accepts it again only when the ray leaves at a grazing
 // angle, |n . d| < mx_self, and such bounces go to the FP64 walk directly.
 
-// Limits of the voxel just entered.  texit, the FP32 exit distance, is off by
+// Limit of the voxel just entered.  texit, the FP32 exit distance, is off by
 // at most kx / |d_a| + 8 eps texit on its exit axis a.  Every voxel list reaches
-// mx_infl past its voxel, so a hit up to mx_infl beyond the exit is tested in
-// this voxel: candidates are accepted up to texit + infl + err, and once the
-// best hit's upper bound is below texit + infl - err no untested triangle can
-// be nearer.
+// mx_infl past its voxel, so a hit up to mx_infl beyond the exit is tested by
+// the time this voxel is done: once the best hit's upper bound is below
+// texit + infl - err no untested triangle can be nearer.
 __device__ __forceinline__ void mix_limits(const TraceConst &tc, Walk<float> &w)
 {
     const float texit = fminf(w.tmx, fminf(w.tmy, w.tmz));
@@ -504,7 +547,6 @@ __device__ __forceinline__ void mix_limits(const TraceConst &tc, Walk<float> &w)
     const float ix = w.ix, iy = w.iy, iz = w.iz;
     const float ia = w.tmx <= w.tmy ? (w.tmx <= w.tmz ? float(ix) : float(iz)) : (w.tmy <= w.tmz ? float(iy) : float(iz));
     const float err = __builtin_fmaf(tc.mx_kx, fabsf(ia), 4.8e-7f * texit);
-    w.tlim = texit + tc.mx_infl + err;
     w.tend = texit + tc.mx_infl - err;
 }
 
@@ -528,7 +570,7 @@ __device__ __forceinline__ void mix_walk_init(const DevScene<float> &s, Walk<flo
     w.eb = 0.0f;
     w.sl = w.sl2 = INF;
     w.sid = w.zid = -1;
-    w.tlim = w.tend = 0.0f;
+    w.tend = 0.0f;
 }
 
 // Flattened walk (as walk_units, SCHED: one kind of unit per wave iteration)
@@ -543,7 +585,7 @@ __device__ bool walk_units_mix(const DevScene<float> &s, const TraceConst &tc, W
         mix_limits(tc, w);
     }
     for (int it = 0; it < budget; ++it) {
-        const bool pending = w.j < w.je;
+        const bool pending = walk_pending(w);
         if constexpr (SCHED) {
             const int n_pending = __popcll(__ballot(pending));
             const int n_walking = __popcll(__ballot(true));
@@ -551,15 +593,15 @@ __device__ bool walk_units_mix(const DevScene<float> &s, const TraceConst &tc, W
             if (pending != tests_now) continue;
         }
         if (pending) {
-            const uint32_t j0 = w.j, jl = w.je - 1;
-            const uint32_t j1 = min(j0 + 1, jl), j2 = min(j0 + 2, jl);
-            w.j = j2 + 1;
-            n_test += j2 - j0 + 1;
+            const Three e = next_three(w);
+            const uint32_t j0 = e.j0, j1 = e.j1, j2 = e.j2;
+            n_test += e.n;
             const int t0 = (int)s.cell_tris[j0];
             const int t1 = (int)s.cell_tris[j1];
             const int t2 = (int)s.cell_tris[j2];
-            // a candidate can matter only while its lower bound is below the best's upper bound
-            const float lim = fminf(w.tlim, w.tbest + w.eb);
+            // every candidate is recorded (as the best, or by its lower bound)
+            // wherever its hit lies: the walk tests each triangle once
+            const float lim = __builtin_huge_valf();
             float tt0, tt1, tt2, e0, e1, e2;
             bool p0, p1, p2;
             // every row load of the unit first, then the three screens
@@ -604,9 +646,9 @@ __device__ bool walk_units_mix(const DevScene<float> &s, const TraceConst &tc, W
 // decides the best candidate, the candidate with the least other lower bound
 // when the two intervals overlap (two triangles at a shared edge) or the best
 // fails, and the candidate near the departure point.  The nearest accepted one
-// is certain when every candidate not tested lies beyond it (the recorded
-// lower bounds; the walk skipped only candidates beyond the best's upper
-// bound tbest + eb) and it lies before the end of the walk.
+// is certain when every candidate not tested lies beyond it (the walk records
+// every candidate it screens: the best, or the lower bound of the others) and
+// it lies before the end of the walk.
 __device__ __forceinline__ bool mix_confirm(const DevScene<double> &s64, const Walk<float> &w, const double *o,
                                             const double *d, int &hit, double &t64)
 {

@@ -115,6 +115,41 @@ def build_grid(verts, dims=(0, 0, 0)):
 
 
 @pytest.mark.parametrize("room_fn,dims", [(rooms.shoebox, (0, 0, 0)), (rooms.fan_hall, (32, 32, 32)),
+                                          (rooms.fan_hall, (7, 5, 3))])
+def test_face_skip_masks_mark_entries_listed_across_each_face(room_fn, dims):
+    """Bit i of skip[voxel][f] is set exactly when entry i (< 32) of the
+    voxel's list is also in the list of its neighbour across face f (-x, +x,
+    -y, +y, -z, +z): the walk, stepping in through f, tested it there."""
+    room = room_fn()
+    G, org, cell, ranges, tris, dist = build_grid(room.verts, dims)
+    v = np.ascontiguousarray(room.verts, dtype=np.float64)
+    d = (C.c_int * 3)(*dims)
+    ncell = G[0] * G[1] * G[2]
+    skip = np.zeros((ncell, 6), np.uint32)
+    assert A.lib().pach_grid_face_skip(v.ctypes.data_as(C.POINTER(C.c_double)), v.shape[0], d,
+                                       skip.ctypes.data_as(C.POINTER(C.c_uint32))) == 0
+    assert tuple(d) == G
+    lists = [tris[a:a + n] for a, n in ranges]
+    n_set = 0
+    for z in range(G[2]):
+        for y in range(G[1]):
+            for x in range(G[0]):
+                c = (z * G[1] + y) * G[0] + x
+                for f in range(6):
+                    q = [x, y, z]
+                    q[f // 2] += 1 if f % 2 else -1
+                    want = 0
+                    if 0 <= q[f // 2] < G[f // 2]:
+                        nb = set(lists[(q[2] * G[1] + q[1]) * G[0] + q[0]].tolist())
+                        for i, t in enumerate(lists[c][:32]):
+                            if int(t) in nb:
+                                want |= 1 << i
+                    assert int(skip[c, f]) == want, (x, y, z, f)
+                    n_set += bin(want).count("1")
+    assert n_set > 0
+
+
+@pytest.mark.parametrize("room_fn,dims", [(rooms.shoebox, (0, 0, 0)), (rooms.fan_hall, (32, 32, 32)),
                                           (rooms.fan_hall, (0, 0, 0))])
 def test_grid_lists_cover_every_triangle_point(room_fn, dims):
     room = room_fn()
