@@ -173,6 +173,14 @@ int pach_grid_build(const double *verts, int T, int dims[3], double origin[3], d
  * the voxel's list is also listed in the neighbour across that face. */
 int pach_grid_face_skip(const double *verts, int T, int dims[3], uint32_t *skip);
 
+/* The walk's empty cubes over an occupancy field (inspection and tests; no
+ * counterpart in the reference): dist [ncell] (0 = occupied, as
+ * pach_grid_build's), oct [ncell][8]: for each empty voxel and ray octant o
+ * (bit 0: d_x < 0, bit 1: d_y < 0, bit 2: d_z < 0) the edge of the largest
+ * empty cube cornered at the voxel and extending along o (voxels outside the
+ * grid count as empty; at most 255), 0 for occupied voxels. */
+int pach_grid_octant_cubes(const int dims[3], const uint8_t *dist, uint8_t *oct);
+
 /* Source origin, per-band power in W (Source.SoundPower, Classes_Sources.cs:96-104)
  * and direction mode.  dirs [n_dirs][3] only for PACH_DIR_SUPPLIED. */
 int pach_source_set(pach_ctx *ctx, const double origin[3], const double power_w[8],

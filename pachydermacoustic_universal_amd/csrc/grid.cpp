@@ -121,6 +121,37 @@ void receiver_cells(const Grid &g, const double *centers, int R, const int *bit,
     linf_distance(g.dims, dist);
 }
 
+void octant_cubes(const int dims[3], const std::vector<uint8_t> &dist, std::vector<uint8_t> &oct)
+{
+    const int gx = dims[0], gy = dims[1], gz = dims[2];
+    const size_t ncell = (size_t)gx * gy * gz;
+    oct.assign(ncell * 8, 0);
+    for (int o = 0; o < 8; ++o) {
+        const int sx = (o & 1) ? -1 : 1, sy = (o & 2) ? -1 : 1, sz = (o & 4) ? -1 : 1;
+        // visit the cells farthest along the octant first: every cube a cell
+        // builds on (one cell further along one, two or three axes) is done
+        auto at = [&](int x, int y, int z) -> int {
+            if (x < 0 || x >= gx || y < 0 || y >= gy || z < 0 || z >= gz) return 255;  // outside: open
+            return oct[(((size_t)z * gy + y) * gx + x) * 8 + o];
+        };
+        for (int iz = 0; iz < gz; ++iz) {
+            const int z = sz > 0 ? gz - 1 - iz : iz;
+            for (int iy = 0; iy < gy; ++iy) {
+                const int y = sy > 0 ? gy - 1 - iy : iy;
+                for (int ix = 0; ix < gx; ++ix) {
+                    const int x = sx > 0 ? gx - 1 - ix : ix;
+                    const size_t c = ((size_t)z * gy + y) * gx + x;
+                    if (dist[c] == 0) continue;  // occupied: 0
+                    const int m = std::min({at(x + sx, y, z), at(x, y + sy, z), at(x, y, z + sz),
+                                            at(x + sx, y + sy, z), at(x + sx, y, z + sz), at(x, y + sy, z + sz),
+                                            at(x + sx, y + sy, z + sz)});
+                    oct[c * 8 + o] = (uint8_t)std::min(m + 1, 255);
+                }
+            }
+        }
+    }
+}
+
 void face_skip(const Grid &g, std::vector<uint32_t> &skip)
 {
     const int gx = g.dims[0], gy = g.dims[1], gz = g.dims[2];
@@ -250,5 +281,15 @@ extern "C" int pach_grid_face_skip(const double *verts, int T, int dims[3], uint
     std::vector<uint32_t> m;
     pach::face_skip(g, m);
     std::copy(m.begin(), m.end(), skip);
+    return 0;
+}
+
+extern "C" int pach_grid_octant_cubes(const int dims[3], const uint8_t *dist, uint8_t *oct)
+{
+    if (!dims || !dist || !oct || dims[0] <= 0 || dims[1] <= 0 || dims[2] <= 0) return -1;
+    const size_t ncell = (size_t)dims[0] * dims[1] * dims[2];
+    std::vector<uint8_t> d(dist, dist + ncell), o;
+    pach::octant_cubes(dims, d, o);
+    std::copy(o.begin(), o.end(), oct);
     return 0;
 }

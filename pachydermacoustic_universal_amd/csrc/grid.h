@@ -25,6 +25,13 @@ void linf_distance(const int dims[3], std::vector<uint8_t> &dist);
 // as occupied
 void receiver_cells(const Grid &g, const double *centers, int R, const int *bit, double radius,
                     std::vector<uint64_t> &mask, std::vector<uint8_t> &dist);
+// For each empty voxel (dist != 0) and each octant o of ray directions
+// (bit 0: d_x < 0, bit 1: d_y < 0, bit 2: d_z < 0), oct[8 voxel + o] = the
+// edge K of the largest cube of empty voxels that has this voxel as its corner
+// and extends K voxels along each axis in o's directions (voxels outside the
+// grid count as empty; capped at 255); 0 for occupied voxels.  A ray in
+// octant o inside the voxel crosses the whole cube in one move.
+void octant_cubes(const int dims[3], const std::vector<uint8_t> &dist, std::vector<uint8_t> &oct);
 // For each voxel and each of its faces f (0: -x, 1: +x, 2: -y, 3: +y, 4: -z,
 // 5: +z), skip[6 voxel + f] has bit i set when entry i (i < 32) of the voxel's
 // list is also listed in the neighbour across f: a walk that steps in through

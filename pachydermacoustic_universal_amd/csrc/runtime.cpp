@@ -72,6 +72,7 @@ struct pach_ctx {
     std::vector<float> tri_rows32;  // [T][12] FP32 rows r2, r0, r1 per triangle
     double row_norm = 0;            // largest |r0|, |r1| (barycentric gradient, 1 / shortest altitude)
     uint8_t *d_cell_dist = nullptr;
+    uint8_t *d_cell_oct = nullptr;  // [ncell][8] empty-cube edges per ray octant
     uint64_t *d_cell_rmask = nullptr;
     int ncell = 0;
     pach::Grid grid;        // host copy (receiver voxels are marked at trace time)
@@ -365,7 +366,7 @@ extern "C" void pach_destroy(pach_ctx *c)
     free_work(c->w64);
     dfree(c->q_o); dfree(c->q_d); dfree(c->q_t); dfree(c->q_tri); dfree(c->q_exact);
     dfree(c->d_surf); dfree(c->d_cell_range); dfree(c->d_cell_tris); dfree(c->d_cell_dist); dfree(c->d_cell_rows);
-    dfree(c->d_cell_skip);
+    dfree(c->d_cell_skip); dfree(c->d_cell_oct);
     dfree(c->d_cell_rmask);
     dfree(c->d_dirs); dfree(c->d_geo);
     dfree(c->d_ctr32); dfree(c->d_ctr64); dfree(c->d_hist);
@@ -424,8 +425,12 @@ int upload_grid(pach_ctx *c, const int dims[3])
     }
     std::vector<uint32_t> skip;
     pach::face_skip(grid, skip);
+    std::vector<uint8_t> oct;  // receiver voxels join the occupancy when receivers are set
+    pach::octant_cubes(grid.dims, grid.dist, oct);
     hipError_t e;
-    if ((e = dalloc(c->d_cell_skip, skip.size())) != hipSuccess ||
+    if ((e = dalloc(c->d_cell_oct, oct.size())) != hipSuccess ||
+        (e = hipMemcpy(c->d_cell_oct, oct.data(), oct.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = dalloc(c->d_cell_skip, skip.size())) != hipSuccess ||
         (e = hipMemcpy(c->d_cell_skip, skip.data(), skip.size() * 4, hipMemcpyHostToDevice)) != hipSuccess ||
         (e = dalloc(c->d_cell_rows, rows.size())) != hipSuccess ||
         (e = hipMemcpy(c->d_cell_rows, rows.data(), rows.size() * sizeof(float4), hipMemcpyHostToDevice)) != hipSuccess ||
@@ -763,6 +768,11 @@ int refresh_receiver_cells(pach_ctx *c)
         dist = c->grid.dist;
     }
     HIPCHK(c, hipMemcpy(c->d_cell_dist, dist.data(), c->ncell, hipMemcpyHostToDevice));
+    // the walk's moves across empty space, from the same occupancy
+    std::vector<uint8_t> oct;
+    pach::octant_cubes(c->g, dist, oct);
+    HIPCHK(c, dalloc(c->d_cell_oct, oct.size()));
+    HIPCHK(c, hipMemcpy(c->d_cell_oct, oct.data(), oct.size(), hipMemcpyHostToDevice));
     c->rmask_dirty = false;
     return PACH_OK;
 }
@@ -942,6 +952,7 @@ template <class R> DevScene<R> dev_scene(pach_ctx *c)
     s.cell_range = (const uint2 *)c->d_cell_range; s.cell_tris = c->d_cell_tris;
     s.cell_rows = c->d_cell_rows;
     s.cell_skip = c->d_cell_skip;
+    s.cell_oct = c->d_cell_oct;
     s.cell_dist = c->d_cell_dist;
     s.cell_rmask = nullptr;
     s.gx = c->g[0]; s.gy = c->g[1]; s.gz = c->g[2];

@@ -225,7 +225,9 @@ __device__ __forceinline__ uint8_t cell_dist(const uint8_t *g, int cell)
 // reference's rule that a hit ends the walk once it lies inside the current
 // voxel (Classes_VoxelGrid_PolyRefractive.cs:172).  Empty space: a voxel at
 // L-inf distance k >= 2 from every occupied voxel lies inside a (2k-1)^3 empty
-// block, crossed in one jump.
+// block, crossed in one jump (walk_steps); the flattened walks jump across the
+// largest empty cube cornered at the voxel on the ray's side (cell_oct), which
+// contains that block's part ahead of the ray and reaches further from walls.
 //
 // Each triangle is tested once per walk where the reference keeps a per-ray
 // mailbox (Classes_VoxelGrid_PolyRefractive.cs:162-164): the flattened walk
@@ -380,20 +382,25 @@ __device__ __forceinline__ void walk_enter(const DevScene<R> &s, Walk<R> &w, uin
         const uint32_t all = rg.y >= 32 ? ~0u : (1u << rg.y) - 1u;
         const uint32_t seen = w.from >= 0 ? s.cell_skip[6 * cell + w.from] : 0u;
         w.todo = all & ~seen;
+    } else {
+        // empty: the edge of the empty cube ahead of the ray, in its octant
+        const int o = (w.ix < R(0) ? 1 : 0) | (w.iy < R(0) ? 2 : 0) | (w.iz < R(0) ? 4 : 0);
+        w.k = s.cell_oct[8 * cell + o];
     }
 }
 
-// One move of the flattened walk: across the empty block around the voxel
-// (k >= 2), else one voxel (the block is the voxel itself), by the same
-// straight-line code for both so stepping and jumping lanes do not diverge.
-// Returns true when the walk leaves the grid.
+// One move of the flattened walk: across the empty cube ahead of the ray
+// (edge k >= 2, cornered at the voxel and extending along the ray's octant),
+// else one voxel (the cube is the voxel itself), by the same straight-line
+// code for both so stepping and jumping lanes do not diverge.  Returns true
+// when the walk leaves the grid.
 template <class R>
 __device__ __forceinline__ bool walk_move(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R dx, R dy, R dz)
 {
     const int h = max(w.k - 1, 0);
-    const int lx = max(w.cx - h, 0), hx = min(w.cx + h, s.gx - 1);
-    const int ly = max(w.cy - h, 0), hy = min(w.cy + h, s.gy - 1);
-    const int lz = max(w.cz - h, 0), hz = min(w.cz + h, s.gz - 1);
+    const int lx = dx < R(0) ? max(w.cx - h, 0) : w.cx, hx = dx < R(0) ? w.cx : min(w.cx + h, s.gx - 1);
+    const int ly = dy < R(0) ? max(w.cy - h, 0) : w.cy, hy = dy < R(0) ? w.cy : min(w.cy + h, s.gy - 1);
+    const int lz = dz < R(0) ? max(w.cz - h, 0) : w.cz, hz = dz < R(0) ? w.cz : min(w.cz + h, s.gz - 1);
     const R bx = axis_exit<R>(ox, dx, w.ix, dx > R(0) ? hx : lx, s.gmin[0], s.cs[0]);
     const R by = axis_exit<R>(oy, dy, w.iy, dy > R(0) ? hy : ly, s.gmin[1], s.cs[1]);
     const R bz = axis_exit<R>(oz, dz, w.iz, dz > R(0) ? hz : lz, s.gmin[2], s.cs[2]);
@@ -403,7 +410,7 @@ __device__ __forceinline__ bool walk_move(const DevScene<R> &s, Walk<R> &w, R ox
     const int fx = (int)floor((px - s.gmin[0]) * s.inv_cs[0]);
     const int fy = (int)floor((py - s.gmin[1]) * s.inv_cs[1]);
     const int fz = (int)floor((pz - s.gmin[2]) * s.inv_cs[2]);
-    // never move backwards; stay inside the block on the other axes
+    // never move backwards; stay inside the cube on the other axes
     // (clamp to [c, h] moving up, [l, c] moving down, c when still)
     int nx = min(max(fx, dx < R(0) ? lx : w.cx), dx > R(0) ? hx : w.cx);
     int ny = min(max(fy, dy < R(0) ? ly : w.cy), dy > R(0) ? hy : w.cy);

@@ -31,6 +31,7 @@ template <class R> struct DevScene {
     const float4 *cell_rows;       // FP32 rows (r2, r0, r1) of each list entry, in list order
     const uint32_t *cell_skip;     // [ncell][6] entries also listed across each face (grid.h face_skip)
     const uint8_t *cell_dist;      // [ncell] L-inf distance (cells) to the nearest occupied cell, 0 = occupied
+    const uint8_t *cell_oct;       // [ncell][8] edge of the empty cube cornered at the cell per ray octant (grid.h)
     const uint64_t *cell_rmask;    // [ncell] receivers touching the voxel (null: test every receiver)
     int gx, gy, gz;
     int ncell;

@@ -114,6 +114,44 @@ def build_grid(verts, dims=(0, 0, 0)):
     return G, org, cell, ranges, tris[:nl.value], dist
 
 
+@pytest.mark.parametrize("seed,dims,p", [(0, (9, 7, 5), 0.08), (1, (6, 11, 4), 0.2), (2, (5, 5, 5), 0.0)])
+def test_octant_cubes_are_the_largest_empty_cubes(seed, dims, p):
+    """oct[v][o] is the largest K whose cube (corner v, K voxels along each
+    axis in o's directions, voxels outside the grid empty) holds no occupied
+    voxel, checked by brute force; 0 on occupied voxels; 255 when the cube
+    never meets one."""
+    rng = np.random.default_rng(seed)
+    gx, gy, gz = dims
+    occ = rng.random((gz, gy, gx)) < p
+    dist = np.where(occ, 0, 1).astype(np.uint8).ravel()
+    oct_ = np.zeros(dist.size * 8, np.uint8)
+    d = (C.c_int * 3)(*dims)
+    assert A.lib().pach_grid_octant_cubes(d, dist.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                          oct_.ctypes.data_as(C.POINTER(C.c_uint8))) == 0
+    oct_ = oct_.reshape(gz, gy, gx, 8)
+
+    def empty(x, y, z, K, s):
+        xs = [x + s[0] * i for i in range(K)]
+        ys = [y + s[1] * i for i in range(K)]
+        zs = [z + s[2] * i for i in range(K)]
+        return not any(occ[c, b, a] for c in zs if 0 <= c < gz for b in ys if 0 <= b < gy
+                       for a in xs if 0 <= a < gx)
+    for z in range(gz):
+        for y in range(gy):
+            for x in range(gx):
+                for o in range(8):
+                    s = (-1 if o & 1 else 1, -1 if o & 2 else 1, -1 if o & 4 else 1)
+                    K = int(oct_[z, y, x, o])
+                    if occ[z, y, x]:
+                        assert K == 0
+                        continue
+                    assert K >= 1 and empty(x, y, z, K, s)
+                    if K < 255:
+                        assert not empty(x, y, z, K + 1, s), (x, y, z, o, K)
+                    else:
+                        assert empty(x, y, z, max(dims) + 1, s)
+
+
 @pytest.mark.parametrize("room_fn,dims", [(rooms.shoebox, (0, 0, 0)), (rooms.fan_hall, (32, 32, 32)),
                                           (rooms.fan_hall, (7, 5, 3))])
 def test_face_skip_masks_mark_entries_listed_across_each_face(room_fn, dims):
