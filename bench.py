@@ -7,11 +7,11 @@ bins, 32^3 voxel grid.  One step is one complete trace (SplitRayTracer.Begin
 -> Combine) of all rays; the inputs are resident in HBM before the timed
 region starts.
 
-Precision (--precision): "mixed" (default) walks the voxel grid with FP32
+Precision (--precision): "fp32" (default) walks the voxel grid and shades in
+FP32 with FP64 histograms, the throughput arithmetic; "mixed" walks with FP32
 triangle tests widened by their error bounds and decides every hit, and does
-all shading, in FP64: its histograms equal the FP64 tracer's bin for bin.
-"fp64" walks in FP64 as well; "fp32" is the all-FP32 walk (statistically
-equivalent, faster).
+all shading, in FP64, so its histograms equal the FP64 tracer's bin for bin;
+"fp64" walks in FP64 as well.
 
 Prints one JSON line: `value` is ray-bounces per second over the timed steps.
 `roofline` prices the octave kernel in algorithmic HBM bytes (128 B per octave
@@ -118,7 +118,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--rays", type=int, default=1_000_000)
     ap.add_argument("--cutoff", type=float, default=3.0)
-    ap.add_argument("--precision", choices=["fp32", "fp64", "mixed"], default="mixed")
+    ap.add_argument("--precision", choices=["fp32", "fp64", "mixed"], default="fp32")
     ap.add_argument("--grid", type=int, default=None,
                     help="voxels per axis (default: 32 for the hall, automatic for the batch)")
     ap.add_argument("--cpu-rays", type=int, default=5000, help="source rays of the CPU sample")
