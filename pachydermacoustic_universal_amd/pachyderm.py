@@ -295,6 +295,13 @@ class Receiver_Bank:
     def Directions_Neg(self, Octave, Rec_Index):
         return self.planes[5:8, Rec_Index, Octave].copy()
 
+    def Reflection_Analysis_Data(self, Rec_Index):
+        """Spherical_Receiver.Reflection_Analysis_Data of one receiver
+        (Classes_Receivers.cs:714, :857-875): [3][8][SampleCT], [0] specular
+        arrivals never scattered, [1] arrivals of scattered rays, [2] specular
+        arrivals of rays scattered before."""
+        return self.planes[8:11, Rec_Index].copy()
+
 
 # ---------------------------------------------------------------- mapping receivers
 class PachMapReceiver(Receiver_Bank):

@@ -107,3 +107,33 @@ def test_progress_counts_completed_source_rays_while_tracing():
     assert len(mid) >= 5, seen[:20]
     assert tr.poll()["rays_done"] == n and tr.stats()["rays_done"] == n
     tr.close()
+
+
+def test_receiver_bank_write_back_scales_every_plane():
+    """Combine_ThreadLocal_Results writes all 11 planes back into the bank and
+    scales them as Receiver_Bank.Scale does: Energy, Dir_Rec_Pos / Neg and
+    Reflection_Analysis_Data / N, Pressure / sqrt(N) (Classes_Receivers.cs:
+    1774-1786, :1088-1102), N the completed source rays; the accessors read
+    those planes, and every plane equals the oracle's."""
+    rt, scene = run(8, seed=21)
+    rb = rt.GetReceiver
+    n = rt.Stats["rays_done"]
+    assert n == 8
+    raw = rt._tr.planes(scaled=False)
+    for k in range(11):
+        want = raw[k] / (np.sqrt(n) if k == 1 else n)
+        np.testing.assert_array_equal(rb.planes[k], want)
+    ora, _ = O.trace(rooms.shoebox(), rays=8, cutoff_s=0.6, seed=21, threads=8,
+                     medium=O.Medium(c=scene.c, atten=scene.atten, rho=scene.rho))
+    ref = O.scale(ora, 8)
+    for k in range(11):
+        np.testing.assert_allclose(rb.planes[k], ref[k], rtol=1e-5, atol=1e-12 * np.abs(ref[k]).max())
+        if k in (0, 1, 2, 5, 8, 9):
+            assert np.abs(ref[k]).max() > 0, k
+    for r in range(rb.Count):
+        for o in range(8):
+            np.testing.assert_array_equal(rb.GetPressureHistogram(o, r), rb.planes[1, r, o])
+            np.testing.assert_array_equal(rb.Directions_Pos(o, r), rb.planes[2:5, r, o])
+            np.testing.assert_array_equal(rb.Directions_Neg(o, r), rb.planes[5:8, r, o])
+        np.testing.assert_array_equal(rb.Reflection_Analysis_Data(r), rb.planes[8:11, r])
+    rt.close()
