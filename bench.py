@@ -126,6 +126,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--batch-rays", type=int, default=0, help="source rays per phase-1 launch (0: automatic)")
+    ap.add_argument("--progress", type=float, default=0, metavar="SECONDS",
+                    help="trace asynchronously and print the completed source rays every SECONDS")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="pach_tune setting, e.g. dist_lds=0")
     ap.add_argument("--autotune", type=int, default=0, metavar="RAYS",
@@ -204,7 +206,19 @@ def main():
         print(f"[bench] autotune: {tuning}", file=sys.stderr, flush=True)
 
     def step(seed):
-        tr.trace(rays=trace_rays, cutoff_m=args.cutoff * c, seed=seed, precision=prec, batch_rays=args.batch_rays)
+        if not args.progress:
+            tr.trace(rays=trace_rays, cutoff_m=args.cutoff * c, seed=seed, precision=prec, batch_rays=args.batch_rays)
+            return tr.stats()
+        # a long trace: run it asynchronously and report completed source rays
+        tr.trace(rays=trace_rays, cutoff_m=args.cutoff * c, seed=seed, precision=prec, batch_rays=args.batch_rays,
+                 async_=True)
+        while tr.poll()["running"]:
+            time.sleep(args.progress)
+            p = tr.poll()
+            print(f"[bench] {p['rays_done']} of {p['rays_total']} source rays done", file=sys.stderr, flush=True)
+        rc = tr.wait()
+        if rc != 0:
+            raise A.PachError(rc, A.lib().pach_last_error(tr.h).decode())
         return tr.stats()
 
     for w in range(args.warmup):

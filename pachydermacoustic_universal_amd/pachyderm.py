@@ -237,8 +237,10 @@ class Receiver_Bank:
     def Scale(self, Ray_CT):
         """Receiver_Bank.Scale (Classes_Receivers.cs:676-679, :1774-1786)."""
         n = float(Ray_CT)
+        p = A.PLANES["pressure"]
+        pressure = self.planes[p] / np.sqrt(n)
         self.planes /= n
-        self.planes[A.PLANES["pressure"]] *= np.sqrt(n)
+        self.planes[p] = pressure
 
     def _filter_channels(self, rec):
         """Directional_Histogram.Create_Filter's envelopes (Classes_Receivers.cs:

@@ -118,3 +118,75 @@ def test_batch_fp32_statistics_match_fp64(tracer):
         rel = np.abs(p32[..., :2] - p64[..., :2]) / p64[..., :2]
         assert rel.max() < 0.02, (i, rel.max())
         assert np.abs(p32[..., 2] - p64[..., 2]).max() < 0.2, i
+
+
+def test_batch_256_rooms_at_shape(tracer):
+    """The batched workload's shape: all 256 rooms of batch_rooms(seed=3) in one
+    launch sequence, each at 1 % of its log-uniform [1e4, 1e6] ray count.
+    Every room's receivers record energy, the bank scales each room by its own
+    rays, almost no energy leaves the closed rooms, and emptying the last
+    room changes no other room's bins."""
+    members, base_rays = rooms.batch_rooms(seed=3, n=256)
+    pk = rooms.pack_rooms(members)
+    rays = np.maximum(1, np.round(np.array(base_rays) * 0.01)).astype(np.int64)
+    cut = 1.0
+    print(f"[case] batch 256 rooms, {rays.sum()} rays, cut={cut}", flush=True)
+    st = A.trace_batch(tracer, pk, rays, cutoff_s=cut, seed=8, precision=A.PREC_FP32)
+    assert st["rays_done"] == rays.sum()
+    raw = tracer.planes(scaled=False)
+    E = A.batch_histograms(tracer, pk, 0)
+    r0 = 0
+    for i in range(len(rays)):
+        k = int(pk.rec_count[i])
+        assert raw[0, r0:r0 + k].sum(axis=(1, 2)).min() > 0, f"room {i}: a receiver got nothing"
+        np.testing.assert_allclose(E[i], raw[0, r0:r0 + k] / rays[i], rtol=1e-14)
+        r0 += k
+    assert st["lost"] / st["e_init"] < 1e-4, st["lost"] / st["e_init"]
+    r2 = rays.copy()
+    r2[-1] = 0
+    A.trace_batch(tracer, pk, r2, cutoff_s=cut, seed=8, precision=A.PREC_FP32)
+    part = tracer.planes(scaled=False)
+    k_last = int(pk.rec_count[-1])
+    np.testing.assert_allclose(part[:, :-k_last], raw[:, :-k_last], rtol=1e-12, atol=0)
+    assert np.all(part[:, -k_last:] == 0)
+
+
+def test_batch_cancel_scales_each_room_by_its_completed_rays(tracer):
+    """A batch cancelled part-way: the rooms whose rays all completed are
+    scaled by their own ray counts, the room in progress by its completed
+    rays, and the rooms not reached hold nothing (Receiver_Bank.Scale per
+    room with the completed count, TaskAll:768-774)."""
+    import time
+
+    members, base_rays = rooms.batch_rooms(seed=3, n=64)
+    pk = rooms.pack_rooms(members)
+    rays = np.maximum(1, np.round(np.array(base_rays) * 0.01)).astype(np.int64)
+    print(f"[case] batch 64 rooms cancelled, {rays.sum()} rays", flush=True)
+    at, c, rho = A.uniform_medium()
+    room = pk.room
+    tracer.upload_scene(room.verts, room.surf, room.alpha, room.scat, room.trans, (0, 0, 0))
+    tracer.set_source(pk.sources[0], np.ones(8), A.DIR_GEODESIC)
+    tracer.set_receivers(room.receivers, room.radius, at, c, c * rho, 1000, 1000)
+    tracer.set_rooms(pk.sources, np.ones((len(rays), 8)), rays, pk.surf_count, pk.rec_count)
+    try:
+        tracer.trace(rays=0, cutoff_m=1.0 * c, seed=3, precision=A.PREC_FP32, batch_rays=512, async_=True)
+        time.sleep(0.3)
+        tracer.cancel()
+        assert tracer.wait() in (0, -5)
+    finally:
+        tracer.clear_rooms()
+    done = tracer.stats()["rays_done"]
+    if done in (0, rays.sum()):
+        pytest.skip(f"cancel landed at {done} rays")
+    base = np.concatenate([[0], np.cumsum(rays)])
+    completed = np.clip(done - base[:-1], 0, rays)
+    raw = tracer.planes(scaled=False)
+    E = A.batch_histograms(tracer, pk, 0)
+    r0 = 0
+    for i in range(len(rays)):
+        k = int(pk.rec_count[i])
+        if completed[i] == 0:
+            assert np.all(raw[:, r0:r0 + k] == 0), i
+        else:
+            np.testing.assert_allclose(E[i], raw[0, r0:r0 + k] / completed[i], rtol=1e-14)
+        r0 += k
