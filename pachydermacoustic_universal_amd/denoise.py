@@ -99,10 +99,16 @@ def denormalize(y, S):
     return E
 
 
-def descriptors(room: RM.Room, receivers=None):
+def ray_descriptor(rays):
+    """The trace's source-ray count as a network input: log10(rays) - 4.5."""
+    return np.float32(np.log10(float(rays)) - 4.5)
+
+
+def descriptors(room: RM.Room, receivers=None, rays=None):
     """Per-receiver descriptors [R, 12]: log10 volume / 4, log10 surface area
     / 4, area-weighted mean absorption per band (8), log10 source-receiver
-    distance / 2, area-weighted mean scattering."""
+    distance / 2, area-weighted mean scattering; with `rays`, a 13th column
+    holds the trace's ray count (ray_descriptor)."""
     v = room.verts
     area_t = 0.5 * np.linalg.norm(np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0]), axis=1)
     area_s = np.bincount(room.surf, weights=area_t, minlength=room.n_surfaces)
@@ -117,6 +123,8 @@ def descriptors(room: RM.Room, receivers=None):
     out[:, :10] = head
     out[:, 10] = np.log10(dist) / 2
     out[:, 11] = scat_mean
+    if rays is not None:
+        out = np.concatenate([out, np.full((out.shape[0], 1), ray_descriptor(rays))], axis=1)
     return out.astype(np.float32)
 
 
@@ -246,10 +254,20 @@ class Denoiser:
         dn.set_params(z["params"])
         return dn
 
-    def denoise(self, E, desc):
-        """Energy histograms [B, 8, bins] (scaled) -> denoised energy histograms."""
+    def denoise(self, E, desc, rays=None):
+        """Energy histograms [B, 8, bins] (scaled) -> denoised energy histograms.
+        desc: [B or 1, n_desc] descriptors; a network that also takes the ray
+        count (n_desc 13) accepts the 12 room descriptors plus `rays`."""
         x, S = normalize(E)
-        y = self.forward(x, np.broadcast_to(desc, (x.shape[0], self.cfg["n_desc"])))  # desc: [B or 1, n_desc]
+        d = np.asarray(desc, dtype=np.float32)
+        d = d.reshape(-1, d.shape[-1])
+        if d.shape[1] == self.cfg["n_desc"] - 1:
+            if rays is None:
+                raise ValueError("this network also takes the trace's source-ray count (rays=)")
+            d = np.concatenate([d, np.full((d.shape[0], 1), ray_descriptor(rays), dtype=np.float32)], axis=1)
+        if d.shape[1] != self.cfg["n_desc"]:
+            raise ValueError(f"descriptors have {d.shape[1]} columns, the network takes {self.cfg['n_desc']}")
+        y = self.forward(x, np.broadcast_to(d, (x.shape[0], self.cfg["n_desc"])))
         return denormalize(y, S)
 
 
@@ -278,28 +296,33 @@ def with_direct(E, dE, dbin):
     return out
 
 
-def make_pairs(tr, room_list, lo_rays, hi_rays, cutoff_s=3.0, lo_repeats=1, seed=0, pack=16, log=None, is_order=-1):
-    """Traces every room at lo_rays (lo_repeats independent seeds) and hi_rays.
+def make_pairs(tr, room_list, lo_rays, hi_rays, cutoff_s=3.0, lo_repeats=1, seed=0, pack=16, log=None, is_order=-1,
+               ray_desc=False, room_ids=None):
+    """Traces every room at lo_rays (lo_repeats independent seeds each; lo_rays
+    may also be a list of ray counts, one input per entry) and hi_rays.
     Returns dict(x [N, 8, bins] normalised low-ray input, t [N, 8, bins]
     normalised high-ray target (same S), S [N, 8, 1] the input's band
-    energies, desc [N, n_desc], room [N], rec [N], the direct sound dE [N, 8]
-    and its bin dbin [N] for the ETCurve)."""
-    xs, ts, ss, ds, ri, rr, des, dbs = [], [], [], [], [], [], [], []
+    energies, desc [N, n_desc] (with ray_desc, the input's ray count as a
+    13th column), lo [N] the input's ray count, room [N], rec [N], the direct
+    sound dE [N, 8] and its bin dbin [N] for the ETCurve)."""
+    los = [int(r) for r in np.atleast_1d(lo_rays) for _ in range(lo_repeats)]
+    ids = list(range(len(room_list))) if room_ids is None else list(room_ids)
+    xs, ts, ss, ds, ri, rr, des, dbs, lr = [], [], [], [], [], [], [], [], []
     for g0 in range(0, len(room_list), pack):
         grp = room_list[g0:g0 + pack]
         pk = RM.pack_rooms(grp)
         n = len(grp)
         t0 = time.perf_counter()
         Ehi = trace_energy(tr, pk, np.full(n, hi_rays), cutoff_s, seed + 7919 * (g0 + 1), is_order=is_order)
-        Elo = [trace_energy(tr, pk, np.full(n, lo_rays), cutoff_s, seed + 7919 * (g0 + 1) + 104729 * (k + 1),
+        Elo = [trace_energy(tr, pk, np.full(n, lo), cutoff_s, seed + 7919 * (g0 + 1) + 104729 * (k + 1),
                             is_order=is_order)
-               for k in range(lo_repeats)]
+               for k, lo in enumerate(los)]
         r0 = 0
         for i, room in enumerate(grp):
             k = int(pk.rec_count[i])
-            d = descriptors(room)
             dE, db = direct_part(room)
-            for E in Elo:
+            for lo, E in zip(los, Elo):
+                d = descriptors(room, rays=lo if ray_desc else None)
                 x, S = normalize(E[r0:r0 + k])
                 sch_hi = np.cumsum(Ehi[r0:r0 + k][..., ::-1], axis=-1)[..., ::-1]
                 t = (np.log10(np.maximum(sch_hi / S, FLOOR)) + 5.0) / 2.5
@@ -309,13 +332,20 @@ def make_pairs(tr, room_list, lo_rays, hi_rays, cutoff_s=3.0, lo_repeats=1, seed
                 ds.append(d)
                 des.append(dE)
                 dbs.append(db)
-                ri.append(np.full(k, g0 + i))
+                lr.append(np.full(k, lo))
+                ri.append(np.full(k, ids[g0 + i]))
                 rr.append(np.arange(k))
             r0 += k
         if log:
             log(f"pairs: rooms {g0}-{g0 + n - 1} traced in {time.perf_counter() - t0:.1f} s")
     return dict(x=np.concatenate(xs), t=np.concatenate(ts), S=np.concatenate(ss), desc=np.concatenate(ds),
-                room=np.concatenate(ri), rec=np.concatenate(rr), dE=np.concatenate(des), dbin=np.concatenate(dbs))
+                lo=np.concatenate(lr), room=np.concatenate(ri), rec=np.concatenate(rr), dE=np.concatenate(des),
+                dbin=np.concatenate(dbs))
+
+
+def merge_pairs(*sets):
+    """Concatenates make_pairs results."""
+    return {k: np.concatenate([s[k] for s in sets]) for k in sets[0]}
 
 
 def early_weights(bins, fs=1000, gain=20.0, tau_s=0.15):
@@ -357,15 +387,22 @@ def train(dn, data, steps, batch=256, lr=1e-3, warmup=100, seed=0, log=None, log
     return np.array(hist), vhist
 
 
-def decay_params(E, fs=1000):
-    """T30, EDT, C80 per (receiver, band) of energy histograms [R, 8, bins]."""
+def decay_params(E, fs=1000, start_bin=None):
+    """T30, EDT, C80 per (receiver, band) of energy histograms [R, 8, bins].
+    C80 is AcousticalMath.Clarity (Classes_Math.cs:559-585) with its
+    startTime at bin start_bin[r] (the direct sound's arrival, ETCurve's
+    ceil(d / c fs)), so the 80 ms early window opens with the direct sound as
+    ISO 3382-1 defines it; with startTime 0 on these absolute-time
+    histograms the window would end before the direct sound of every
+    receiver more than 27 m from the source."""
     out = np.full(E.shape[:2] + (3,), np.nan)
     for r in range(E.shape[0]):
+        t0 = 0.0 if start_bin is None else (float(start_bin[r]) + 1e-6) / fs
         for b in range(E.shape[1]):
             if E[r, b].sum() <= 0:
                 continue
             sch = A.schroeder_integral(E[r, b])
-            out[r, b] = (A.t_x(sch, 30, fs), A.early_decay_time(sch, fs), A.clarity(E[r, b], fs))
+            out[r, b] = (A.t_x(sch, 30, fs), A.early_decay_time(sch, fs), A.clarity(E[r, b], fs, start_time=t0))
     return out
 
 

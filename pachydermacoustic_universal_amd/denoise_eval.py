@@ -11,7 +11,15 @@ Reports per (receiver, band) T30 / EDT relative errors and C80 absolute
 errors (dB) against the reference, the share inside the just-noticeable
 differences (5 %, 5 %, 1 dB), and wall times: the plain reference trace, the
 low-ray trace, the denoiser (normalisation, network, inverse), so the
-denoised path's time can be set against the plain trace's.
+denoised path's time can be set against the plain trace's.  A network that
+takes the ray count (13 descriptors) denoises every ladder count; one that
+does not denoises --rays only.
+
+Time to JND accuracy: for each share q in --shares, the wall time of the
+cheapest ladder trace (plain, or plain + network) whose T30, EDT and C80 each
+have at least a share q of the (receiver, band) pairs inside 5 % / 5 % / 1 dB
+of the reference; its ratio to the reference trace's wall time is what the
+denoised path must bring under 1/20.
 """
 from __future__ import annotations
 
@@ -34,8 +42,10 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref-rays", type=int, default=10_000_000)
     ap.add_argument("--rays", type=int, default=20_000)
-    ap.add_argument("--ladder", default="100000,400000",
+    ap.add_argument("--ladder", default="50000,100000,200000,400000",
                     help="further plain ray counts, for the plain trace's error against its time")
+    ap.add_argument("--shares", default="0.9,0.95,1.0",
+                    help="shares of (receiver, band) pairs inside the JND that define JND accuracy")
     ap.add_argument("--cutoff", type=float, default=3.0)
     ap.add_argument("--is-order", type=int, default=0,
                     help="IS_Order of every trace; with 0 the direct sound comes from Direct_Sound (ETCurve)")
@@ -44,6 +54,7 @@ def main(argv=None):
     ap.add_argument("--autotune", type=int, default=-1,
                     help="sample source rays of pach_autotune (-1: sized by the tuner, 0: no tuning)")
     ap.add_argument("--report", default="")
+    ap.add_argument("--dump", default="", help="npz of the plain histograms of every ladder count")
     args = ap.parse_args(argv)
 
     room = RM.concert_hall()
@@ -93,26 +104,44 @@ def main(argv=None):
             np.savez_compressed(args.ref_cache, E=E_ref.astype(np.float32), rays=args.ref_rays,
                                 is_order=args.is_order, info=json.dumps(rep["ref"]))
     log(f"reference: {rep['ref']}")
-    P_ref = DN.decay_params(etc(E_ref))
+    # C80 from the direct sound's arrival (Clarity's startTime), the same bin for every trace
+    sb = dbin if args.is_order >= 0 else None
+    P_ref = DN.decay_params(etc(E_ref), start_bin=sb)
 
     dn = DN.Denoiser.load(args.weights, max_batch=64)
     desc = DN.descriptors(room)
+    takes_rays = dn.cfg["n_desc"] == desc.shape[1] + 1
+    rep["weights_meta"] = json.loads(str(np.load(args.weights, allow_pickle=False)["meta"]))
     rep["plain"], rep["denoised"] = {}, {}
-    for rays in [args.rays] + [int(r) for r in args.ladder.split(",") if r]:
+    dump = {}
+    for rays in sorted(set([args.rays] + [int(float(r)) for r in args.ladder.split(",") if r])):
         E, wall, st = plain(rays, seed=1000 + rays)
-        rep["plain"][rays] = {"wall_s": wall, "errors": DN.summarize(DN.errors(DN.decay_params(etc(E)), P_ref))}
+        if args.dump:
+            dump[f"E{rays}"] = E.astype(np.float32)
+        rep["plain"][rays] = {"wall_s": wall, "errors": DN.summarize(DN.errors(DN.decay_params(etc(E), start_bin=sb), P_ref))}
         log(f"plain {rays}: {rep['plain'][rays]}")
-        if rays == args.rays:
+        if rays == args.rays or takes_rays:
             t0 = time.perf_counter()
-            E_dn = dn.denoise(E, desc)
+            E_dn = dn.denoise(E, desc, rays=rays)
             t_dn = time.perf_counter() - t0
-            rep["denoised"] = {"rays": rays, "trace_wall_s": wall, "denoise_wall_s": t_dn,
-                               "total_wall_s": wall + t_dn,
-                               "time_fraction_of_ref": (wall + t_dn) / rep["ref"]["wall_s"],
-                               "errors": DN.summarize(DN.errors(DN.decay_params(etc(E_dn)), P_ref))}
-            log(f"denoised {rays}: {rep['denoised']}")
+            rep["denoised"][rays] = {"trace_wall_s": wall, "denoise_wall_s": t_dn, "wall_s": wall + t_dn,
+                                     "time_fraction_of_ref": (wall + t_dn) / rep["ref"]["wall_s"],
+                                     "errors": DN.summarize(DN.errors(DN.decay_params(etc(E_dn), start_bin=sb), P_ref))}
+            log(f"denoised {rays}: {rep['denoised'][rays]}")
     dn.close()
     tr.close()
+    if args.dump:
+        np.savez_compressed(args.dump, E_ref=E_ref.astype(np.float32), dE=dE, dbin=dbin, desc=desc, **dump)
+    rep["time_to_jnd"] = {}
+    for q in [float(v) for v in args.shares.split(",") if v]:
+        row = {}
+        for kind in ("plain", "denoised"):
+            ok = [(r, v["wall_s"]) for r, v in sorted(rep[kind].items())
+                  if min(v["errors"][k]["within"] for k in ("t30", "edt", "c80")) >= q]
+            row[kind] = ({"rays": ok[0][0], "wall_s": ok[0][1], "fraction_of_ref": ok[0][1] / rep["ref"]["wall_s"]}
+                         if ok else None)
+        rep["time_to_jnd"][q] = row
+        log(f"time to JND accuracy at share {q}: {row}")
     out = json.dumps(rep, indent=1, default=str)
     print(out, flush=True)
     if args.report:

@@ -264,6 +264,74 @@ def concert_hall(seed=5, n_receivers=64, radius=0.5, name="concert_hall") -> Roo
                 volume=_tri_volume(tris), meta={"seed": seed})
 
 
+def concert_variant(seed, n_receivers=64, radius=0.5) -> Room:
+    """A concert hall of the concert_hall kind with seeded proportions (the
+    denoiser's training rooms of that class; concert_hall itself is never
+    one of them): a W x L x H shell with W U(20, 34) m, L U(32, 52) m,
+    H U(11, 22) m in ~1 m quads (walls and ceiling jittered +-5 cm, the floor
+    flat), 4-7 columns of seat-block boxes (0.8 m deep, 0.40-0.55 m high,
+    open to the floor) at a row pitch U(0.9, 1.05) m over the audience area,
+    0-2 levels of subdivided balcony slabs along the side walls, per-polygon
+    absorption U(0.02, 0.40) and scattering U(0.05, 0.70) per band, the
+    source on the stage and receivers on a jittered 8 x 8 grid 1.25 m over the
+    audience floor (~8,000-20,000 triangles)."""
+    rng = np.random.default_rng(2_000_003 + seed)
+    W, Lh, H = rng.uniform(20, 34), rng.uniform(32, 52), rng.uniform(11, 22)
+
+    def shell_map(u, v, w):
+        return (-W / 2 + W * u, Lh * v, H * w)
+
+    quads = _deformed_box(shell_map, int(round(W)), int(round(Lh)), int(round(H)), 0.05, rng)
+    floor = np.abs(quads[..., 2]) < 0.08
+    quads[..., 2][floor] = 0.0
+    tris, surf = _quads_to_tris(quads)
+    tris, surf = [tris], [surf]
+    nsurf = int(surf[0].max()) + 1
+    # balconies: 0-2 levels, 0.2 m clear of the side walls
+    levels = [z for z in (5.0, 9.0)[:int(rng.integers(0, 3))] if z + 4.0 < H]
+    bal_depth = rng.uniform(2.0, 3.5)
+    for z in levels:
+        for side in (-1, 1):
+            xa, xb = side * (W / 2 - 0.2), side * (W / 2 - 0.2 - bal_depth)
+            lo = (min(xa, xb), 0.22 * Lh, z)
+            hi = (max(xa, xb), 0.9 * Lh, z + 0.8)
+
+            def slab_map(u, v, w, lo=lo, hi=hi):
+                return (lo[0] + (hi[0] - lo[0]) * u, lo[1] + (hi[1] - lo[1]) * v, lo[2] + (hi[2] - lo[2]) * w)
+
+            q = _deformed_box(slab_map, 3, int(round(0.68 * Lh)), 1, 0.0, rng)[:, ::-1]
+            t, sf = _quads_to_tris(q)
+            tris.append(t)
+            surf.append(sf + nsurf)
+            nsurf += q.shape[0]
+    # seat blocks in columns with 0.2 m aisles
+    ncol = int(rng.integers(4, 8))
+    x_lo, x_hi = -W / 2 + 1.5, W / 2 - 1.5
+    bw = (x_hi - x_lo - 0.2 * (ncol - 1)) / ncol
+    pitch = rng.uniform(0.9, 1.05)
+    bh = rng.uniform(0.40, 0.55)
+    y0, y1 = 0.27 * Lh, 0.87 * Lh
+    for col in range(ncol):
+        x0 = x_lo + col * (bw + 0.2)
+        for row in range(int((y1 - y0) // pitch)):
+            yb = y0 + row * pitch
+            b = _box_tris((x0, yb, 0.0), (x0 + bw, yb + 0.8, bh), bottom=False)
+            tris.append(b)
+            surf.append(nsurf + np.arange(b.shape[0], dtype=np.int32) // 2)
+            nsurf += b.shape[0] // 2
+    tris = np.concatenate(tris)
+    surf = np.concatenate(surf).astype(np.int32)
+    alpha = rng.uniform(0.02, 0.40, (nsurf, NBANDS))
+    scat = rng.uniform(0.05, 0.70, (nsurf, NBANDS))
+    gx, gy = np.meshgrid(np.linspace(-0.375 * W, 0.375 * W, 8), np.linspace(0.3 * Lh, 0.86 * Lh, 8))
+    recs = np.stack([gx.ravel() + rng.uniform(-0.4, 0.4, 64), gy.ravel() + rng.uniform(-0.4, 0.4, 64),
+                     np.full(64, 1.25)], axis=1)[:n_receivers]
+    src = np.array([rng.uniform(-3.0, 3.0), rng.uniform(2.0, 0.12 * Lh + 2.0), 1.5])
+    return Room(name=f"concert_variant{seed}", verts=tris, surf=surf, alpha=alpha, scat=scat,
+                trans=np.zeros((nsurf, NBANDS)), source=src, receivers=recs, radius=float(radius),
+                volume=_tri_volume(tris), meta={"dims": (W, Lh, H), "columns": ncol, "balconies": len(levels)})
+
+
 def small_workload() -> Room:
     """Plain method, small: the 12-triangle shoebox."""
     return shoebox()
