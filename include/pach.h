@@ -244,6 +244,24 @@ int pach_map_histogram(pach_ctx *ctx, int plane, double *out, int scaled);
  * once its push lands past that; default 2^25). */
 int pach_tune(pach_ctx *ctx, const char *key, int64_t value);
 
+/* Exact (order-independent) histogram accumulation for later traces (on = 1)
+ * or FP64 atomics (0, the default).  Replaces the reference's unsynchronised
+ * FP64 adds into the shared histograms (Directional_Histogram.Add,
+ * Classes_Receivers.cs:1159-1166, from every task of TaskAll:217-282), whose
+ * sums depend on the thread schedule.  With on = 1 each deposit is scaled by a
+ * power of two chosen from the source power and receiver radius (not the ray
+ * count, so a deposit is the same integer in every trace of a scene), rounded
+ * to an integer of up to 123 bits and added as 32-bit limbs into int64 sums,
+ * which are converted to FP64 when the trace ends (and after each convergence
+ * revolution): the histograms are then bit-for-bit the same whatever the launch
+ * shape, batch size or schedule, at 4x the histogram memory (4 x 11 x R x 8 x
+ * SampleCT int64) and up to 4 integer atomics per plane per deposit.  The unit
+ * is 2^-76 of the largest possible deposit u = 2 P / (pi r^2), and the range
+ * holds 16 deposits of u per bin from each of 2^40 source rays; the spherical
+ * receivers only (a trace with mapping receivers returns PACH_EINVAL).  A
+ * deposit beyond the range makes the trace return PACH_ERANGE. */
+int pach_exact_histograms(pach_ctx *ctx, int on);
+
 /* Polygon_Scene.shoot (Classes_Scenes.cs:757-819; Hare Voxel_Grid.Shoot, whose
  * contract is restated at Classes_VoxelGrid_PolyRefractive.cs:70-316) for n
  * rays on the uploaded scene: origins, dirs [n][3] (unit directions); out

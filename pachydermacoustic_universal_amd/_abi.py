@@ -125,6 +125,7 @@ SIGNATURES = [
     ("pach_map_receivers_clear", C.c_int, [C.c_void_p]),
     ("pach_map_histogram", C.c_int, [C.c_void_p, C.c_int, _dp, C.c_int]),
     ("pach_tune", C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
+    ("pach_exact_histograms", C.c_int, [C.c_void_p, C.c_int]),
     ("pach_direct_sound", C.c_int, [_dp, C.c_int, _ip, _dp, C.c_int, _dp, _dp, _dp, C.c_int, _dp, C.c_double,
                                     _dp, _dp, _ip]),
     ("pach_convergence_state_size", C.c_int64, [C.c_int, C.c_int, C.c_int]),
@@ -394,6 +395,10 @@ class Tracer:
 
     def tune(self, key, value):
         self._chk(lib().pach_tune(self.h, key.encode(), int(value)))
+
+    def exact_histograms(self, on=True):
+        """Order-independent (bit-reproducible) histogram accumulation for later traces."""
+        self._chk(lib().pach_exact_histograms(self.h, 1 if on else 0))
 
     def poll(self):
         pr = Progress()

@@ -101,6 +101,10 @@ struct pach_ctx {
     double4 *d_ctr64 = nullptr;
     double *d_hist = nullptr;
     size_t hist_n = 0;  // per plane
+    // exact accumulation (pach_exact_histograms): int64 limb sums [4][kPlanes][hist_n]
+    bool hist_exact = false;
+    long long *d_fx = nullptr;
+    size_t fx_cap = 0;  // values per limb allocated
 
     // mapping receivers (pach_map_receivers_set)
     bool map_on = false;
@@ -369,7 +373,7 @@ extern "C" void pach_destroy(pach_ctx *c)
     dfree(c->d_cell_skip); dfree(c->d_cell_oct);
     dfree(c->d_cell_rmask);
     dfree(c->d_dirs); dfree(c->d_geo);
-    dfree(c->d_ctr32); dfree(c->d_ctr64); dfree(c->d_hist);
+    dfree(c->d_ctr32); dfree(c->d_ctr64); dfree(c->d_hist); dfree(c->d_fx);
     dfree(c->f_env); dfree(c->f_car); dfree(c->f_part); dfree(c->f_out); dfree(c->f_tab);
     dfree(c->d_map_ctr); dfree(c->d_map_start); dfree(c->d_map_list); dfree(c->d_map_hist);
     dfree(c->d_cnt); dfree(c->d_scal); dfree(c->d_band); dfree(c->d_rooms); dfree(c->d_live);
@@ -1074,6 +1078,42 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
     rv.hist = c->d_hist;
     rv.plane_stride = c->hist_n;
     rv.has_map = 0;
+    double fx_inv_se = 0, fx_inv_sp = 0;
+    if (c->hist_exact) {
+        if (c->map_on) return fail(c, PACH_EINVAL, "exact histograms cover the spherical receivers, not a map");
+        const size_t need = (size_t)kPlanes * c->hist_n;
+        if (c->fx_cap < need) {
+            dfree(c->d_fx);
+            c->fx_cap = 0;
+            if (dalloc(c->d_fx, 4 * need) != hipSuccess) return fail(c, PACH_ENOMEM, "exact histogram allocation");
+            c->fx_cap = need;
+        }
+        // Fixed-point scales from the scene alone (never the ray count, so a
+        // deposit becomes the same integer in every trace of the scene): a
+        // deposit is at most u = P / (pi r^2) (intensity <= the source power P,
+        // half-chord <= r, SizeMod 1 / (pi r^3)), doubled for FP32 rounding,
+        // and a bin takes at most 16 deposits from each of up to 2^40 source
+        // rays.  2^s maps that bound, 2^44 u, to 2^120 (the flush needs the
+        // sums below 2^127), so one unit is 2^-76 u.
+        double pmax = 0;
+        for (const RoomTab &t : rooms)
+            for (int o = 0; o < kBands; ++o) pmax = std::max(pmax, t.power[o]);
+        const double ue = 2.0 * std::max(pmax, 1e-300) / (M_PI * c->radius * c->radius);
+        const double up = std::sqrt(ue * c->rho_c);
+        const int se = 120 - 44 - (int)std::ceil(std::log2(ue));
+        const int sp = 120 - 44 - (int)std::ceil(std::log2(up));
+        if (se < -1000 || se > 1000 || sp < -1000 || sp > 1000)
+            return fail(c, PACH_EINVAL, "source power outside the exact histograms' range");
+        rv.fx = c->d_fx;
+        rv.fx_se = std::ldexp(1.0, se);
+        rv.fx_sp = std::ldexp(1.0, sp);
+        fx_inv_se = std::ldexp(1.0, -se);
+        fx_inv_sp = std::ldexp(1.0, -sp);
+    }
+    // exact accumulation: the limb sums into the FP64 planes
+    auto fx_flush = [&]() -> hipError_t {
+        return rv.fx ? launch_fx_flush(c->stream, c->d_fx, c->d_hist, c->hist_n, fx_inv_se, fx_inv_sp) : hipSuccess;
+    };
     if (c->map_on) {
         if (rooms.size() != 1) return fail(c, PACH_EINVAL, "mapping receivers trace one room");
         // Direct_Time of each map receiver from the current source (Map_Receiver ctor)
@@ -1130,6 +1170,7 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
     hipStream_t st = c->stream;
     HIPCHK(c, hipMemcpyAsync(c->d_band, &bt, sizeof(bt), hipMemcpyHostToDevice, st));
     HIPCHK(c, hipMemsetAsync(c->d_hist, 0, c->hist_n * kPlanes * sizeof(double), st));
+    if (rv.fx) HIPCHK(c, hipMemsetAsync(c->d_fx, 0, 4 * c->hist_n * kPlanes * sizeof(long long), st));
     if (c->map_on) HIPCHK(c, hipMemsetAsync(c->d_map_hist, 0, c->map_hist_n * 8 * sizeof(double), st));
     HIPCHK(c, hipMemsetAsync(c->d_cnt, 0, sizeof(DevCounters), st));
     HIPCHK(c, hipMemsetAsync(c->d_scal, 0, 8 * sizeof(unsigned), st));
@@ -1253,6 +1294,7 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
             ++revolutions;
             if (c->cancel.load()) break;
             bool conv = true;
+            HIPCHK(c, fx_flush());
             for (int k = 0; k < 2; ++k) {
                 if (recs[k] < 0) continue;
                 const size_t off = ((size_t)recs[k] * kBands + hb) * c->sct;
@@ -1277,6 +1319,7 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
         c->room_tab = rooms;
     }
     c->rays_done = started;
+    HIPCHK(c, fx_flush());
     DevCounters hc{};
     HIPCHK(c, hipMemcpy(&hc, c->d_cnt, sizeof(hc), hipMemcpyDeviceToHost));
     pach_stats &S = c->stats;
@@ -1320,6 +1363,7 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
     S.converged = converged ? 1 : 0;
     c->oct_rays = S.octave_rays;
     if (hc.v[C_DROP]) return fail(c, PACH_ENOMEM, "octave-ray queue overflow: rays dropped");
+    if (hc.v[C_FXOVF]) return fail(c, PACH_ERANGE, "deposit beyond the exact histograms' range");
     if (conv_mode) return c->cancel.load() && !converged ? PACH_ECANCEL : PACH_OK;
     if (c->cancel.load() && started < N) return PACH_ECANCEL;
     return PACH_OK;
@@ -1458,6 +1502,14 @@ extern "C" int pach_map_histogram(pach_ctx *c, int plane, double *out, int scale
             for (size_t i = 0; i < c->map_hist_n; ++i) out[i] *= f;
         }
     }
+    return PACH_OK;
+}
+
+extern "C" int pach_exact_histograms(pach_ctx *c, int on)
+{
+    if (!c || (on != 0 && on != 1)) return PACH_EINVAL;
+    if (c->running) return fail(c, PACH_ESTATE, "trace running");
+    c->hist_exact = on != 0;
     return PACH_OK;
 }
 

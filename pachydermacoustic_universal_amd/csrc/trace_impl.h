@@ -736,6 +736,50 @@ __device__ __forceinline__ void counters_end(const LaneCounters &lc, DevCounters
 }
 
 // p: the pressure sqrt(e rho_c), computed by the caller in the ray's precision
+// Exact accumulation of v x scale (a power of two, so the product is exact):
+// the rounded magnitude is split into four 32-bit limbs (each split is exact:
+// the remainder keeps only low mantissa bits of the value), and each nonzero
+// limb, negated for a negative value, is added to its int64 sum.  Sums of up
+// to 2^30 deposits per bin cannot overflow.  Returns false (nothing added)
+// for a value at or beyond 2^123 after scaling.
+__device__ __forceinline__ bool fx_add(long long *fx, uint64_t limb_stride, uint64_t i, double v, double scale)
+{
+    double t = fabs(v * scale);
+    if (!(t < 0x1p123)) return false;
+    const double h3 = floor(t * 0x1p-96);
+    t -= h3 * 0x1p96;
+    const double h2 = floor(t * 0x1p-64);
+    t -= h2 * 0x1p64;
+    const double h1 = floor(t * 0x1p-32);
+    t -= h1 * 0x1p32;
+    const double h0 = rint(t);
+    const double h[4] = {h0, h1, h2, h3};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (h[k] != 0.0) {
+            const long long q = (long long)h[k];
+            atomicAdd((unsigned long long *)(fx + k * limb_stride + i),
+                      (unsigned long long)(v < 0 ? -q : q));
+        }
+    }
+    return true;
+}
+
+// The planes of one deposit through exact accumulation (see DevRecv::fx).
+__device__ __forceinline__ bool hist_add_fx(long long *fx, uint64_t stride, uint64_t idx, double e, double p,
+                                         double ndx, double ndy, double ndz, int ana, double se, double sp)
+{
+    const uint64_t ls = kPlanes * stride;
+    const double vx = (double)(float)(ndx * e), vy = (double)(float)(ndy * e), vz = (double)(float)(ndz * e);
+    bool ok = fx_add(fx, ls, idx, e, se);
+    ok &= fx_add(fx, ls, stride + idx, p, sp);
+    ok &= fx_add(fx, ls, (ndx > 0 ? 2 : 5) * stride + idx, vx, se);
+    ok &= fx_add(fx, ls, (ndy > 0 ? 3 : 6) * stride + idx, vy, se);
+    ok &= fx_add(fx, ls, (ndz > 0 ? 4 : 7) * stride + idx, vz, se);
+    ok &= fx_add(fx, ls, (8 + ana) * stride + idx, e, se);
+    return ok;
+}
+
 __device__ __forceinline__ void hist_add(const double *base_unused, double *hist, uint64_t stride,
                                          uint64_t idx, double e, double p, double ndx,
                                          double ndy, double ndz, int ana)
@@ -811,13 +855,25 @@ __device__ __forceinline__ void receiver_test(const DevRecv<R> &rv, int r, R ox,
     if constexpr (!BROAD) {
         const uint64_t idx = ((uint64_t)r * kBands + band) * rv.sct + sample;
         const double e = (double)I * band_gain<R>(rv, band, dist) * rv.sizemod * ts;
-        hist_add(nullptr, rv.hist, rv.plane_stride, idx, e, pressure<R>(e, rv.rho_c), ndx, ndy, ndz, ana);
+        if (rv.fx) {
+            if (!hist_add_fx(rv.fx, rv.plane_stride, idx, e, pressure<R>(e, rv.rho_c), ndx, ndy, ndz, ana,
+                             rv.fx_se, rv.fx_sp))
+                lc.add(C_FXOVF);
+        } else {
+            hist_add(nullptr, rv.hist, rv.plane_stride, idx, e, pressure<R>(e, rv.rho_c), ndx, ndy, ndz, ana);
+        }
         lc.add(C_DEP);
     } else {
         for (int o = lo; o <= hi; ++o) {
             const uint64_t idx = ((uint64_t)r * kBands + o) * rv.sct + sample;
             const double e = (double)E[o] * band_gain<R>(rv, o, dist) * rv.sizemod * ts;
-            hist_add(nullptr, rv.hist, rv.plane_stride, idx, e, pressure<R>(e, rv.rho_c), ndx, ndy, ndz, 0);
+            if (rv.fx) {
+                if (!hist_add_fx(rv.fx, rv.plane_stride, idx, e, pressure<R>(e, rv.rho_c), ndx, ndy, ndz, 0,
+                                 rv.fx_se, rv.fx_sp))
+                    lc.add(C_FXOVF);
+            } else {
+                hist_add(nullptr, rv.hist, rv.plane_stride, idx, e, pressure<R>(e, rv.rho_c), ndx, ndy, ndz, 0);
+            }
             lc.add(C_DEP);
         }
     }

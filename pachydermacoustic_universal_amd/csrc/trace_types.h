@@ -92,6 +92,13 @@ template <class R> struct DevRecv {
     int fs, sct;
     double *hist;         // [kPlanes][n][8][sct]
     uint64_t plane_stride;
+    // Exact accumulation (nullptr: FP64 atomics into hist): every deposit is
+    // scaled by 2^s (fx_se for the energy-class planes, fx_sp for pressure),
+    // rounded to an integer of up to 124 bits and added as four signed 32-bit
+    // limbs into int64 sums [4][kPlanes][n][8][sct]; k_fx_flush turns the
+    // sums into hist.  Integer sums do not depend on the order of the adds.
+    long long *fx;
+    double fx_se, fx_sp;
     int has_map;          // 1: the receivers are the map (RecMain = PachMapReceiver)
     DevMap map;
 };
@@ -178,6 +185,7 @@ enum : int {
     C_SHADED,    // lanes shaded (per lane)
     C_EXACT,     // mixed walk: bounces the FP32 screen could not decide (redone by the FP64 walk)
     C_EXACT0,    // mixed walk: grazing departures (walked in FP64 from the start)
+    C_FXOVF,     // exact accumulation: deposits beyond the fixed-point range (never expected)
     C_N
 };
 
@@ -186,6 +194,12 @@ struct DevCounters {
     double lost;
     double pad;
 };
+
+// Exact accumulation (DevRecv::fx): writes hist [kPlanes][stride] from the
+// limb sums fx [4][kPlanes][stride], scaled by inv_sp on plane 1 (pressure)
+// and inv_se on the others (trace_f64.hip).
+hipError_t launch_fx_flush(hipStream_t st, const long long *fx, double *hist, uint64_t stride, double inv_se,
+                           double inv_sp);
 
 // launchers (trace_f32.hip / trace_f64.hip)
 template <class R>
