@@ -137,3 +137,28 @@ def test_receiver_bank_write_back_scales_every_plane():
             np.testing.assert_array_equal(rb.Directions_Neg(o, r), rb.planes[5:8, r, o])
         np.testing.assert_array_equal(rb.Reflection_Analysis_Data(r), rb.planes[8:11, r])
     rt.close()
+
+
+def test_pac1_file_of_a_gpu_trace(tmp_path):
+    """Write_Pac1 (Classes_IO.cs:57-261) of a GPU trace: the direct sound and
+    the ray-traced receiver bank go to a .pac1 file whose ray-traced section
+    reads back as the bank's scaled planes, and whose energy plane equals the
+    oracle's trace of the same rays (bins within 1e-5 relative)."""
+    from pachydermacoustic_universal_amd import Direct_Sound, pac1
+
+    rt, scene = run(8, seed=21)
+    rb = rt.GetReceiver
+    ds = Direct_Sound(rt.Source, rb, scene, [0, 7])
+    ds.Begin()
+    f = tmp_path / "gpu_trace.pac1"
+    pac1.write_pac1(f, ds, rb, date="12/04/2026 10:00:00 AM")
+    d = pac1.read_pac1(f)
+    sec = d["ray_traced"][0]
+    assert sec["sample_ct"] == rb.SampleCT
+    np.testing.assert_array_equal(sec["planes"], rb.planes[:8])
+    ora, _ = O.trace(rooms.shoebox(), rays=8, cutoff_s=0.6, seed=21, threads=8,
+                     medium=O.Medium(c=scene.c, atten=scene.atten, rho=scene.rho))
+    ref = O.scale(ora, 8)
+    np.testing.assert_allclose(sec["planes"][0], ref[0], rtol=1e-5, atol=1e-12 * ref[0].max())
+    assert len(d["direct"]) == 1 and d["direct"][0]["receivers"][0]["time"] == ds.Time(0)
+    rt.close()
