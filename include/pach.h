@@ -213,10 +213,14 @@ int pach_rooms_set(pach_ctx *ctx, int n, const double *src, const double *power,
  * Each receiver records at 1000 Hz, sample_ct samples, times relative to its
  * direct arrival, deposits I 10^(-0.1 a t) / (pi r^2) at the chord midpoint;
  * directional: Directional_Histogram (pressure sqrt(E rho c), direction
- * planes) else Histogram (pressure sqrt(E)).  pach_receivers_set still
- * supplies the medium. */
+ * planes) else Histogram (pressure sqrt(E)); both drop deposits outside
+ * [0, sample_ct).  time1pt = 1 (the ctor's Time1Pt, :65-93, which sets
+ * SampleCT to 1; sample_ct must be 1): Histogram_1Pt (Classes_Receivers.cs:
+ * 1369-1402), every deposit's energy added to sample 0 of plane 0 whatever
+ * its time, the other planes left zero.  pach_receivers_set still supplies
+ * the medium. */
 int pach_map_receivers_set(pach_ctx *ctx, const double *centers, int M, double increment, int directional,
-                           int sample_ct);
+                           int sample_ct, int time1pt);
 int pach_map_receivers_clear(pach_ctx *ctx);
 /* plane 0-7 (energy, pressure, +x, +y, +z, -x, -y, -z): out [M][8][sample_ct],
  * scaled by the source rays traced (pressure by their square root). */

@@ -549,7 +549,7 @@ struct or_map {
     double *ctr;      /* [M][3] */
     double *direct;   /* [M] Direct_Time */
     double r, r2, sizemod, c, rho_c, atten[8];
-    int directional, sample_ct;
+    int directional, sample_ct, time1pt;
     double bmin[3], bmax[3], vd[3];
     int ct[3];
     int *start;       /* [ncell + 1], cell = (x * ctY + y) * ctZ + z */
@@ -663,6 +663,11 @@ int or_map_voxel(const or_map *m, int x, int y, int z, int *out, int cap)
 static void map_add(const or_map *m, double **P, int i, int band, double time, double e, const double *d,
                     int64_t *dep)
 {
+    if (m->time1pt) {
+        if (P[0]) P[0][(size_t)i * 8 + band] += e;
+        if (dep) (*dep)++;
+        return;
+    }
     int sample = (int)(time * 1000);
     if (sample >= m->sample_ct || sample < 0) return;
     size_t idx = ((size_t)i * 8 + band) * m->sample_ct + sample;
@@ -793,6 +798,11 @@ static void map_walk(const or_map *m, double **P, const double *o, const double 
         if (!more) break;
     }
     free(seen);
+}
+
+void or_map_set_time1pt(or_map *m, int on)
+{
+    m->time1pt = on && m->sample_ct == 1;
 }
 
 void or_map_check_octave(const or_map *m, double **planes, const double o[3], const double d[3],

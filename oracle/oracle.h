@@ -102,6 +102,10 @@ typedef struct or_map or_map;
 or_map *or_map_create(const double *centers, int M, double increment, const double src[3], double c,
                       double rho_c, const double atten[8], int directional, int sample_ct);
 void or_map_destroy(or_map *m);
+/* Time1Pt (Pach_Map_Receiver.cs:65-93; needs sample_ct 1): Histogram_1Pt.Add
+ * (Classes_Receivers.cs:1393-1396) adds every deposit's energy to sample 0,
+ * whatever its time, and nothing to the other planes. */
+void or_map_set_time1pt(or_map *m, int on);
 int or_map_grid(const or_map *m, int ct[3], double box_min[3], double voxel[3]);
 /* receivers listed in voxel (x, y, z): count, written to out up to cap */
 int or_map_voxel(const or_map *m, int x, int y, int z, int *out, int cap);

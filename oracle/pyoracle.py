@@ -119,6 +119,7 @@ def lib():
         L.or_map_create.restype = C.c_void_p
         L.or_map_create.argtypes = [_dp, C.c_int, C.c_double, _dp, C.c_double, C.c_double, _dp, C.c_int, C.c_int]
         L.or_map_destroy.argtypes = [C.c_void_p]
+        L.or_map_set_time1pt.argtypes = [C.c_void_p, C.c_int]
         L.or_map_grid.restype = C.c_int
         L.or_map_grid.argtypes = [C.c_void_p, _ip, _dp, _dp]
         L.or_map_voxel.restype = C.c_int
@@ -341,15 +342,17 @@ def geodesic_faces():
 class Map:
     """Mapping receivers (or_map): spheres of diameter `increment` at `centers`."""
 
-    def __init__(self, centers, increment, src, medium, directional=True, sample_ct=1000):
+    def __init__(self, centers, increment, src, medium, directional=True, sample_ct=1000, time1pt=False):
         self.centers = np.ascontiguousarray(centers, dtype=np.float64).reshape(-1, 3)
         s = np.ascontiguousarray(src, dtype=np.float64)
         at = np.ascontiguousarray(medium.atten, dtype=np.float64)
-        self.M, self.sample_ct = self.centers.shape[0], int(sample_ct)
+        self.M, self.sample_ct = self.centers.shape[0], 1 if time1pt else int(sample_ct)
         self.h = lib().or_map_create(_p(self.centers), self.M, float(increment), _p(s), float(medium.c),
                                      float(medium.rho_c), _p(at), int(directional), self.sample_ct)
         if not self.h:
             raise ValueError("oracle map creation failed")
+        if time1pt:
+            lib().or_map_set_time1pt(self.h, 1)
 
     def __del__(self):
         if getattr(self, "h", None):

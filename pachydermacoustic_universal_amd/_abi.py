@@ -121,7 +121,7 @@ SIGNATURES = [
     ("pach_receivers_set", C.c_int, [C.c_void_p, _dp, C.c_int, C.c_double, _dp, C.c_double,
                                      C.c_double, C.c_int, C.c_int]),
     ("pach_rooms_set", C.c_int, [C.c_void_p, C.c_int, _dp, _dp, C.POINTER(C.c_int64), _ip, _ip]),
-    ("pach_map_receivers_set", C.c_int, [C.c_void_p, _dp, C.c_int, C.c_double, C.c_int, C.c_int]),
+    ("pach_map_receivers_set", C.c_int, [C.c_void_p, _dp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int]),
     ("pach_map_receivers_clear", C.c_int, [C.c_void_p]),
     ("pach_map_histogram", C.c_int, [C.c_void_p, C.c_int, _dp, C.c_int]),
     ("pach_tune", C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
@@ -468,11 +468,14 @@ class Tracer:
         self._chk(lib().pach_image_sources_stats(self.h, C.byref(st)))
         return st.as_dict()
 
-    def set_map(self, centers, increment, directional=True, sample_ct=1000):
-        """Mapping receivers (pach_map_receivers_set) for the next traces."""
+    def set_map(self, centers, increment, directional=True, sample_ct=1000, time1pt=False):
+        """Mapping receivers (pach_map_receivers_set) for the next traces;
+        time1pt: single-sample histograms (sample_ct 1)."""
         ctr = np.ascontiguousarray(centers, dtype=np.float64).reshape(-1, 3)
+        if time1pt:
+            sample_ct = 1
         self._chk(lib().pach_map_receivers_set(self.h, _p(ctr), ctr.shape[0], float(increment), int(directional),
-                                               int(sample_ct)))
+                                               int(sample_ct), int(bool(time1pt))))
         self.map_n, self.map_sct = ctr.shape[0], int(sample_ct)
 
     def clear_map(self):

@@ -108,7 +108,7 @@ struct pach_ctx {
 
     // mapping receivers (pach_map_receivers_set)
     bool map_on = false;
-    int map_n = 0, map_sct = 0, map_directional = 1;
+    int map_n = 0, map_sct = 0, map_directional = 1, map_time1pt = 0;
     double map_r = 0;
     std::vector<double> map_ctr;  // [M][3]
     double map_bmin[3] = {0, 0, 0}, map_bmax[3] = {0, 0, 0}, map_vd[3] = {1, 1, 1};
@@ -1141,6 +1141,7 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
         m.rho_c = c->rho_c;
         m.directional = c->map_directional;
         m.sct = c->map_sct;
+        m.time1pt = c->map_time1pt;
         m.hist = c->d_map_hist;
         m.plane_stride = c->map_hist_n;
         rv.has_map = 1;
@@ -1403,9 +1404,10 @@ extern "C" int pach_rooms_set(pach_ctx *c, int n, const double *src, const doubl
 // diameter `increment`, their bounding box, voxels of 4 x 4 x 2.5 increments,
 // each listing (ascending) the receivers whose sphere reaches into it.
 extern "C" int pach_map_receivers_set(pach_ctx *c, const double *centers, int M, double increment, int directional,
-                                      int sample_ct)
+                                      int sample_ct, int time1pt)
 {
     if (!c || !centers || M <= 0 || !(increment > 0) || sample_ct <= 0) return PACH_EINVAL;
+    if (time1pt != 0 && (time1pt != 1 || sample_ct != 1)) return PACH_EINVAL;  // SampleCT = Time1Pt ? 1 : ...
     if (c->running) return fail(c, PACH_ESTATE, "trace running");
     for (int i = 0; i < 3 * M; ++i)
         if (!std::isfinite(centers[i])) return fail(c, PACH_EINVAL, "map: non-finite centre");
@@ -1467,6 +1469,7 @@ extern "C" int pach_map_receivers_set(pach_ctx *c, const double *centers, int M,
     c->map_r = r;
     c->map_sct = sample_ct;
     c->map_directional = directional ? 1 : 0;
+    c->map_time1pt = time1pt;
     c->map_hist_n = per_plane;
     for (int k = 0; k < 3; ++k) {
         c->map_bmin[k] = bmin[k];

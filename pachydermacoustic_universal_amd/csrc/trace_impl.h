@@ -1073,8 +1073,10 @@ __device__ void map_check(const DevRecv<R> &rv, const R *oR, const R *dR, R ex, 
             }
             if (earlier) continue;
             const double raydist = t / m.c + t_sum - C.w;
-            const int sample = (int)(raydist * 1000);
-            if (sample >= m.sct || sample < 0) continue;
+            // Histogram.Add drops samples outside [0, SampleCT); Histogram_1Pt.Add
+            // (Classes_Receivers.cs:1393-1396) adds every deposit to sample 0
+            const int sample = m.time1pt ? 0 : (int)(raydist * 1000);
+            if (!m.time1pt && (sample >= m.sct || sample < 0)) continue;
             const int b_lo = broadband ? 0 : band, b_hi = broadband ? kBands - 1 : band;
             for (int oc = b_lo; oc <= b_hi; ++oc) {
                 double Ei;
@@ -1083,7 +1085,9 @@ __device__ void map_check(const DevRecv<R> &rv, const R *oR, const R *dR, R ex, 
                 const double e = Ei * pow(10.0, -.1 * rv.band->atten[oc] * raydist) * m.sizemod;
                 const uint64_t idx = ((uint64_t)j * kBands + oc) * m.sct + sample;
                 unsafeAtomicAdd(m.hist + idx, e);
-                if (m.directional) {
+                if (m.time1pt) {
+                    // the energy plane only
+                } else if (m.directional) {
                     unsafeAtomicAdd(m.hist + m.plane_stride + idx, sqrt(e * m.rho_c));
                     for (int a = 0; a < 3; ++a) {
                         const double dk = -d[a];

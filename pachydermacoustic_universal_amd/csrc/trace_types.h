@@ -76,6 +76,7 @@ struct DevMap {
     int n;
     double r2, sizemod, c, rho_c;
     int directional, sct;
+    int time1pt;             // Histogram_1Pt: every deposit to bin 0 of the energy plane (sct 1)
     double *hist;            // 8 planes (E, P, +x, +y, +z, -x, -y, -z) of [n][8][sct], 1000 Hz
     uint64_t plane_stride;
 };

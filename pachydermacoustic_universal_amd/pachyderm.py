@@ -316,13 +316,12 @@ class PachMapReceiver(Receiver_Bank):
     (Partition :99-181); spheres of diameter Increment, recording at 1000 Hz
     (the rate the reference fixes), times relative to each receiver's direct
     arrival.  A SplitRayTracer with this bank as RecMain traces into the map
-    (pach_map_receivers_set).  Time1Pt (single-sample histograms) is not
-    provided."""
+    (pach_map_receivers_set).  With Time1Pt each receiver keeps one sample
+    per band (SampleCT 1, :93): Histogram_1Pt adds every deposit's energy
+    there whatever its time (Classes_Receivers.cs:1393-1396)."""
 
     def __init__(self, Map_Mesh, Src_Pt, SampleRate_in, Increment_in, Sc, Cutoff_time, Time1Pt=False,
                  Z_displacementIn=False, DirectionalIn=True, RecOnVertex=False, Offset_Mesh=False):
-        if Time1Pt:
-            raise NotImplementedError("single-sample map histograms (Time1Pt) are not provided")
         V = np.asarray(Map_Mesh[0], dtype=np.float64).reshape(-1, 3)
         polys = [list(p) for p in Map_Mesh[1]]
         self.increment = float(Increment_in)
@@ -330,10 +329,11 @@ class PachMapReceiver(Receiver_Bank):
         self.Z_Displacement = bool(Z_displacementIn)
         self.Rec_Vertex = bool(RecOnVertex)
         self.Mesh_Offset = bool(Offset_Mesh)
+        self.Time1Pt = bool(Time1Pt)
         self.Src = np.asarray(Src_Pt.Origin, dtype=np.float64)
         self.CutOffTime = float(Cutoff_time)
         self.SampleRate = 1000
-        self.SampleCT = int(np.floor(self.CutOffTime * self.SampleRate / 1000))
+        self.SampleCT = 1 if self.Time1Pt else int(np.floor(self.CutOffTime * self.SampleRate / 1000))
         self.SWL = Src_Pt.SWL()
         self.scene = Sc
         self.Radius = self.increment * 0.5
@@ -472,7 +472,7 @@ class SplitRayTracer(Simulation_Type):
         if isinstance(rb, PachMapReceiver):
             # the medium and cut-off still come through the receiver set-up
             self._tr.set_receivers(rb.Origins[:1], rb.Radius, r.atten, r.c, r.rho * r.c, 1000, rb.SampleCT)
-            self._tr.set_map(rb.Origins, rb.increment, rb.Directional, rb.SampleCT)
+            self._tr.set_map(rb.Origins, rb.increment, rb.Directional, rb.SampleCT, rb.Time1Pt)
         else:
             if getattr(self._tr, "map_n", 0):
                 self._tr.clear_map()

@@ -35,18 +35,18 @@ def box():
     return rooms.shoebox(14, 10, 4, alpha=0.15, scat=0.3, source=SRC, receivers=[(7.3, 5.2, 1.4)], radius=0.5)
 
 
-def run_pair(tr, room, centers, inc, *, rays, mode, seed=5, cutoff=0.6, directional=True):
+def run_pair(tr, room, centers, inc, *, rays, mode, seed=5, cutoff=0.6, directional=True, time1pt=False):
     at, c, rho = A.uniform_medium()
     med = O.Medium(c=c, atten=at, rho=rho)
     sct = int(cutoff * 1000)
     tr.upload_scene(room.verts, room.surf, room.alpha, room.scat, room.trans, (8, 8, 8))
     tr.set_source(room.source, np.ones(8), A.DIR_GEODESIC)
     tr.set_receivers(room.receivers, room.radius, at, c, c * rho, 1000, sct)
-    tr.set_map(centers, inc, directional, sct)
+    tr.set_map(centers, inc, directional, sct, time1pt=time1pt)
     tr.trace(rays=rays, cutoff_m=cutoff * c, seed=seed, mode=mode, precision=A.PREC_FP64)
     got = tr.map_planes(scaled=False)
     st = tr.stats()
-    m = O.Map(centers, inc, room.source, med, directional=directional, sample_ct=sct)
+    m = O.Map(centers, inc, room.source, med, directional=directional, sample_ct=sct, time1pt=time1pt)
     want, cnt = O.trace(room, rays=rays, cutoff_s=cutoff, seed=seed, mode=mode, medium=med, map_rx=m,
                         grid=(8, 8, 8), threads=8)
     return got, want, st, cnt
@@ -145,3 +145,23 @@ def test_split_ray_tracer_with_a_map_mesh():
         e = E[i][: max(0, 500 - arrival)].sum() * np.sqrt(405.0)
         assert spl[i] == pytest.approx(10 * np.log10(e / 1e-12), rel=1e-12)
     rt.close()
+
+
+def test_time1pt_map_matches_oracle_and_the_windowed_sums(tracer):
+    """Time1Pt (Histogram_1Pt): one sample per receiver and band holding every
+    deposit's energy; equal to the oracle within 1e-9 of the largest value,
+    the same deposit count, no pressure or direction planes, and at least the
+    time sum of the windowed map of the same trace (which drops deposits
+    outside [0, SampleCT))."""
+    print("[case] map receivers Time1Pt", flush=True)
+    got, want, st, cnt = run_pair(tracer, box(), plane_centers(), 1.0, rays=2000, mode=A.MODE_STOCHASTIC,
+                                  time1pt=True)
+    assert got.shape == (8, 48, 8, 1)
+    assert st["deposits"] == cnt["deposits"] and want[0].sum() > 0
+    np.testing.assert_allclose(got[0], want[0], rtol=0, atol=1e-9 * np.abs(want[0]).max())
+    assert not got[1:].any()
+    win, _, st_w, _ = run_pair(tracer, box(), plane_centers(), 1.0, rays=2000, mode=A.MODE_STOCHASTIC)
+    tot = win[0].sum(-1)
+    assert np.all(got[0][..., 0] >= tot * (1 - 1e-12))
+    assert st["deposits"] >= st_w["deposits"]
+    tracer.clear_map()

@@ -164,3 +164,41 @@ def test_trace_with_a_map_deposits_and_conserves_counts():
     assert cnt["deposits"] > 0 and (pl[0].sum((1, 2)) > 0).all()
     pl2, _ = O.trace(room, rays=300, cutoff_s=0.5, seed=2, medium=med, map_rx=m, grid=(8, 8, 8), threads=3)
     np.testing.assert_allclose(pl2, pl, rtol=1e-12, atol=0)
+
+
+def test_time1pt_keeps_every_deposit_in_one_sample():
+    """Histogram_1Pt.Add (Classes_Receivers.cs:1393-1396): with Time1Pt each
+    deposit's energy goes to sample 0 of its band whatever its time -- also
+    one arriving before the direct sound (t_r < 0), which Histogram.Add drops
+    -- and the pressure and direction planes stay empty."""
+    _, centers = plane_map()
+    med = medium()
+    m1 = O.Map(centers, INC, SRC, med, directional=True, time1pt=True)
+    assert m1.sample_ct == 1
+    k = 10
+    C = centers[k]
+    d = np.array([1.0, 0.0, 0.0])
+    o, end = C + np.array([-5.0, 0.0, 0.0]), C + np.array([30.0, 0.0, 0.0])
+    direct = np.linalg.norm(SRC - C) / med.c
+    I, band = 2.5, 3
+    for t_sum in (0.013, -0.05):  # inside the window, and 50 ms before the direct sound
+        pl = m1.planes()
+        m1.check_octave(pl, o, d, end, I, band, t_sum)
+        tr = 5.0 / med.c + t_sum - direct
+        want = I * 10 ** (-0.1 * med.atten[band] * tr) / (np.pi * 0.25)
+        assert pl.shape == (8, len(centers), 8, 1)
+        assert pl[0, k, band, 0] == pytest.approx(want, rel=1e-12)
+        assert pl[0, k].sum() == pytest.approx(want, rel=1e-12)
+        assert not pl[1:].any()
+        # the windowed histogram keeps the first and drops the second
+        mw, _ = plane_map()
+        plw = mw.planes()
+        mw.check_octave(plw, o, d, end, I, band, t_sum)
+        assert plw[0, k].sum() == pytest.approx(want if tr >= 0 else 0.0, rel=1e-12)
+    # a broadband segment (from positive coordinates: that walk steers by the
+    # signs of the origin): every band, each to its sample 0
+    E = np.linspace(1.0, 2.0, 8)
+    pl = m1.planes()
+    m1.check_broadband(pl, C + np.array([-3.5, 0.0, 0.0]), d, end, E, 0.02)
+    tr = 3.5 / med.c + 0.02 - direct
+    np.testing.assert_allclose(pl[0, k, :, 0], E * 10 ** (-0.1 * med.atten * tr) / (np.pi * 0.25), rtol=1e-12)
