@@ -24,6 +24,8 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 namespace {
@@ -34,6 +36,37 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 constexpr int kBM = 128, kBN = 128, kBK = 64, kLdsK = kBK + 8;
 constexpr int kChunks = kBM * kBK / 8 / 256;  // 16-byte chunks per operand tile per thread
 constexpr int kThreads = 256;
+
+// f(std::integral_constant<int, 0>{}) ... f(std::integral_constant<int, N - 1>{}):
+// a loop whose index is a compile-time constant in each copy, so it can pick
+// a register array by index without dynamic indexing
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, I...>)
+{
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F> __device__ __forceinline__ void static_for(F &&f)
+{
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// Workgroup barrier that orders LDS accesses only: __syncthreads() also
+// waits for every outstanding global load (s_waitcnt vmcnt(0) before
+// s_barrier), which would drain the K tiles in flight at every step.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// v where ok, else zero, word by word (a select of two uint4 values is
+// lowered through scratch memory)
+__device__ __forceinline__ uint4 keep4(bool ok, const uint4 &v)
+{
+    const uint32_t m = ok ? ~0u : 0u;
+    return make_uint4(v.x & m, v.y & m, v.z & m, v.w & m);
+}
 
 __device__ __forceinline__ uint16_t f2bf(float f)
 {
@@ -46,7 +79,10 @@ __device__ __forceinline__ uint16_t f2bf(float f)
 
 // tanh through one exponential: tanh y = 1 - 2 / (e^(2y) + 1), which saturates
 // to +-1 for large |y| (e^(2y) overflows to inf or underflows to 0)
-__device__ __forceinline__ float tanh_fast(float y) { return 1.0f - 2.0f / (__expf(2.0f * y) + 1.0f); }
+__device__ __forceinline__ float tanh_fast(float y)
+{
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(__expf(2.0f * y) + 1.0f);
+}
 
 __device__ __forceinline__ float gelu(float x)
 {
@@ -79,8 +115,18 @@ struct Epi {
 // rows are bounds-checked one by one); blockIdx.z takes K range
 // [z * kchunk, (z + 1) * kchunk), kchunk a multiple of kBK.  With `partial`
 // the raw tile goes to partial[z] and the epilogue is skipped.
+//
+// Pipeline: an operand copied in k-contiguous chunks keeps kPD K tiles in
+// flight in registers (the products with TA false stream their large A
+// operand so; their grids hold only about two workgroups per CU, so one tile
+// in flight left each K step waiting a full memory latency).  The other
+// operand and every operand of the TA products keep one tile in flight.  Per
+// step the loads are issued in the order they are consumed, so waiting for
+// the oldest never waits for a younger one.
+constexpr int kPD = 2;
 template <bool TA, bool TB>
-__global__ void __launch_bounds__(kThreads, 3)
+__global__ void __attribute__((amdgpu_flat_work_group_size(kThreads, kThreads),
+                               amdgpu_waves_per_eu(TA || kPD == 1 ? 3 : 2, TA || kPD == 1 ? 3 : 2)))
 k_gemm(const uint16_t *__restrict__ A, int lda, const uint16_t *__restrict__ B, int ldb, int M, int N, int K,
        int kchunk, Epi epi, float *__restrict__ partial)
 {
@@ -124,26 +170,43 @@ k_gemm(const uint16_t *__restrict__ A, int lda, const uint16_t *__restrict__ B, 
     // 8 (b / 16) .. + 7 at columns 8 (b % 16) .. + 7, b = t % 128, and
     // transposes the block in registers into 8 k-contiguous chunks.
     constexpr bool XA = TA, XB = !TB;
+    constexpr int PD = XA ? 1 : kPD;  // A tiles in flight
     constexpr int NDA = XA ? 1 : kChunks, NDB = XB ? 1 : kChunks;
     const bool blk_a = XA && tid < 128;
     const bool blk_b = XB && (XA ? tid >= 128 : tid < 128);
     const int bt = tid & 127, bkg = bt >> 4, bmg = bt & 15;
-    uint4 rda[NDA], rdb[NDB], rt[8];
-    auto load = [&](int k0) {
+    uint4 rda[PD][NDA], rdb[NDB], rt[8];
+    // In the products with TA false (UNC) loads are unconditional: an element
+    // outside the operand (or a tile past the K range, fetched ahead and
+    // never used) is read from a clamped address inside it and replaced by
+    // zero, so the loop is straight-line code in which the compiler's memory
+    // waits count only the loads a step needs, not every load in flight.
+    constexpr bool UNC = !XA;
+    // A's k-contiguous chunks of tile k0 into register set `set`
+    auto load_a = [&](auto set, int k0) __attribute__((always_inline)) {
         if constexpr (!XA) {
 #pragma unroll
             for (int pass = 0; pass < kChunks; ++pass) {
                 const int c = tid + pass * kThreads, r = c >> 3, kc = (c & 7) * 8;
-                rda[pass] = (m0 + r < M && k0 + kc < ke) ? *(const uint4 *)(A + (size_t)(m0 + r) * lda + k0 + kc)
-                                                        : make_uint4(0, 0, 0, 0);
+                const bool ok = m0 + r < M && k0 + kc < ke;
+                const uint4 v = *(const uint4 *)(A + (size_t)min(m0 + r, M - 1) * lda + min(k0 + kc, ke - 8));
+                rda[set][pass] = keep4(ok, v);
             }
         }
+    };
+    // the rest of tile k0: B's chunks, or the 8 x 8 blocks of a k-major operand
+    auto load = [&](int k0) __attribute__((always_inline)) {
         if constexpr (!XB) {
 #pragma unroll
             for (int pass = 0; pass < kChunks; ++pass) {
                 const int c = tid + pass * kThreads, r = c >> 3, kc = (c & 7) * 8;
-                rdb[pass] = (n0 + r < N && k0 + kc < ke) ? *(const uint4 *)(B + (size_t)(n0 + r) * ldb + k0 + kc)
-                                                        : make_uint4(0, 0, 0, 0);
+                const bool ok = n0 + r < N && k0 + kc < ke;
+                if constexpr (UNC) {
+                    const uint4 v = *(const uint4 *)(B + (size_t)min(n0 + r, N - 1) * ldb + min(k0 + kc, ke - 8));
+                    rdb[pass] = keep4(ok, v);
+                } else {
+                    rdb[pass] = ok ? *(const uint4 *)(B + (size_t)(n0 + r) * ldb + k0 + kc) : make_uint4(0, 0, 0, 0);
+                }
             }
         }
         if (blk_a || blk_b) {
@@ -152,19 +215,27 @@ k_gemm(const uint16_t *__restrict__ A, int lda, const uint16_t *__restrict__ B, 
 #pragma unroll
             for (int rr = 0; rr < 8; ++rr) {
                 const int k = k0 + bkg * 8 + rr;
-                rt[rr] = (k < ke && col < ncol) ? *(const uint4 *)(X + (size_t)k * ld + col) : make_uint4(0, 0, 0, 0);
+                const bool ok = k < ke && col < ncol;
+                if constexpr (UNC) {
+                    const uint4 v = *(const uint4 *)(X + (size_t)min(k, ke - 1) * ld + min(col, ncol - 8));
+                    rt[rr] = keep4(ok, v);
+                } else {
+                    rt[rr] = ok ? *(const uint4 *)(X + (size_t)k * ld + col) : make_uint4(0, 0, 0, 0);
+                }
             }
         }
     };
-    auto word = [](const uint4 &v, int d) { return d == 0 ? v.x : (d == 1 ? v.y : (d == 2 ? v.z : v.w)); };
-    auto store = [&]() {
+    auto word = [](const uint4 &v, int d) __attribute__((always_inline)) { return d == 0 ? v.x : (d == 1 ? v.y : (d == 2 ? v.z : v.w)); };
+    auto store_a = [&](auto set) __attribute__((always_inline)) {
         if constexpr (!XA) {
 #pragma unroll
             for (int pass = 0; pass < kChunks; ++pass) {
                 const int c = tid + pass * kThreads;
-                *(uint4 *)(As + lds_off(c >> 3, c & 7)) = rda[pass];
+                *(uint4 *)(As + lds_off(c >> 3, c & 7)) = rda[set][pass];
             }
         }
+    };
+    auto store = [&]() __attribute__((always_inline)) {
         if constexpr (!XB) {
 #pragma unroll
             for (int pass = 0; pass < kChunks; ++pass) {
@@ -187,11 +258,8 @@ k_gemm(const uint16_t *__restrict__ A, int lda, const uint16_t *__restrict__ B, 
             }
         }
     };
-    if (kb < ke) load(kb);
-    for (int k0 = kb; k0 < ke; k0 += kBK) {
-        store();
-        __syncthreads();
-        if (k0 + kBK < ke) load(k0 + kBK);  // next tile in flight during the MFMAs
+    // the MFMAs of the tile in LDS
+    auto mma = [&]() __attribute__((always_inline)) {
 #pragma unroll
         for (int kk = 0; kk < kBK; kk += 16) {
             // operand layout of the 32x32x16 MFMA: lane l supplies row (l % 32)
@@ -210,13 +278,41 @@ k_gemm(const uint16_t *__restrict__ A, int lda, const uint16_t *__restrict__ B, 
                 for (int j = 0; j < 2; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
         }
-        __syncthreads();
+    };
+    // prologue: A tiles 0 .. PD - 1 and the first tile of the rest
+    const int nsteps = kb < ke ? (ke - kb + kBK - 1) / kBK : 0;
+    if (nsteps > 0) {
+        load(kb);
+        static_for<PD>([&](auto u) __attribute__((always_inline)) { load_a(u, kb + (int)u * kBK); });
     }
+    // one K step with A in register set u (compile-time): consume tile k0,
+    // then issue the loads of tile k0 + kBK (rest) and k0 + PD kBK (A)
+    auto step = [&](auto u, int k0) __attribute__((always_inline)) {
+        store_a(u);
+        store();
+        lds_barrier();
+        if (UNC || k0 + kBK < ke) load(k0 + kBK);
+        load_a(u, k0 + PD * kBK);
+        mma();
+        lds_barrier();
+    };
+    int st = 0;
+    for (; st + PD <= nsteps; st += PD)
+        static_for<PD>([&](auto u) __attribute__((always_inline)) { step(u, kb + (st + (int)u) * kBK); });
+    static_for<PD>([&](auto u) __attribute__((always_inline)) {
+        if (st + (int)u < nsteps) step(u, kb + (st + (int)u) * kBK);
+    });
     // result layout: lane l, register v holds C[8 (v / 4) + 4 (l / 32) + v % 4][l % 32].
     // One 32 x 32 block at a time, every input the epilogue reads (position
     // rows, pre-activations, the output it accumulates into) is loaded for the
     // whole block before its first store, so those loads are in flight
-    // together rather than one round trip per element.
+    // together rather than one round trip per element.  Which inputs exist is
+    // uniform over the grid (scalar branches), a lane's 16 rows are offsets
+    // 8 (v / 4) + v % 4 < 32 from its first row, so every address is one
+    // 64-bit base plus a 32-bit offset, and the position-table row (row mod
+    // npos) is stepped from the first row's instead of divided per element.
+    const int ldo = epi.ldo;
+    auto roff = [](int v) __attribute__((always_inline)) { return (v >> 2) * 8 + (v & 3); };
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -225,39 +321,65 @@ k_gemm(const uint16_t *__restrict__ A, int lda, const uint16_t *__restrict__ B, 
             const int rb = m0 + wm * 64 + i * 32 + (lane >> 5) * 4;
             const bool cok = col < N;
             if (partial) {
-                float *pz = partial + (size_t)tz * M * N;
+                float *pz = partial + (size_t)tz * M * N + (size_t)rb * N + col;
 #pragma unroll
-                for (int v = 0; v < 16; ++v) {
-                    const int row = rb + (v >> 2) * 8 + (v & 3);
-                    if (cok && row < M) pz[(size_t)row * N + col] = acc[i][j][v];
-                }
+                for (int v = 0; v < 16; ++v)
+                    if (cok && rb + roff(v) < M) pz[roff(v) * N] = acc[i][j][v];
                 continue;
             }
-            const float bias = (epi.bias && cok) ? epi.bias[col] : 0.0f;
-            float posv[16], pre[16], old[16];
+            const size_t base = (size_t)rb * ldo + col;
+            float r[16];
 #pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                const int row = rb + (v >> 2) * 8 + (v & 3);
-                const bool ok = cok && row < M;
-                const size_t o = (size_t)row * epi.ldo + col;
-                posv[v] = (epi.pos && ok) ? epi.pos[(size_t)(row % epi.npos) * N + col] : 0.0f;
-                pre[v] = (epi.dgelu && ok) ? epi.dgelu[o] : 0.0f;
-                old[v] = (epi.out && epi.accumulate && ok) ? epi.out[o] : 0.0f;
+            for (int v = 0; v < 16; ++v) r[v] = acc[i][j][v];
+            if (epi.bias) {
+                const float bias = cok ? epi.bias[col] : 0.0f;
+#pragma unroll
+                for (int v = 0; v < 16; ++v) r[v] += bias;
             }
+            float pre[16], old[16];
+            if (epi.pos) {
+                const int np = epi.npos;
+                const int p0 = rb % np;
+                const float *pc = epi.pos + col;
+                float posv[16];
 #pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                const int row = rb + (v >> 2) * 8 + (v & 3);
-                if (!cok || row >= M) continue;
-                const size_t o = (size_t)row * epi.ldo + col;
-                float r = acc[i][j][v];
-                if (epi.bias) r += bias;
-                if (epi.pos) r += posv[v];
-                if (epi.dgelu) r *= gelu_grad(pre[v]);
-                if (epi.out) {
-                    if (epi.accumulate) r += old[v];
-                    epi.out[o] = r;
+                for (int v = 0; v < 16; ++v) {
+                    int pr = p0 + roff(v);
+                    if (np >= 32) pr = pr >= np ? pr - np : pr;
+                    else pr %= np;
+                    posv[v] = (cok && rb + roff(v) < M) ? pc[(size_t)pr * N] : 0.0f;
                 }
-                if (epi.out_bf) epi.out_bf[o] = f2bf(epi.gelu ? gelu(r) : r);
+#pragma unroll
+                for (int v = 0; v < 16; ++v) r[v] += posv[v];
+            }
+            if (epi.dgelu) {
+                const float *pp = epi.dgelu + base;
+#pragma unroll
+                for (int v = 0; v < 16; ++v) pre[v] = (cok && rb + roff(v) < M) ? pp[roff(v) * ldo] : 0.0f;
+#pragma unroll
+                for (int v = 0; v < 16; ++v) r[v] *= gelu_grad(pre[v]);
+            }
+            if (epi.out) {
+                float *po = epi.out + base;
+                if (epi.accumulate) {
+#pragma unroll
+                    for (int v = 0; v < 16; ++v) old[v] = (cok && rb + roff(v) < M) ? po[roff(v) * ldo] : 0.0f;
+#pragma unroll
+                    for (int v = 0; v < 16; ++v) r[v] += old[v];
+                }
+#pragma unroll
+                for (int v = 0; v < 16; ++v)
+                    if (cok && rb + roff(v) < M) po[roff(v) * ldo] = r[v];
+            }
+            if (epi.out_bf) {
+                uint16_t *pb = epi.out_bf + base;
+                if (epi.gelu) {
+#pragma unroll
+                    for (int v = 0; v < 16; ++v) r[v] = gelu(r[v]);
+                }
+#pragma unroll
+                for (int v = 0; v < 16; ++v)
+                    if (cok && rb + roff(v) < M) pb[roff(v) * ldo] = f2bf(r[v]);
             }
         }
 }
