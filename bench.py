@@ -20,7 +20,9 @@ bounce + 40 B per spawned ray) per second of its own launch time against the
 and names the resource that actually binds (`binding`: VALU issue, from the
 profiled VALU instructions per bounce).  `cpu_baseline` times the C FP64
 restatement on the host's cores over a bounded sample.  Other workloads
-(--workload) report their own metric in the same form.
+(--workload) report their own metric in the same form; `--workload jnd`
+measures the second half of the metric: the wall time a plain and a
+denoised trace of the 50k-triangle concert hall need to reach JND accuracy.
 """
 from __future__ import annotations
 
@@ -138,8 +140,14 @@ def main():
                     help="image_sources workload: first objects of the CPU sample's sequences")
     ap.add_argument("--map-increment", type=float, default=0.5,
                     help="map workload: receiver spacing and diameter (m)")
-    ap.add_argument("--workload", choices=["hall", "batch", "image_sources", "filters", "map"], default="hall",
-                    help="hall: the 2k-triangle hall; batch: 256 packed rooms in one launch sequence")
+    ap.add_argument("--workload", choices=["hall", "batch", "image_sources", "filters", "map", "jnd"], default="hall",
+                    help="hall: the 2k-triangle hall; batch: 256 packed rooms in one launch sequence; jnd: "
+                         "time to JND-accurate T30/EDT/C80 on the concert hall, denoised vs plain")
+    ap.add_argument("--jnd-ref-rays", type=int, default=1_000_000, help="jnd workload: rays of the reference trace")
+    ap.add_argument("--jnd-ladder", default="20000,35000,50000,70000,100000,140000,200000",
+                    help="jnd workload: ray counts traced plainly and denoised")
+    ap.add_argument("--jnd-share", type=float, default=0.95,
+                    help="jnd workload: share of (receiver, band) pairs that must be inside every JND")
     ap.add_argument("--batch-ray-scale", type=float, default=0.01,
                     help="batch workload: scale of each room's log-uniform [1e4, 1e6] ray count")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
@@ -168,6 +176,8 @@ def main():
         return bench_image_sources(args, A, rooms)
     if args.workload == "filters":
         return bench_filters(args, A, rooms)
+    if args.workload == "jnd":
+        return bench_jnd(args, A, rooms)
     at, c, rho = A.uniform_medium()
     prec = {"fp64": A.PREC_FP64, "mixed": A.PREC_MIXED}.get(args.precision, A.PREC_FP32)
     tr = A.Tracer(0)
@@ -447,6 +457,99 @@ def bench_filters(args, A, rooms):
     }
     print(json.dumps(out))
     tr.close()
+
+
+def bench_jnd(args, A, rooms):
+    """Time to JND accuracy on the procedural concert hall (50,120 triangles,
+    64 receivers, 3 s): a plain reference trace of --jnd-ref-rays, then every
+    ladder count traced plainly and denoised by the shipped network (IS_Order
+    0, the direct sound added as ETCurve does, C80 from its arrival).  A trace
+    is JND-accurate when T30, EDT and C80 each have at least --jnd-share of the
+    64 x 8 (receiver, band) pairs inside 5 % / 5 % / 1 dB of the reference.
+    `value` is the wall time of the cheapest JND-accurate denoised trace
+    (trace + network); `plain_s` that of the cheapest plain one; both also as
+    a fraction of the 1e7-ray trace's time, extrapolated linearly in rays from
+    the reference trace."""
+    from pachydermacoustic_universal_amd import denoise as DN
+
+    at, c, rho = A.uniform_medium()
+    room = rooms.concert_hall()
+    tr = A.Tracer(0)
+    tr.upload_scene(room.verts, room.surf, room.alpha, room.scat, room.trans, (0, 0, 0))
+    tr.set_source(room.source, np.ones(8), A.DIR_GEODESIC)
+    tr.set_receivers(room.receivers, room.radius, at, c, c * rho, 1000, int(round(args.cutoff * 1000)))
+    dE, dbin = DN.direct_part(room)
+    desc = DN.descriptors(room)
+
+    def etc(E):
+        return DN.with_direct(E, dE, dbin)
+
+    def trace(rays, seed):
+        t0 = time.perf_counter()
+        tr.trace(rays=rays, cutoff_m=args.cutoff * c, seed=seed, is_order=0)
+        wall = time.perf_counter() - t0
+        return tr.histogram(0, scaled=True), wall, tr.stats()
+
+    for w in range(args.warmup):
+        trace(20000, 99)
+        print(f"[bench] warmup {w} done", file=sys.stderr, flush=True)
+    E_ref, ref_wall, st = trace(args.jnd_ref_rays, 1)
+    print(f"[bench] reference {args.jnd_ref_rays} rays: {ref_wall:.1f} s", file=sys.stderr, flush=True)
+    oct_b = st["bounces_late"] + st["bounces_verylate"]
+    achieved = (oct_b * BYTES_OCT_BOUNCE + st["spawned"] * BYTES_SPAWN) / (st["ms_octave"] / 1e3) / 1e9
+    P_ref = DN.decay_params(etc(E_ref), start_bin=dbin)
+    dn = DN.Denoiser.load(max_batch=64)
+    rows = []
+    t_all = time.perf_counter()
+    for rays in sorted(int(float(r)) for r in args.jnd_ladder.split(",") if r):
+        E, wall, _ = trace(rays, 1000 + rays)
+        t0 = time.perf_counter()
+        E_dn = dn.denoise(E, desc, rays=rays)
+        t_dn = time.perf_counter() - t0
+        row = {"rays": rays, "plain_s": wall, "denoised_s": wall + t_dn}
+        for kind, X in (("plain", E), ("denoised", E_dn)):
+            e = DN.summarize(DN.errors(DN.decay_params(etc(X), start_bin=dbin), P_ref))
+            row[kind] = {k: {"mean": e[k]["mean"], "within": e[k]["within"]} for k in e}
+        rows.append(row)
+        print(f"[bench] {rays} rays: plain {wall:.2f} s, shares within JND "
+              f"{[round(row['plain'][k]['within'], 3) for k in ('t30', 'edt', 'c80')]}; denoised "
+              f"{[round(row['denoised'][k]['within'], 3) for k in ('t30', 'edt', 'c80')]}", file=sys.stderr, flush=True)
+    dn.close()
+    tr.close()
+    dt = time.perf_counter() - t_all
+    t_1e7 = ref_wall * 1e7 / args.jnd_ref_rays
+
+    def first(kind):
+        for r in rows:
+            if min(r[kind][k]["within"] for k in ("t30", "edt", "c80")) >= args.jnd_share:
+                return r
+        return None
+
+    fp, fd = first("plain"), first("denoised")
+    out = {
+        "metric": f"time to JND-accurate T30/EDT/C80 ({100 * args.jnd_share:.0f} % of pairs), denoised",
+        "value": fd["denoised_s"] if fd else None, "unit": "s", "n_gpus": 1, "steps": 1, "warmup": args.warmup,
+        "ms_per_step": dt * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 trace (f64 histograms), bf16 network", "data": "synthetic",
+        "config": {"workload": "procedural concert hall, 50,120 tris, 64 receivers, 3 s cut-off, IS_Order 0 "
+                               "(direct sound added), JND 5 % T30 / 5 % EDT / 1 dB C80",
+                   "reference_rays": args.jnd_ref_rays, "reference_s": ref_wall,
+                   "ladder": [r["rays"] for r in rows]},
+        "plain_s": fp["plain_s"] if fp else None, "plain_rays": fp["rays"] if fp else None,
+        "denoised_rays": fd["rays"] if fd else None,
+        "speedup_vs_plain": (fp["plain_s"] / fd["denoised_s"]) if (fp and fd) else None,
+        "fraction_of_1e7_trace": {"plain": fp["plain_s"] / t_1e7 if fp else None,
+                                  "denoised": fd["denoised_s"] / t_1e7 if fd else None,
+                                  "t_1e7_s": t_1e7, "note": "1e7-ray time extrapolated linearly from the reference"},
+        "ladder": rows,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_octave",
+                     "avg_launch_ms": st["ms_octave"] / max(st["launches_octave"], 1),
+                     "launches": st["launches_octave"],
+                     "achieved_note": "the reference trace's octave kernel, algorithmic bytes as the hall workload"},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
 
 
 if __name__ == "__main__":
