@@ -181,6 +181,27 @@ void face_skip(const Grid &g, std::vector<uint32_t> &skip)
             }
 }
 
+void cell_records(const Grid &g, const std::vector<uint32_t> &skip, const std::vector<uint64_t> &mask,
+                  const std::vector<uint8_t> &dist, const std::vector<uint8_t> &oct, std::vector<uint32_t> &rec)
+{
+    // word layout of one record: 0 first, 1 count, 2-7 skip masks by face,
+    // 8-9 receiver mask (low, high), 10-11 octant edges (bytes 0-7),
+    // 12 distance, 13-15 unused
+    const size_t ncell = (size_t)g.dims[0] * g.dims[1] * g.dims[2];
+    rec.assign(ncell * 16, 0u);
+    for (size_t c = 0; c < ncell; ++c) {
+        uint32_t *r = rec.data() + 16 * c;
+        r[0] = g.ranges[2 * c];
+        r[1] = g.ranges[2 * c + 1];
+        for (int f = 0; f < 6; ++f) r[2 + f] = skip[6 * c + f];
+        const uint64_t m = mask.empty() ? 0 : mask[c];
+        r[8] = (uint32_t)m;
+        r[9] = (uint32_t)(m >> 32);
+        for (int o = 0; o < 8; ++o) r[10 + o / 4] |= (uint32_t)oct[8 * c + o] << (8 * (o % 4));
+        r[12] = dist[c];
+    }
+}
+
 int build_grid(const double *verts, int T, const int dims_in[3], Grid &g, std::string &err)
 {
     if (!verts || T <= 0) { err = "grid: empty scene"; return -1; }

@@ -37,6 +37,12 @@ void octant_cubes(const int dims[3], const std::vector<uint8_t> &dist, std::vect
 // list is also listed in the neighbour across f: a walk that steps in through
 // face f has already tested that triangle in the neighbour.
 void face_skip(const Grid &g, std::vector<uint32_t> &skip);
+
+// The per-cell records of the flattened walks (trace_types.h CellRec), 16
+// uint32 words per cell: list range, face skip masks (face_skip), receiver
+// mask (mask empty: none), octant cube edges and distance.
+void cell_records(const Grid &g, const std::vector<uint32_t> &skip, const std::vector<uint64_t> &mask,
+                  const std::vector<uint8_t> &dist, const std::vector<uint8_t> &oct, std::vector<uint32_t> &rec);
 // dims_in: requested voxel counts (any <= 0: chosen from the triangle count)
 int build_grid(const double *verts, int T, const int dims_in[3], Grid &g, std::string &err);
 

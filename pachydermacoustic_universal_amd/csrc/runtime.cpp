@@ -74,6 +74,8 @@ struct pach_ctx {
     uint8_t *d_cell_dist = nullptr;
     uint8_t *d_cell_oct = nullptr;  // [ncell][8] empty-cube edges per ray octant
     uint64_t *d_cell_rmask = nullptr;
+    CellRec *d_cell_rec = nullptr;     // [ncell] packed per-voxel records of the flattened walks
+    std::vector<uint32_t> cell_skip;   // host copy of the face skip masks (records are rebuilt with receivers)
     int ncell = 0;
     pach::Grid grid;        // host copy (receiver voxels are marked at trace time)
     std::vector<double> verts;  // host copy of the triangles (grid rebuilds)
@@ -372,6 +374,7 @@ extern "C" void pach_destroy(pach_ctx *c)
     dfree(c->d_surf); dfree(c->d_cell_range); dfree(c->d_cell_tris); dfree(c->d_cell_dist); dfree(c->d_cell_rows);
     dfree(c->d_cell_skip); dfree(c->d_cell_oct);
     dfree(c->d_cell_rmask);
+    dfree(c->d_cell_rec);
     dfree(c->d_dirs); dfree(c->d_geo);
     dfree(c->d_ctr32); dfree(c->d_ctr64); dfree(c->d_hist); dfree(c->d_fx);
     dfree(c->f_env); dfree(c->f_car); dfree(c->f_part); dfree(c->f_out); dfree(c->f_tab);
@@ -431,7 +434,12 @@ int upload_grid(pach_ctx *c, const int dims[3])
     pach::face_skip(grid, skip);
     std::vector<uint8_t> oct;  // receiver voxels join the occupancy when receivers are set
     pach::octant_cubes(grid.dims, grid.dist, oct);
+    std::vector<uint32_t> rec;
+    pach::cell_records(grid, skip, {}, grid.dist, oct, rec);
     hipError_t e;
+    if ((e = dalloc(c->d_cell_rec, rec.size() / 16)) != hipSuccess ||
+        (e = hipMemcpy(c->d_cell_rec, rec.data(), rec.size() * 4, hipMemcpyHostToDevice)) != hipSuccess)
+        return fail(c, PACH_EHIP, std::string("grid upload: ") + hipGetErrorString(e));
     if ((e = dalloc(c->d_cell_oct, oct.size())) != hipSuccess ||
         (e = hipMemcpy(c->d_cell_oct, oct.data(), oct.size(), hipMemcpyHostToDevice)) != hipSuccess ||
         (e = dalloc(c->d_cell_skip, skip.size())) != hipSuccess ||
@@ -451,6 +459,7 @@ int upload_grid(pach_ctx *c, const int dims[3])
         c->cs[k] = grid.cell[k];
     }
     c->ncell = grid.dims[0] * grid.dims[1] * grid.dims[2];
+    c->cell_skip = std::move(skip);
     c->grid = std::move(grid);
     c->rmask_dirty = true;
     return PACH_OK;
@@ -777,6 +786,10 @@ int refresh_receiver_cells(pach_ctx *c)
     pach::octant_cubes(c->g, dist, oct);
     HIPCHK(c, dalloc(c->d_cell_oct, oct.size()));
     HIPCHK(c, hipMemcpy(c->d_cell_oct, oct.data(), oct.size(), hipMemcpyHostToDevice));
+    std::vector<uint32_t> rec;
+    pach::cell_records(c->grid, c->cell_skip, c->rmask_on ? mask : std::vector<uint64_t>{}, dist, oct, rec);
+    HIPCHK(c, dalloc(c->d_cell_rec, rec.size() / 16));
+    HIPCHK(c, hipMemcpy(c->d_cell_rec, rec.data(), rec.size() * 4, hipMemcpyHostToDevice));
     c->rmask_dirty = false;
     return PACH_OK;
 }
@@ -959,6 +972,7 @@ template <class R> DevScene<R> dev_scene(pach_ctx *c)
     s.cell_oct = c->d_cell_oct;
     s.cell_dist = c->d_cell_dist;
     s.cell_rmask = nullptr;
+    s.cell_rec = c->d_cell_rec;
     s.gx = c->g[0]; s.gy = c->g[1]; s.gz = c->g[2];
     s.ncell = c->ncell;
     s.dist_lds = 0;

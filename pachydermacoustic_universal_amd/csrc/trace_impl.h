@@ -371,21 +371,23 @@ __device__ __forceinline__ void walk_enter(const DevScene<R> &s, Walk<R> &w, uin
 {
     ++n_vox;
     const int cell = (w.cz * s.gy + w.cy) * s.gx + w.cx;
-    w.k = cell_dist<LDS>(s.cell_dist, cell);
+    const CellRec *rec = s.cell_rec + cell;  // every field below lies on this record's cache line
+    if constexpr (LDS) w.k = pach_lds_dist[cell];
+    else w.k = (int)rec->dist;
     w.j = w.je = 0;
     w.todo = 0;
     if (w.k == 0) {
-        if (s.cell_rmask) w.rmask |= s.cell_rmask[cell];
-        const uint2 rg = s.cell_range[cell];
+        if (s.cell_rmask) w.rmask |= rec->rmask;
+        const uint2 rg = *(const uint2 *)&rec->first;
         w.j = rg.x;
         w.je = rg.x + rg.y;
         const uint32_t all = rg.y >= 32 ? ~0u : (1u << rg.y) - 1u;
-        const uint32_t seen = w.from >= 0 ? s.cell_skip[6 * cell + w.from] : 0u;
+        const uint32_t seen = w.from >= 0 ? rec->skip[w.from] : 0u;
         w.todo = all & ~seen;
     } else {
         // empty: the edge of the empty cube ahead of the ray, in its octant
         const int o = (w.ix < R(0) ? 1 : 0) | (w.iy < R(0) ? 2 : 0) | (w.iz < R(0) ? 4 : 0);
-        w.k = s.cell_oct[8 * cell + o];
+        w.k = rec->oct[o];
     }
 }
 

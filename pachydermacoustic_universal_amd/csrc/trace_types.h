@@ -11,6 +11,24 @@ constexpr int kPlanes = 11;
 constexpr int kBlock = 256;  // 4 wave64 per workgroup
 constexpr int kMaxLdsDist = 32768;  // voxel-distance bytes staged in LDS per workgroup
 
+// Everything a flattened walk reads on entering a voxel, in one 64-byte
+// record (one cache line per voxel entry instead of one per array): the list
+// range, per entry face the mask of list entries the neighbour across that
+// face also lists (face order -x, +x, -y, +y, -z, +z by the face the walk
+// came in through: grid.h face_skip), the receivers whose spheres touch the
+// voxel, the edge of the largest empty cube cornered at the voxel per ray
+// octant (grid.h octant_cubes) and the L-inf distance to the nearest occupied
+// voxel (receiver voxels count as occupied when receiver masks are on).
+struct alignas(64) CellRec {
+    uint32_t first, count;
+    uint32_t skip[6];
+    uint64_t rmask;
+    uint8_t oct[8];
+    uint32_t dist;
+    uint32_t pad[3];
+};
+static_assert(sizeof(CellRec) == 64, "one cell record per 64 bytes");
+
 template <class R> struct V4;
 template <> struct V4<float> { using T = float4; };
 template <> struct V4<double> { using T = double4; };
@@ -33,6 +51,7 @@ template <class R> struct DevScene {
     const uint8_t *cell_dist;      // [ncell] L-inf distance (cells) to the nearest occupied cell, 0 = occupied
     const uint8_t *cell_oct;       // [ncell][8] edge of the empty cube cornered at the cell per ray octant (grid.h)
     const uint64_t *cell_rmask;    // [ncell] receivers touching the voxel (null: test every receiver)
+    const CellRec *cell_rec;       // [ncell] the flattened walks' view of the same data
     int gx, gy, gz;
     int ncell;
     int dist_lds;                  // 1: k_octave stages cell_dist in LDS
