@@ -218,8 +218,12 @@ int build_grid(const double *verts, int T, const int dims_in[3], Grid &g, std::s
     for (int k = 0; k < 3; ++k) ext[k] = std::max(bmax[k] - bmin[k], 1e-6);
     int G[3] = {dims_in ? dims_in[0] : 0, dims_in ? dims_in[1] : 0, dims_in ? dims_in[2] : 0};
     if (G[0] <= 0 || G[1] <= 0 || G[2] <= 0) {
-        // about 8 voxels per triangle, shaped like the bounding box
-        const double target = std::min(std::max(8.0 * T, 4096.0), 2.0e6);
+        // about 8 voxels per triangle, 16 from 8,192 triangles on (large scenes
+        // trade more voxel steps for fewer triangle tests; small ones keep
+        // their distance field small enough for LDS), shaped like the
+        // bounding box
+        const double per = T >= 8192 ? 16.0 : 8.0;
+        const double target = std::min(std::max(per * T, 4096.0), 4.0e6);
         const double unit = std::cbrt(ext[0] * ext[1] * ext[2] / target);
         for (int k = 0; k < 3; ++k) G[k] = std::max(1, std::min(256, (int)std::lround(ext[k] / unit)));
     }
