@@ -107,6 +107,7 @@ struct Epi {
     uint16_t *out_bf = nullptr;    // BF16 [M, ldo] of (gelu ? gelu(result) : result)
     int gelu = 0;
     int ldo = 0;
+    float *colpart = nullptr;      // [M tiles][N]: each tile's column sums of the result (after dgelu)
 };
 
 // C = op(A) op(B).  TA: A is stored [K, M] (row-major, lda), else [M, K].
@@ -313,6 +314,10 @@ k_gemm(const uint16_t *__restrict__ A, int lda, const uint16_t *__restrict__ B, 
     // npos) is stepped from the first row's instead of divided per element.
     const int ldo = epi.ldo;
     auto roff = [](int v) __attribute__((always_inline)) { return (v >> 2) * 8 + (v & 3); };
+    // column sums of this tile (bias gradients): a lane's 16 rows, then the
+    // other half-wave's 16 rows of the same column, then the wave's two row
+    // blocks, then the two waves along M, in that fixed order
+    float csum[2] = {0.0f, 0.0f};
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -359,6 +364,13 @@ k_gemm(const uint16_t *__restrict__ A, int lda, const uint16_t *__restrict__ B, 
 #pragma unroll
                 for (int v = 0; v < 16; ++v) r[v] *= gelu_grad(pre[v]);
             }
+            if (epi.colpart) {
+                float cs = 0.0f;
+#pragma unroll
+                for (int v = 0; v < 16; ++v) cs += rb + roff(v) < M ? r[v] : 0.0f;
+                const float other = __shfl_xor(cs, 32);
+                csum[j] += lane < 32 ? cs + other : other + cs;
+            }
             if (epi.out) {
                 float *po = epi.out + base;
                 if (epi.accumulate) {
@@ -382,6 +394,22 @@ k_gemm(const uint16_t *__restrict__ A, int lda, const uint16_t *__restrict__ B, 
                     if (cok && rb + roff(v) < M) pb[roff(v) * ldo] = f2bf(r[v]);
             }
         }
+    if (epi.colpart) {
+        // the wave of the upper row half hands its sums to the lower one
+        __shared__ float cs_up[2][64];
+        if (wm == 1 && lane < 32) {
+            cs_up[wn][lane] = csum[0];
+            cs_up[wn][32 + lane] = csum[1];
+        }
+        lds_barrier();
+        if (wm == 0 && lane < 32) {
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int col = n0 + wn * 64 + j * 32 + lane;
+                if (col < N) epi.colpart[(size_t)ty * N + col] = csum[j] + cs_up[wn][j * 32 + lane];
+            }
+        }
+    }
 }
 
 // Sums of `parts` FP32 slices of n values (split-K partial tiles, column-sum
@@ -593,11 +621,12 @@ struct pach_dn {
     float *H = nullptr, *Y = nullptr;
     std::vector<uint16_t *> Hb, Ub, Vb;
     std::vector<float *> Upre, Vpre;
-    float *dY = nullptr, *dH = nullptr, *dV = nullptr, *dU = nullptr, *dC1 = nullptr;
+    float *dY = nullptr, *dH = nullptr, *dC1 = nullptr;
     uint16_t *dYb = nullptr, *dHb = nullptr, *dVb = nullptr, *dUb = nullptr;
     float *partial = nullptr;
     size_t partial_n = 0;
     float *colpart = nullptr;
+    float *tilecs = nullptr;  // [M tiles][N] column sums written by GEMM epilogues
     double *lossp = nullptr;
     size_t lossp_n = 0;
     float *x = nullptr, *desc = nullptr, *tgt = nullptr, *out = nullptr;
@@ -645,7 +674,7 @@ int gemm(pach_dn *d, bool ta, bool tb, const uint16_t *A, int lda, const uint16_
     float *part = nullptr;
     if (splits > 1) {
         if ((size_t)splits * M * N > d->partial_n || !epi.out || epi.accumulate || epi.bias || epi.pos || epi.dgelu ||
-            epi.out_bf || epi.ldo != N)
+            epi.out_bf || epi.colpart || epi.ldo != N)
             return dn_fail(d, PACH_EINVAL, "gemm: split-K takes a plain FP32 output");
         part = d->partial;
     }
@@ -660,6 +689,15 @@ int gemm(pach_dn *d, bool ta, bool tb, const uint16_t *A, int lda, const uint16_
         hipLaunchKernelGGL(k_splitk_sum, dim3(nblk(n)), dim3(256), 0, d->st, (const float *)part, splits, n, epi.out);
         DNCHK(d, hipGetLastError());
     }
+    return PACH_OK;
+}
+
+// out[n] = sum over the M tiles of a GEMM's epilogue column sums (fixed order)
+int colsum_tiles(pach_dn *d, int M, int N, float *out)
+{
+    const int tiles = (M + kBM - 1) / kBM;
+    hipLaunchKernelGGL(k_colsum_fin, dim3(nblk(N)), dim3(256), 0, d->st, (const float *)d->tilecs, tiles, N, out);
+    DNCHK(d, hipGetLastError());
     return PACH_OK;
 }
 
@@ -745,23 +783,25 @@ int backward(pach_dn *d, int B)
         pw.out = d->g + d->oW2[i];
         pw.ldo = D;
         if ((rc = gemm(d, true, false, d->Vb[i], 2 * D, d->dHb, D, 2 * D, D, M, pw, wsplits(2 * D, D, M)))) return rc;
+        // dV = (dH W2^T) gelu'(Vpre): only its BF16 copy and its column sums
+        // (the b1 gradient) are used, so no FP32 copy is written
         Epi dv;
         dv.dgelu = d->Vpre[i];
-        dv.out = d->dV;
         dv.out_bf = d->dVb;
+        dv.colpart = d->tilecs;
         dv.ldo = 2 * D;
         if ((rc = gemm(d, false, true, d->dHb, D, d->wb + d->oW2[i], D, M, 2 * D, D, dv))) return rc;
-        if ((rc = colsum(d, d->dV, M, 2 * D, d->g + d->ob1[i]))) return rc;
+        if ((rc = colsum_tiles(d, M, 2 * D, d->g + d->ob1[i]))) return rc;
         pw.out = d->g + d->oW1[i];
         pw.ldo = 2 * D;
         if ((rc = gemm(d, true, false, d->Ub[i], D, d->dVb, 2 * D, D, 2 * D, M, pw, wsplits(D, 2 * D, M)))) return rc;
         Epi du;
         du.dgelu = d->Upre[i];
-        du.out = d->dU;
         du.out_bf = d->dUb;
+        du.colpart = d->tilecs;
         du.ldo = D;
         if ((rc = gemm(d, false, true, d->dVb, 2 * D, d->wb + d->oW1[i], 2 * D, M, D, 2 * D, du))) return rc;
-        if ((rc = colsum(d, d->dU, M, D, d->g + d->obc[i]))) return rc;
+        if ((rc = colsum_tiles(d, M, D, d->g + d->obc[i]))) return rc;
         hipLaunchKernelGGL(k_im2col, dim3(nblk((size_t)M * 3 * D / 8)), dim3(256), 0, d->st,
                            (const uint16_t *)d->Hb[i], B, P, D, d->C1b);
         DNCHK(d, hipGetLastError());
@@ -875,10 +915,11 @@ extern "C" int pach_dn_create(pach_dn **out, int device, const pach_dn_config *c
                 bad(dn_alloc(d, d->v, o)) || bad(dn_alloc(d, d->wb, o)) || bad(dn_alloc(d, d->X0b, M * KE)) ||
                 bad(dn_alloc(d, d->C1b, M * 3 * D)) || bad(dn_alloc(d, d->H, M * D)) || bad(dn_alloc(d, d->Y, M * F)) ||
                 bad(dn_alloc(d, d->dY, M * F)) || bad(dn_alloc(d, d->dYb, M * F)) || bad(dn_alloc(d, d->dH, M * D)) ||
-                bad(dn_alloc(d, d->dHb, M * D)) || bad(dn_alloc(d, d->dV, M * 2 * D)) ||
-                bad(dn_alloc(d, d->dVb, M * 2 * D)) || bad(dn_alloc(d, d->dU, M * D)) || bad(dn_alloc(d, d->dUb, M * D)) ||
+                bad(dn_alloc(d, d->dHb, M * D)) ||
+                bad(dn_alloc(d, d->dVb, M * 2 * D)) || bad(dn_alloc(d, d->dUb, M * D)) ||
                 bad(dn_alloc(d, d->dC1, M * 3 * D)) || bad(dn_alloc(d, d->partial, d->partial_n)) ||
                 bad(dn_alloc(d, d->colpart, (size_t)kColParts * std::max<int64_t>(3 * D, std::max<int64_t>(KE, F)))) ||
+                bad(dn_alloc(d, d->tilecs, ((M + kBM - 1) / kBM) * (size_t)(2 * D))) ||
                 bad(dn_alloc(d, d->lossp, d->lossp_n)) || bad(dn_alloc(d, d->x, (size_t)c.max_batch * 8 * c.bins)) ||
                 bad(dn_alloc(d, d->tgt, (size_t)c.max_batch * 8 * c.bins)) ||
                 bad(dn_alloc(d, d->out, (size_t)c.max_batch * 8 * c.bins)) ||
