@@ -692,13 +692,13 @@ int gemm(pach_dn *d, bool ta, bool tb, const uint16_t *A, int lda, const uint16_
     return PACH_OK;
 }
 
-// out[n] = sum over the M tiles of a GEMM's epilogue column sums (fixed order)
+int colsum(pach_dn *d, const float *in, int M, int N, float *out);
+
+// out[n] = sum over the M tiles of a GEMM's epilogue column sums: the tile
+// sums form a [tiles][N] matrix reduced like any other (fixed order)
 int colsum_tiles(pach_dn *d, int M, int N, float *out)
 {
-    const int tiles = (M + kBM - 1) / kBM;
-    hipLaunchKernelGGL(k_colsum_fin, dim3(nblk(N)), dim3(256), 0, d->st, (const float *)d->tilecs, tiles, N, out);
-    DNCHK(d, hipGetLastError());
-    return PACH_OK;
+    return colsum(d, d->tilecs, (M + kBM - 1) / kBM, N, out);
 }
 
 int colsum(pach_dn *d, const float *in, int M, int N, float *out)
