@@ -372,23 +372,30 @@ __device__ __forceinline__ void walk_enter(const DevScene<R> &s, Walk<R> &w, uin
     ++n_vox;
     const int cell = (w.cz * s.gy + w.cy) * s.gx + w.cx;
     const CellRec *rec = s.cell_rec + cell;  // every field below lies on this record's cache line
-    if constexpr (LDS) w.k = pach_lds_dist[cell];
-    else w.k = (int)rec->dist;
-    w.j = w.je = 0;
-    w.todo = 0;
-    if (w.k == 0) {
-        if (s.cell_rmask) w.rmask |= rec->rmask;
-        const uint2 rg = *(const uint2 *)&rec->first;
-        w.j = rg.x;
-        w.je = rg.x + rg.y;
-        const uint32_t all = rg.y >= 32 ? ~0u : (1u << rg.y) - 1u;
-        const uint32_t seen = w.from >= 0 ? rec->skip[w.from] : 0u;
-        w.todo = all & ~seen;
-    } else {
-        // empty: the edge of the empty cube ahead of the ray, in its octant
-        const int o = (w.ix < R(0) ? 1 : 0) | (w.iy < R(0) ? 2 : 0) | (w.iz < R(0) ? 4 : 0);
-        w.k = rec->oct[o];
-    }
+    // every load is issued before any is used, occupied voxel or not, so a
+    // voxel costs one memory latency
+    const uint4 head = *(const uint4 *)&rec->first;  // first, count, skip[0], skip[1]
+    const uint4 tail = *(const uint4 *)&rec->rmask;  // receiver mask (2 words), octant edges (2 words)
+    const uint32_t seen = w.from >= 0 ? rec->skip[max(w.from, 0)] : 0u;
+    int k;
+    if constexpr (LDS) k = pach_lds_dist[cell];
+    else k = (int)rec->dist;
+    // the edge of the empty cube ahead of the ray, in its octant (bit a set
+    // when the ray runs down axis a)
+    int o;
+    if constexpr (sizeof(R) == 4)
+        o = (int)((__float_as_uint(w.ix) >> 31) | ((__float_as_uint(w.iy) >> 30) & 2u) |
+                  ((__float_as_uint(w.iz) >> 29) & 4u));
+    else
+        o = (w.ix < R(0) ? 1 : 0) | (w.iy < R(0) ? 2 : 0) | (w.iz < R(0) ? 4 : 0);
+    const uint32_t edge = ((o < 4 ? tail.z : tail.w) >> (8 * (o & 3))) & 0xffu;
+    const bool occ = k == 0;
+    w.k = occ ? 0 : (int)edge;
+    if (s.cell_rmask) w.rmask |= occ ? (((uint64_t)tail.y << 32) | tail.x) : 0ull;
+    w.j = occ ? head.x : 0u;
+    w.je = occ ? head.x + head.y : 0u;
+    const uint32_t all = head.y >= 32 ? ~0u : (1u << head.y) - 1u;
+    w.todo = occ ? all & ~seen : 0u;
 }
 
 // One move of the flattened walk: across the empty cube ahead of the ray
@@ -399,13 +406,17 @@ __device__ __forceinline__ void walk_enter(const DevScene<R> &s, Walk<R> &w, uin
 template <class R>
 __device__ __forceinline__ bool walk_move(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R dx, R dy, R dz)
 {
+    const R INF = R(__builtin_huge_val());
     const int h = max(w.k - 1, 0);
-    const int lx = dx < R(0) ? max(w.cx - h, 0) : w.cx, hx = dx < R(0) ? w.cx : min(w.cx + h, s.gx - 1);
-    const int ly = dy < R(0) ? max(w.cy - h, 0) : w.cy, hy = dy < R(0) ? w.cy : min(w.cy + h, s.gy - 1);
-    const int lz = dz < R(0) ? max(w.cz - h, 0) : w.cz, hz = dz < R(0) ? w.cz : min(w.cz + h, s.gz - 1);
-    const R bx = axis_exit<R>(ox, dx, w.ix, dx > R(0) ? hx : lx, s.gmin[0], s.cs[0]);
-    const R by = axis_exit<R>(oy, dy, w.iy, dy > R(0) ? hy : ly, s.gmin[1], s.cs[1]);
-    const R bz = axis_exit<R>(oz, dz, w.iz, dz > R(0) ? hz : lz, s.gmin[2], s.cs[2]);
+    const R hr = R(h);
+    // the cube's far faces lie h voxel widths past the current voxel's exits
+    // (an axis the ray does not advance along keeps its infinite exit); a cube
+    // reaching past the grid is not clipped: the ray then leaves the grid
+    // inside the cube, and the cell reached below lies outside it
+    const R ex = s.cs[0] * fabs(w.ix), ey = s.cs[1] * fabs(w.iy), ez = s.cs[2] * fabs(w.iz);
+    const R bx = ex < INF ? mad<R>(hr, ex, w.tmx) : w.tmx;
+    const R by = ey < INF ? mad<R>(hr, ey, w.tmy) : w.tmy;
+    const R bz = ez < INF ? mad<R>(hr, ez, w.tmz) : w.tmz;
     const int ax = (bx < by) ? (bx < bz ? 0 : 2) : (by < bz ? 1 : 2);
     const R tb = ax == 0 ? bx : (ax == 1 ? by : bz);
     const R px = mad<R>(dx, tb, ox), py = mad<R>(dy, tb, oy), pz = mad<R>(dz, tb, oz);
@@ -413,13 +424,15 @@ __device__ __forceinline__ bool walk_move(const DevScene<R> &s, Walk<R> &w, R ox
     const int fy = (int)floor((py - s.gmin[1]) * s.inv_cs[1]);
     const int fz = (int)floor((pz - s.gmin[2]) * s.inv_cs[2]);
     // never move backwards; stay inside the cube on the other axes
-    // (clamp to [c, h] moving up, [l, c] moving down, c when still)
-    int nx = min(max(fx, dx < R(0) ? lx : w.cx), dx > R(0) ? hx : w.cx);
-    int ny = min(max(fy, dy < R(0) ? ly : w.cy), dy > R(0) ? hy : w.cy);
-    int nz = min(max(fz, dz < R(0) ? lz : w.cz), dz > R(0) ? hz : w.cz);
-    nx = ax == 0 ? (dx > R(0) ? hx + 1 : lx - 1) : nx;
-    ny = ax == 1 ? (dy > R(0) ? hy + 1 : ly - 1) : ny;
-    nz = ax == 2 ? (dz > R(0) ? hz + 1 : lz - 1) : nz;
+    // (clamp to [c, c + h] moving up, [c - h, c] moving down, c when still);
+    // across the exit face on the exit axis
+    int nx = min(max(fx, dx < R(0) ? w.cx - h : w.cx), dx > R(0) ? w.cx + h : w.cx);
+    int ny = min(max(fy, dy < R(0) ? w.cy - h : w.cy), dy > R(0) ? w.cy + h : w.cy);
+    int nz = min(max(fz, dz < R(0) ? w.cz - h : w.cz), dz > R(0) ? w.cz + h : w.cz);
+    const int sx = dx > R(0) ? h + 1 : -h - 1, sy = dy > R(0) ? h + 1 : -h - 1, sz = dz > R(0) ? h + 1 : -h - 1;
+    nx = ax == 0 ? w.cx + sx : nx;
+    ny = ax == 1 ? w.cy + sy : ny;
+    nz = ax == 2 ? w.cz + sz : nz;
     if ((unsigned)nx >= (unsigned)s.gx || (unsigned)ny >= (unsigned)s.gy || (unsigned)nz >= (unsigned)s.gz)
         return true;
     // a single step enters through the face towards the voxel left (-x when
@@ -488,12 +501,12 @@ __device__ bool walk_units(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R
             // any hit nearer than the best so far, inside this voxel or not
             const R tlim = w.tbest;
             // the tests first: their row loads then need not wait for the index loads
+            // as selects, not a branch: a branch would defer the id loads behind the tests
             auto take = [&](bool hit, int t, R tt) __attribute__((always_inline)) {
-                if (hit && t != w.best && !(Tmin<R>::excl && t == excl) &&
-                    (tt < w.tbest || (tt == w.tbest && t < w.best))) {
-                    w.tbest = tt;
-                    w.best = t;
-                }
+                const bool nearer = hit & (t != w.best) & !(Tmin<R>::excl && t == excl) &
+                                    ((tt < w.tbest) | ((tt == w.tbest) & (t < w.best)));
+                w.tbest = nearer ? tt : w.tbest;
+                w.best = nearer ? t : w.best;
             };
             {
                 // three list entries per unit, merged in list order, as
