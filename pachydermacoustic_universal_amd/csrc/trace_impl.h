@@ -1438,6 +1438,11 @@ __device__ __forceinline__ bool octave_shade(const TraceConst &tc, const DevScen
         lc.add(C_MISS);
         return false;
     }
+    // the surface's data is read first, so its loads overlap the receiver checks
+    const int sf = s.surf[ht];
+    const auto N = s.nrm[ht];
+    const auto M = s.mat[sf * kBands + band];  // (1 - alpha, s, lo, hi)
+    const R tau = tc.has_trans ? s.trans[sf * kBands + band] : R(0);
     const R hx = r.o[0] + r.d[0] * leg, hy = r.o[1] + r.d[1] * leg, hz = r.o[2] + r.d[2] * leg;
     {
         const R qx = hx - r.o[0], qy = hy - r.o[1], qz = hz - r.o[2];
@@ -1449,16 +1454,13 @@ __device__ __forceinline__ bool octave_shade(const TraceConst &tc, const DevScen
     r.t += (double)leg / rv.c;
     r.o[0] = hx; r.o[1] = hy; r.o[2] = hz;
     r.tri = ht;
-    const int sf = s.surf[ht];
-    const auto N = s.nrm[ht];
-    const auto M = s.mat[sf * kBands + band];  // (1 - alpha, s, lo, hi)
     const R cos_t = N.x * r.d[0] + N.y * r.d[1] + N.z * r.d[2];
     r.I *= M.x;
     const uint32_t kev = r.k++;
     if (tc.mode == 0) {
         uint32_t w[4];
         philox4x32_10(kev, r.uid_lo, r.uid_hi, r.src, tc.key0, tc.key1, w);
-        const bool transmit = tc.has_trans && unif<R>(w[0]) < s.trans[sf * kBands + band];
+        const bool transmit = tc.has_trans && unif<R>(w[0]) < tau;
         bool diffuse;
         if (!late3) {
             // Scatter_Late (Classes_Materials.cs:4629-4702)
