@@ -376,10 +376,12 @@ __device__ __forceinline__ void walk_enter(const DevScene<R> &s, Walk<R> &w, uin
     // voxel costs one memory latency
     const uint4 head = *(const uint4 *)&rec->first;  // first, count, skip[0], skip[1]
     const uint4 tail = *(const uint4 *)&rec->rmask;  // receiver mask (2 words), octant edges (2 words)
-    const uint32_t seen = w.from >= 0 ? rec->skip[max(w.from, 0)] : 0u;
     int k;
     if constexpr (LDS) k = pach_lds_dist[cell];
     else k = (int)rec->dist;
+    // face mask of the entry face, loaded whether there is one or not (a
+    // conditional load would make the wave wait for all of them there)
+    const uint32_t seen = rec->skip[max(w.from, 0)] & (w.from >= 0 ? ~0u : 0u);
     // the edge of the empty cube ahead of the ray, in its octant (bit a set
     // when the ray runs down axis a)
     int o;
@@ -1426,30 +1428,30 @@ __device__ __forceinline__ bool octave_shade(const TraceConst &tc, const DevScen
     const RoomTab &room = tc.rooms[r.meta >> M_ROOM_SHIFT];
     const int ana = (r.meta & M_SCAT) ? 1 : ((r.meta & M_EVER) ? 2 : 0);
     lc.add(late3 ? C_VLATE : C_LATE);
-    if (ht < 0) {
-        const R ex = r.o[0] + r.d[0] * R(1000000), ey = r.o[1] + r.d[1] * R(1000000),
-                ez = r.o[2] + r.d[2] * R(1000000);
-        const R qx = ex - r.o[0], qy = ey - r.o[1], qz = ez - r.o[2];
-        if constexpr (MAP) map_check<R, false>(rv, r.o, r.d, ex, ey, ez, nullptr, r.I, band, r.t, lc);
-        else
-            check_receivers<R, false>(rv, room, cand, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2],
-                               qx * qx + qy * qy + qz * qz, nullptr, r.I, band, 0, 0, r.t, ana, lc);
-        lc.lose((double)r.I);
-        lc.add(C_MISS);
-        return false;
-    }
-    // the surface's data is read first, so its loads overlap the receiver checks
-    const int sf = s.surf[ht];
-    const auto N = s.nrm[ht];
+    const bool miss = ht < 0;
+    // the surface's data is read first, so its loads overlap the receiver
+    // checks (a miss reads triangle 0's and does not use it)
+    const int hs = miss ? 0 : ht;
+    const int sf = s.surf[hs];
+    const auto N = s.nrm[hs];
     const auto M = s.mat[sf * kBands + band];  // (1 - alpha, s, lo, hi)
     const R tau = tc.has_trans ? s.trans[sf * kBands + band] : R(0);
-    const R hx = r.o[0] + r.d[0] * leg, hy = r.o[1] + r.d[1] * leg, hz = r.o[2] + r.d[2] * leg;
+    // the segment checked against the receivers ends at the hit, or 1e6 m out
+    // for a ray that leaves the scene; one call site for both keeps the
+    // kernel's code small
+    const R lg = miss ? R(1000000) : leg;
+    const R hx = r.o[0] + r.d[0] * lg, hy = r.o[1] + r.d[1] * lg, hz = r.o[2] + r.d[2] * lg;
     {
         const R qx = hx - r.o[0], qy = hy - r.o[1], qz = hz - r.o[2];
         if constexpr (MAP) map_check<R, false>(rv, r.o, r.d, hx, hy, hz, nullptr, r.I, band, r.t, lc);
         else
             check_receivers<R, false>(rv, room, cand, r.o[0], r.o[1], r.o[2], r.d[0], r.d[1], r.d[2],
                                qx * qx + qy * qy + qz * qz, nullptr, r.I, band, 0, 0, r.t, ana, lc);
+    }
+    if (miss) {
+        lc.lose((double)r.I);
+        lc.add(C_MISS);
+        return false;
     }
     r.t += (double)leg / rv.c;
     r.o[0] = hx; r.o[1] = hy; r.o[2] = hz;
