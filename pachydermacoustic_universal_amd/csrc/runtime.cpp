@@ -993,8 +993,10 @@ template <class R> DevScene<R> dev_scene(pach_ctx *c)
 // at most 12 eps L + 8 eps |t| over |n . d| plus 3 eps |t| (rounded inputs,
 // three fused products, the reciprocal), its hit point by that plus
 // eps (2 L + D), its barycentric coordinates by G times that plus 11 eps L G;
-// each bound is doubled.  A voxel exit is off by at most 8 eps L / |d_axis|
-// + 8 eps t.  FP64 can hit the triangle a ray leaves again only when the
+// each bound is doubled.  A voxel exit, c s + q with s = cs / d and
+// q = (cs [d > 0] + gmin - o) / d (the walk's axis lines), is off by at most
+// (7 D + 2 L) eps / |d_axis| + eps t <= 16 eps L / |d_axis| + eps t; the bound
+// used is 32 eps L / |d_axis| + 8 eps t.  FP64 can hit the triangle a ray leaves again only when the
 // departure point's FP64 distance from its plane, ~2.2e-16 (L + D), exceeds
 // 1e-10 |n . d|.
 void mix_bounds(const pach_ctx *c, TraceConst &tc)
@@ -1014,7 +1016,7 @@ void mix_bounds(const pach_ctx *c, TraceConst &tc)
     tc.mx_kt0 = (float)kt0;
     tc.mx_ku = (float)(2 * G * kt);
     tc.mx_ku0 = (float)(2 * G * (kt0 + eps * (2 * L + D) + 11 * eps * L));
-    tc.mx_kx = (float)(8 * eps * L);
+    tc.mx_kx = (float)(32 * eps * L);
     // grid.cpp lists a triangle in every voxel its box, grown by 1e-3 of a voxel per side, touches
     tc.mx_infl = (float)(0.9e-3 * std::min({c->cs[0], c->cs[1], c->cs[2]}));
     tc.mx_self = (float)std::max(1e-4, 16 * 1.1102230246251565e-16 * (L + D) / 1e-10);

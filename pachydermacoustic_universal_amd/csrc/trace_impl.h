@@ -239,7 +239,11 @@ __device__ __forceinline__ uint8_t cell_dist(const uint8_t *g, int cell)
 template <class R> struct Walk {
     int cx, cy, cz;
     R tmx, tmy, tmz;  // t where the ray leaves the current voxel along each axis
-    R ix, iy, iz;     // 1 / d (infinite for a zero component)
+    // exit t of voxel c along axis a: c * sa + qa (sa = cs_a / d_a, one voxel
+    // width in t with the direction's sign; sa = 0 and qa = +inf for d_a == 0,
+    // so the same fused multiply-add gives the infinite exit)
+    R sx, sy, sz;
+    R qx, qy, qz;
     int best;
     R tbest;
     uint64_t rmask;   // receivers whose spheres touch a voxel the walk crossed
@@ -255,31 +259,35 @@ template <class R> struct Walk {
     R tend;           // current voxel: the walk ends once tbest + eb <= tend
 };
 
-// t where the ray leaves voxel c along one axis: the upper face when d > 0,
-// the lower one when d < 0, never when d == 0.  Written as selects so a wave
-// whose lanes move in different directions runs one straight-line sequence.
+// Slope and offset of the voxel exits along one axis: voxel c is left at
+// t = c * sa + qa, through its upper face when d > 0 and its lower one when
+// d < 0, never when d == 0 (sa = 0, qa = +inf).
 template <class R>
-__device__ __forceinline__ R axis_exit(R o, R d, R inv, int c, const R gmin, const R cs)
+__device__ __forceinline__ void axis_line(R o, R d, const R gmin, const R cs, R &sa, R &qa)
 {
     const R INF = R(__builtin_huge_val());
-    const R v = (mad<R>(R(c + (d > R(0) ? 1 : 0)), cs, gmin) - o) * inv;
-    return d != R(0) ? v : INF;
+    const R inv = rcp<R>(d);
+    sa = d != R(0) ? cs * inv : R(0);
+    qa = d != R(0) ? ((d > R(0) ? cs : R(0)) + gmin - o) * inv : INF;
 }
+
+// t where the ray leaves voxel c along the axis of (sa, qa)
+template <class R> __device__ __forceinline__ R axis_exit(R sa, R qa, int c) { return mad<R>(R(c), sa, qa); }
 
 template <class R>
 __device__ __forceinline__ void walk_init(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R dx, R dy, R dz)
 {
     const R INF = R(__builtin_huge_val());
-    w.ix = dx != R(0) ? rcp<R>(dx) : INF;
-    w.iy = dy != R(0) ? rcp<R>(dy) : INF;
-    w.iz = dz != R(0) ? rcp<R>(dz) : INF;
+    axis_line<R>(ox, dx, s.gmin[0], s.cs[0], w.sx, w.qx);
+    axis_line<R>(oy, dy, s.gmin[1], s.cs[1], w.sy, w.qy);
+    axis_line<R>(oz, dz, s.gmin[2], s.cs[2], w.sz, w.qz);
     w.rmask = 0;
     w.cx = min(max((int)floor((ox - s.gmin[0]) * s.inv_cs[0]), 0), s.gx - 1);
     w.cy = min(max((int)floor((oy - s.gmin[1]) * s.inv_cs[1]), 0), s.gy - 1);
     w.cz = min(max((int)floor((oz - s.gmin[2]) * s.inv_cs[2]), 0), s.gz - 1);
-    w.tmx = axis_exit<R>(ox, dx, w.ix, w.cx, s.gmin[0], s.cs[0]);
-    w.tmy = axis_exit<R>(oy, dy, w.iy, w.cy, s.gmin[1], s.cs[1]);
-    w.tmz = axis_exit<R>(oz, dz, w.iz, w.cz, s.gmin[2], s.cs[2]);
+    w.tmx = axis_exit<R>(w.sx, w.qx, w.cx);
+    w.tmy = axis_exit<R>(w.sy, w.qy, w.cy);
+    w.tmz = axis_exit<R>(w.sz, w.qz, w.cz);
     w.best = -1;
     w.tbest = INF;
     w.j = w.je = 0;
@@ -323,9 +331,9 @@ __device__ bool walk_steps(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R
             const int lx = max(w.cx - h, 0), hx = min(w.cx + h, s.gx - 1);
             const int ly = max(w.cy - h, 0), hy = min(w.cy + h, s.gy - 1);
             const int lz = max(w.cz - h, 0), hz = min(w.cz + h, s.gz - 1);
-            const R bx = axis_exit<R>(ox, dx, w.ix, dx > R(0) ? hx : lx, s.gmin[0], s.cs[0]);
-            const R by = axis_exit<R>(oy, dy, w.iy, dy > R(0) ? hy : ly, s.gmin[1], s.cs[1]);
-            const R bz = axis_exit<R>(oz, dz, w.iz, dz > R(0) ? hz : lz, s.gmin[2], s.cs[2]);
+            const R bx = axis_exit<R>(w.sx, w.qx, dx > R(0) ? hx : lx);
+            const R by = axis_exit<R>(w.sy, w.qy, dy > R(0) ? hy : ly);
+            const R bz = axis_exit<R>(w.sz, w.qz, dz > R(0) ? hz : lz);
             const int ax = (bx < by) ? (bx < bz ? 0 : 2) : (by < bz ? 1 : 2);
             const R tb = ax == 0 ? bx : (ax == 1 ? by : bz);
             const R px = ox + dx * tb, py = oy + dy * tb, pz = oz + dz * tb;
@@ -341,23 +349,23 @@ __device__ bool walk_steps(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R
             else nz = dz > R(0) ? hz + 1 : lz - 1;
             if (nx < 0 || nx >= s.gx || ny < 0 || ny >= s.gy || nz < 0 || nz >= s.gz) return true;
             w.cx = nx; w.cy = ny; w.cz = nz;
-            w.tmx = axis_exit<R>(ox, dx, w.ix, nx, s.gmin[0], s.cs[0]);
-            w.tmy = axis_exit<R>(oy, dy, w.iy, ny, s.gmin[1], s.cs[1]);
-            w.tmz = axis_exit<R>(oz, dz, w.iz, nz, s.gmin[2], s.cs[2]);
+            w.tmx = axis_exit<R>(w.sx, w.qx, nx);
+            w.tmy = axis_exit<R>(w.sy, w.qy, ny);
+            w.tmz = axis_exit<R>(w.sz, w.qz, nz);
         } else {
             // one voxel step along the axis whose boundary comes first
             if (w.tmx < w.tmy && w.tmx < w.tmz) {
                 w.cx += dx > R(0) ? 1 : -1;
                 if (w.cx < 0 || w.cx >= s.gx) return true;
-                w.tmx += s.cs[0] * fabs(w.ix);
+                w.tmx += fabs(w.sx);
             } else if (w.tmy < w.tmz) {
                 w.cy += dy > R(0) ? 1 : -1;
                 if (w.cy < 0 || w.cy >= s.gy) return true;
-                w.tmy += s.cs[1] * fabs(w.iy);
+                w.tmy += fabs(w.sy);
             } else {
                 w.cz += dz > R(0) ? 1 : -1;
                 if (w.cz < 0 || w.cz >= s.gz) return true;
-                w.tmz += s.cs[2] * fabs(w.iz);
+                w.tmz += fabs(w.sz);
             }
         }
     }
@@ -386,10 +394,10 @@ __device__ __forceinline__ void walk_enter(const DevScene<R> &s, Walk<R> &w, uin
     // when the ray runs down axis a)
     int o;
     if constexpr (sizeof(R) == 4)
-        o = (int)((__float_as_uint(w.ix) >> 31) | ((__float_as_uint(w.iy) >> 30) & 2u) |
-                  ((__float_as_uint(w.iz) >> 29) & 4u));
+        o = (int)((__float_as_uint(w.sx) >> 31) | ((__float_as_uint(w.sy) >> 30) & 2u) |
+                  ((__float_as_uint(w.sz) >> 29) & 4u));
     else
-        o = (w.ix < R(0) ? 1 : 0) | (w.iy < R(0) ? 2 : 0) | (w.iz < R(0) ? 4 : 0);
+        o = (w.sx < R(0) ? 1 : 0) | (w.sy < R(0) ? 2 : 0) | (w.sz < R(0) ? 4 : 0);
     const uint32_t edge = ((o < 4 ? tail.z : tail.w) >> (8 * (o & 3))) & 0xffu;
     const bool occ = k == 0;
     w.k = occ ? 0 : (int)edge;
@@ -408,17 +416,15 @@ __device__ __forceinline__ void walk_enter(const DevScene<R> &s, Walk<R> &w, uin
 template <class R>
 __device__ __forceinline__ bool walk_move(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R dx, R dy, R dz)
 {
-    const R INF = R(__builtin_huge_val());
     const int h = max(w.k - 1, 0);
     const R hr = R(h);
     // the cube's far faces lie h voxel widths past the current voxel's exits
     // (an axis the ray does not advance along keeps its infinite exit); a cube
     // reaching past the grid is not clipped: the ray then leaves the grid
     // inside the cube, and the cell reached below lies outside it
-    const R ex = s.cs[0] * fabs(w.ix), ey = s.cs[1] * fabs(w.iy), ez = s.cs[2] * fabs(w.iz);
-    const R bx = ex < INF ? mad<R>(hr, ex, w.tmx) : w.tmx;
-    const R by = ey < INF ? mad<R>(hr, ey, w.tmy) : w.tmy;
-    const R bz = ez < INF ? mad<R>(hr, ez, w.tmz) : w.tmz;
+    const R bx = mad<R>(hr, fabs(w.sx), w.tmx);
+    const R by = mad<R>(hr, fabs(w.sy), w.tmy);
+    const R bz = mad<R>(hr, fabs(w.sz), w.tmz);
     const int ax = (bx < by) ? (bx < bz ? 0 : 2) : (by < bz ? 1 : 2);
     const R tb = ax == 0 ? bx : (ax == 1 ? by : bz);
     const R px = mad<R>(dx, tb, ox), py = mad<R>(dy, tb, oy), pz = mad<R>(dz, tb, oz);
@@ -442,9 +448,9 @@ __device__ __forceinline__ bool walk_move(const DevScene<R> &s, Walk<R> &w, R ox
     const R da = ax == 0 ? dx : (ax == 1 ? dy : dz);
     w.from = h == 0 ? 2 * ax + (da > R(0) ? 0 : 1) : -1;
     w.cx = nx; w.cy = ny; w.cz = nz;
-    w.tmx = axis_exit<R>(ox, dx, w.ix, nx, s.gmin[0], s.cs[0]);
-    w.tmy = axis_exit<R>(oy, dy, w.iy, ny, s.gmin[1], s.cs[1]);
-    w.tmz = axis_exit<R>(oz, dz, w.iz, nz, s.gmin[2], s.cs[2]);
+    w.tmx = axis_exit<R>(w.sx, w.qx, nx);
+    w.tmy = axis_exit<R>(w.sy, w.qy, ny);
+    w.tmz = axis_exit<R>(w.sz, w.qz, nz);
     return false;
 }
 
@@ -563,14 +569,15 @@ __device__ bool walk_units(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R
 // mx_infl past its voxel, so a hit up to mx_infl beyond the exit is tested by
 // the time this voxel is done: once the best hit's upper bound is below
 // texit + infl - err no untested triangle can be nearer.
-__device__ __forceinline__ void mix_limits(const TraceConst &tc, Walk<float> &w)
+__device__ __forceinline__ void mix_limits(const DevScene<float> &s, const TraceConst &tc, Walk<float> &w)
 {
     const float texit = fminf(w.tmx, fminf(w.tmy, w.tmz));
-    // select values, not fields: a conditional over two fields is an lvalue, and
-    // loading through the selected address keeps the whole walk state in memory
-    const float ix = w.ix, iy = w.iy, iz = w.iz;
+    // |1 / d_a| = |s_a| / cs_a.  Select values, not fields: a conditional over
+    // two fields is an lvalue, and loading through the selected address keeps
+    // the whole walk state in memory
+    const float ix = fabsf(w.sx) * s.inv_cs[0], iy = fabsf(w.sy) * s.inv_cs[1], iz = fabsf(w.sz) * s.inv_cs[2];
     const float ia = w.tmx <= w.tmy ? (w.tmx <= w.tmz ? float(ix) : float(iz)) : (w.tmy <= w.tmz ? float(iy) : float(iz));
-    const float err = __builtin_fmaf(tc.mx_kx, fabsf(ia), 4.8e-7f * texit);
+    const float err = __builtin_fmaf(tc.mx_kx, ia, 4.8e-7f * texit);
     w.tend = texit + tc.mx_infl - err;
 }
 
@@ -606,7 +613,7 @@ __device__ bool walk_units_mix(const DevScene<float> &s, const TraceConst &tc, W
 {
     if (w.k < 0) {
         walk_enter<float, LDS>(s, w, n_vox);
-        mix_limits(tc, w);
+        mix_limits(s, tc, w);
     }
     for (int it = 0; it < budget; ++it) {
         const bool pending = walk_pending(w);
@@ -660,7 +667,7 @@ __device__ bool walk_units_mix(const DevScene<float> &s, const TraceConst &tc, W
         if (w.best >= 0 && w.tbest + w.eb <= w.tend) return true;
         if (walk_move<float>(s, w, ox, oy, oz, dx, dy, dz)) return true;
         walk_enter<float, LDS>(s, w, n_vox);
-        mix_limits(tc, w);
+        mix_limits(s, tc, w);
     }
     return false;
 }
@@ -1453,7 +1460,10 @@ __device__ __forceinline__ bool octave_shade(const TraceConst &tc, const DevScen
         lc.add(C_MISS);
         return false;
     }
-    r.t += (double)leg / rv.c;
+    // the FP64 path divides as the reference does; the FP32 path's legs carry
+    // FP32 rounding already, and a multiply saves the FP64 division
+    if constexpr (sizeof(R) == 8) r.t += (double)leg / rv.c;
+    else r.t += (double)leg * rv.inv_c;
     r.o[0] = hx; r.o[1] = hy; r.o[2] = hz;
     r.tri = ht;
     const R cos_t = N.x * r.d[0] + N.y * r.d[1] + N.z * r.d[2];
