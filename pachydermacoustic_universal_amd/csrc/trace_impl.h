@@ -34,8 +34,10 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
 {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
-        uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
-        uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+        // one 32 x 32 -> 64-bit product per pair (a single v_mad_u64_u32)
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
         uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
         c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
         k0 += 0x9E3779B9u;
@@ -913,17 +915,19 @@ __device__ void check_receivers(const DevRecv<R> &rv, const RoomTab &rm, uint64_
                                 R dy, R dz, R sqd, const R *E, R I, int band, int lo, int hi, double tsum, int ana,
                                 LaneCounters &lc)
 {
-    // voxel mask bits are receiver indices relative to the room's first receiver
+    // voxel mask bits are receiver indices relative to the room's first
+    // receiver.  The room's range is read once: the deposits' atomics would
+    // otherwise make every iteration read it again
+    const int base = rm.rec_base, count = rm.rec_count;
     if (rv.masked) {
-        cand &= rm.rec_count >= 64 ? ~0ull : ((1ull << rm.rec_count) - 1);
+        cand &= count >= 64 ? ~0ull : ((1ull << count) - 1);
         while (cand) {
             const int r = __builtin_ctzll(cand);
             cand &= cand - 1;
-            receiver_test<R, BROAD>(rv, rm.rec_base + r, ox, oy, oz, dx, dy, dz, sqd, E, I, band, lo, hi, tsum, ana,
-                                    lc);
+            receiver_test<R, BROAD>(rv, base + r, ox, oy, oz, dx, dy, dz, sqd, E, I, band, lo, hi, tsum, ana, lc);
         }
     } else {
-        for (int r = rm.rec_base; r < rm.rec_base + rm.rec_count; ++r)
+        for (int r = base; r < base + count; ++r)
             receiver_test<R, BROAD>(rv, r, ox, oy, oz, dx, dy, dz, sqd, E, I, band, lo, hi, tsum, ana, lc);
     }
 }
