@@ -181,6 +181,15 @@ int pach_grid_face_skip(const double *verts, int T, int dims[3], uint32_t *skip)
  * grid count as empty; at most 255), 0 for occupied voxels. */
 int pach_grid_octant_cubes(const int dims[3], const uint8_t *dist, uint8_t *oct);
 
+/* The walk's empty boxes over the same occupancy field (inspection and tests;
+ * no counterpart in the reference): ext [ncell][3][8]: for each empty voxel,
+ * axis a and ray octant o, the length E >= K (K = the octant cube's edge) of
+ * the longest empty box cornered at the voxel that extends K voxels along the
+ * other two axes and E along a in o's directions (voxels outside the grid
+ * count as empty; at most 255); 0 for occupied voxels.  A ray running fastest
+ * along a (the smallest |cell size / d|) crosses that box in one move. */
+int pach_grid_octant_boxes(const int dims[3], const uint8_t *dist, uint8_t *ext);
+
 /* Source origin, per-band power in W (Source.SoundPower, Classes_Sources.cs:96-104)
  * and direction mode.  dirs [n_dirs][3] only for PACH_DIR_SUPPLIED. */
 int pach_source_set(pach_ctx *ctx, const double origin[3], const double power_w[8],

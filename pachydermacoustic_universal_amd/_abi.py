@@ -117,6 +117,7 @@ SIGNATURES = [
                                   C.POINTER(C.c_uint32), C.POINTER(C.c_int64), C.POINTER(C.c_uint8)]),
     ("pach_grid_face_skip", C.c_int, [_dp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_uint32)]),
     ("pach_grid_octant_cubes", C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_uint8), C.POINTER(C.c_uint8)]),
+    ("pach_grid_octant_boxes", C.c_int, [C.POINTER(C.c_int), C.POINTER(C.c_uint8), C.POINTER(C.c_uint8)]),
     ("pach_source_set", C.c_int, [C.c_void_p, _dp, _dp, C.c_int, _dp, C.c_int64]),
     ("pach_receivers_set", C.c_int, [C.c_void_p, _dp, C.c_int, C.c_double, _dp, C.c_double,
                                      C.c_double, C.c_int, C.c_int]),

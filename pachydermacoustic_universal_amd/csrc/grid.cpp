@@ -7,7 +7,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <deque>
+#include <thread>
 
 namespace pach {
 
@@ -152,6 +154,81 @@ void octant_cubes(const int dims[3], const std::vector<uint8_t> &dist, std::vect
     }
 }
 
+void octant_boxes(const int dims[3], const std::vector<uint8_t> &dist, const std::vector<uint8_t> &oct,
+                  std::vector<uint8_t> &ext)
+{
+    const int gx = dims[0], gy = dims[1], gz = dims[2];
+    const size_t ncell = (size_t)gx * gy * gz;
+    // occupied voxels of every box [x0, x1) x [y0, y1) x [z0, z1) from one
+    // table of prefix sums
+    std::vector<int32_t> P((size_t)(gx + 1) * (gy + 1) * (gz + 1), 0);
+    auto at = [&](int x, int y, int z) -> int32_t & { return P[((size_t)z * (gy + 1) + y) * (gx + 1) + x]; };
+    for (int z = 1; z <= gz; ++z)
+        for (int y = 1; y <= gy; ++y)
+            for (int x = 1; x <= gx; ++x)
+                at(x, y, z) = (dist[((size_t)(z - 1) * gy + (y - 1)) * gx + (x - 1)] == 0) + at(x - 1, y, z) +
+                              at(x, y - 1, z) + at(x, y, z - 1) - at(x - 1, y - 1, z) - at(x - 1, y, z - 1) -
+                              at(x, y - 1, z - 1) + at(x - 1, y - 1, z - 1);
+    const int g[3] = {gx, gy, gz};
+    // the cells c .. c + s (n - 1) along one axis as a half-open range clipped to the grid
+    auto span = [&](int c, int s, int n, int a, int &lo, int &hi) {
+        lo = s > 0 ? c : c - n + 1;
+        hi = s > 0 ? c + n : c + 1;
+        lo = std::max(lo, 0);
+        hi = std::min(hi, g[a]);
+    };
+    auto empty = [&](const int c[3], const int s[3], const int n[3]) {
+        int lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) {
+            span(c[a], s[a], n[a], a, lo[a], hi[a]);
+            if (lo[a] >= hi[a]) return true;  // wholly outside the grid
+        }
+        const int32_t occ = at(hi[0], hi[1], hi[2]) - at(lo[0], hi[1], hi[2]) - at(hi[0], lo[1], hi[2]) -
+                            at(hi[0], hi[1], lo[2]) + at(lo[0], lo[1], hi[2]) + at(lo[0], hi[1], lo[2]) +
+                            at(hi[0], lo[1], lo[2]) - at(lo[0], lo[1], lo[2]);
+        return occ == 0;
+    };
+    ext.assign(ncell * 24, 0);
+    auto octant = [&](int o) {
+        const int s[3] = {(o & 1) ? -1 : 1, (o & 2) ? -1 : 1, (o & 4) ? -1 : 1};
+        for (int z = 0; z < gz; ++z)
+            for (int y = 0; y < gy; ++y)
+                for (int x = 0; x < gx; ++x) {
+                    const size_t c = ((size_t)z * gy + y) * gx + x;
+                    if (dist[c] == 0) continue;
+                    const int k = oct[8 * c + o], cc[3] = {x, y, z};
+                    for (int a = 0; a < 3; ++a) {
+                        // cells to the grid's edge along a, the box's reach past it
+                        const int edge = s[a] > 0 ? g[a] - cc[a] : cc[a] + 1;
+                        int e;
+                        if (k >= 255 || k >= edge) {
+                            e = 255;  // the cube already reaches past the grid along a
+                        } else {
+                            int n[3] = {k, k, k};
+                            n[a] = edge;
+                            if (empty(cc, s, n)) {
+                                e = 255;
+                            } else {
+                                // largest e in [k, edge) with an empty box: k holds, edge does not
+                                int lo = k, hi = edge;
+                                while (hi - lo > 1) {
+                                    const int m = (lo + hi) / 2;
+                                    n[a] = m;
+                                    if (empty(cc, s, n)) lo = m;
+                                    else hi = m;
+                                }
+                                e = lo;
+                            }
+                        }
+                        ext[24 * c + 8 * a + o] = (uint8_t)std::min(e, 255);
+                    }
+                }
+    };
+    std::vector<std::thread> th;
+    for (int o = 0; o < 8; ++o) th.emplace_back(octant, o);
+    for (auto &t : th) t.join();
+}
+
 void face_skip(const Grid &g, std::vector<uint32_t> &skip)
 {
     const int gx = g.dims[0], gy = g.dims[1], gz = g.dims[2];
@@ -182,21 +259,32 @@ void face_skip(const Grid &g, std::vector<uint32_t> &skip)
 }
 
 void cell_records(const Grid &g, const std::vector<uint32_t> &skip, const std::vector<uint64_t> &mask,
-                  const std::vector<uint8_t> &dist, const std::vector<uint8_t> &oct, std::vector<uint32_t> &rec)
+                  const std::vector<uint8_t> &dist, const std::vector<uint8_t> &oct, const std::vector<uint8_t> &ext,
+                  std::vector<uint32_t> &rec)
 {
     // word layout of one record: 0 first, 1 count, 2-7 skip masks by face,
-    // 8-9 receiver mask (low, high), 10-11 octant edges (bytes 0-7),
-    // 12 distance, 13-15 unused
+    // 8-9 receiver mask (low, high), 10-11 octant edges (bytes 0-7), 12
+    // distance, 13-15 unused.  An empty voxel has no list, skip masks or
+    // receivers: words 0-1, 2-3 and 8-9 hold its box lengths along x, y and z
+    // (ext [axis][octant], one byte per octant), so the words the walk loads
+    // anyway carry them
     const size_t ncell = (size_t)g.dims[0] * g.dims[1] * g.dims[2];
     rec.assign(ncell * 16, 0u);
     for (size_t c = 0; c < ncell; ++c) {
         uint32_t *r = rec.data() + 16 * c;
-        r[0] = g.ranges[2 * c];
-        r[1] = g.ranges[2 * c + 1];
-        for (int f = 0; f < 6; ++f) r[2 + f] = skip[6 * c + f];
-        const uint64_t m = mask.empty() ? 0 : mask[c];
-        r[8] = (uint32_t)m;
-        r[9] = (uint32_t)(m >> 32);
+        if (dist[c] == 0) {
+            r[0] = g.ranges[2 * c];
+            r[1] = g.ranges[2 * c + 1];
+            for (int f = 0; f < 6; ++f) r[2 + f] = skip[6 * c + f];
+            const uint64_t m = mask.empty() ? 0 : mask[c];
+            r[8] = (uint32_t)m;
+            r[9] = (uint32_t)(m >> 32);
+        } else {
+            static const int word[3] = {0, 2, 8};
+            for (int a = 0; a < 3; ++a)
+                for (int o = 0; o < 8; ++o)
+                    r[word[a] + o / 4] |= (uint32_t)ext[24 * c + 8 * a + o] << (8 * (o % 4));
+        }
         for (int o = 0; o < 8; ++o) r[10 + o / 4] |= (uint32_t)oct[8 * c + o] << (8 * (o % 4));
         r[12] = dist[c];
     }
@@ -316,5 +404,16 @@ extern "C" int pach_grid_octant_cubes(const int dims[3], const uint8_t *dist, ui
     std::vector<uint8_t> d(dist, dist + ncell), o;
     pach::octant_cubes(dims, d, o);
     std::copy(o.begin(), o.end(), oct);
+    return 0;
+}
+
+extern "C" int pach_grid_octant_boxes(const int dims[3], const uint8_t *dist, uint8_t *ext)
+{
+    if (!dims || !dist || !ext || dims[0] <= 0 || dims[1] <= 0 || dims[2] <= 0) return -1;
+    const size_t ncell = (size_t)dims[0] * dims[1] * dims[2];
+    std::vector<uint8_t> d(dist, dist + ncell), o, e;
+    pach::octant_cubes(dims, d, o);
+    pach::octant_boxes(dims, d, o, e);
+    std::copy(e.begin(), e.end(), ext);
     return 0;
 }

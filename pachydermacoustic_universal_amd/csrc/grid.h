@@ -32,6 +32,14 @@ void receiver_cells(const Grid &g, const double *centers, int R, const int *bit,
 // grid count as empty; capped at 255); 0 for occupied voxels.  A ray in
 // octant o inside the voxel crosses the whole cube in one move.
 void octant_cubes(const int dims[3], const std::vector<uint8_t> &dist, std::vector<uint8_t> &oct);
+// For each empty voxel, octant o and axis a, ext[24 voxel + 8 a + o] = the
+// length E >= K (K = oct[8 voxel + o]) of the longest box of empty voxels
+// cornered at the voxel, extending K voxels along the other two axes and E
+// along a, in o's directions (voxels outside the grid count as empty; capped
+// at 255); 0 for occupied voxels.  A ray that runs fastest along a crosses
+// the box in one move.
+void octant_boxes(const int dims[3], const std::vector<uint8_t> &dist, const std::vector<uint8_t> &oct,
+                  std::vector<uint8_t> &ext);
 // For each voxel and each of its faces f (0: -x, 1: +x, 2: -y, 3: +y, 4: -z,
 // 5: +z), skip[6 voxel + f] has bit i set when entry i (i < 32) of the voxel's
 // list is also listed in the neighbour across f: a walk that steps in through
@@ -39,10 +47,12 @@ void octant_cubes(const int dims[3], const std::vector<uint8_t> &dist, std::vect
 void face_skip(const Grid &g, std::vector<uint32_t> &skip);
 
 // The per-cell records of the flattened walks (trace_types.h CellRec), 16
-// uint32 words per cell: list range, face skip masks (face_skip), receiver
-// mask (mask empty: none), octant cube edges and distance.
+// uint32 words per cell: list range, face skip masks (face_skip; occupied
+// voxels) or box lengths (octant_boxes; empty voxels), receiver mask (mask
+// empty: none), octant cube edges and distance.
 void cell_records(const Grid &g, const std::vector<uint32_t> &skip, const std::vector<uint64_t> &mask,
-                  const std::vector<uint8_t> &dist, const std::vector<uint8_t> &oct, std::vector<uint32_t> &rec);
+                  const std::vector<uint8_t> &dist, const std::vector<uint8_t> &oct, const std::vector<uint8_t> &ext,
+                  std::vector<uint32_t> &rec);
 // dims_in: requested voxel counts (any <= 0: chosen from the triangle count)
 int build_grid(const double *verts, int T, const int dims_in[3], Grid &g, std::string &err);
 

@@ -434,8 +434,10 @@ int upload_grid(pach_ctx *c, const int dims[3])
     pach::face_skip(grid, skip);
     std::vector<uint8_t> oct;  // receiver voxels join the occupancy when receivers are set
     pach::octant_cubes(grid.dims, grid.dist, oct);
+    std::vector<uint8_t> ext;
+    pach::octant_boxes(grid.dims, grid.dist, oct, ext);
     std::vector<uint32_t> rec;
-    pach::cell_records(grid, skip, {}, grid.dist, oct, rec);
+    pach::cell_records(grid, skip, {}, grid.dist, oct, ext, rec);
     hipError_t e;
     if ((e = dalloc(c->d_cell_rec, rec.size() / 16)) != hipSuccess ||
         (e = hipMemcpy(c->d_cell_rec, rec.data(), rec.size() * 4, hipMemcpyHostToDevice)) != hipSuccess)
@@ -786,8 +788,10 @@ int refresh_receiver_cells(pach_ctx *c)
     pach::octant_cubes(c->g, dist, oct);
     HIPCHK(c, dalloc(c->d_cell_oct, oct.size()));
     HIPCHK(c, hipMemcpy(c->d_cell_oct, oct.data(), oct.size(), hipMemcpyHostToDevice));
+    std::vector<uint8_t> ext;
+    pach::octant_boxes(c->g, dist, oct, ext);
     std::vector<uint32_t> rec;
-    pach::cell_records(c->grid, c->cell_skip, c->rmask_on ? mask : std::vector<uint64_t>{}, dist, oct, rec);
+    pach::cell_records(c->grid, c->cell_skip, c->rmask_on ? mask : std::vector<uint64_t>{}, dist, oct, ext, rec);
     HIPCHK(c, dalloc(c->d_cell_rec, rec.size() / 16));
     HIPCHK(c, hipMemcpy(c->d_cell_rec, rec.data(), rec.size() * 4, hipMemcpyHostToDevice));
     c->rmask_dirty = false;

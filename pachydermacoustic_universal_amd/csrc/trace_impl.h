@@ -252,7 +252,8 @@ template <class R> struct Walk {
     uint32_t j, je;   // flattened walk: list window [j, j + 32) of the current voxel, list end je
     uint32_t todo;    // flattened walk: entries of the window still to test (bit i: entry j + i)
     int from;         // face of the current voxel the walk stepped in through (-1: none)
-    int k;            // flattened walk: distance value of the current voxel
+    int k;            // flattened walk: the current move's box edges minus one along x, y, z in
+                      // bytes 0-2 (0 in an occupied voxel); -1 before the first voxel
     // mixed walk only (unused, and removed by the compiler, elsewhere)
     R eb;             // error bound of tbest
     R sl, sl2;        // least and second least lower bound t - e of the other candidates (-inf: undecidable)
@@ -374,8 +375,10 @@ __device__ bool walk_steps(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R
     return false;
 }
 
-// Enter the walk's current voxel: its distance value, and for an occupied
-// voxel its triangle-list range and receiver mask.
+// Enter the walk's current voxel: for an occupied voxel its triangle-list
+// range and receiver mask, for an empty one the empty box ahead of the ray:
+// the octant cube's edge K on two axes and, on the axis the ray runs fastest
+// along in voxels (the smallest |s_a|), the box length E >= K.
 template <class R, bool LDS>
 __device__ __forceinline__ void walk_enter(const DevScene<R> &s, Walk<R> &w, uint32_t &n_vox)
 {
@@ -400,46 +403,57 @@ __device__ __forceinline__ void walk_enter(const DevScene<R> &s, Walk<R> &w, uin
                   ((__float_as_uint(w.sz) >> 29) & 4u));
     else
         o = (w.sx < R(0) ? 1 : 0) | (w.sy < R(0) ? 2 : 0) | (w.sz < R(0) ? 4 : 0);
+    const R ax = fabs(w.sx), ay = fabs(w.sy), az = fabs(w.sz);
+    const int dom = (ax <= ay && ax <= az) ? 0 : (ay <= az ? 1 : 2);
+    // an empty voxel's box lengths along x, y, z sit in head.xy, head.zw, tail.xy
+    const uint32_t blo = dom == 0 ? head.x : (dom == 1 ? head.z : tail.x);
+    const uint32_t bhi = dom == 0 ? head.y : (dom == 1 ? head.w : tail.y);
+    const uint32_t elong = ((o < 4 ? blo : bhi) >> (8 * (o & 3))) & 0xffu;
     const uint32_t edge = ((o < 4 ? tail.z : tail.w) >> (8 * (o & 3))) & 0xffu;
     const bool occ = k == 0;
-    w.k = occ ? 0 : (int)edge;
+    // the box edges minus one per axis: 0 in an occupied voxel (one step)
+    w.k = occ ? 0 : (int)((edge - 1u) * 0x010101u + ((elong - edge) << (8 * dom)));
     if (s.cell_rmask) w.rmask |= occ ? (((uint64_t)tail.y << 32) | tail.x) : 0ull;
     w.j = occ ? head.x : 0u;
     w.je = occ ? head.x + head.y : 0u;
     const uint32_t all = head.y >= 32 ? ~0u : (1u << head.y) - 1u;
     w.todo = occ ? all & ~seen : 0u;
+    // every word is needed on every path: without this the compiler moves the
+    // loads of the words only an empty voxel uses behind the occupancy test,
+    // a second round trip
+    asm volatile("" ::"v"(head.z), "v"(head.w), "v"(tail.z), "v"(tail.w));
 }
 
-// One move of the flattened walk: across the empty cube ahead of the ray
-// (edge k >= 2, cornered at the voxel and extending along the ray's octant),
-// else one voxel (the cube is the voxel itself), by the same straight-line
-// code for both so stepping and jumping lanes do not diverge.  Returns true
-// when the walk leaves the grid.
+// One move of the flattened walk: across the empty box ahead of the ray
+// (edges from w.k, cornered at the voxel and extending along the ray's
+// octant), else one voxel (the box is the voxel itself), by the same
+// straight-line code for both so stepping and jumping lanes do not diverge.
+// Returns true when the walk leaves the grid.
 template <class R>
 __device__ __forceinline__ bool walk_move(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R dx, R dy, R dz)
 {
-    const int h = max(w.k - 1, 0);
-    const R hr = R(h);
-    // the cube's far faces lie h voxel widths past the current voxel's exits
-    // (an axis the ray does not advance along keeps its infinite exit); a cube
+    // voxels the box reaches past the current one per axis
+    const int hx = w.k & 0xff, hy = (w.k >> 8) & 0xff, hz = (w.k >> 16) & 0xff;
+    // the box's far faces lie h voxel widths past the current voxel's exits
+    // (an axis the ray does not advance along keeps its infinite exit); a box
     // reaching past the grid is not clipped: the ray then leaves the grid
-    // inside the cube, and the cell reached below lies outside it
-    const R bx = mad<R>(hr, fabs(w.sx), w.tmx);
-    const R by = mad<R>(hr, fabs(w.sy), w.tmy);
-    const R bz = mad<R>(hr, fabs(w.sz), w.tmz);
+    // inside the box, and the cell reached below lies outside it
+    const R bx = mad<R>(R(hx), fabs(w.sx), w.tmx);
+    const R by = mad<R>(R(hy), fabs(w.sy), w.tmy);
+    const R bz = mad<R>(R(hz), fabs(w.sz), w.tmz);
     const int ax = (bx < by) ? (bx < bz ? 0 : 2) : (by < bz ? 1 : 2);
     const R tb = ax == 0 ? bx : (ax == 1 ? by : bz);
     const R px = mad<R>(dx, tb, ox), py = mad<R>(dy, tb, oy), pz = mad<R>(dz, tb, oz);
     const int fx = (int)floor((px - s.gmin[0]) * s.inv_cs[0]);
     const int fy = (int)floor((py - s.gmin[1]) * s.inv_cs[1]);
     const int fz = (int)floor((pz - s.gmin[2]) * s.inv_cs[2]);
-    // never move backwards; stay inside the cube on the other axes
+    // never move backwards; stay inside the box on the other axes
     // (clamp to [c, c + h] moving up, [c - h, c] moving down, c when still);
     // across the exit face on the exit axis
-    int nx = min(max(fx, dx < R(0) ? w.cx - h : w.cx), dx > R(0) ? w.cx + h : w.cx);
-    int ny = min(max(fy, dy < R(0) ? w.cy - h : w.cy), dy > R(0) ? w.cy + h : w.cy);
-    int nz = min(max(fz, dz < R(0) ? w.cz - h : w.cz), dz > R(0) ? w.cz + h : w.cz);
-    const int sx = dx > R(0) ? h + 1 : -h - 1, sy = dy > R(0) ? h + 1 : -h - 1, sz = dz > R(0) ? h + 1 : -h - 1;
+    int nx = min(max(fx, dx < R(0) ? w.cx - hx : w.cx), dx > R(0) ? w.cx + hx : w.cx);
+    int ny = min(max(fy, dy < R(0) ? w.cy - hy : w.cy), dy > R(0) ? w.cy + hy : w.cy);
+    int nz = min(max(fz, dz < R(0) ? w.cz - hz : w.cz), dz > R(0) ? w.cz + hz : w.cz);
+    const int sx = dx > R(0) ? hx + 1 : -hx - 1, sy = dy > R(0) ? hy + 1 : -hy - 1, sz = dz > R(0) ? hz + 1 : -hz - 1;
     nx = ax == 0 ? w.cx + sx : nx;
     ny = ax == 1 ? w.cy + sy : ny;
     nz = ax == 2 ? w.cz + sz : nz;
@@ -448,7 +462,7 @@ __device__ __forceinline__ bool walk_move(const DevScene<R> &s, Walk<R> &w, R ox
     // a single step enters through the face towards the voxel left (-x when
     // moving in +x); a jump leaves no neighbour whose list was tested
     const R da = ax == 0 ? dx : (ax == 1 ? dy : dz);
-    w.from = h == 0 ? 2 * ax + (da > R(0) ? 0 : 1) : -1;
+    w.from = w.k == 0 ? 2 * ax + (da > R(0) ? 0 : 1) : -1;
     w.cx = nx; w.cy = ny; w.cz = nz;
     w.tmx = axis_exit<R>(w.sx, w.qx, nx);
     w.tmy = axis_exit<R>(w.sy, w.qy, ny);

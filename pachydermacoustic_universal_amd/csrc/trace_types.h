@@ -18,7 +18,11 @@ constexpr int kMaxLdsDist = 32768;  // voxel-distance bytes staged in LDS per wo
 // came in through: grid.h face_skip), the receivers whose spheres touch the
 // voxel, the edge of the largest empty cube cornered at the voxel per ray
 // octant (grid.h octant_cubes) and the L-inf distance to the nearest occupied
-// voxel (receiver voxels count as occupied when receiver masks are on).
+// voxel (receiver voxels count as occupied when receiver masks are on).  An
+// empty voxel has no list, skip masks or receivers: its box lengths per axis
+// and octant (grid.h octant_boxes; one byte per octant) lie in first and
+// count (x), skip[0] and skip[1] (y) and rmask (z), words the walk loads for
+// every voxel.
 struct alignas(64) CellRec {
     uint32_t first, count;
     uint32_t skip[6];

@@ -152,6 +152,58 @@ def test_octant_cubes_are_the_largest_empty_cubes(seed, dims, p):
                         assert empty(x, y, z, max(dims) + 1, s)
 
 
+@pytest.mark.parametrize("seed,dims,p", [(0, (9, 7, 5), 0.08), (1, (6, 11, 4), 0.2), (2, (5, 5, 5), 0.0),
+                                         (3, (12, 4, 6), 0.05)])
+def test_octant_boxes_are_the_longest_empty_boxes(seed, dims, p):
+    """ext[v][a][o] is the largest E >= K (K = the octant cube's edge) whose
+    box (corner v, E voxels along axis a and K along the other two, in o's
+    directions, voxels outside the grid empty) holds no occupied voxel,
+    checked by brute force; 0 on occupied voxels; 255 when the box never
+    meets one.  A slip in the axis or octant order, or a box one voxel too
+    long, fails here."""
+    rng = np.random.default_rng(seed)
+    gx, gy, gz = dims
+    occ = rng.random((gz, gy, gx)) < p
+    dist = np.where(occ, 0, 1).astype(np.uint8).ravel()
+    u8 = C.POINTER(C.c_uint8)
+    d = (C.c_int * 3)(*dims)
+    oct_ = np.zeros(dist.size * 8, np.uint8)
+    ext = np.zeros(dist.size * 24, np.uint8)
+    assert A.lib().pach_grid_octant_cubes(d, dist.ctypes.data_as(u8), oct_.ctypes.data_as(u8)) == 0
+    assert A.lib().pach_grid_octant_boxes(d, dist.ctypes.data_as(u8), ext.ctypes.data_as(u8)) == 0
+    oct_ = oct_.reshape(gz, gy, gx, 8)
+    ext = ext.reshape(gz, gy, gx, 3, 8)
+
+    def empty(x, y, z, n, s):
+        xs = [x + s[0] * i for i in range(n[0])]
+        ys = [y + s[1] * i for i in range(n[1])]
+        zs = [z + s[2] * i for i in range(n[2])]
+        return not any(occ[c, b, a] for c in zs if 0 <= c < gz for b in ys if 0 <= b < gy
+                       for a in xs if 0 <= a < gx)
+    longer = 0
+    for z in range(gz):
+        for y in range(gy):
+            for x in range(gx):
+                for o in range(8):
+                    s = (-1 if o & 1 else 1, -1 if o & 2 else 1, -1 if o & 4 else 1)
+                    K = int(oct_[z, y, x, o])
+                    for a in range(3):
+                        E = int(ext[z, y, x, a, o])
+                        if occ[z, y, x]:
+                            assert E == 0
+                            continue
+                        assert E >= K
+                        longer += E > K
+                        n = [min(K, max(dims) + 1)] * 3
+                        n[a] = min(E, max(dims) + 1)
+                        assert empty(x, y, z, n, s), (x, y, z, a, o, K, E)
+                        if E < 255:
+                            n[a] = E + 1
+                            assert not empty(x, y, z, n, s), (x, y, z, a, o, K, E)
+    if p > 0:
+        assert longer > 0  # boxes do reach past the cubes
+
+
 @pytest.mark.parametrize("room_fn,dims", [(rooms.shoebox, (0, 0, 0)), (rooms.fan_hall, (32, 32, 32)),
                                           (rooms.fan_hall, (7, 5, 3))])
 def test_face_skip_masks_mark_entries_listed_across_each_face(room_fn, dims):
