@@ -156,6 +156,9 @@ __device__ __forceinline__ Rows load_rows(const DevScene<float> &s, uint32_t j)
     return Rows{rows[0], rows[1], rows[2]};
 }
 
+// LIM false: no limit on t (the caller compares t with its nearest hit
+// itself), and tout is set whether or not the ray hits.
+template <bool LIM = true>
 __device__ __forceinline__ bool tri_hit_rows(const Rows &q, float ox, float oy, float oz, float dx, float dy,
                                              float dz, float tlim, float &tout)
 {
@@ -166,9 +169,10 @@ __device__ __forceinline__ bool tri_hit_rows(const Rows &q, float ox, float oy, 
     const float px = ox + dx * tt, py = oy + dy * tt, pz = oz + dz * tt;
     const float u = r0.x * px + r0.y * py + r0.z * pz + r0.w;
     const float v = r1.x * px + r1.y * py + r1.z * pz + r1.w;
-    const bool ok = (tt > 0.0f) & (tt <= tlim) & !(u < -4e-6f) & !(u > 1.000004f) & !(v < -4e-6f) &
+    const bool ok = (tt > 0.0f) & (!LIM || tt <= tlim) & !(u < -4e-6f) & !(u > 1.000004f) & !(v < -4e-6f) &
                     !(u + v > 1.000004f);
-    tout = ok ? tt : tout;
+    if constexpr (LIM) tout = ok ? tt : tout;
+    else tout = tt;
     return ok;
 }
 
@@ -525,9 +529,11 @@ __device__ bool walk_units(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R
             // any hit nearer than the best so far, inside this voxel or not
             const R tlim = w.tbest;
             // the tests first: their row loads then need not wait for the index loads
-            // as selects, not a branch: a branch would defer the id loads behind the tests
+            // as selects, not a branch: a branch would defer the id loads behind
+            // the tests.  The best triangle tested again gives its own t, which
+            // neither order test takes
             auto take = [&](bool hit, int t, R tt) __attribute__((always_inline)) {
-                const bool nearer = hit & (t != w.best) & !(Tmin<R>::excl && t == excl) &
+                const bool nearer = hit & !(Tmin<R>::excl && t == excl) &
                                     ((tt < w.tbest) | ((tt == w.tbest) & (t < w.best)));
                 w.tbest = nearer ? tt : w.tbest;
                 w.best = nearer ? t : w.best;
@@ -547,9 +553,10 @@ __device__ bool walk_units(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R
                 if constexpr (sizeof(R) == 4) {
                     // every row load of the unit first, then the three tests
                     const Rows q0 = load_rows(s, j0), q1 = load_rows(s, j1), q2 = load_rows(s, j2);
-                    h0 = tri_hit_rows(q0, ox, oy, oz, dx, dy, dz, tlim, tt0);
-                    h1 = tri_hit_rows(q1, ox, oy, oz, dx, dy, dz, tlim, tt1);
-                    h2 = tri_hit_rows(q2, ox, oy, oz, dx, dy, dz, tlim, tt2);
+                    // unlimited: take() keeps a hit only if it is nearer than the best so far
+                    h0 = tri_hit_rows<false>(q0, ox, oy, oz, dx, dy, dz, tlim, tt0);
+                    h1 = tri_hit_rows<false>(q1, ox, oy, oz, dx, dy, dz, tlim, tt1);
+                    h2 = tri_hit_rows<false>(q2, ox, oy, oz, dx, dy, dz, tlim, tt2);
                 } else {
                     h0 = tri_hit<R>(s, t0, j0, ox, oy, oz, dx, dy, dz, tlim, tt0);
                     h1 = tri_hit<R>(s, t1, j1, ox, oy, oz, dx, dy, dz, tlim, tt1);
