@@ -182,12 +182,15 @@ int pach_grid_face_skip(const double *verts, int T, int dims[3], uint32_t *skip)
 int pach_grid_octant_cubes(const int dims[3], const uint8_t *dist, uint8_t *oct);
 
 /* The walk's empty boxes over the same occupancy field (inspection and tests;
- * no counterpart in the reference): ext [ncell][3][8]: for each empty voxel,
- * axis a and ray octant o, the length E >= K (K = the octant cube's edge) of
- * the longest empty box cornered at the voxel that extends K voxels along the
- * other two axes and E along a in o's directions (voxels outside the grid
- * count as empty; at most 255); 0 for occupied voxels.  A ray running fastest
- * along a (the smallest |cell size / d|) crosses that box in one move. */
+ * no counterpart in the reference): ext [ncell][3][8]: for each empty voxel
+ * and ray octant o, the box a ray in octant o crosses in one move, cornered at
+ * the voxel and extending in o's directions (voxels outside the grid count as
+ * empty; lengths at most 255).  With K the octant cube's edge and E_a the
+ * length of the longest empty box K long on the two axes other than a: first
+ * axis A = the a with the largest E_a (the lowest on a tie), second axis B =
+ * the longer of the other two (the one after A on a tie), E2 = the longest
+ * empty length on B with A at E_A and the third axis at K.  ext[v][0][o] =
+ * E_A, ext[v][1][o] = E2, ext[v][2][o] = A | B << 2; 0 for occupied voxels. */
 int pach_grid_octant_boxes(const int dims[3], const uint8_t *dist, uint8_t *ext);
 
 /* Source origin, per-band power in W (Source.SoundPower, Classes_Sources.cs:96-104)

@@ -197,31 +197,43 @@ void octant_boxes(const int dims[3], const std::vector<uint8_t> &dist, const std
                     const size_t c = ((size_t)z * gy + y) * gx + x;
                     if (dist[c] == 0) continue;
                     const int k = oct[8 * c + o], cc[3] = {x, y, z};
-                    for (int a = 0; a < 3; ++a) {
+                    // the longest empty box along axis a, the box n[] holding on the
+                    // other axes (n[a] = k holds: a sub-box of an empty box)
+                    auto longest = [&](int a, int n[3]) {
                         // cells to the grid's edge along a, the box's reach past it
                         const int edge = s[a] > 0 ? g[a] - cc[a] : cc[a] + 1;
-                        int e;
-                        if (k >= 255 || k >= edge) {
-                            e = 255;  // the cube already reaches past the grid along a
-                        } else {
-                            int n[3] = {k, k, k};
-                            n[a] = edge;
-                            if (empty(cc, s, n)) {
-                                e = 255;
-                            } else {
-                                // largest e in [k, edge) with an empty box: k holds, edge does not
-                                int lo = k, hi = edge;
-                                while (hi - lo > 1) {
-                                    const int m = (lo + hi) / 2;
-                                    n[a] = m;
-                                    if (empty(cc, s, n)) lo = m;
-                                    else hi = m;
-                                }
-                                e = lo;
-                            }
+                        if (k >= 255 || k >= edge) return 255;  // the cube already reaches past the grid
+                        n[a] = edge;
+                        if (empty(cc, s, n)) return 255;
+                        // largest length in [k, edge) with an empty box: k holds, edge does not
+                        int lo = k, hi = edge;
+                        while (hi - lo > 1) {
+                            const int m = (lo + hi) / 2;
+                            n[a] = m;
+                            if (empty(cc, s, n)) lo = m;
+                            else hi = m;
                         }
-                        ext[24 * c + 8 * a + o] = (uint8_t)std::min(e, 255);
+                        return lo;
+                    };
+                    int e[3];
+                    for (int a = 0; a < 3; ++a) {
+                        int n[3] = {k, k, k};
+                        e[a] = longest(a, n);
                     }
+                    // the box: the axis with the longest single-axis box (the lowest
+                    // axis on a tie), then the longer of the other two (the next
+                    // axis on a tie) lengthened with the first at its length
+                    int dom = 0;
+                    for (int a = 1; a < 3; ++a)
+                        if (e[a] > e[dom]) dom = a;
+                    const int a1 = (dom + 1) % 3, a2 = (dom + 2) % 3;
+                    const int sec = e[a1] >= e[a2] ? a1 : a2;
+                    int n[3] = {k, k, k};
+                    n[dom] = e[dom];
+                    const int e2 = longest(sec, n);
+                    ext[24 * c + o] = (uint8_t)e[dom];
+                    ext[24 * c + 8 + o] = (uint8_t)e2;
+                    ext[24 * c + 16 + o] = (uint8_t)(dom | sec << 2);
                 }
     };
     std::vector<std::thread> th;
@@ -265,9 +277,9 @@ void cell_records(const Grid &g, const std::vector<uint32_t> &skip, const std::v
     // word layout of one record: 0 first, 1 count, 2-7 skip masks by face,
     // 8-9 receiver mask (low, high), 10-11 octant edges (bytes 0-7), 12
     // distance, 13-15 unused.  An empty voxel has no list, skip masks or
-    // receivers: words 0-1, 2-3 and 8-9 hold its box lengths along x, y and z
-    // (ext [axis][octant], one byte per octant), so the words the walk loads
-    // anyway carry them
+    // receivers: words 0-1, 2-3 and 8-9 hold its boxes (ext [plane][octant],
+    // one byte per octant: first axis length, second axis length, axes), so
+    // the words the walk loads anyway carry them
     const size_t ncell = (size_t)g.dims[0] * g.dims[1] * g.dims[2];
     rec.assign(ncell * 16, 0u);
     for (size_t c = 0; c < ncell; ++c) {

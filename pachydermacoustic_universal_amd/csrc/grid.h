@@ -32,12 +32,16 @@ void receiver_cells(const Grid &g, const double *centers, int R, const int *bit,
 // grid count as empty; capped at 255); 0 for occupied voxels.  A ray in
 // octant o inside the voxel crosses the whole cube in one move.
 void octant_cubes(const int dims[3], const std::vector<uint8_t> &dist, std::vector<uint8_t> &oct);
-// For each empty voxel, octant o and axis a, ext[24 voxel + 8 a + o] = the
-// length E >= K (K = oct[8 voxel + o]) of the longest box of empty voxels
-// cornered at the voxel, extending K voxels along the other two axes and E
-// along a, in o's directions (voxels outside the grid count as empty; capped
-// at 255); 0 for occupied voxels.  A ray that runs fastest along a crosses
-// the box in one move.
+// For each empty voxel and octant o, the empty box a ray in octant o crosses
+// in one move, cornered at the voxel and extending in o's directions (voxels
+// outside the grid count as empty; lengths capped at 255): with K =
+// oct[8 voxel + o] and E_a the length of the longest empty box that is K long
+// on the two axes other than a, the first axis A is the one with the largest
+// E_a (the lowest on a tie), the second B the longer of the other two (the
+// next after A on a tie); the box is E_A long on A, E2 (the longest empty
+// length on B with A at E_A and the third axis at K) on B and K on the third.
+// ext[24 voxel + o] = E_A, ext[24 voxel + 8 + o] = E2, ext[24 voxel + 16 + o]
+// = A | B << 2; all 0 for occupied voxels.
 void octant_boxes(const int dims[3], const std::vector<uint8_t> &dist, const std::vector<uint8_t> &oct,
                   std::vector<uint8_t> &ext);
 // For each voxel and each of its faces f (0: -x, 1: +x, 2: -y, 3: +y, 4: -z,

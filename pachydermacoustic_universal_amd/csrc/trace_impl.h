@@ -380,9 +380,8 @@ __device__ bool walk_steps(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R
 }
 
 // Enter the walk's current voxel: for an occupied voxel its triangle-list
-// range and receiver mask, for an empty one the empty box ahead of the ray:
-// the octant cube's edge K on two axes and, on the axis the ray runs fastest
-// along in voxels (the smallest |s_a|), the box length E >= K.
+// range and receiver mask, for an empty one the empty box ahead of the ray in
+// its octant (the octant cube lengthened along two axes, grid.h octant_boxes).
 template <class R, bool LDS>
 __device__ __forceinline__ void walk_enter(const DevScene<R> &s, Walk<R> &w, uint32_t &n_vox)
 {
@@ -407,16 +406,19 @@ __device__ __forceinline__ void walk_enter(const DevScene<R> &s, Walk<R> &w, uin
                   ((__float_as_uint(w.sz) >> 29) & 4u));
     else
         o = (w.sx < R(0) ? 1 : 0) | (w.sy < R(0) ? 2 : 0) | (w.sz < R(0) ? 4 : 0);
-    const R ax = fabs(w.sx), ay = fabs(w.sy), az = fabs(w.sz);
-    const int dom = (ax <= ay && ax <= az) ? 0 : (ay <= az ? 1 : 2);
-    // an empty voxel's box lengths along x, y, z sit in head.xy, head.zw, tail.xy
-    const uint32_t blo = dom == 0 ? head.x : (dom == 1 ? head.z : tail.x);
-    const uint32_t bhi = dom == 0 ? head.y : (dom == 1 ? head.w : tail.y);
-    const uint32_t elong = ((o < 4 ? blo : bhi) >> (8 * (o & 3))) & 0xffu;
-    const uint32_t edge = ((o < 4 ? tail.z : tail.w) >> (8 * (o & 3))) & 0xffu;
+    // an empty voxel's box in this octant (grid.h octant_boxes): its first
+    // axis length in head.xy, its second axis length in head.zw, the two axes
+    // in tail.xy, one byte per octant; the cube's edge in tail.zw
+    const bool hi4 = o >= 4;
+    const uint32_t sh = 8u * (uint32_t)(o & 3);
+    const uint32_t la = ((hi4 ? head.y : head.x) >> sh) & 0xffu;
+    const uint32_t lb = ((hi4 ? head.w : head.z) >> sh) & 0xffu;
+    const uint32_t axes = ((hi4 ? tail.y : tail.x) >> sh) & 0xffu;
+    const uint32_t edge = ((hi4 ? tail.w : tail.z) >> sh) & 0xffu;
+    const uint32_t a0 = axes & 3u, a1 = (axes >> 2) & 3u;
     const bool occ = k == 0;
     // the box edges minus one per axis: 0 in an occupied voxel (one step)
-    w.k = occ ? 0 : (int)((edge - 1u) * 0x010101u + ((elong - edge) << (8 * dom)));
+    w.k = occ ? 0 : (int)((edge - 1u) * 0x010101u + ((la - edge) << (8u * a0)) + ((lb - edge) << (8u * a1)));
     if (s.cell_rmask) w.rmask |= occ ? (((uint64_t)tail.y << 32) | tail.x) : 0ull;
     w.j = occ ? head.x : 0u;
     w.je = occ ? head.x + head.y : 0u;

@@ -19,10 +19,10 @@ constexpr int kMaxLdsDist = 32768;  // voxel-distance bytes staged in LDS per wo
 // voxel, the edge of the largest empty cube cornered at the voxel per ray
 // octant (grid.h octant_cubes) and the L-inf distance to the nearest occupied
 // voxel (receiver voxels count as occupied when receiver masks are on).  An
-// empty voxel has no list, skip masks or receivers: its box lengths per axis
-// and octant (grid.h octant_boxes; one byte per octant) lie in first and
-// count (x), skip[0] and skip[1] (y) and rmask (z), words the walk loads for
-// every voxel.
+// empty voxel has no list, skip masks or receivers: its boxes per octant
+// (grid.h octant_boxes; one byte per octant) lie in first and count (first
+// axis length), skip[0] and skip[1] (second axis length) and rmask (the two
+// axes), words the walk loads for every voxel.
 struct alignas(64) CellRec {
     uint32_t first, count;
     uint32_t skip[6];

@@ -155,12 +155,13 @@ def test_octant_cubes_are_the_largest_empty_cubes(seed, dims, p):
 @pytest.mark.parametrize("seed,dims,p", [(0, (9, 7, 5), 0.08), (1, (6, 11, 4), 0.2), (2, (5, 5, 5), 0.0),
                                          (3, (12, 4, 6), 0.05)])
 def test_octant_boxes_are_the_longest_empty_boxes(seed, dims, p):
-    """ext[v][a][o] is the largest E >= K (K = the octant cube's edge) whose
-    box (corner v, E voxels along axis a and K along the other two, in o's
-    directions, voxels outside the grid empty) holds no occupied voxel,
-    checked by brute force; 0 on occupied voxels; 255 when the box never
-    meets one.  A slip in the axis or octant order, or a box one voxel too
-    long, fails here."""
+    """For each empty voxel and octant: E_a, the longest empty box K long on
+    the two axes other than a (K = the octant cube's edge), by brute force;
+    the box's first axis A is the a with the largest E_a (lowest on a tie),
+    its second B the longer of the other two (the one after A on a tie), and
+    its B length the longest empty one with A at E_A and the third axis at K
+    -- the box empty and one voxel more on A or B not.  A slip in the axis or
+    octant order, a wrong tie rule or a box one voxel too long fails here."""
     rng = np.random.default_rng(seed)
     gx, gy, gz = dims
     occ = rng.random((gz, gy, gx)) < p
@@ -173,33 +174,49 @@ def test_octant_boxes_are_the_longest_empty_boxes(seed, dims, p):
     assert A.lib().pach_grid_octant_boxes(d, dist.ctypes.data_as(u8), ext.ctypes.data_as(u8)) == 0
     oct_ = oct_.reshape(gz, gy, gx, 8)
     ext = ext.reshape(gz, gy, gx, 3, 8)
+    big = max(dims) + 1
 
     def empty(x, y, z, n, s):
-        xs = [x + s[0] * i for i in range(n[0])]
-        ys = [y + s[1] * i for i in range(n[1])]
-        zs = [z + s[2] * i for i in range(n[2])]
+        xs = [x + s[0] * i for i in range(min(n[0], big))]
+        ys = [y + s[1] * i for i in range(min(n[1], big))]
+        zs = [z + s[2] * i for i in range(min(n[2], big))]
         return not any(occ[c, b, a] for c in zs if 0 <= c < gz for b in ys if 0 <= b < gy
                        for a in xs if 0 <= a < gx)
+
+    def longest(x, y, z, s, n, a, K):
+        m = list(n)
+        L = K
+        while L < 255:
+            m[a] = L + 1
+            if not empty(x, y, z, m, s):
+                return L
+            if L + 1 >= big:
+                return 255
+            L += 1
+        return 255
     longer = 0
     for z in range(gz):
         for y in range(gy):
             for x in range(gx):
                 for o in range(8):
+                    ea, eb, ax = (int(v) for v in ext[z, y, x, :, o])
+                    if occ[z, y, x]:
+                        assert (ea, eb, ax) == (0, 0, 0)
+                        continue
                     s = (-1 if o & 1 else 1, -1 if o & 2 else 1, -1 if o & 4 else 1)
                     K = int(oct_[z, y, x, o])
-                    for a in range(3):
-                        E = int(ext[z, y, x, a, o])
-                        if occ[z, y, x]:
-                            assert E == 0
-                            continue
-                        assert E >= K
-                        longer += E > K
-                        n = [min(K, max(dims) + 1)] * 3
-                        n[a] = min(E, max(dims) + 1)
-                        assert empty(x, y, z, n, s), (x, y, z, a, o, K, E)
-                        if E < 255:
-                            n[a] = E + 1
-                            assert not empty(x, y, z, n, s), (x, y, z, a, o, K, E)
+                    E = [longest(x, y, z, s, [K, K, K], a, K) for a in range(3)]
+                    A_ = max(range(3), key=lambda a: (E[a], -a))
+                    a1, a2 = (A_ + 1) % 3, (A_ + 2) % 3
+                    B_ = a1 if E[a1] >= E[a2] else a2
+                    assert ax == A_ | B_ << 2, (x, y, z, o, E, ax)
+                    assert ea == E[A_], (x, y, z, o, E, ea)
+                    n = [K, K, K]
+                    n[A_] = E[A_]
+                    assert eb == longest(x, y, z, s, n, B_, K), (x, y, z, o, eb)
+                    n[B_] = eb
+                    assert empty(x, y, z, n, s)
+                    longer += (ea > K) + (eb > K)
     if p > 0:
         assert longer > 0  # boxes do reach past the cubes
 
