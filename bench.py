@@ -222,10 +222,15 @@ def main():
         # a long trace: run it asynchronously and report completed source rays
         tr.trace(rays=trace_rays, cutoff_m=args.cutoff * c, seed=seed, precision=prec, batch_rays=args.batch_rays,
                  async_=True)
+        # polled every 20 ms so the timed region ends when the trace does,
+        # reported every `progress` seconds
+        t_rep = time.perf_counter()
         while tr.poll()["running"]:
-            time.sleep(args.progress)
-            p = tr.poll()
-            print(f"[bench] {p['rays_done']} of {p['rays_total']} source rays done", file=sys.stderr, flush=True)
+            time.sleep(0.02)
+            if time.perf_counter() - t_rep >= args.progress:
+                t_rep = time.perf_counter()
+                p = tr.poll()
+                print(f"[bench] {p['rays_done']} of {p['rays_total']} source rays done", file=sys.stderr, flush=True)
         rc = tr.wait()
         if rc != 0:
             raise A.PachError(rc, A.lib().pach_last_error(tr.h).decode())
