@@ -568,11 +568,16 @@ __device__ bool walk_units(const DevScene<R> &s, Walk<R> &w, R ox, R oy, R oz, R
                 take(h1, t1, tt1);
                 take(h2, t2, tt2);
             }
+            // the voxel's last entries tested and the nearest hit inside it:
+            // the walk ends here rather than in the next move unit
+            if (!walk_pending(w) && w.best >= 0 && w.tbest <= texit) return true;
             continue;
         }
         if (w.best >= 0 && w.tbest <= texit) return true;
         if (walk_move<R>(s, w, ox, oy, oz, dx, dy, dz)) return true;
         walk_enter<R, LDS>(s, w, n_vox);
+        // nothing to test here and the nearest hit before this voxel's exit
+        if (!walk_pending(w) && w.best >= 0 && w.tbest <= fmin(w.tmx, fmin(w.tmy, w.tmz))) return true;
     }
     return false;
 }
