@@ -117,6 +117,8 @@ struct pach_ctx {
     int map_ct[3] = {1, 1, 1};
     double4 *d_map_ctr = nullptr;
     uint32_t *d_map_start = nullptr, *d_map_list = nullptr;
+    float4 *d_map_lctr = nullptr;  // per list entry: FP32 centre, w = receiver index (bits)
+    float map_scr = 0;             // FP32 screen widening for the rounding of d_map_lctr
     double *d_map_hist = nullptr;
     size_t map_hist_n = 0;  // per plane
 
@@ -378,7 +380,7 @@ extern "C" void pach_destroy(pach_ctx *c)
     dfree(c->d_dirs); dfree(c->d_geo);
     dfree(c->d_ctr32); dfree(c->d_ctr64); dfree(c->d_hist); dfree(c->d_fx);
     dfree(c->f_env); dfree(c->f_car); dfree(c->f_part); dfree(c->f_out); dfree(c->f_tab);
-    dfree(c->d_map_ctr); dfree(c->d_map_start); dfree(c->d_map_list); dfree(c->d_map_hist);
+    dfree(c->d_map_ctr); dfree(c->d_map_start); dfree(c->d_map_list); dfree(c->d_map_lctr); dfree(c->d_map_hist);
     dfree(c->d_cnt); dfree(c->d_scal); dfree(c->d_band); dfree(c->d_rooms); dfree(c->d_live);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -1148,7 +1150,15 @@ template <class R> int run_trace(pach_ctx *c, const pach_params p)
         m.ctr = c->d_map_ctr;
         m.start = c->d_map_start;
         m.list = c->d_map_list;
+        m.lctr = c->d_map_lctr;
+        m.scr = c->map_scr;
         for (int k = 0; k < 3; ++k) {
+            // a deposit needs a point of the segment within r of a centre, so
+            // inside [bmin, bmax]; widened by r and by 1e-5 of the box's
+            // coordinates against the rounding of segment ends and of FP32
+            const double pad = c->map_r + 1e-5 * (1 + std::max(std::fabs(c->map_bmin[k]), std::fabs(c->map_bmax[k])));
+            m.lo[k] = (float)(c->map_bmin[k] - pad);
+            m.hi[k] = (float)(c->map_bmax[k] + pad);
             m.bmin[k] = c->map_bmin[k];
             m.bmax[k] = c->map_bmax[k];
             m.vd[k] = c->map_vd[k];
@@ -1476,6 +1486,25 @@ extern "C" int pach_map_receivers_set(pach_ctx *c, const double *centers, int M,
         list.insert(list.end(), cells[q].begin(), cells[q].end());
     }
     if (list.empty()) list.push_back(0);
+    // The walk screens each listed receiver in FP32 from a copy of its centre
+    // stored with the entry (one load per entry).  Rounding a centre C to FP32
+    // moves it by e <= sqrt(3) 2^-24 max|C|, which moves the line's distance
+    // from it by at most e; for a sphere the line meets (distance <= r) the
+    // screened discriminant r^2 - distance^2 then falls by at most
+    // 2 r e + e^2, widened here by a factor of 2.
+    std::vector<float4> lctr(list.size());
+    double cmax = 0;
+    for (int i = 0; i < 3 * M; ++i) cmax = std::max(cmax, std::fabs(centers[i]));
+    for (size_t q = 0; q < list.size(); ++q) {
+        const double *P = centers + 3 * (size_t)list[q];
+        uint32_t j = list[q];
+        float w;
+        std::memcpy(&w, &j, 4);
+        lctr[q] = make_float4((float)P[0], (float)P[1], (float)P[2], w);
+    }
+    const double e = std::sqrt(3.0) * std::ldexp(cmax, -24);
+    HIPCHK(c, dalloc(c->d_map_lctr, lctr.size()));
+    HIPCHK(c, hipMemcpy(c->d_map_lctr, lctr.data(), lctr.size() * sizeof(float4), hipMemcpyHostToDevice));
     HIPCHK(c, dalloc(c->d_map_ctr, M));
     HIPCHK(c, dalloc(c->d_map_start, (size_t)ncell + 1));
     HIPCHK(c, dalloc(c->d_map_list, list.size()));
@@ -1487,6 +1516,7 @@ extern "C" int pach_map_receivers_set(pach_ctx *c, const double *centers, int M,
     c->map_ctr.assign(centers, centers + 3 * M);
     c->map_n = M;
     c->map_r = r;
+    c->map_scr = (float)(2 * (2 * r * e + e * e));
     c->map_sct = sample_ct;
     c->map_directional = directional ? 1 : 0;
     c->map_time1pt = time1pt;

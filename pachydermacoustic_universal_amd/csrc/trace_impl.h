@@ -1087,73 +1087,102 @@ __device__ void map_check(const DevRecv<R> &rv, const R *oR, const R *dR, R ex, 
     const double o[3] = {(double)oR[0], (double)oR[1], (double)oR[2]};
     const double d[3] = {(double)dR[0], (double)dR[1], (double)dR[2]};
     const double end[3] = {(double)ex, (double)ey, (double)ez};
+    // A deposit needs the point of the segment nearest a centre (the test's
+    // t in (0, |segment|)) within r of it, so inside the receivers' box: a
+    // segment whose bounding box misses that box (widened by r) deposits nothing.
+    if (fmax(oR[0], ex) < (R)m.lo[0] || fmin(oR[0], ex) > (R)m.hi[0] || fmax(oR[1], ey) < (R)m.lo[1] ||
+        fmin(oR[1], ey) > (R)m.hi[1] || fmax(oR[2], ez) < (R)m.lo[2] || fmin(oR[2], ez) > (R)m.hi[2])
+        return;
     MapWalk w;
     int Xe[3];
     if (!map_walk_init(m, o, d, end, broadband, w, Xe)) return;
     const double qx = end[0] - o[0], qy = end[1] - o[1], qz = end[2] - o[2];
     const double sqd = qx * qx + qy * qy + qz * qz;
+    const float dfx = (float)d[0], dfy = (float)d[1], dfz = (float)d[2], r2f = (float)m.r2;
+    // FP32 screen of one list entry: the line misses the sphere by more than
+    // the FP32 rounding of this test (1e-5 |m|^2 against ~1e-6) and of the
+    // entry's FP32 centre (m.scr) can hide, so the FP64 test would reject it too
+    auto screen = [&](const float4 &L) __attribute__((always_inline)) {
+        float fx, fy, fz;
+        if constexpr (std::is_same<R, float>::value) {
+            fx = oR[0] - L.x;
+            fy = oR[1] - L.y;
+            fz = oR[2] - L.z;
+        } else {
+            fx = (float)(o[0] - (double)L.x);
+            fy = (float)(o[1] - (double)L.y);
+            fz = (float)(o[2] - (double)L.z);
+        }
+        const float fb = fx * dfx + fy * dfy + fz * dfz;
+        const float fm = fx * fx + fy * fy + fz * fz;
+        return !(fb * fb - (fm - r2f) < -1e-5f * fm - 1e-12f - m.scr);
+    };
     for (int visit = 0;; ++visit) {
         const int cell = map_cell(m, w);
-        for (uint32_t k = m.start[cell]; k < m.start[cell + 1]; ++k) {
-            const uint32_t j = m.list[k];
-            const double4 C = m.ctr[j];
-            {
-                // FP32 screen: the line misses the sphere by more than the
-                // FP32 rounding of this test can hide (1e-5 |m|^2 against
-                // ~1e-6), so the FP64 test below would reject it too
-                const float fx = (float)(o[0] - C.x), fy = (float)(o[1] - C.y), fz = (float)(o[2] - C.z);
-                const float fb = fx * (float)d[0] + fy * (float)d[1] + fz * (float)d[2];
-                const float fm = fx * fx + fy * fy + fz * fz;
-                if (fb * fb - (fm - (float)m.r2) < -1e-5f * fm - 1e-12f) continue;
-            }
-            const double mx = o[0] - C.x, my = o[1] - C.y, mz = o[2] - C.z;
-            const double b = mx * d[0] + my * d[1] + mz * d[2];
-            const double c = (mx * mx + my * my + mz * mz) - m.r2;
-            if (c > 0 && b > 0) continue;
-            const double discr = b * b - c;
-            if (discr < 0) continue;
-            const double t1 = -b - sqrt(discr), t2 = -b + sqrt(discr);
-            const double t = (t1 + t2) / 2;
-            if (!(t > 0 && t * t < sqd)) continue;
-            // tested once per segment: skip a receiver an earlier voxel of the
-            // walk lists (the walk is replayed from its start; hits are rare)
-            bool earlier = false;
-            if (visit > 0) {
-                MapWalk v;
-                int Xv[3];
-                map_walk_init(m, o, d, end, broadband, v, Xv);
-                for (int q = 0; q < visit && !earlier; ++q) {
-                    earlier = map_listed(m, map_cell(m, v), j);
-                    map_walk_step(m, v, Xv, broadband);
-                }
-            }
-            if (earlier) continue;
-            const double raydist = t / m.c + t_sum - C.w;
-            // Histogram.Add drops samples outside [0, SampleCT); Histogram_1Pt.Add
-            // (Classes_Receivers.cs:1393-1396) adds every deposit to sample 0
-            const int sample = m.time1pt ? 0 : (int)(raydist * 1000);
-            if (!m.time1pt && (sample >= m.sct || sample < 0)) continue;
-            const int b_lo = broadband ? 0 : band, b_hi = broadband ? kBands - 1 : band;
-            for (int oc = b_lo; oc <= b_hi; ++oc) {
-                double Ei;
-                if constexpr (BROAD) Ei = (double)E[oc];
-                else Ei = (double)I;
-                const double e = Ei * pow(10.0, -.1 * rv.band->atten[oc] * raydist) * m.sizemod;
-                const uint64_t idx = ((uint64_t)j * kBands + oc) * m.sct + sample;
-                unsafeAtomicAdd(m.hist + idx, e);
-                if (m.time1pt) {
-                    // the energy plane only
-                } else if (m.directional) {
-                    unsafeAtomicAdd(m.hist + m.plane_stride + idx, sqrt(e * m.rho_c));
-                    for (int a = 0; a < 3; ++a) {
-                        const double dk = -d[a];
-                        const double vv = (double)(float)(dk * e);
-                        unsafeAtomicAdd(m.hist + (uint64_t)(dk > 0 ? 2 + a : 5 + a) * m.plane_stride + idx, vv);
+        const uint32_t k1 = m.start[cell + 1];
+        // four entries per step, their loads issued together
+        for (uint32_t k = m.start[cell]; k < k1; k += 4) {
+            float4 L[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) L[u] = m.lctr[min(k + (uint32_t)u, k1 - 1)];
+            uint32_t pass = 0;
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (k + (uint32_t)u < k1 && screen(L[u])) pass |= 1u << u;
+            while (pass) {
+                const int u = __builtin_ctz(pass);
+                pass &= pass - 1;
+                const uint32_t j = __float_as_uint(u == 0 ? L[0].w : (u == 1 ? L[1].w : (u == 2 ? L[2].w : L[3].w)));
+                const double4 C = m.ctr[j];
+                const double mx = o[0] - C.x, my = o[1] - C.y, mz = o[2] - C.z;
+                const double b = mx * d[0] + my * d[1] + mz * d[2];
+                const double c = (mx * mx + my * my + mz * mz) - m.r2;
+                if (c > 0 && b > 0) continue;
+                const double discr = b * b - c;
+                if (discr < 0) continue;
+                const double t1 = -b - sqrt(discr), t2 = -b + sqrt(discr);
+                const double t = (t1 + t2) / 2;
+                if (!(t > 0 && t * t < sqd)) continue;
+                // tested once per segment: skip a receiver an earlier voxel of the
+                // walk lists (the walk is replayed from its start; hits are rare)
+                bool earlier = false;
+                if (visit > 0) {
+                    MapWalk v;
+                    int Xv[3];
+                    map_walk_init(m, o, d, end, broadband, v, Xv);
+                    for (int q = 0; q < visit && !earlier; ++q) {
+                        earlier = map_listed(m, map_cell(m, v), j);
+                        map_walk_step(m, v, Xv, broadband);
                     }
-                } else {
-                    unsafeAtomicAdd(m.hist + m.plane_stride + idx, sqrt(e));
                 }
-                lc.add(C_DEP);
+                if (earlier) continue;
+                const double raydist = t / m.c + t_sum - C.w;
+                // Histogram.Add drops samples outside [0, SampleCT); Histogram_1Pt.Add
+                // (Classes_Receivers.cs:1393-1396) adds every deposit to sample 0
+                const int sample = m.time1pt ? 0 : (int)(raydist * 1000);
+                if (!m.time1pt && (sample >= m.sct || sample < 0)) continue;
+                const int b_lo = broadband ? 0 : band, b_hi = broadband ? kBands - 1 : band;
+                for (int oc = b_lo; oc <= b_hi; ++oc) {
+                    double Ei;
+                    if constexpr (BROAD) Ei = (double)E[oc];
+                    else Ei = (double)I;
+                    const double e = Ei * pow(10.0, -.1 * rv.band->atten[oc] * raydist) * m.sizemod;
+                    const uint64_t idx = ((uint64_t)j * kBands + oc) * m.sct + sample;
+                    unsafeAtomicAdd(m.hist + idx, e);
+                    if (m.time1pt) {
+                        // the energy plane only
+                    } else if (m.directional) {
+                        unsafeAtomicAdd(m.hist + m.plane_stride + idx, sqrt(e * m.rho_c));
+                        for (int a = 0; a < 3; ++a) {
+                            const double dk = -d[a];
+                            const double vv = (double)(float)(dk * e);
+                            unsafeAtomicAdd(m.hist + (uint64_t)(dk > 0 ? 2 + a : 5 + a) * m.plane_stride + idx, vv);
+                        }
+                    } else {
+                        unsafeAtomicAdd(m.hist + m.plane_stride + idx, sqrt(e));
+                    }
+                    lc.add(C_DEP);
+                }
             }
         }
         if (!map_walk_step(m, w, Xe, broadband)) break;

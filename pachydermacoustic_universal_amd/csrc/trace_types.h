@@ -94,6 +94,9 @@ struct DevMap {
     const double4 *ctr;      // [n] centre, w = Direct_Time (|source - centre| / c)
     const uint32_t *start;   // [ncell + 1] first list entry of each cell
     const uint32_t *list;    // receiver indices
+    const float4 *lctr;      // [list entries] FP32 centre of each entry's receiver, w = its index (bits)
+    float scr;               // absolute widening of the FP32 screen for the rounding of lctr
+    float lo[3], hi[3];      // the receiver box widened by one radius (segment-box screen)
     double bmin[3], bmax[3], vd[3];
     int ct[3];
     int n;
