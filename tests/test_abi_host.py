@@ -293,3 +293,41 @@ def test_grid_rejects_bad_input():
     nl = C.c_int64(0)
     assert A.lib().pach_grid_build(v.ctypes.data_as(C.POINTER(C.c_double)), 1, d, None, None, None,
                                    None, C.byref(nl), None) == -1
+
+
+@pytest.mark.parametrize("r,scale", [(0.25, 40.0), (0.05, 150.0)])
+def test_map_entry_screen_never_rejects_a_line_the_fp64_test_accepts(r, scale):
+    """The mapping receivers' FP32 screen (trace_impl.h map_check) uses an FP32
+    copy Cf of each centre and widens its threshold by 2 (2 r e + e^2), e =
+    sqrt(3) 2^-24 max|C| (runtime.cpp pach_map_receivers_set).  On lines that
+    pass within r of a centre by 1e-9 .. 1e-4 of r (where the FP64 test's
+    discriminant is barely >= 0) the screen must accept every one the FP64
+    test accepts; without the widening it would reject some of them."""
+    rng = np.random.default_rng(11)
+    n = 400_000
+    f32 = np.float32
+    o = rng.uniform(-scale, scale, (n, 3)).astype(f32)
+    d = rng.normal(size=(n, 3))
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(f32)
+    # a unit normal to d, and the centre at distance rho from the line
+    a = np.cross(d.astype(np.float64), rng.normal(size=(n, 3)))
+    a /= np.linalg.norm(a, axis=1, keepdims=True)
+    rho = r * (1 - 10.0 ** rng.uniform(-9, -4, n))
+    t = rng.uniform(-3, 3, n)
+    C = o.astype(np.float64) + t[:, None] * d.astype(np.float64) + rho[:, None] * a
+    # the FP64 test of map_check: b^2 - c >= 0
+    m = o.astype(np.float64) - C
+    b = np.einsum("ij,ij->i", m, d.astype(np.float64))
+    c = np.einsum("ij,ij->i", m, m) - r * r
+    hit = b * b - c >= 0
+    assert hit.sum() > n // 4
+    e = np.sqrt(3.0) * np.ldexp(np.abs(C).max(), -24)
+    scr = f32(2 * (2 * r * e + e * e))
+    # the FP32 screen, operation by operation in FP32
+    fx = o - C.astype(f32)
+    fb = (fx[:, 0] * d[:, 0] + fx[:, 1] * d[:, 1]) + fx[:, 2] * d[:, 2]
+    fm = (fx[:, 0] * fx[:, 0] + fx[:, 1] * fx[:, 1]) + fx[:, 2] * fx[:, 2]
+    lhs = fb * fb - (fm - f32(r * r))
+    base = -f32(1e-5) * fm - f32(1e-12)
+    assert not np.any(hit & (lhs < base - scr))
+    assert np.any(hit & (lhs < base)), "the widening is not exercised"
