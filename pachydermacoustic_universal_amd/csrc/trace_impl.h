@@ -2002,8 +2002,8 @@ hipError_t launch_shoot(hipStream_t st, const TraceConst &tc, const DevScene<R> 
 
 // ---------------------------------------------------------------- launchers
 // Register cap of the mapping-receiver variant: the compiler's allocation
-// (231 VGPRs, two waves per SIMD); a cap at three waves spills 88 registers
-// and runs 15 % slower on the hall's 1,184-receiver map.
+// (254 VGPRs, two waves per SIMD); a cap at three waves spills 232 bytes per
+// lane and runs 8 % slower on the hall's 1,184-receiver map.
 constexpr int kMapOcc = 0;
 template <class R>
 hipError_t launch_phase1(hipStream_t st, const TraceConst &tc, const DevScene<R> &s,
